@@ -1,0 +1,182 @@
+/* rtrex.h -- C ABI of the MI355X (gfx950) RTRExtractor decomposition path.
+ *
+ * Drop-in boundary for the reference's GPU layer (RTRex_CUDA/gpu_*.cuh) and its
+ * CPU twin (RTRex_Sequential/Escape/Decomposition.h): graph load -> CSR ->
+ * weighted triangle supports -> cascading clean -> greedy extraction.
+ *
+ * Conventions
+ *   - C linkage, plain pointers and sizes; no C++ or torch types.
+ *   - Every call returns an rtrex_status; nothing calls exit().  The message of
+ *     the last failure on the calling thread is in rtrex_last_error().
+ *   - Host arrays are caller-allocated; the library copies in and out.
+ *   - A graph handle owns its device memory and one HIP stream.  Different
+ *     handles may be used from different host threads; one handle must not be
+ *     used by two threads at once (same as the reference's DeviceGraph).
+ *   - CSR layout is the reference's (Escape/Graph.h:64-106): offsets int64[n+1],
+ *     nbors int64[nnz], nnz = 2m directed slots, lists sorted by id, the graph
+ *     simple and undirected.  Slot indices in outputs refer to that layout.
+ *   - Edge status bytes use the reference's characters: 'Y' alive, 'N' deleted,
+ *     'S' queued for deletion (Decomposition.h:127).
+ *   - Limits: nnz < 2^32 and n < 2^32 (device ids and slots are 32-bit).
+ */
+#ifndef RTREX_H
+#define RTREX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RTREX_OK = 0,
+    RTREX_EINVAL = 1,   /* bad argument or malformed graph                   */
+    RTREX_EDEVICE = 2,  /* HIP runtime error or no usable gfx950 device      */
+    RTREX_ENOMEM = 3,   /* device or host allocation failed                  */
+    RTREX_EIO = 4,      /* file could not be read / is truncated             */
+    RTREX_ESTATE = 5,   /* phase called out of order                         */
+    RTREX_EBUSY = 6     /* server/batch queue full                           */
+} rtrex_status;
+
+typedef struct rtrex_graph_s* rtrex_graph;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* rtrex_last_error(void);
+
+/* Library build string (kernels, target). */
+const char* rtrex_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Graph load.  Replaces loadGraph/loadGraph_Escape (GraphIO.cpp:19-88):    */
+/* header "n m", '#' comments and blank lines skipped, the first two        */
+/* integers of each line are the edge (extra columns such as weights are    */
+/* ignored, GraphIO.cpp:40-41), fewer than m edge lines is RTREX_EIO        */
+/* (GraphIO.cpp:67-71).  Edges beyond m are ignored.  Returns the edges as  */
+/* written (one per line, not mirrored).                                     */
+rtrex_status rtrex_escape_header(const char* path, int64_t* n, int64_t* m);
+rtrex_status rtrex_escape_read(const char* path, int64_t m, uint32_t* u,
+                               uint32_t* v);
+
+/* ------------------------------------------------------------------------ */
+/* Graph lifecycle.                                                          */
+
+/* Host CSR -> device.  Replaces transferGraphToDevice + allocateDeviceState
+ * (gpu_graph.cuh:47-57; gpu_graph.cu:46-135).  The partner slot of every edge
+ * (CGraph::getPartnerMap, Graph.cpp:320-340) is derived on the device.  The
+ * CSR must be simple, symmetric and sorted (RTREX_EINVAL otherwise). */
+rtrex_status rtrex_graph_create_csr(const int64_t* offsets,
+                                    const int64_t* nbors, int64_t n,
+                                    int64_t nnz, int device, rtrex_graph* out);
+
+/* Undirected edge list (each edge once, any order) -> device CSR.  Replaces
+ * the Escape loader's mirroring (GraphIO.cpp:54-62) + makeCSR
+ * (Graph.cpp:344-384) + getPartnerMap: the CSR is built on the device with a
+ * radix sort.  Self-loops, duplicate edges and ids >= n are RTREX_EINVAL. */
+rtrex_status rtrex_graph_create_edges(const uint32_t* u, const uint32_t* v,
+                                      int64_t n, int64_t m, int device,
+                                      rtrex_graph* out);
+
+/* Releases all device memory of the handle.  Replaces freeDeviceGraph
+ * (gpu_graph.cuh:62).  NULL is accepted. */
+void rtrex_graph_destroy(rtrex_graph g);
+
+rtrex_status rtrex_graph_info(rtrex_graph g, int64_t* n, int64_t* nnz);
+
+/* Copies the device CSR back in the reference layout (for makeCSR parity). */
+rtrex_status rtrex_graph_get_csr(rtrex_graph g, int64_t* offsets,
+                                 int64_t* nbors);
+
+/* Device bytes a graph of this size holds while resident (all phases). */
+int64_t rtrex_graph_footprint_bytes(int64_t n, int64_t nnz);
+
+/* ------------------------------------------------------------------------ */
+/* Phases (each also callable alone, in this order).                         */
+
+/* Weighted triangle supports.  Replaces gpuCommonNbr
+ * (gpu_triangle_count.cuh:23) / commonNbr (Decomposition.h:42-117).  The
+ * per-edge and per-vertex FP64 sums are accumulated in the reference's order
+ * (ascending third vertex per edge; lexicographic per vertex), so they are
+ * bit-identical to the sequential reference.  *total = triangle count. */
+rtrex_status rtrex_count(rtrex_graph g, int64_t* total);
+
+/* Current supports, per directed slot (both copies of an edge are equal) and
+ * per vertex.  Replaces syncWeightsDeviceToHost (gpu_graph.cuh:67-71).  Any
+ * pointer may be NULL. */
+rtrex_status rtrex_get_supports(rtrex_graph g, double* perEdge,
+                                double* perVertex, int64_t* total);
+
+/* Cascading clean to convergence.  Replaces gpuInitialClean
+ * (gpu_clean.cuh:30) / initialClean (Decomposition.h:302-460): every edge
+ * with support < eps * 1/(d_u d_v) is deleted, with cascades, until none
+ * remains.  There is no round cap. */
+rtrex_status rtrex_initial_clean(rtrex_graph g, double eps);
+
+/* Status byte per directed slot ('Y','N','S').  Replaces
+ * syncEdgeStatusDeviceToHost (gpu_graph.cuh:76). */
+rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status);
+
+/* Deletes the k undirected edges pairs[2k] (each must be alive) and cleans to
+ * convergence.  Replaces gpuDeleteAndClean (gpu_clean.cuh:44-45) /
+ * deleteAndClean (Decomposition.h:147-283). */
+rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
+                                    int64_t k, double eps);
+
+/* Greedy extraction after rtrex_initial_clean.  Replaces disjointExtractCUDA
+ * (RTRex_CUDA/main.cu:130-338) / disjointExtract (Decomposition.h:538-940).
+ * cluster_map[n] receives the reference's cluster id of every vertex (ids in
+ * the order the sequential reference creates them); *n_clusters the count. */
+rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
+                           int64_t* n_clusters);
+
+/* count + initial clean + extract in one call (RTRex.cpp:193). */
+rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
+                             int64_t* n_clusters);
+
+/* Per-cluster statistics over the original graph, populateStats
+ * (Decomposition.h:473-491): arrays of length n_clusters, indexed by id. */
+rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
+                                 int64_t* n_edges, int64_t* cut);
+
+/* Decision audit.  Every threshold or ratio decision whose margin is within
+ * the FP64 error bound of the reference's own summation is counted as
+ * ambiguous; with zero ambiguous decisions the survivor and cluster sets are
+ * provably those of the sequential reference. */
+typedef struct {
+    int64_t clean_rounds;         /* frontier rounds, all clean calls        */
+    int64_t edges_removed;        /* undirected edges deleted by cleaning    */
+    int64_t triangles_removed;
+    int64_t ambiguous_clean;      /* near-threshold removal/survival         */
+    int64_t ambiguous_ratio;      /* near-tie ratio-prefix decisions         */
+    int64_t seeds;                /* clusters formed                          */
+    int64_t components;           /* alive components at extraction start    */
+    int64_t largest_component;    /* vertices in the largest one             */
+} rtrex_audit;
+rtrex_status rtrex_get_audit(rtrex_graph g, rtrex_audit* out);
+
+/* Device time of the last run of each phase, milliseconds (HIP events). */
+typedef struct {
+    double build_ms, count_ms, clean_ms, extract_ms;
+    double count_edge_kernel_ms;  /* per-edge support kernel(s) alone        */
+    double count_vertex_kernel_ms;/* per-vertex fold kernel(s) alone         */
+} rtrex_timings;
+rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
+
+/* ------------------------------------------------------------------------ */
+/* Batched decomposition (no reference counterpart: the reference runs one   */
+/* graph per process).  n_graphs independent graphs given as concatenated    */
+/* edge lists with LOCAL ids: graph i has vtx[i] vertices and edges          */
+/* [eoff[i], eoff[i+1]) of u/v.  All graphs are resident at once and every   */
+/* phase runs over all of them per launch.  cluster_map (sum of vtx) gets    */
+/* per-graph cluster ids; n_clusters[i] per graph.  Results equal running    */
+/* each graph alone.                                                         */
+rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
+                                   const int64_t* eoff, const uint32_t* u,
+                                   const uint32_t* v, double eps, int device,
+                                   int64_t* cluster_map, int64_t* n_clusters,
+                                   rtrex_timings* timings, rtrex_audit* audit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTREX_H */

@@ -1,0 +1,71 @@
+/* CPU oracle for the RTRExtractor decomposition path (count -> clean -> extract).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a project-written C++ restatement of the
+ * reference's sequential algorithm (RTRex_Sequential, mode 's').  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and only as
+ * the checker.  The product (rtrex_cuda_amd) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  The restatement
+ * is pinned by tests/test_oracle.py against: the golden vectors produced by the
+ * reference itself (oracle/_ref/ref_dump built from the reference sources),
+ * brute-force triangle enumeration, K_n closed forms and the order-independent
+ * fixpoint of the clean phase.
+ *
+ * All arrays use the reference's CSR layout: offsets int64[n+1], nbors
+ * int64[nnz] with nnz = 2m directed slots, each adjacency list sorted by id.
+ */
+#ifndef RTREX_ORACLE_H
+#define RTREX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* COO (both directions already present, as the Escape loader emits them,
+ * GraphIO.cpp:54-62) -> CSR sorted by (src, dst) (Graph.cpp:344-384). */
+int oracle_make_csr(int64_t n, int64_t nnz, const int64_t* src,
+                    const int64_t* dst, int64_t* offsets, int64_t* nbors);
+
+/* Weighted triangle supports, commonNbr (Decomposition.h:42-117). */
+int oracle_count(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                 int64_t* total, double* perEdge, double* perVertex);
+
+/* initialClean with sequential seeding (Decomposition.h:302-460) applied to
+ * supports produced by oracle_count.  perEdge/perVertex/total are updated in
+ * place; status receives 'Y' or 'N' per directed slot. */
+int oracle_initial_clean(int64_t n, const int64_t* offsets,
+                         const int64_t* nbors, double eps, double* perEdge,
+                         double* perVertex, int64_t* total, uint8_t* status);
+
+/* deleteAndClean (Decomposition.h:147-283) on an existing state: the k
+ * undirected edges in pairs[2k] (each must be 'Y') are pushed in order, marked
+ * 'S' and then cascaded to convergence. */
+int oracle_delete_and_clean(int64_t n, const int64_t* offsets,
+                            const int64_t* nbors, double eps,
+                            const int64_t* pairs, int64_t k, double* perEdge,
+                            double* perVertex, int64_t* total,
+                            uint8_t* status);
+
+/* Full disjointExtract (Decomposition.h:538-940): count, clean, then greedy
+ * extraction.  Outputs: cluster_map[n] (cluster id per vertex, ids in
+ * extraction order), *n_clusters, survivors status[nnz] after the initial
+ * clean (may be NULL), and phase wall times in seconds t[3] (may be NULL).
+ * If flat != NULL it receives, per cluster in id order: nVertices, nEdges,
+ * cut, then the vertices in the reference's iteration order; flat_cap bounds
+ * it (n_clusters*3 + n suffices) and *flat_len receives the length used. */
+int oracle_decompose(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                     double eps, int64_t* cluster_map, int64_t* n_clusters,
+                     uint8_t* status_after_clean, int64_t* flat,
+                     int64_t flat_cap, int64_t* flat_len, double* t);
+
+/* connectedComp (Decomposition.h:1023-1068): component id per vertex in BFS
+ * discovery order; returns the number of components in *n_comp. */
+int oracle_connected_components(int64_t n, const int64_t* offsets,
+                                const int64_t* nbors, int64_t* comp,
+                                int64_t* n_comp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

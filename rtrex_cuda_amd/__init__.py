@@ -1,0 +1,272 @@
+"""rtrex_cuda_amd -- MI355X (gfx950) RTRExtractor decomposition path.
+
+Python bindings (ctypes) over the C ABI in ``include/rtrex.h``, implemented by
+``librtrex_hip.so`` (hand-written HIP kernels).  The method names mirror the
+reference's GPU layer (RTRex_CUDA/gpu_*.cuh) and CPU driver
+(RTRex_Sequential/Escape/Decomposition.h):
+
+    DeviceGraph.from_csr(offsets, nbors)     transferGraphToDevice + allocateDeviceState
+    g.gpu_common_nbr()                       gpuCommonNbr / commonNbr
+    g.sync_weights()                         syncWeightsDeviceToHost
+    g.gpu_initial_clean(eps)                 gpuInitialClean / initialClean
+    g.sync_edge_status()                     syncEdgeStatusDeviceToHost
+    g.gpu_delete_and_clean(pairs, eps)       gpuDeleteAndClean / deleteAndClean
+    g.disjoint_extract(eps)                  disjointExtractCUDA / disjointExtract
+    g.populate_stats()                       populateStats
+    g.close()                                freeDeviceGraph
+
+Errors raise ``RtrexError`` carrying the library's status and message.  There
+is no CPU fallback: without the built library or a gfx950 device every call
+fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(PKG, "librtrex_hip.so")
+GEN_LIB = os.path.join(PKG, "librtrex_gen.so")
+
+STATUS = {0: "OK", 1: "EINVAL", 2: "EDEVICE", 3: "ENOMEM", 4: "EIO", 5: "ESTATE", 6: "EBUSY"}
+
+# Every symbol include/rtrex.h declares.
+EXPORTS = [
+    "rtrex_last_error", "rtrex_version", "rtrex_escape_header", "rtrex_escape_read",
+    "rtrex_graph_create_csr", "rtrex_graph_create_edges", "rtrex_graph_destroy",
+    "rtrex_graph_info", "rtrex_graph_get_csr", "rtrex_graph_footprint_bytes",
+    "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
+    "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
+    "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_decompose",
+]
+
+
+class RtrexError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"rtrex {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Audit(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in (
+        "clean_rounds", "edges_removed", "triangles_removed", "ambiguous_clean",
+        "ambiguous_ratio", "seeds", "components", "largest_component")]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Timings(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "build_ms", "count_ms", "clean_ms", "extract_ms",
+        "count_edge_kernel_ms", "count_vertex_kernel_ms")]
+
+    def as_dict(self) -> dict:
+        return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+_lib = None
+_gen = None
+P64, PU32, PD, PU8 = C.POINTER(C.c_int64), C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+
+
+def lib() -> C.CDLL:
+    """Loads librtrex_hip.so (build it first: python -m rtrex_cuda_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HIP_LIB):
+            raise RtrexError(2, f"{HIP_LIB} is not built; run python -m rtrex_cuda_amd.build")
+        L = C.CDLL(HIP_LIB)
+        G = C.c_void_p
+        L.rtrex_last_error.restype = C.c_char_p
+        L.rtrex_version.restype = C.c_char_p
+        sig = {
+            "rtrex_escape_header": [C.c_char_p, P64, P64],
+            "rtrex_escape_read": [C.c_char_p, C.c_int64, PU32, PU32],
+            "rtrex_graph_create_csr": [P64, P64, C.c_int64, C.c_int64, C.c_int, C.POINTER(G)],
+            "rtrex_graph_create_edges": [PU32, PU32, C.c_int64, C.c_int64, C.c_int, C.POINTER(G)],
+            "rtrex_graph_info": [G, P64, P64],
+            "rtrex_graph_get_csr": [G, P64, P64],
+            "rtrex_count": [G, P64],
+            "rtrex_get_supports": [G, PD, PD, P64],
+            "rtrex_initial_clean": [G, C.c_double],
+            "rtrex_get_edge_status": [G, PU8],
+            "rtrex_delete_and_clean": [G, P64, C.c_int64, C.c_double],
+            "rtrex_extract": [G, C.c_double, P64, P64],
+            "rtrex_decompose": [G, C.c_double, P64, P64],
+            "rtrex_cluster_stats": [G, P64, P64, P64],
+            "rtrex_get_audit": [G, C.POINTER(Audit)],
+            "rtrex_get_timings": [G, C.POINTER(Timings)],
+            "rtrex_batch_decompose": [C.c_int64, P64, P64, PU32, PU32, C.c_double, C.c_int,
+                                      P64, P64, C.POINTER(Timings), C.POINTER(Audit)],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.rtrex_graph_destroy.argtypes = [G]
+        L.rtrex_graph_destroy.restype = None
+        L.rtrex_graph_footprint_bytes.argtypes = [C.c_int64, C.c_int64]
+        L.rtrex_graph_footprint_bytes.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RtrexError(rc, lib().rtrex_last_error().decode())
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def version() -> str:
+    return lib().rtrex_version().decode()
+
+
+def footprint_bytes(n: int, nnz: int) -> int:
+    return int(lib().rtrex_graph_footprint_bytes(n, nnz))
+
+
+# --------------------------------------------------------------------- loader
+def load_escape(path: str):
+    """Escape text file -> (n, u, v) with u/v uint32, one entry per line
+    (loadGraph_Escape, GraphIO.cpp:19-74)."""
+    n = C.c_int64()
+    m = C.c_int64()
+    b = os.fsencode(path)
+    _check(lib().rtrex_escape_header(b, C.byref(n), C.byref(m)))
+    u = np.empty(m.value, np.uint32)
+    v = np.empty(m.value, np.uint32)
+    _check(lib().rtrex_escape_read(b, m.value, _p(u, PU32), _p(v, PU32)))
+    return int(n.value), u, v
+
+
+# ---------------------------------------------------------------------- graph
+class DeviceGraph:
+    """A graph resident in HBM with its per-edge / per-vertex state."""
+
+    def __init__(self, handle: C.c_void_p, n: int, nnz: int):
+        self._h = handle
+        self.n = n
+        self.nnz = nnz
+
+    @classmethod
+    def from_csr(cls, offsets, nbors, device: int = 0) -> "DeviceGraph":
+        off = np.ascontiguousarray(offsets, np.int64)
+        nb = np.ascontiguousarray(nbors, np.int64)
+        h = C.c_void_p()
+        _check(lib().rtrex_graph_create_csr(_p(off, P64), _p(nb, P64), len(off) - 1,
+                                            len(nb), device, C.byref(h)))
+        return cls(h, len(off) - 1, len(nb))
+
+    @classmethod
+    def from_edges(cls, n: int, u, v, device: int = 0) -> "DeviceGraph":
+        u = np.ascontiguousarray(u, np.uint32)
+        v = np.ascontiguousarray(v, np.uint32)
+        h = C.c_void_p()
+        _check(lib().rtrex_graph_create_edges(_p(u, PU32), _p(v, PU32), n, len(u), device,
+                                              C.byref(h)))
+        return cls(h, n, 2 * len(u))
+
+    def close(self) -> None:  # freeDeviceGraph
+        if self._h:
+            lib().rtrex_graph_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def csr(self):
+        off = np.empty(self.n + 1, np.int64)
+        nb = np.empty(self.nnz, np.int64)
+        _check(lib().rtrex_graph_get_csr(self._h, _p(off, P64), _p(nb, P64)))
+        return off, nb
+
+    def gpu_common_nbr(self) -> int:
+        t = C.c_int64()
+        _check(lib().rtrex_count(self._h, C.byref(t)))
+        return int(t.value)
+
+    def sync_weights(self):
+        pe = np.empty(self.nnz, np.float64)
+        pv = np.empty(self.n, np.float64)
+        t = C.c_int64()
+        _check(lib().rtrex_get_supports(self._h, _p(pe, PD), _p(pv, PD), C.byref(t)))
+        return pe, pv, int(t.value)
+
+    def gpu_initial_clean(self, eps: float) -> None:
+        _check(lib().rtrex_initial_clean(self._h, float(eps)))
+
+    def sync_edge_status(self) -> np.ndarray:
+        st = np.empty(self.nnz, np.uint8)
+        _check(lib().rtrex_get_edge_status(self._h, _p(st, PU8)))
+        return st
+
+    def gpu_delete_and_clean(self, pairs, eps: float) -> None:
+        pr = np.ascontiguousarray(pairs, np.int64).reshape(-1)
+        _check(lib().rtrex_delete_and_clean(self._h, _p(pr, P64), len(pr) // 2, float(eps)))
+
+    def disjoint_extract(self, eps: float):
+        cm = np.empty(self.n, np.int64)
+        k = C.c_int64()
+        _check(lib().rtrex_extract(self._h, float(eps), _p(cm, P64), C.byref(k)))
+        return cm, int(k.value)
+
+    def decompose(self, eps: float):
+        cm = np.empty(self.n, np.int64)
+        k = C.c_int64()
+        _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64), C.byref(k)))
+        return cm, int(k.value)
+
+    def populate_stats(self, n_clusters: int):
+        nv = np.empty(n_clusters, np.int64)
+        ne = np.empty(n_clusters, np.int64)
+        cut = np.empty(n_clusters, np.int64)
+        _check(lib().rtrex_cluster_stats(self._h, _p(nv, P64), _p(ne, P64), _p(cut, P64)))
+        return nv, ne, cut
+
+    def audit(self) -> dict:
+        a = Audit()
+        _check(lib().rtrex_get_audit(self._h, C.byref(a)))
+        return a.as_dict()
+
+    def timings(self) -> dict:
+        t = Timings()
+        _check(lib().rtrex_get_timings(self._h, C.byref(t)))
+        return t.as_dict()
+
+
+def batch_decompose(graphs, eps: float, device: int = 0):
+    """graphs: list of (n, u, v) with local ids.  Returns (list of cluster maps,
+    list of cluster counts, timings, audit)."""
+    vtx = np.array([g[0] for g in graphs], np.int64)
+    sizes = np.array([len(g[1]) for g in graphs], np.int64)
+    eoff = np.zeros(len(graphs) + 1, np.int64)
+    np.cumsum(sizes, out=eoff[1:])
+    u = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.uint32) for g in graphs]))
+    v = np.ascontiguousarray(np.concatenate([np.asarray(g[2], np.uint32) for g in graphs]))
+    cm = np.empty(int(vtx.sum()), np.int64)
+    nc = np.empty(len(graphs), np.int64)
+    t = Timings()
+    a = Audit()
+    _check(lib().rtrex_batch_decompose(len(graphs), _p(vtx, P64), _p(eoff, P64), _p(u, PU32),
+                                       _p(v, PU32), float(eps), device, _p(cm, P64),
+                                       _p(nc, P64), C.byref(t), C.byref(a)))
+    vb = np.zeros(len(graphs) + 1, np.int64)
+    np.cumsum(vtx, out=vb[1:])
+    maps = [cm[vb[i]:vb[i + 1]] for i in range(len(graphs))]
+    return maps, [int(x) for x in nc], t.as_dict(), a.as_dict()

@@ -1,0 +1,413 @@
+// C ABI of librtrex_hip.so (declared in include/rtrex.h).
+//
+// Every entry point validates its arguments, maps HIP failures to status codes
+// and records a message for rtrex_last_error(); nothing exits the process
+// (the reference's CUDA_CHECK prints and calls exit(1), gpu_common.cuh:31-39).
+// The library runs only on a gfx950 device; there is no CPU fallback.
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace rtrex {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+rtrex_status fail(rtrex_status code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+rtrex_status hip_fail(hipError_t e, const char* what, const char* file,
+                      int line) {
+    std::string m = std::string("HIP error ") + hipGetErrorString(e) + " in " +
+                    what + " (" + file + ":" + std::to_string(line) + ")";
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? RTREX_ENOMEM : RTREX_EDEVICE, m);
+}
+
+namespace {
+
+rtrex_status open_device(int device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(RTREX_EDEVICE, "no HIP device available (this library "
+                                   "runs only on MI355X / gfx950)");
+    if (device < 0 || device >= count)
+        return fail(RTREX_EINVAL, "device index out of range");
+    hipDeviceProp_t prop;
+    RT_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(RTREX_EDEVICE, std::string("device is ") + prop.gcnArchName +
+                                       ", this build targets gfx950 only");
+    RT_HIP(hipSetDevice(device));
+    return RTREX_OK;
+}
+
+rtrex_status new_handle(int device, rtrex_graph* out) {
+    if (!out) return fail(RTREX_EINVAL, "null output handle");
+    *out = nullptr;
+    RT_TRY(open_device(device));
+    rtrex_graph h = new rtrex_graph_s();
+    h->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (auto& ev : h->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) {
+        rtrex_graph_destroy(h);
+        return hip_fail(e, "stream/event creation", __FILE__, __LINE__);
+    }
+    *out = h;
+    return RTREX_OK;
+}
+
+rtrex_status check(rtrex_graph g) {
+    if (!g) return fail(RTREX_EINVAL, "null graph handle");
+    RT_HIP(hipSetDevice(g->device));
+    return RTREX_OK;
+}
+
+// Whole-file Escape parser; the rules of loadGraph_Escape (GraphIO.cpp:19-74).
+struct EscapeFile {
+    std::vector<char> buf;
+    size_t pos = 0;
+    bool next_line(const char** b, const char** e) {
+        while (pos < buf.size()) {
+            const char* s = buf.data() + pos;
+            const char* nl = static_cast<const char*>(
+                std::memchr(s, '\n', buf.size() - pos));
+            const char* t = nl ? nl : buf.data() + buf.size();
+            pos = (size_t)(t - buf.data()) + (nl ? 1 : 0);
+            if (s[0] == '#') continue;  // comment line (GraphIO.cpp:34-35)
+            const char* q = s;
+            while (q < t && std::isspace((unsigned char)*q)) ++q;
+            if (q == t) continue;  // blank line (GraphIO.cpp:37-38)
+            *b = q;
+            *e = t;
+            return true;
+        }
+        return false;
+    }
+};
+
+bool two_ints(const char* b, const char* e, int64_t* x, int64_t* y) {
+    std::string line(b, e);
+    char* p = nullptr;
+    errno = 0;
+    long long a = std::strtoll(line.c_str(), &p, 10);
+    if (p == line.c_str() || errno) return false;
+    char* q = nullptr;
+    long long c = std::strtoll(p, &q, 10);
+    if (q == p || errno) return false;
+    *x = a;
+    *y = c;
+    return true;
+}
+
+rtrex_status read_file(const char* path, EscapeFile* f) {
+    if (!path) return fail(RTREX_EINVAL, "null path");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(RTREX_EIO, std::string("could not open file ") + path);
+    std::fseek(fp, 0, SEEK_END);
+    long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    f->buf.resize(sz > 0 ? (size_t)sz : 0);
+    size_t got = sz > 0 ? std::fread(f->buf.data(), 1, (size_t)sz, fp) : 0;
+    std::fclose(fp);
+    if ((long)got != sz) return fail(RTREX_EIO, "short read");
+    return RTREX_OK;
+}
+
+}  // namespace
+}  // namespace rtrex
+
+using namespace rtrex;
+
+extern "C" {
+
+const char* rtrex_last_error(void) { return g_last_error.c_str(); }
+
+const char* rtrex_version(void) {
+    return "rtrex-mi355x 1.0 (gfx950; count/clean/extract HIP kernels)";
+}
+
+rtrex_status rtrex_escape_header(const char* path, int64_t* n, int64_t* m) {
+    EscapeFile f;
+    RT_TRY(read_file(path, &f));
+    const char *b, *e;
+    int64_t x, y;
+    if (!f.next_line(&b, &e) || !two_ints(b, e, &x, &y) || x < 0 || y < 0)
+        return fail(RTREX_EIO, "missing or malformed header line 'n m'");
+    if (n) *n = x;
+    if (m) *m = y;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_escape_read(const char* path, int64_t m, uint32_t* u,
+                               uint32_t* v) {
+    EscapeFile f;
+    RT_TRY(read_file(path, &f));
+    const char *b, *e;
+    int64_t x, y;
+    if (!f.next_line(&b, &e) || !two_ints(b, e, &x, &y))
+        return fail(RTREX_EIO, "missing header");
+    if (m != y) return fail(RTREX_EINVAL, "m does not match the header");
+    int64_t i = 0;
+    while (i < m && f.next_line(&b, &e)) {
+        if (!two_ints(b, e, &x, &y))
+            return fail(RTREX_EIO, "edge line " + std::to_string(i) +
+                                       " does not start with two integers");
+        if (x < 0 || y < 0 || x > UINT32_MAX || y > UINT32_MAX)
+            return fail(RTREX_EINVAL, "vertex id outside uint32");
+        u[i] = (uint32_t)x;
+        v[i] = (uint32_t)y;
+        ++i;
+    }
+    if (i < m)  // GraphIO.cpp:67-71
+        return fail(RTREX_EIO, "expected " + std::to_string(m) + " edges, only got " +
+                                   std::to_string(i));
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_graph_create_csr(const int64_t* offsets,
+                                    const int64_t* nbors, int64_t n,
+                                    int64_t nnz, int device, rtrex_graph* out) {
+    rtrex_graph h = nullptr;
+    RT_TRY(new_handle(device, &h));
+    rtrex_status s = build_from_csr(h, offsets, nbors, n, nnz);
+    if (s != RTREX_OK) {
+        std::string msg = g_last_error;
+        rtrex_graph_destroy(h);
+        set_error(msg);
+        return s;
+    }
+    *out = h;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_graph_create_edges(const uint32_t* u, const uint32_t* v,
+                                      int64_t n, int64_t m, int device,
+                                      rtrex_graph* out) {
+    rtrex_graph h = nullptr;
+    RT_TRY(new_handle(device, &h));
+    rtrex_status s = build_from_edges(h, u, v, n, m);
+    if (s != RTREX_OK) {
+        std::string msg = g_last_error;
+        rtrex_graph_destroy(h);
+        set_error(msg);
+        return s;
+    }
+    *out = h;
+    return RTREX_OK;
+}
+
+void rtrex_graph_destroy(rtrex_graph g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (void* p : g->allocs) (void)hipFree(p);
+    for (auto& ev : g->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+rtrex_status rtrex_graph_info(rtrex_graph g, int64_t* n, int64_t* nnz) {
+    RT_TRY(check(g));
+    if (n) *n = g->g.n;
+    if (nnz) *nnz = (int64_t)g->g.nnz;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_graph_get_csr(rtrex_graph g, int64_t* offsets,
+                                 int64_t* nbors) {
+    RT_TRY(check(g));
+    std::vector<uint32_t> o(g->g.n + 1ull), b(g->g.nnz);
+    RT_HIP(hipMemcpyAsync(o.data(), g->g.off, o.size() * 4, hipMemcpyDeviceToHost, g->stream));
+    if (g->g.nnz)
+        RT_HIP(hipMemcpyAsync(b.data(), g->g.nb, b.size() * 4, hipMemcpyDeviceToHost, g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    if (offsets) std::copy(o.begin(), o.end(), offsets);
+    if (nbors) std::copy(b.begin(), b.end(), nbors);
+    return RTREX_OK;
+}
+
+int64_t rtrex_graph_footprint_bytes(int64_t n, int64_t nnz) {
+    if (n < 0 || nnz < 0) return -1;
+    const int64_t m = nnz / 2;
+    // resident: off, nb, deg, part, s2e, st + per-edge / per-vertex state
+    int64_t resident = 4 * (n + 1) + nnz * (4 + 4 + 4 + 1) + m * (4 + 4 + 8 + 8 + 4 + 1 + 4) +
+                       n * (4 + 8 + 8 + 4 + 4 + 4) + 256;
+    // peak scratch: build (keys + tmp + src + canon + rank) or extraction
+    int64_t build = nnz * (8 + 8 + 4 + 4 + 4);
+    int64_t extract = n * (8 + 8 + 4 * 8 + 8 + 8 + 16) + nnz * 8 + m * 4;
+    return resident + std::max(build, extract);
+}
+
+rtrex_status rtrex_count(rtrex_graph g, int64_t* total) {
+    RT_TRY(check(g));
+    RT_TRY(run_count(g));
+    unsigned long long t3 = 0;
+    RT_HIP(hipMemcpyAsync(&t3, &g->ctr->tri_total, 8, hipMemcpyDeviceToHost, g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    if (total) *total = (int64_t)(t3 / 3);
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_get_supports(rtrex_graph g, double* perEdge,
+                                double* perVertex, int64_t* total) {
+    RT_TRY(check(g));
+    if (g->phase < PH_COUNTED) return fail(RTREX_ESTATE, "count has not run");
+    hipStream_t s = g->stream;
+    if (perEdge && g->g.nnz) {
+        std::vector<double> sup(g->g.m);
+        std::vector<uint32_t> s2e(g->g.nnz);
+        RT_HIP(hipMemcpyAsync(sup.data(), g->g.sup, sup.size() * 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipMemcpyAsync(s2e.data(), g->g.s2e, s2e.size() * 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < g->g.nnz; ++i) perEdge[i] = sup[s2e[i]];
+    }
+    if (perVertex && g->g.n)
+        RT_HIP(hipMemcpyAsync(perVertex, g->g.pv, g->g.n * 8ull, hipMemcpyDeviceToHost, s));
+    unsigned long long c[2] = {0, 0};
+    RT_HIP(hipMemcpyAsync(c, &g->ctr->tri_total, 16, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (total) *total = (int64_t)(c[0] / 3 - c[1]);
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_initial_clean(rtrex_graph g, double eps) {
+    RT_TRY(check(g));
+    return run_initial_clean(g, eps);
+}
+
+rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status) {
+    RT_TRY(check(g));
+    if (!status) return fail(RTREX_EINVAL, "null status array");
+    std::vector<uint8_t> st(g->g.nnz);
+    if (g->g.nnz)
+        RT_HIP(hipMemcpyAsync(st.data(), g->g.st, st.size(), hipMemcpyDeviceToHost, g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    for (uint64_t i = 0; i < g->g.nnz; ++i)
+        status[i] = st[i] == ST_ALIVE ? 'Y' : st[i] == ST_DEAD ? 'N' : 'S';
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
+                                    int64_t k, double eps) {
+    RT_TRY(check(g));
+    return run_delete_and_clean(g, pairs, k, eps);
+}
+
+rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
+                           int64_t* n_clusters) {
+    RT_TRY(check(g));
+    RT_TRY(run_extract(g, eps));
+    if (cluster_map)
+        std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
+    if (n_clusters) *n_clusters = g->n_clusters;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
+                             int64_t* n_clusters) {
+    RT_TRY(check(g));
+    RT_TRY(run_count(g));
+    RT_TRY(run_initial_clean(g, eps));
+    RT_TRY(run_extract(g, eps));
+    if (cluster_map)
+        std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
+    if (n_clusters) *n_clusters = g->n_clusters;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
+                                 int64_t* n_edges, int64_t* cut) {
+    RT_TRY(check(g));
+    return run_cluster_stats(g, n_vertices, n_edges, cut);
+}
+
+rtrex_status rtrex_get_audit(rtrex_graph g, rtrex_audit* out) {
+    RT_TRY(check(g));
+    if (!out) return fail(RTREX_EINVAL, "null output");
+    Counters c;
+    RT_HIP(hipMemcpyAsync(&c, g->ctr, sizeof(c), hipMemcpyDeviceToHost, g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    out->clean_rounds = g->audit.clean_rounds;
+    out->edges_removed = (int64_t)c.edges_removed;
+    out->triangles_removed = (int64_t)c.tri_removed;
+    out->ambiguous_clean = (int64_t)c.ambiguous_clean;
+    out->ambiguous_ratio = (int64_t)c.ambiguous_ratio;
+    out->seeds = g->phase >= PH_EXTRACTED ? g->n_clusters : 0;
+    out->components = g->audit.components;
+    out->largest_component = g->audit.largest_component;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out) {
+    RT_TRY(check(g));
+    if (!out) return fail(RTREX_EINVAL, "null output");
+    out->build_ms = g->tm.build_ms;
+    out->count_ms = g->tm.count_ms;
+    out->clean_ms = g->tm.clean_ms;
+    out->extract_ms = g->tm.extract_ms;
+    out->count_edge_kernel_ms = g->tm.count_edge_ms;
+    out->count_vertex_kernel_ms = g->tm.count_vertex_ms;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
+                                   const int64_t* eoff, const uint32_t* u,
+                                   const uint32_t* v, double eps, int device,
+                                   int64_t* cluster_map, int64_t* n_clusters,
+                                   rtrex_timings* timings, rtrex_audit* audit) {
+    if (n_graphs <= 0 || !vtx || !eoff) return fail(RTREX_EINVAL, "bad batch description");
+    // One disjoint union: graph i's vertices are shifted by the vertices of
+    // the graphs before it.  Triangles, components and seeds never cross
+    // graphs, so every phase runs over all graphs per launch.
+    std::vector<int64_t> vbase(n_graphs + 1, 0);
+    for (int64_t i = 0; i < n_graphs; ++i) {
+        if (vtx[i] < 0 || eoff[i + 1] < eoff[i]) return fail(RTREX_EINVAL, "bad batch sizes");
+        vbase[i + 1] = vbase[i] + vtx[i];
+    }
+    const int64_t N = vbase[n_graphs], M = eoff[n_graphs] - eoff[0];
+    if (N >= (int64_t)UINT32_MAX) return fail(RTREX_EINVAL, "batch too large");
+    std::vector<uint32_t> U(M), V(M);
+    for (int64_t i = 0; i < n_graphs; ++i)
+        for (int64_t k = eoff[i]; k < eoff[i + 1]; ++k) {
+            if ((int64_t)u[k] >= vtx[i] || (int64_t)v[k] >= vtx[i])
+                return fail(RTREX_EINVAL, "edge endpoint outside its graph");
+            U[k - eoff[0]] = (uint32_t)(u[k] + vbase[i]);
+            V[k - eoff[0]] = (uint32_t)(v[k] + vbase[i]);
+        }
+    rtrex_graph h = nullptr;
+    RT_TRY(new_handle(device, &h));
+    struct Guard {
+        rtrex_graph h;
+        ~Guard() {
+            std::string msg = g_last_error;
+            rtrex_graph_destroy(h);
+            set_error(msg);
+        }
+    } guard{h};
+    RT_TRY(build_from_edges(h, U.data(), V.data(), N, M));
+    h->seg_vtx = vbase;
+    RT_TRY(run_count(h));
+    RT_TRY(run_initial_clean(h, eps));
+    RT_TRY(run_extract(h, eps));
+    if (cluster_map) std::copy(h->cluster_map_host.begin(), h->cluster_map_host.end(), cluster_map);
+    if (n_clusters) std::copy(h->seg_clusters.begin(), h->seg_clusters.end(), n_clusters);
+    if (timings) RT_TRY(rtrex_get_timings(h, timings));
+    if (audit) RT_TRY(rtrex_get_audit(h, audit));
+    return RTREX_OK;
+}
+
+}  // extern "C"

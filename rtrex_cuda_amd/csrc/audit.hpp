@@ -1,0 +1,130 @@
+// Order-independent certification of cleaning decisions.
+//
+// The reference's support of edge f at any moment is its count-phase value S0
+// (bit-identical here) minus the weights of f's dead triangles, subtracted one
+// by one in the order its LIFO cascade happened to kill them
+// (Decomposition.h:217-223).  Any decision "support < eps/(d_u d_v)" the
+// reference takes is therefore a property of (S0, D, order) with D the
+// multiset of dead-triangle weights.  A decision is certain when it comes out
+// the same for EVERY order of D:
+//   1. E = S0 - sum(D) is formed exactly enough in double-double; every order
+//      rounds |D| times by at most ulp(S0)/2 each (running values never exceed
+//      S0), so |fold - E| <= |D| ulp(S0)/2 decides most cases;
+//   2. otherwise, for |D| <= kMaxPerm every distinct order is evaluated with
+//      the reference's own double subtractions.
+// An undecided case is reported (ambiguous_clean) and resolved by this
+// library's own value.
+//
+// Monotonicity (fl(a - t) <= fl(b - t) for a <= b, and fl(a - t) <= a for
+// t > 0) makes "removed" = "some prefix of the kills drops below the
+// threshold" = "the whole current set D does", which is what is certified.
+#pragma once
+
+#include "dev.hpp"
+#include "internal.hpp"
+
+namespace rtrex {
+
+constexpr int kMaxPerm = 8;
+
+enum : int { CERT_KEEP = 0, CERT_REMOVE = 1, CERT_UNDECIDED = 2 };
+
+__device__ __forceinline__ double ulp_of(double x) {
+    if (!(x > 0)) return 0.0;
+    return ldexp(1.0, ilogb(x) - 52);
+}
+
+// Cheap test valid for any history of at most k subtractions from s0: both
+// this library's value v and the reference's lie within k ulp(s0)/2 of the
+// exact difference, hence within k ulp(s0) of each other.
+__device__ __forceinline__ double cheap_bound(uint32_t k, double s0) {
+    return (double)k * ulp_of(s0) + 0x1.0p-1060;
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s,
+                                        double& e) {
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Next lexicographic permutation of a[0..k) (ascending start); false at end.
+__device__ __forceinline__ bool next_perm(double* a, int k) {
+    int i = k - 2;
+    while (i >= 0 && !(a[i] < a[i + 1])) --i;
+    if (i < 0) return false;
+    int j = k - 1;
+    while (!(a[i] < a[j])) --j;
+    double t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+    for (int l = i + 1, r = k - 1; l < r; ++l, --r) {
+        t = a[l];
+        a[l] = a[r];
+        a[r] = t;
+    }
+    return true;
+}
+
+// Certifies "support(f) < thr" for edge f = {x, y} with count-phase support s0.
+// A triangle {x, y, z} counts as dead when one of its other two edges is
+// DEAD or QUEUED (qdead) / DEAD only (!qdead), or when z == extra_dead (the
+// triangle shared with an edge being deleted right now).  Runs on one lane.
+__device__ int certify(const GraphDev& g, uint32_t x, uint32_t y, double s0,
+                       double thr, bool qdead, uint32_t extra_dead) {
+    const uint32_t dx = g.deg[x], dy = g.deg[y];
+    uint32_t a = g.off[x], ae = g.off[x + 1];
+    uint32_t b = g.off[y], be = g.off[y + 1];
+    double hi = s0, lo = 0.0;
+    double t8[kMaxPerm];
+    int nd = 0;
+    while (a < ae && b < be) {
+        const uint32_t za = g.nb[a], zb = g.nb[b];
+        if (za < zb) {
+            ++a;
+        } else if (zb < za) {
+            ++b;
+        } else {
+            const uint8_t sa = g.st[a], sb = g.st[b];
+            bool dead = sa == ST_DEAD || sb == ST_DEAD || za == extra_dead;
+            if (qdead) dead = dead || sa == ST_QUEUED || sb == ST_QUEUED;
+            if (dead) {
+                const double t = tri_w(dx, dy, g.deg[za]);
+                double s, e;
+                two_sum(hi, -t, s, e);
+                hi = s;
+                lo += e;
+                if (nd < kMaxPerm) t8[nd] = t;
+                ++nd;
+            }
+            ++a;
+            ++b;
+        }
+    }
+    const double u = ulp_of(s0);
+    const double half = 0.5 * (double)nd * u * (1.0 + 0x1.0p-40) + 0x1.0p-1060;
+    double d, de;
+    two_sum(hi, -thr, d, de);
+    d += de + lo;
+    if (d >= half) return CERT_KEEP;
+    if (d < -half) return CERT_REMOVE;
+    if (nd > kMaxPerm) return CERT_UNDECIDED;
+    // every distinct order, with the reference's plain subtractions
+    for (int i = 1; i < nd; ++i)  // insertion sort ascending
+        for (int j = i; j > 0 && t8[j] < t8[j - 1]; --j) {
+            double t = t8[j];
+            t8[j] = t8[j - 1];
+            t8[j - 1] = t;
+        }
+    bool seen_lt = false, seen_ge = false;
+    do {
+        double v = s0;
+        for (int i = 0; i < nd; ++i) v -= t8[i];
+        if (v < thr) seen_lt = true;
+        else seen_ge = true;
+        if (seen_lt && seen_ge) return CERT_UNDECIDED;
+    } while (next_perm(t8, nd));
+    return seen_lt ? CERT_REMOVE : CERT_KEEP;
+}
+
+}  // namespace rtrex

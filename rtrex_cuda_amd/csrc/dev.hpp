@@ -1,0 +1,57 @@
+// Device-side helpers shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rtrex {
+
+constexpr int WAVE = 64;
+
+// Position of x in the sorted range a[lo, hi), or -1.  Same contract as the
+// reference's getEdgeBinary (Graph.cpp:130-149), on 32-bit slots.
+__device__ __forceinline__ int64_t find_slot(const uint32_t* __restrict__ a,
+                                             uint32_t lo, uint32_t hi,
+                                             uint32_t x) {
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t y = a[mid];
+        if (y == x) return mid;
+        if (y < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return -1;
+}
+
+// First position in a[lo, hi) with a[pos] >= x.
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ a,
+                                                uint32_t lo, uint32_t hi,
+                                                uint32_t x) {
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Triangle weight 1/(d_a d_b d_c): the degree product is formed exactly in 64
+// bits and converted once (Decomposition.h:97,213,760,785).
+__device__ __forceinline__ double tri_w(uint32_t a, uint32_t b, uint32_t c) {
+    return 1.0 / (double)((uint64_t)a * (uint64_t)b * (uint64_t)c);
+}
+
+// Cleaning threshold eps * 1/(d_a d_b) (Decomposition.h:239-254,395-397).
+__device__ __forceinline__ double edge_thr(double eps, uint32_t a, uint32_t b) {
+    double ew = 1.0 / (double)((uint64_t)a * (uint64_t)b);
+    return eps * ew;
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) {
+    return __ballot(p);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+}  // namespace rtrex

@@ -1,0 +1,781 @@
+// Greedy extraction (seed loop) on the device.
+//
+// Reference: disjointExtract (Decomposition.h:538-940), populateStats
+// (:473-491); CUDA port: disjointExtractCUDA (RTRex_CUDA/main.cu:130-338),
+// gpuExtractInner / extractInnerKernel / gpuMarkClusterEdges
+// (RTRex_CUDA/gpu_extract.cu:9-317).
+//
+// Independence.  A seed's cluster is itself, its neighbours over alive edges
+// and two-hop vertices reached through alive triangles; deleting the cluster's
+// edges and cascading only touches alive edges reachable through alive
+// triangles.  So every connected component of the alive graph (after the
+// initial clean) evolves on its own, its seeds taken in the global
+// (degree, id) order restricted to it.  Components are extracted in parallel,
+// one wave each, pulled from a work queue in one persistent launch; there is
+// no host round trip per seed (the CUDA port makes ~10 launches and O(n + m)
+// PCIe traffic per seed).
+//
+// Per seed u (all 64 lanes of the wave):
+//   1. alive neighbours A of u join the cluster (slot order);
+//   2. the reference's inner loop (:728-805) in its exact order: pairs (v, w)
+//      with w at or after v in A, both passing the degree filter d > d_u/eps;
+//      internal += 1/(d_u d_v d_w); for x in alive N(lower) with (higher, x)
+//      alive and x not in {lower, higher, u}: if x is a neighbour of u the
+//      weight joins `internal` when x > v, w, else it accumulates into the
+//      two-hop weight of x.  Lanes probe 64 ids of N(lower) at a time; the
+//      `internal` fold and each candidate's fold keep the reference's order
+//      (lane order inside a chunk, chunks and pairs in sequence);
+//   3. candidates ascending by id, potential = fold of their weights,
+//      std::sort-identical ordering by weight (introsort.hpp), greedy prefix
+//      with the strict '>' ratio rule (:847-864);
+//   4. every alive edge at a cluster vertex is queued and the deletion
+//      cascades (deleteAndClean semantics, LIFO) inside the wave.
+// Per-vertex supports read in step 3 went through the initial clean's atomic
+// subtractions, so they may differ from the reference's by a few ulps; each
+// ratio decision is audited against that bound (ambiguous_ratio).
+#include <algorithm>
+
+#include "audit.hpp"
+#include "dev.hpp"
+#include "internal.hpp"
+#include "introsort.hpp"
+
+namespace rtrex {
+namespace {
+
+constexpr int T = 256;
+constexpr int XT = 64;  // one wave per workgroup
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t LDS_CAND = 1024;
+
+using isort::Rec;
+
+// ---------------------------------------------------------------- setup
+__global__ void __launch_bounds__(T) k_seed_keys(const GraphDev g,
+                                                 const uint32_t* __restrict__ vseg,
+                                                 int Bd, int Bi,
+                                                 uint64_t* __restrict__ keys) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < g.n; v += gridDim.x * T) {
+        uint64_t sg = vseg ? vseg[v] : 0;
+        keys[v] = (sg << (Bd + Bi)) | ((uint64_t)g.deg[v] << Bi) | v;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_ranks(const uint64_t* __restrict__ keys,
+                                             uint32_t n, int Bi,
+                                             uint32_t* __restrict__ order,
+                                             uint32_t* __restrict__ rank) {
+    const uint64_t mask = (1ull << Bi) - 1;
+    for (uint32_t r = blockIdx.x * T + threadIdx.x; r < n; r += gridDim.x * T) {
+        uint32_t v = (uint32_t)(keys[r] & mask);
+        order[r] = v;
+        rank[v] = r;
+    }
+}
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    uint32_t p = parent[x];
+    while (p != x) {
+        uint32_t gp = parent[p];
+        if (gp != p) parent[x] = gp;  // path halving
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(T) k_uf_init(uint32_t n, uint32_t* parent,
+                                               uint32_t* csize, uint32_t* cseed) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
+        parent[v] = v;
+        csize[v] = 0;
+        cseed[v] = NONE;
+    }
+}
+
+// Lock-free union of the endpoints of every alive edge (smaller root wins).
+__global__ void __launch_bounds__(T) k_uf_hook(const GraphDev g, uint32_t* parent) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint32_t sl = g.eslot[e];
+        if (g.st[sl] != ST_ALIVE) continue;
+        uint32_t a = uf_find(parent, g.esrc[e]);
+        uint32_t b = uf_find(parent, g.nb[sl]);
+        while (a != b) {
+            if (a > b) {
+                uint32_t t = a;
+                a = b;
+                b = t;
+            }
+            uint32_t old = atomicCAS(&parent[b], b, a);
+            if (old == b) break;
+            b = uf_find(parent, old);
+            a = uf_find(parent, a);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_uf_compress(uint32_t n, uint32_t* parent,
+                                                   uint32_t* csize) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
+        uint32_t r = uf_find(parent, v);
+        parent[v] = r;
+        atomicAdd(&csize[r], 1u);
+    }
+}
+
+// Vertices of size-1 components have no alive edge: each is a singleton
+// cluster seeded by itself.  The rest are keyed (root, seed rank).
+__global__ void __launch_bounds__(T) k_comp_keys(uint32_t n,
+                                                 const uint32_t* __restrict__ parent,
+                                                 const uint32_t* __restrict__ csize,
+                                                 const uint32_t* __restrict__ rank,
+                                                 uint32_t* __restrict__ cseed,
+                                                 uint64_t* __restrict__ keys,
+                                                 unsigned* nkeys) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
+        uint32_t r = parent[v];
+        if (csize[r] < 2) {
+            cseed[v] = v;
+        } else {
+            keys[atomicAdd(nkeys, 1u)] = ((uint64_t)r << 32) | rank[v];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_comp_layout(const uint64_t* __restrict__ keys,
+                                                   uint32_t nk,
+                                                   const uint32_t* __restrict__ order,
+                                                   const GraphDev g,
+                                                   uint32_t* __restrict__ cv,
+                                                   uint32_t* __restrict__ vdeg,
+                                                   uint32_t* __restrict__ flag) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nk; i += gridDim.x * T) {
+        uint32_t v = order[(uint32_t)(keys[i] & 0xffffffffu)];
+        cv[i] = v;
+        vdeg[i] = g.deg[v];
+        flag[i] = (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_comp_starts(const uint32_t* __restrict__ flag,
+                                                   const uint32_t* __restrict__ fpos,
+                                                   uint32_t nk,
+                                                   uint32_t* __restrict__ cstart) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nk; i += gridDim.x * T)
+        if (flag[i]) cstart[fpos[i]] = i;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[fpos[nk]] = nk;
+}
+
+// ---------------------------------------------------------------- extraction
+struct XArgs {
+    GraphDev g;
+    double eps;
+    const uint32_t* cv;      // component-grouped vertices, seed order inside
+    const uint32_t* cstart;  // ncomp + 1 boundaries into cv
+    const uint64_t* dpre;    // prefix of degrees over cv (scratch placement)
+    uint32_t ncomp;
+    uint32_t* cseed;         // seed of each vertex's cluster, NONE if not yet
+    uint32_t* mk;            // neighbour stamp (seed + 1)
+    uint32_t* cm;            // candidate stamp (seed + 1)
+    double* tw;              // two-hop weight per candidate vertex
+    uint32_t* mem;           // [start, start+len): u, then alive neighbours
+    uint32_t* cand;          // [2 start, ...): candidates, padded for sorting
+    Rec* grec;               // [start, ...): candidate records (large case)
+    double* gpv;             // [start, ...): prefetched supports (large case)
+    uint32_t* wl;            // [dpre[start], ...): deletion worklist (LIFO)
+    Counters* ctr;
+};
+
+// Cascade decision for alive edge f whose support just became v (threshold
+// thr); `third` is the third vertex of f's triangle with the edge being
+// deleted.  Certain outside the cheap bound, otherwise certified against every
+// subtraction order (audit.hpp).
+__device__ bool decide(const GraphDev& g, uint32_t f, double v, double thr,
+                       uint32_t third, unsigned long long& amb) {
+    const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
+    if (v < thr - cb) return true;
+    if (v >= thr + cb) return false;
+    const uint32_t sl = g.eslot[f];
+    int c = certify(g, g.esrc[f], g.nb[sl], g.sup0[f], thr, false, third);
+    if (c == CERT_UNDECIDED) {
+        amb++;
+        return v < thr;
+    }
+    return c == CERT_REMOVE;
+}
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const int l = lane_id();
+    return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+// Ascending bitonic sort of a[0, np) by the wave (np a power of two).
+__device__ void wave_bitonic(uint32_t* a, uint32_t np) {
+    const int lane = lane_id();
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < np; i += XT) {
+                uint32_t l = i ^ j;
+                if (l > i) {
+                    uint32_t ai = a[i], al = a[l];
+                    bool up = (i & k) == 0;
+                    if ((ai > al) == up) {
+                        a[i] = al;
+                        a[l] = ai;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(XT) k_extract(XArgs A) {
+    __shared__ Rec srec[LDS_CAND];
+    __shared__ double spv[LDS_CAND];
+    __shared__ double sbd[LDS_CAND];
+    __shared__ uint32_t skey[LDS_CAND];
+    __shared__ int s_maxind;
+    const GraphDev& g = A.g;
+    const int lane = lane_id();
+    unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
+
+    while (true) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(&A.ctr->work_next, 1u);
+        c = __shfl(c, 0, 64);
+        if (c >= A.ncomp) break;
+        const uint32_t start = A.cstart[c], len = A.cstart[c + 1] - start;
+        const uint64_t ebase = A.dpre[start];
+
+        for (uint32_t r = 0; r < len; ++r) {
+            const uint32_t u = A.cv[start + r];
+            if (A.cseed[u] != NONE) continue;
+            const uint32_t du = g.deg[u];
+            const double lim = (double)du / A.eps;
+            const uint32_t stamp = u + 1;
+            n_seeds++;
+
+            // 1. alive neighbours (Decomposition.h:620-631)
+            uint32_t nA = 0;
+            const uint32_t ub = g.off[u], ue = g.off[u + 1];
+            for (uint32_t p = ub; p < ue; p += XT) {
+                const uint32_t j = p + lane;
+                const bool alive = j < ue && g.st[j] != ST_DEAD;
+                const uint32_t v = j < ue ? g.nb[j] : 0;
+                const uint64_t bal = ballot64(alive);
+                if (alive) {
+                    A.mem[start + 1 + nA + __popcll(bal & lanes_below())] = v;
+                    A.mk[v] = stamp;
+                    A.cseed[v] = u;
+                }
+                nA += (uint32_t)__popcll(bal);
+            }
+            if (lane == 0) {
+                A.mem[start] = u;
+                A.cseed[u] = u;
+            }
+            __syncthreads();
+            if (nA == 0) continue;  // singleton cluster {u}
+
+            // 2. inner loop (Decomposition.h:728-805)
+            double internal = 0.0;
+            uint32_t nc = 0;
+            const uint64_t cb = 2ull * start;
+            for (uint32_t a = 0; a < nA; ++a) {
+                const uint32_t v = A.mem[start + 1 + a];
+                const uint32_t dv = g.deg[v];
+                if ((double)dv > lim) continue;
+                for (uint32_t b = a; b < nA; ++b) {
+                    const uint32_t w = A.mem[start + 1 + b];
+                    const uint32_t dw = g.deg[w];
+                    if ((double)dw > lim) continue;
+                    internal += tri_w(du, dv, dw);
+                    const uint32_t lo = dv < dw ? v : w, hi = dv < dw ? w : v;
+                    uint32_t p = g.off[lo];
+                    const uint32_t le = g.off[lo + 1];
+                    uint32_t hlo = g.off[hi];
+                    const uint32_t he = g.off[hi + 1];
+                    for (; p < le; p += XT) {
+                        const uint32_t l = p + lane;
+                        const bool in = l < le;
+                        const uint32_t x = in ? g.nb[l] : 0;
+                        const bool ok = in && g.st[l] != ST_DEAD && x != lo &&
+                                        x != hi && x != u;
+                        const uint32_t pos = in ? lower_bound(g.nb, hlo, he, x) : hlo;
+                        const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
+                        const bool hit = ok && pos < he && g.nb[pos] == x &&
+                                         g.st[pos] != ST_DEAD;
+                        const double t = hit ? tri_w(g.deg[x], dv, dw) : 0.0;
+                        const bool isn = hit && A.mk[x] == stamp;
+                        uint64_t ib = ballot64(isn && x > v && x > w);
+                        while (ib) {  // wave-uniform
+                            const int bl = __ffsll((unsigned long long)ib) - 1;
+                            internal += __shfl(t, bl, 64);
+                            ib &= ib - 1;
+                        }
+                        const bool ct = hit && !isn;
+                        const bool fresh = ct && A.cm[x] != stamp;
+                        const uint64_t fb = ballot64(fresh);
+                        if (fresh) {
+                            A.cm[x] = stamp;
+                            A.tw[x] = t;
+                            A.cand[cb + nc + __popcll(fb & lanes_below())] = x;
+                        } else if (ct) {
+                            A.tw[x] += t;
+                        }
+                        nc += (uint32_t)__popcll(fb);
+                        hlo = nh;
+                    }
+                    __syncthreads();  // candidate weights of this pair are final
+                }
+            }
+
+            // 3. candidates ascending (std::set), potential, sort, prefix
+            uint32_t np = 1;
+            while (np < nc) np <<= 1;
+            const bool small = np <= LDS_CAND;
+            uint32_t* keys = small ? skey : A.cand + cb;
+            for (uint32_t i = lane; i < np; i += XT)
+                keys[i] = i < nc ? A.cand[cb + i] : NONE;
+            __syncthreads();
+            if (nc > 1) wave_bitonic(keys, np);
+            Rec* rec = small ? srec : A.grec + start;
+            double* pvv = small ? spv : A.gpv + start;
+            double* bdd = sbd;  // per-candidate support bounds (LDS case only)
+            double potential = 0.0;
+            for (uint32_t p = 0; p < nc; p += XT) {  // ascending x (Decomposition.h:815-827)
+                const uint32_t i = p + lane;
+                const bool in = i < nc;
+                const uint32_t x = in ? keys[i] : 0;
+                const double wx = in ? A.tw[x] : 0.0;
+                if (in) {
+                    rec[i].vertex = x;
+                    rec[i].wgt = wx;
+                }
+                const uint32_t cnt = min(64u, nc - p);
+                for (uint32_t b2 = 0; b2 < cnt; ++b2) potential += __shfl(wx, (int)b2, 64);
+            }
+            __syncthreads();
+            if (lane == 0) isort::sort(rec, rec + nc);
+            __syncthreads();
+            // supports of the candidates in sorted order
+            for (uint32_t i = lane; i < nc; i += XT) {
+                const uint32_t x = rec[i].vertex;
+                pvv[i] = g.pv[x];
+                if (small) bdd[i] = cheap_bound(g.vtcnt[x], g.pv0[x]);
+            }
+            __syncthreads();
+            if (lane == 0) {
+                const double D0 = internal + potential;
+                double ratio = internal / D0;
+                double eratio = 0.0;
+                int maxind = -1;
+                double num = 0.0, den = 0.0, eden = 0.0;
+                const bool audit = ratio == ratio;  // NaN ratio: every '>' is false
+                for (uint32_t i = 0; i < nc; ++i) {
+                    num += rec[i].wgt;
+                    den += pvv[i];
+                    const double bi = small ? bdd[i]
+                                            : cheap_bound(g.vtcnt[rec[i].vertex],
+                                                           g.pv0[rec[i].vertex]);
+                    eden += bi + 0x1.0p-52 * fabs(den);
+                    const double D = D0 + den;
+                    const double nr = (internal + num) / D;
+                    if (audit) {
+                        const double enr =
+                            1.01 * nr * ((eden + 0x1.0p-52 * D) / D) + 0x1.0p-51 * nr;
+                        if (fabs(nr - ratio) <= enr + eratio) amb_r++;
+                        if (nr > ratio) eratio = enr;
+                    }
+                    if (nr > ratio) {
+                        ratio = nr;
+                        maxind = (int)i;
+                    }
+                }
+                s_maxind = maxind;
+            }
+            __syncthreads();
+            const int maxind = s_maxind;
+            for (int i = lane; i <= maxind; i += XT) A.cseed[rec[i].vertex] = u;
+            __syncthreads();
+
+            // 4. queue every alive edge of the cluster (Decomposition.h:897-914)
+            uint32_t top = 0;
+            const uint32_t nmem = 1 + nA + (uint32_t)(maxind + 1);
+            for (uint32_t q = 0; q < nmem; ++q) {
+                const uint32_t z = q < 1 + nA ? A.mem[start + q] : rec[q - 1 - nA].vertex;
+                const uint32_t zb = g.off[z], ze = g.off[z + 1];
+                for (uint32_t p = zb; p < ze; p += XT) {
+                    const uint32_t j = p + lane;
+                    const bool al = j < ze && g.st[j] == ST_ALIVE;
+                    const uint64_t bal = ballot64(al);
+                    if (al) {
+                        A.wl[ebase + top + __popcll(bal & lanes_below())] = g.s2e[j];
+                        g.st[j] = ST_QUEUED;
+                        g.st[g.part[j]] = ST_QUEUED;
+                    }
+                    top += (uint32_t)__popcll(bal);
+                }
+                __syncthreads();
+            }
+
+            // ... and cascade (deleteAndClean, Decomposition.h:147-283)
+            while (top > 0) {
+                --top;
+                const uint32_t e = A.wl[ebase + top];
+                const uint32_t sl = g.eslot[e];
+                const uint32_t x = g.esrc[e], y = g.nb[sl];
+                const uint32_t dx = g.deg[x], dy = g.deg[y];
+                const uint32_t lo = dx < dy ? x : y, hi = dx < dy ? y : x;
+                const uint32_t dlo = dx < dy ? dx : dy, dhi = dx < dy ? dy : dx;
+                uint32_t p = g.off[lo];
+                const uint32_t le = g.off[lo + 1];
+                uint32_t hlo = g.off[hi];
+                const uint32_t he = g.off[hi + 1];
+                for (; p < le; p += XT) {
+                    const uint32_t j = p + lane;
+                    const bool in = j < le;
+                    const uint8_t sj = in ? g.st[j] : ST_DEAD;
+                    const uint32_t w = in ? g.nb[j] : 0;
+                    const uint32_t pos = in ? lower_bound(g.nb, hlo, he, w) : hlo;
+                    const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
+                    bool hit = in && sj != ST_DEAD && pos < he && g.nb[pos] == w;
+                    const uint8_t sh = hit ? g.st[pos] : ST_DEAD;
+                    hit = hit && sh != ST_DEAD;
+                    const uint32_t dw = hit ? g.deg[w] : 1;
+                    const double t = hit ? tri_w(dx, dy, dw) : 0.0;
+                    uint64_t hb = ballot64(hit);
+                    n_tri += (lane == 0) ? (unsigned long long)__popcll(hb) : 0ull;
+                    while (hb) {  // the popped edge's own accumulators, in w order
+                        const int bl = __ffsll((unsigned long long)hb) - 1;
+                        const double tv = __shfl(t, bl, 64);
+                        if (lane == 0) {
+                            g.sup[e] -= tv;
+                            g.pv[x] -= tv;
+                            g.pv[y] -= tv;
+                        }
+                        hb &= hb - 1;
+                    }
+                    bool push1 = false, push2 = false;
+                    uint32_t fj = 0, fh = 0;
+                    if (hit) {
+                        fj = g.s2e[j];
+                        fh = g.s2e[pos];
+                        const double v1 = g.sup[fj] - t;
+                        const double v2 = g.sup[fh] - t;
+                        g.sup[fj] = v1;
+                        g.sup[fh] = v2;
+                        g.pv[w] -= t;
+                        g.touched[fj] = 1;
+                        g.touched[fh] = 1;
+                        // (lo, w) then (hi, w), as Decomposition.h:238-264.
+                        if (sj == ST_ALIVE)
+                            push1 = decide(g, fj, v1, edge_thr(A.eps, dlo, dw), hi, amb_c);
+                        if (sh == ST_ALIVE)
+                            push2 = decide(g, fh, v2, edge_thr(A.eps, dhi, dw), lo, amb_c);
+                    }
+                    const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
+                    // exclusive prefix of pushes over lanes
+                    uint32_t pre = 0;
+                    {
+                        uint32_t sc = np2;
+                        for (int d = 1; d < 64; d <<= 1) {
+                            uint32_t o = __shfl_up(sc, d, 64);
+                            if (lane >= d) sc += o;
+                        }
+                        pre = sc - np2;
+                    }
+                    const uint32_t tot = __shfl(pre + np2, 63, 64);
+                    if (push1) {
+                        A.wl[ebase + top + pre] = fj;
+                        g.st[j] = ST_QUEUED;
+                        g.st[g.part[j]] = ST_QUEUED;
+                    }
+                    if (push2) {
+                        A.wl[ebase + top + pre + (uint32_t)push1] = fh;
+                        g.st[pos] = ST_QUEUED;
+                        g.st[g.part[pos]] = ST_QUEUED;
+                    }
+                    top += tot;
+                    hlo = nh;
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    g.st[sl] = ST_DEAD;
+                    g.st[g.part[sl]] = ST_DEAD;
+                }
+                n_edges++;
+                __syncthreads();
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&A.ctr->seeds, n_seeds);
+        atomicAdd(&A.ctr->tri_removed, n_tri);
+        atomicAdd(&A.ctr->edges_removed, n_edges);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        amb_c += __shfl_xor(amb_c, d, 64);
+        amb_r += __shfl_xor(amb_r, d, 64);
+    }
+    if (lane == 0) {
+        if (amb_c) atomicAdd(&A.ctr->ambiguous_clean, amb_c);
+        if (amb_r) atomicAdd(&A.ctr->ambiguous_ratio, amb_r);
+    }
+}
+
+}  // namespace
+}  // namespace rtrex
+
+namespace rtrex {
+namespace {
+
+// Cluster ids: seeds numbered in seed order (Decomposition.h:612,892-893).
+__global__ void __launch_bounds__(T) k_seed_flags(uint32_t n,
+                                                  const uint32_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ cseed,
+                                                  uint32_t* __restrict__ flag) {
+    for (uint32_t r = blockIdx.x * T + threadIdx.x; r < n; r += gridDim.x * T) {
+        const uint32_t v = order[r];
+        flag[r] = cseed[v] == v ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_cluster_ids(uint32_t n,
+                                                   const uint32_t* __restrict__ cseed,
+                                                   const uint32_t* __restrict__ rank,
+                                                   const uint32_t* __restrict__ cidr,
+                                                   uint32_t* __restrict__ cid,
+                                                   unsigned* err) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
+        const uint32_t sd = cseed[v];
+        if (sd >= n) {  // every vertex must have been assigned a cluster
+            cid[v] = NONE;
+            atomicOr(err, ERR_SCRATCH);
+            continue;
+        }
+        cid[v] = cidr[rank[sd]];
+    }
+}
+
+// populateStats over the original graph (Decomposition.h:473-491).
+__global__ void __launch_bounds__(T) k_cluster_stats(const GraphDev g,
+                                                     const uint32_t* __restrict__ cid,
+                                                     unsigned long long* nv,
+                                                     unsigned long long* ne,
+                                                     unsigned long long* cut) {
+    for (uint32_t z = blockIdx.x * T + threadIdx.x; z < g.n; z += gridDim.x * T) {
+        const uint32_t c = cid[z];
+        unsigned long long in = 0, out = 0;
+        for (uint32_t i = g.off[z]; i < g.off[z + 1]; ++i) {
+            if (cid[g.nb[i]] == c) in++;
+            else out++;
+        }
+        atomicAdd(&nv[c], 1ull);
+        if (in) atomicAdd(&ne[c], in);
+        if (out) atomicAdd(&cut[c], out);
+    }
+}
+
+int bits_for(uint64_t n) {
+    int b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+}  // namespace
+
+rtrex_status run_extract(rtrex_graph h, double eps) {
+    if (h->phase < PH_CLEANED)
+        return fail(RTREX_ESTATE, "extract needs count and initial clean first");
+    if (h->phase >= PH_EXTRACTED)
+        return fail(RTREX_ESTATE, "graph already extracted; run count again");
+    if (!(eps > 0)) return fail(RTREX_EINVAL, "eps must be > 0");
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    const uint32_t n = g.n;
+    RT_HIP(hipEventRecord(h->ev[5], s));
+    Scratch sc;
+    uint64_t *keys, *tmp;
+    uint32_t *order, *rank, *parent, *csize, *vseg = nullptr;
+    RT_TRY(sc.get(&keys, n));
+    RT_TRY(sc.get(&tmp, n));
+    RT_TRY(sc.get(&order, n));
+    RT_TRY(sc.get(&rank, n));
+    RT_TRY(sc.get(&parent, n));
+    RT_TRY(sc.get(&csize, n));
+    const int Bi = bits_for((uint64_t)n + 1);
+    int Bs = 0;
+    if (!h->seg_vtx.empty()) {
+        const size_t ns = h->seg_vtx.size() - 1;
+        Bs = bits_for(ns + 1);
+        std::vector<uint32_t> seg(n);
+        for (size_t q = 0; q < ns; ++q)
+            for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v) seg[v] = (uint32_t)q;
+        RT_TRY(sc.get(&vseg, n));
+        RT_HIP(hipMemcpyAsync(vseg, seg.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    if (Bs + 2 * Bi > 64) return fail(RTREX_EINVAL, "too many vertices/segments");
+    const unsigned bn = blocks_for(n, T, 8192);
+    // seed order: (segment, degree, id) -- pairCompareSecond (Graph.cpp:40-45)
+    k_seed_keys<<<bn, T, 0, s>>>(g, vseg, Bi, Bi, keys);
+    RT_HIP(hipGetLastError());
+    RT_TRY(radix_sort_u64(keys, tmp, n, Bs + 2 * Bi, s));
+    k_ranks<<<bn, T, 0, s>>>(keys, n, Bi, order, rank);
+    // alive components
+    k_uf_init<<<bn, T, 0, s>>>(n, parent, csize, h->cseed);
+    k_uf_hook<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, parent);
+    k_uf_compress<<<bn, T, 0, s>>>(n, parent, csize);
+    unsigned* nk_d = &h->ctr->frontier_next;
+    RT_HIP(hipMemsetAsync(nk_d, 0, sizeof(unsigned), s));
+    k_comp_keys<<<bn, T, 0, s>>>(n, parent, csize, rank, h->cseed, keys, nk_d);
+    RT_HIP(hipGetLastError());
+    uint32_t nk = 0;
+    RT_HIP(hipMemcpyAsync(&nk, nk_d, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    h->audit.components = 0;
+    h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
+    if (nk) {
+        RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
+        uint32_t *cv, *vdeg, *flag, *fpos, *cstart;
+        uint64_t* dpre;
+        RT_TRY(sc.get(&cv, nk));
+        RT_TRY(sc.get(&vdeg, nk));
+        RT_TRY(sc.get(&flag, nk));
+        RT_TRY(sc.get(&fpos, (uint64_t)nk + 1));
+        RT_TRY(sc.get(&dpre, (uint64_t)nk + 1));
+        const unsigned bk = blocks_for(nk, T, 8192);
+        k_comp_layout<<<bk, T, 0, s>>>(keys, nk, order, g, cv, vdeg, flag);
+        RT_HIP(hipGetLastError());
+        RT_TRY(exclusive_scan_u32_32(flag, fpos, nk, s));
+        uint32_t ncomp = 0;
+        RT_HIP(hipMemcpyAsync(&ncomp, fpos + nk, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        RT_TRY(sc.get(&cstart, (uint64_t)ncomp + 1));
+        k_comp_starts<<<bk, T, 0, s>>>(flag, fpos, nk, cstart);
+        RT_HIP(hipGetLastError());
+        RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
+        {
+            std::vector<uint32_t> cs(ncomp + 1);
+            RT_HIP(hipMemcpyAsync(cs.data(), cstart, (ncomp + 1) * 4ull,
+                                  hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            int64_t big = 0;
+            for (uint32_t c = 0; c < ncomp; ++c)
+                big = std::max<int64_t>(big, cs[c + 1] - cs[c]);
+            h->audit.components = ncomp;
+            h->audit.largest_component = big;
+        }
+        uint32_t *mk, *cm, *mem, *cand, *wl;
+        double *tw, *gpv;
+        Rec* grec;
+        RT_TRY(sc.get(&mk, n));
+        RT_TRY(sc.get(&cm, n));
+        RT_TRY(sc.get(&mem, n));
+        RT_TRY(sc.get(&cand, 2ull * n));
+        RT_TRY(sc.get(&tw, n));
+        RT_TRY(sc.get(&gpv, n));
+        RT_TRY(sc.get(&grec, n));
+        RT_TRY(sc.get(&wl, g.nnz));
+        RT_HIP(hipMemsetAsync(mk, 0, (uint64_t)n * 4, s));
+        RT_HIP(hipMemsetAsync(cm, 0, (uint64_t)n * 4, s));
+        RT_HIP(hipMemsetAsync(&h->ctr->work_next, 0, sizeof(unsigned), s));
+        XArgs a;
+        a.g = g;
+        a.eps = eps;
+        a.cv = cv;
+        a.cstart = cstart;
+        a.dpre = dpre;
+        a.ncomp = ncomp;
+        a.cseed = h->cseed;
+        a.mk = mk;
+        a.cm = cm;
+        a.tw = tw;
+        a.mem = mem;
+        a.cand = cand;
+        a.grec = grec;
+        a.gpv = gpv;
+        a.wl = wl;
+        a.ctr = h->ctr;
+        const unsigned grid = std::min<uint32_t>(ncomp, 8192);
+        k_extract<<<grid, XT, 0, s>>>(a);
+        RT_HIP(hipGetLastError());
+    }
+    // cluster ids in seed order
+    uint32_t *flag2, *cidr;
+    RT_TRY(sc.get(&flag2, n));
+    RT_TRY(sc.get(&cidr, (uint64_t)n + 1));
+    if (!h->cid) RT_TRY(dalloc(h, &h->cid, n));
+    k_seed_flags<<<bn, T, 0, s>>>(n, order, h->cseed, flag2);
+    RT_TRY(exclusive_scan_u32_32(flag2, cidr, n, s));
+    RT_HIP(hipMemsetAsync(&h->ctr->error, 0, sizeof(unsigned), s));
+    k_cluster_ids<<<bn, T, 0, s>>>(n, h->cseed, rank, cidr, h->cid, &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    unsigned cerr = 0;
+    RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
+    std::vector<uint32_t> cid(n), cidr_h;
+    uint32_t total = 0;
+    RT_HIP(hipMemcpyAsync(cid.data(), h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(&total, cidr + n, 4, hipMemcpyDeviceToHost, s));
+    if (!h->seg_vtx.empty()) {
+        cidr_h.resize((uint64_t)n + 1);
+        RT_HIP(hipMemcpyAsync(cidr_h.data(), cidr, ((uint64_t)n + 1) * 4,
+                              hipMemcpyDeviceToHost, s));
+    }
+    RT_HIP(hipEventRecord(h->ev[6], s));
+    RT_HIP(hipEventSynchronize(h->ev[6]));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
+    h->tm.extract_ms = ms;
+    h->cluster_map_host.assign(cid.begin(), cid.end());
+    h->n_clusters = total;
+    if (!h->seg_vtx.empty()) {
+        // Per-graph ids: ranks of segment q start at seg_vtx[q].
+        const size_t ns = h->seg_vtx.size() - 1;
+        h->seg_clusters.assign(ns, 0);
+        for (size_t q = 0; q < ns; ++q) {
+            const uint32_t b0 = cidr_h[h->seg_vtx[q]], b1 = cidr_h[h->seg_vtx[q + 1]];
+            h->seg_clusters[q] = (int64_t)b1 - b0;
+            for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v)
+                h->cluster_map_host[v] -= b0;
+        }
+    }
+    h->phase = PH_EXTRACTED;
+    return RTREX_OK;
+}
+
+rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
+                               int64_t* cut) {
+    if (h->phase < PH_EXTRACTED || !h->cid)
+        return fail(RTREX_ESTATE, "extract has not run");
+    const uint64_t k = (uint64_t)h->n_clusters;
+    hipStream_t s = h->stream;
+    Scratch sc;
+    unsigned long long *a, *b, *c;
+    RT_TRY(sc.get(&a, k));
+    RT_TRY(sc.get(&b, k));
+    RT_TRY(sc.get(&c, k));
+    RT_HIP(hipMemsetAsync(a, 0, k * 8, s));
+    RT_HIP(hipMemsetAsync(b, 0, k * 8, s));
+    RT_HIP(hipMemsetAsync(c, 0, k * 8, s));
+    k_cluster_stats<<<blocks_for(h->g.n, T, 8192), T, 0, s>>>(h->g, h->cid, a, b, c);
+    RT_HIP(hipGetLastError());
+    std::vector<unsigned long long> ha(k), hb(k), hc(k);
+    RT_HIP(hipMemcpyAsync(ha.data(), a, k * 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(hb.data(), b, k * 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(hc.data(), c, k * 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    for (uint64_t i = 0; i < k; ++i) {
+        if (nv) nv[i] = (int64_t)ha[i];
+        if (ne) ne[i] = (int64_t)(hb[i] / 2);
+        if (cut) cut[i] = (int64_t)hc[i];
+    }
+    return RTREX_OK;
+}
+
+}  // namespace rtrex

@@ -1,0 +1,317 @@
+// Device-resident graph: CSR upload or on-device CSR construction, partner
+// slots and undirected edge numbering.
+//
+// Reference counterparts: transferGraphToDevice/allocateDeviceState
+// (RTRex_CUDA/gpu_graph.cu:46-135), makeCSR + sortById (Graph.cpp:344-384,
+// 201-205) and getPartnerMap (Graph.cpp:320-340).
+//
+// Edge-list build: both directions of every edge become a u64 key
+// (src << B | dst), B = bits needed for n; an LSD radix sort orders them by
+// (src, dst) exactly as makeCSR's std::sort of pairs does, offsets come from a
+// degree histogram + exclusive scan.  Partner slots are found by a binary
+// search of src in dst's sorted list (one dependent search per slot, coalesced
+// outer loop); edge ids are the ranks of the canonical slots (src < dst).
+#include <algorithm>
+
+#include "dev.hpp"
+#include "internal.hpp"
+
+namespace rtrex {
+namespace {
+
+constexpr int T = 256;
+
+// src of every slot: the vertex whose [off[v], off[v+1]) range holds it.
+__global__ void __launch_bounds__(T) k_slot_src(const uint32_t* __restrict__ off,
+                                                uint32_t n, uint64_t nnz,
+                                                uint32_t* __restrict__ src) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        // Largest v with off[v] <= i (lists may be empty).
+        uint32_t lo = 0, hi = n;  // answer in [lo, hi)
+        while (hi - lo > 1) {
+            uint32_t mid = lo + ((hi - lo) >> 1);
+            if (off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        src[i] = lo;
+    }
+}
+
+// Degrees and the CSR validity checks of the uploaded graph: ids in range,
+// no self-loops, each list strictly increasing (sorted, no duplicates).
+__global__ void __launch_bounds__(T) k_check_csr(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ nb,
+    const uint32_t* __restrict__ src, uint32_t n, uint64_t nnz,
+    uint32_t* __restrict__ deg, unsigned* err) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        uint32_t s = src[i], x = nb[i];
+        unsigned e = 0;
+        if (x >= n) e |= ERR_RANGE;
+        if (x == s) e |= ERR_SELF_LOOP;
+        if (i + 1 < off[s + 1] && nb[i + 1] <= x) e |= ERR_UNSORTED_OR_DUP;
+        if (e) atomicOr(err, e);
+    }
+    for (uint64_t v = (uint64_t)blockIdx.x * T + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * T)
+        deg[v] = off[v + 1] - off[v];
+}
+
+__global__ void __launch_bounds__(T) k_partner(const uint32_t* __restrict__ off,
+                                               const uint32_t* __restrict__ nb,
+                                               const uint32_t* __restrict__ src,
+                                               uint64_t nnz,
+                                               uint32_t* __restrict__ part,
+                                               uint32_t* __restrict__ canon,
+                                               unsigned* err) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        uint32_t s = src[i], x = nb[i];
+        int64_t p = find_slot(nb, off[x], off[x + 1], s);
+        if (p < 0) {
+            atomicOr(err, ERR_ASYMMETRIC);
+            p = i;
+        }
+        part[i] = (uint32_t)p;
+        canon[i] = x > s ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_edge_ids(const uint32_t* __restrict__ nb,
+                                                const uint32_t* __restrict__ src,
+                                                const uint32_t* __restrict__ canon,
+                                                const uint32_t* __restrict__ rank,
+                                                uint64_t nnz,
+                                                uint32_t* __restrict__ s2e,
+                                                uint32_t* __restrict__ esrc,
+                                                uint32_t* __restrict__ eslot) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        if (canon[i]) {
+            uint32_t e = rank[i];
+            s2e[i] = e;
+            esrc[e] = src[i];
+            eslot[e] = (uint32_t)i;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_edge_ids_mirror(
+    const uint32_t* __restrict__ part, const uint32_t* __restrict__ canon,
+    const uint32_t* __restrict__ rank, uint64_t nnz, uint32_t* __restrict__ s2e) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T)
+        if (!canon[i]) s2e[i] = rank[part[i]];
+}
+
+// ---- edge-list build --------------------------------------------------------
+__global__ void __launch_bounds__(T) k_make_keys(const uint32_t* __restrict__ u,
+                                                 const uint32_t* __restrict__ v,
+                                                 uint64_t m, uint32_t n, int B,
+                                                 uint64_t* __restrict__ keys,
+                                                 unsigned* err) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < m;
+         i += (uint64_t)gridDim.x * T) {
+        uint32_t a = u[i], b = v[i];
+        unsigned e = 0;
+        if (a >= n || b >= n) {
+            e |= ERR_RANGE;
+            a = b = 0;
+        }
+        if (a == b) e |= ERR_SELF_LOOP;
+        if (e) atomicOr(err, e);
+        keys[2 * i] = ((uint64_t)a << B) | b;
+        keys[2 * i + 1] = ((uint64_t)b << B) | a;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_split_keys(const uint64_t* __restrict__ keys,
+                                                  uint64_t nnz, int B,
+                                                  uint32_t* __restrict__ nb,
+                                                  uint32_t* __restrict__ src,
+                                                  uint32_t* __restrict__ deg,
+                                                  unsigned* err) {
+    const uint64_t mask = (1ull << B) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        uint64_t k = keys[i];
+        uint32_t s = (uint32_t)(k >> B);
+        nb[i] = (uint32_t)(k & mask);
+        src[i] = s;
+        atomicAdd(&deg[s], 1u);
+        if (i > 0 && keys[i - 1] == k) atomicOr(err, ERR_UNSORTED_OR_DUP);
+    }
+}
+
+int bits_for(uint64_t n) {
+    int b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+std::string describe(unsigned err) {
+    std::string s;
+    if (err & ERR_RANGE) s += " vertex id out of range;";
+    if (err & ERR_SELF_LOOP) s += " self-loop;";
+    if (err & ERR_UNSORTED_OR_DUP) s += " unsorted list or duplicate edge;";
+    if (err & ERR_ASYMMETRIC) s += " not symmetric (missing reverse slot);";
+    return s;
+}
+
+rtrex_status alloc_state(rtrex_graph h, uint32_t n, uint64_t nnz) {
+    GraphDev& g = h->g;
+    const uint64_t m = nnz / 2;
+    g.n = n;
+    g.nnz = nnz;
+    g.m = (uint32_t)m;
+    RT_TRY(dalloc(h, &g.off, (uint64_t)n + 1));
+    RT_TRY(dalloc(h, &g.nb, nnz));
+    RT_TRY(dalloc(h, &g.deg, n));
+    RT_TRY(dalloc(h, &g.part, nnz));
+    RT_TRY(dalloc(h, &g.s2e, nnz));
+    RT_TRY(dalloc(h, &g.esrc, m));
+    RT_TRY(dalloc(h, &g.eslot, m));
+    RT_TRY(dalloc(h, &g.st, nnz));
+    RT_TRY(dalloc(h, &g.sup, m));
+    RT_TRY(dalloc(h, &g.sup0, m));
+    RT_TRY(dalloc(h, &g.tcnt, m));
+    RT_TRY(dalloc(h, &g.pv, n));
+    RT_TRY(dalloc(h, &g.pv0, n));
+    RT_TRY(dalloc(h, &g.vtcnt, n));
+    RT_TRY(dalloc(h, &g.touched, m));
+    RT_TRY(dalloc(h, &g.claim, m));
+    RT_TRY(dalloc(h, &h->cseed, n));
+    RT_TRY(dalloc(h, &h->ctr, 1));
+    RT_HIP(hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream));
+    return RTREX_OK;
+}
+
+// Shared tail of both builds: partner slots, edge ids, state reset.
+rtrex_status finish(rtrex_graph h, uint32_t* src) {
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    const uint64_t nnz = g.nnz;
+    unsigned* err = &h->ctr->error;
+    Scratch sc;
+    uint32_t *canon, *rank;
+    RT_TRY(sc.get(&canon, nnz));
+    RT_TRY(sc.get(&rank, nnz + 1));
+    const unsigned bs = blocks_for(nnz, T, 65536);
+    k_partner<<<bs, T, 0, s>>>(g.off, g.nb, src, nnz, g.part, canon, err);
+    RT_TRY(exclusive_scan_u32_32(canon, rank, nnz, s));
+    unsigned hostErr = 0;
+    uint32_t ncanon = 0;
+    RT_HIP(hipMemcpyAsync(&hostErr, err, sizeof(unsigned),
+                          hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(&ncanon, rank + nnz, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (hostErr)
+        return fail(RTREX_EINVAL, "graph is not a simple undirected CSR:" +
+                                      describe(hostErr));
+    if ((uint64_t)ncanon * 2 != nnz)
+        return fail(RTREX_EINVAL, "graph is not symmetric");
+    k_edge_ids<<<bs, T, 0, s>>>(g.nb, src, canon, rank, nnz, g.s2e, g.esrc,
+                                g.eslot);
+    k_edge_ids_mirror<<<bs, T, 0, s>>>(g.part, canon, rank, nnz, g.s2e);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, nnz, s));
+    RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
+    RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
+    RT_HIP(hipStreamSynchronize(s));
+    h->phase = PH_BUILT;
+    return RTREX_OK;
+}
+
+}  // namespace
+
+rtrex_status build_from_csr(rtrex_graph h, const int64_t* off,
+                            const int64_t* nb, int64_t n, int64_t nnz) {
+    if (n < 0 || nnz < 0 || n >= (int64_t)UINT32_MAX ||
+        nnz >= (int64_t)UINT32_MAX)
+        return fail(RTREX_EINVAL, "n and nnz must be in [0, 2^32-1)");
+    if (nnz % 2) return fail(RTREX_EINVAL, "nnz must be even (undirected)");
+    if (!off || (nnz && !nb)) return fail(RTREX_EINVAL, "null CSR array");
+    if (off[0] != 0 || off[n] != nnz)
+        return fail(RTREX_EINVAL, "offsets must start at 0 and end at nnz");
+    std::vector<uint32_t> off32(n + 1), nb32(nnz);
+    for (int64_t v = 0; v <= n; ++v) {
+        if (v && off[v] < off[v - 1])
+            return fail(RTREX_EINVAL, "offsets must be non-decreasing");
+        off32[v] = (uint32_t)off[v];
+    }
+    for (int64_t i = 0; i < nnz; ++i) {
+        if (nb[i] < 0 || nb[i] >= n)
+            return fail(RTREX_EINVAL, "neighbour id out of range");
+        nb32[i] = (uint32_t)nb[i];
+    }
+    RT_TRY(alloc_state(h, (uint32_t)n, (uint64_t)nnz));
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    RT_HIP(hipEventRecord(h->ev[0], s));
+    RT_HIP(hipMemcpyAsync(g.off, off32.data(), (n + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    if (nnz)
+        RT_HIP(hipMemcpyAsync(g.nb, nb32.data(), nnz * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+    Scratch sc;
+    uint32_t* src;
+    RT_TRY(sc.get(&src, nnz));
+    const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
+    k_slot_src<<<bs, T, 0, s>>>(g.off, g.n, nnz, src);
+    k_check_csr<<<bs, T, 0, s>>>(g.off, g.nb, src, g.n, nnz, g.deg,
+                                 &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    RT_TRY(finish(h, src));
+    RT_HIP(hipEventRecord(h->ev[1], s));
+    RT_HIP(hipEventSynchronize(h->ev[1]));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    h->tm.build_ms = ms;
+    return RTREX_OK;
+}
+
+rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
+                              const uint32_t* v, int64_t n, int64_t m) {
+    if (n < 0 || m < 0 || n >= (int64_t)UINT32_MAX ||
+        2 * m >= (int64_t)UINT32_MAX)
+        return fail(RTREX_EINVAL, "n and 2m must be in [0, 2^32-1)");
+    if (m && (!u || !v)) return fail(RTREX_EINVAL, "null edge array");
+    const uint64_t nnz = 2 * (uint64_t)m;
+    RT_TRY(alloc_state(h, (uint32_t)n, nnz));
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    Scratch sc;
+    uint32_t *du, *dv, *src;
+    uint64_t *keys, *tmp;
+    RT_TRY(sc.get(&du, m));
+    RT_TRY(sc.get(&dv, m));
+    RT_TRY(sc.get(&src, nnz));
+    RT_TRY(sc.get(&keys, nnz));
+    RT_TRY(sc.get(&tmp, nnz));
+    RT_HIP(hipEventRecord(h->ev[0], s));
+    if (m) {
+        RT_HIP(hipMemcpyAsync(du, u, m * 4, hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(dv, v, m * 4, hipMemcpyHostToDevice, s));
+    }
+    const int B = bits_for((uint64_t)n + 1);
+    const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
+    k_make_keys<<<bs, T, 0, s>>>(du, dv, m, g.n, B, keys, &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    RT_TRY(radix_sort_u64(keys, tmp, nnz, 2 * B, s));
+    RT_HIP(hipMemsetAsync(g.deg, 0, (uint64_t)g.n * 4, s));
+    k_split_keys<<<bs, T, 0, s>>>(keys, nnz, B, g.nb, src, g.deg,
+                                  &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    RT_TRY(exclusive_scan_u32_32(g.deg, g.off, g.n, s));
+    RT_TRY(finish(h, src));
+    RT_HIP(hipEventRecord(h->ev[1], s));
+    RT_HIP(hipEventSynchronize(h->ev[1]));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    h->tm.build_ms = ms;
+    return RTREX_OK;
+}
+
+}  // namespace rtrex

@@ -1,0 +1,209 @@
+// Internal definitions shared by the HIP translation units of librtrex_hip.so.
+//
+// Device data layout of one resident graph (all arrays in HBM, 32-bit ids):
+//   off[n+1]   u32  CSR offsets (reference layout, lists sorted by id)
+//   nb[nnz]    u32  neighbour ids
+//   deg[n]     u32  original degree (never changes; every weight uses it)
+//   part[nnz]  u32  slot of the reverse copy of each directed slot
+//   s2e[nnz]   u32  undirected edge id of each slot; edge ids are the ranks of
+//                   the canonical slots (the copy stored in the lower-id
+//                   endpoint's list), so edge order = lexicographic (x<y)
+//   esrc[m], eslot[m]  u32  lower-id endpoint and canonical slot per edge
+//   st[nnz]    u8   per-slot status, both copies always equal
+//   sup[m]     f64  current support of each undirected edge
+//   sup0[m]    f64  support after the count phase (audit bound)
+//   tcnt[m]    u32  triangles on the edge after the count phase
+//   pv[n]      f64  current per-vertex support; pv0[n] after count
+//   touched[m] u8   edge lost at least one triangle since the count phase
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rtrex.h"
+
+namespace rtrex {
+
+// Per-slot status codes.  The reference's characters map as Y=ALIVE,
+// S=QUEUED, N=DEAD; PENDING is an edge claimed for the next peel round.
+enum : uint8_t { ST_ALIVE = 0, ST_QUEUED = 1, ST_DEAD = 2, ST_PENDING = 3 };
+
+struct GraphDev {
+    uint32_t* off = nullptr;
+    uint32_t* nb = nullptr;
+    uint32_t* deg = nullptr;
+    uint32_t* part = nullptr;
+    uint32_t* s2e = nullptr;
+    uint32_t* esrc = nullptr;
+    uint32_t* eslot = nullptr;
+    uint8_t* st = nullptr;
+    double* sup = nullptr;
+    double* sup0 = nullptr;
+    uint32_t* tcnt = nullptr;
+    double* pv = nullptr;
+    double* pv0 = nullptr;
+    uint32_t* vtcnt = nullptr;
+    uint8_t* touched = nullptr;
+    uint32_t* claim = nullptr;
+    uint32_t n = 0, m = 0;
+    uint64_t nnz = 0;
+};
+
+// Counters written by kernels (one device block per graph).
+struct Counters {
+    unsigned long long tri_total;      // triangles (count phase)
+    unsigned long long tri_removed;    // triangles removed by cleaning
+    unsigned long long edges_removed;  // edges removed by cleaning
+    unsigned long long ambiguous_clean;
+    unsigned long long ambiguous_ratio;
+    unsigned long long seeds;
+    unsigned int frontier_next;        // size of the next peel frontier
+    unsigned int error;                // validation / capacity flags
+    unsigned int work_next;            // extraction work queue head
+    unsigned int changed;              // union-find progress flag
+    unsigned int pad[2];
+};
+
+enum : uint32_t {
+    ERR_SELF_LOOP = 1u,
+    ERR_UNSORTED_OR_DUP = 2u,
+    ERR_RANGE = 4u,
+    ERR_ASYMMETRIC = 8u,
+    ERR_SCRATCH = 16u,
+};
+
+struct Timings {
+    double build_ms = 0, count_ms = 0, clean_ms = 0, extract_ms = 0;
+    double count_edge_ms = 0, count_vertex_ms = 0;
+};
+
+struct AuditHost {
+    int64_t clean_rounds = 0, edges_removed = 0, triangles_removed = 0;
+    int64_t ambiguous_clean = 0, ambiguous_ratio = 0, seeds = 0;
+    int64_t components = 0, largest_component = 0;
+};
+
+enum Phase { PH_NONE = 0, PH_BUILT = 1, PH_COUNTED = 2, PH_CLEANED = 3, PH_EXTRACTED = 4 };
+
+}  // namespace rtrex
+
+// The opaque handle of the C ABI.
+struct rtrex_graph_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rtrex::GraphDev g;
+    rtrex::Counters* ctr = nullptr;  // device
+    std::vector<void*> allocs;       // owned device allocations
+    rtrex::Phase phase = rtrex::PH_NONE;
+    double eps = 0;
+    // Extraction results (device): seed vertex of each vertex's cluster,
+    // and the cluster id of every vertex.
+    uint32_t* cseed = nullptr;
+    uint32_t* cid = nullptr;  // cluster id per vertex (global numbering)
+    int64_t n_clusters = -1;
+    std::vector<int64_t> seg_clusters;  // per segment (batch)
+    std::vector<int64_t> cluster_map_host;
+    // Batch layout (one graph = one segment): vertex segment boundaries.
+    std::vector<int64_t> seg_vtx;  // empty for a single graph
+    uint32_t* nearst = nullptr;  // per edge: last peel round it was near-threshold
+    uint32_t round_stamp = 0;
+    rtrex::Timings tm;
+    rtrex::AuditHost audit;
+    hipEvent_t ev[8] = {};
+};
+
+namespace rtrex {
+
+// ---- error reporting (api.cpp) -------------------------------------------
+void set_error(const std::string& msg);
+rtrex_status fail(rtrex_status code, const std::string& msg);
+rtrex_status hip_fail(hipError_t e, const char* what, const char* file,
+                      int line);
+
+#define RT_HIP(call)                                                       \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess)                                              \
+            return ::rtrex::hip_fail(e_, #call, __FILE__, __LINE__);       \
+    } while (0)
+
+#define RT_TRY(call)                                                       \
+    do {                                                                   \
+        rtrex_status s_ = (call);                                          \
+        if (s_ != RTREX_OK) return s_;                                     \
+    } while (0)
+
+// ---- device allocation owned by a handle ---------------------------------
+template <class T>
+rtrex_status dalloc(rtrex_graph h, T** p, uint64_t count) {
+    void* q = nullptr;
+    uint64_t bytes = count ? count * sizeof(T) : sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RTREX_ENOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                      " bytes failed: " + hipGetErrorString(e));
+    }
+    h->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return RTREX_OK;
+}
+
+// Scratch that lives for one call.
+struct Scratch {
+    std::vector<void*> ptrs;
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    rtrex_status get(T** p, uint64_t count) {
+        void* q = nullptr;
+        uint64_t bytes = count ? count * sizeof(T) : sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(RTREX_ENOMEM, "scratch hipMalloc of " +
+                                          std::to_string(bytes) + " bytes failed");
+        }
+        ptrs.push_back(q);
+        *p = static_cast<T*>(q);
+        return RTREX_OK;
+    }
+};
+
+// ---- primitives (sort.hip) ------------------------------------------------
+// In-place LSD radix sort of u64 keys on bits [0, bits); tmp has n entries.
+rtrex_status radix_sort_u64(uint64_t* keys, uint64_t* tmp, uint64_t n,
+                            int bits, hipStream_t s);
+// Exclusive scan of u32 counts into u64 (out has n+1 entries, out[n] = sum).
+rtrex_status exclusive_scan_u32(const uint32_t* in, uint64_t* out, uint64_t n,
+                                hipStream_t s);
+// Same, u32 output (caller guarantees the sum fits in 32 bits).
+rtrex_status exclusive_scan_u32_32(const uint32_t* in, uint32_t* out,
+                                   uint64_t n, hipStream_t s);
+
+// ---- phases ---------------------------------------------------------------
+rtrex_status build_from_csr(rtrex_graph h, const int64_t* off,
+                            const int64_t* nb, int64_t n, int64_t nnz);
+rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
+                              const uint32_t* v, int64_t n, int64_t m);
+rtrex_status run_count(rtrex_graph h);
+rtrex_status run_initial_clean(rtrex_graph h, double eps);
+rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
+                                  int64_t k, double eps);
+rtrex_status run_extract(rtrex_graph h, double eps);
+rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
+                               int64_t* cut);
+
+inline unsigned blocks_for(uint64_t work, unsigned per_block,
+                           unsigned cap = 1u << 20) {
+    uint64_t b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+}  // namespace rtrex
