@@ -116,6 +116,13 @@ rtrex_status rtrex_initial_clean(rtrex_graph g, double eps);
  * syncEdgeStatusDeviceToHost (gpu_graph.cuh:76). */
 rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status);
 
+/* Survivors of the last rtrex_initial_clean, packed: bit (e % 32) of
+ * bits[e / 32] is set when undirected edge e is alive, edges numbered in
+ * (u < v) lexicographic order -- the order of the reference's sorted survivor
+ * dump (Decomposition.h:419-452).  Stays valid through extraction.  nwords
+ * must be >= (nnz/2 + 31) / 32. */
+rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords);
+
 /* Deletes the k undirected edges pairs[2k] (each must be alive) and cleans to
  * convergence.  Replaces gpuDeleteAndClean (gpu_clean.cuh:44-45) /
  * deleteAndClean (Decomposition.h:147-283). */

@@ -39,6 +39,7 @@ EXPORTS = [
     "rtrex_graph_create_csr", "rtrex_graph_create_edges", "rtrex_graph_destroy",
     "rtrex_graph_info", "rtrex_graph_get_csr", "rtrex_graph_footprint_bytes",
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
+    "rtrex_get_survivors",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_decompose",
 ]
@@ -94,6 +95,7 @@ def lib() -> C.CDLL:
             "rtrex_get_supports": [G, PD, PD, P64],
             "rtrex_initial_clean": [G, C.c_double],
             "rtrex_get_edge_status": [G, PU8],
+            "rtrex_get_survivors": [G, PU32, C.c_int64],
             "rtrex_delete_and_clean": [G, P64, C.c_int64, C.c_double],
             "rtrex_extract": [G, C.c_double, P64, P64],
             "rtrex_decompose": [G, C.c_double, P64, P64],
@@ -215,6 +217,15 @@ class DeviceGraph:
         st = np.empty(self.nnz, np.uint8)
         _check(lib().rtrex_get_edge_status(self._h, _p(st, PU8)))
         return st
+
+    def survivors(self) -> np.ndarray:
+        """Bitmap of the edges alive after the initial clean, one bool per
+        undirected edge in (u < v) lexicographic order."""
+        m = self.nnz // 2
+        words = np.zeros((m + 31) // 32, np.uint32)
+        _check(lib().rtrex_get_survivors(self._h, _p(words, PU32), len(words)))
+        bits = np.unpackbits(words.view(np.uint8), bitorder="little")
+        return bits[:m].astype(bool)
 
     def gpu_delete_and_clean(self, pairs, eps: float) -> None:
         pr = np.ascontiguousarray(pairs, np.int64).reshape(-1)

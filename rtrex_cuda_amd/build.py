@@ -2,6 +2,7 @@
 
     librtrex_hip.so   HIP kernels for gfx950 + the C ABI of include/rtrex.h
     librtrex_gen.so   seeded synthetic graph generators (host C++)
+    librtrex_hostcheck.so  host builds of device routines, for CPU tests
 
 Run ``python -m rtrex_cuda_amd.build`` (or ``__graft_entry__.build()``).
 hipcc cross-compiles for gfx950 without a GPU present.
@@ -16,6 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 HIP_LIB = os.path.join(PKG, "librtrex_hip.so")
 GEN_LIB = os.path.join(PKG, "librtrex_gen.so")
+CHECK_LIB = os.path.join(PKG, "librtrex_hostcheck.so")
 
 HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip"]
 
@@ -51,8 +53,17 @@ def build_gen(out: str = GEN_LIB) -> str:
     return out
 
 
+def build_hostcheck(out: str = CHECK_LIB) -> str:
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", "-o", tmp,
+          os.path.join(CSRC, "hostcheck.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_gen()
+    build_hostcheck()
     build_hip()
 
 

@@ -301,6 +301,17 @@ rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status) {
     return RTREX_OK;
 }
 
+rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords) {
+    RT_TRY(check(g));
+    if (!g->surv) return fail(RTREX_ESTATE, "initial clean has not run");
+    const int64_t need = ((int64_t)g->g.m + 31) / 32;
+    if (!bits || nwords < need) return fail(RTREX_EINVAL, "survivor buffer too small");
+    if (need)
+        RT_HIP(hipMemcpyAsync(bits, g->surv, need * 4, hipMemcpyDeviceToHost, g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    return RTREX_OK;
+}
+
 rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
                                     int64_t k, double eps) {
     RT_TRY(check(g));

@@ -66,18 +66,65 @@ __device__ __forceinline__ bool next_perm(double* a, int k) {
     return true;
 }
 
-// Certifies "support(f) < thr" for edge f = {x, y} with count-phase support s0.
-// A triangle {x, y, z} counts as dead when one of its other two edges is
+// Accumulates one dead triangle of weight t into the certification state.
+struct CertAcc {
+    double hi, lo;
+    double t8[kMaxPerm];
+    int nd;
+    __device__ explicit CertAcc(double s0) : hi(s0), lo(0.0), nd(0) {}
+    __device__ void add(double t) {
+        double s, e;
+        two_sum(hi, -t, s, e);
+        hi = s;
+        lo += e;
+        if (nd < kMaxPerm) t8[nd] = t;
+        ++nd;
+    }
+    // Outcome over every order of the accumulated subtractions from s0.
+    __device__ int finish(double s0, double thr) {
+        const double u = ulp_of(s0);
+        const double half = 0.5 * (double)nd * u * (1.0 + 0x1.0p-40) + 0x1.0p-1060;
+        double d, de;
+        two_sum(hi, -thr, d, de);
+        d += de + lo;
+        if (d >= half) return CERT_KEEP;
+        if (d < -half) return CERT_REMOVE;
+        if (nd > kMaxPerm) return CERT_UNDECIDED;
+        for (int i = 1; i < nd; ++i)  // insertion sort ascending
+            for (int j = i; j > 0 && t8[j] < t8[j - 1]; --j) {
+                double t = t8[j];
+                t8[j] = t8[j - 1];
+                t8[j - 1] = t;
+            }
+        bool seen_lt = false, seen_ge = false;
+        do {  // every distinct order, with the reference's plain subtractions
+            double v = s0;
+            for (int i = 0; i < nd; ++i) v -= t8[i];
+            if (v < thr) seen_lt = true;
+            else seen_ge = true;
+            if (seen_lt && seen_ge) return CERT_UNDECIDED;
+        } while (next_perm(t8, nd));
+        return seen_lt ? CERT_REMOVE : CERT_KEEP;
+    }
+};
+
+__device__ __forceinline__ bool tri_dead(uint8_t sa, uint8_t sb, bool qdead) {
+    bool dead = sa == ST_DEAD || sb == ST_DEAD;
+    if (qdead) dead = dead || sa == ST_QUEUED || sb == ST_QUEUED;
+    return dead;
+}
+
+// Certifies "support(f) < thr" for edge f = {x, y} with count-phase support
+// s0.  A triangle {x, y, z} counts as dead when one of its other two edges is
 // DEAD or QUEUED (qdead) / DEAD only (!qdead), or when z == extra_dead (the
-// triangle shared with an edge being deleted right now).  Runs on one lane.
+// triangle shared with an edge being deleted right now).  Runs on one lane;
+// enumerates f's triangles by merging the two sorted lists.
 __device__ int certify(const GraphDev& g, uint32_t x, uint32_t y, double s0,
                        double thr, bool qdead, uint32_t extra_dead) {
     const uint32_t dx = g.deg[x], dy = g.deg[y];
     uint32_t a = g.off[x], ae = g.off[x + 1];
     uint32_t b = g.off[y], be = g.off[y + 1];
-    double hi = s0, lo = 0.0;
-    double t8[kMaxPerm];
-    int nd = 0;
+    CertAcc acc(s0);
     while (a < ae && b < be) {
         const uint32_t za = g.nb[a], zb = g.nb[b];
         if (za < zb) {
@@ -85,46 +132,39 @@ __device__ int certify(const GraphDev& g, uint32_t x, uint32_t y, double s0,
         } else if (zb < za) {
             ++b;
         } else {
-            const uint8_t sa = g.st[a], sb = g.st[b];
-            bool dead = sa == ST_DEAD || sb == ST_DEAD || za == extra_dead;
-            if (qdead) dead = dead || sa == ST_QUEUED || sb == ST_QUEUED;
-            if (dead) {
-                const double t = tri_w(dx, dy, g.deg[za]);
-                double s, e;
-                two_sum(hi, -t, s, e);
-                hi = s;
-                lo += e;
-                if (nd < kMaxPerm) t8[nd] = t;
-                ++nd;
-            }
+            if (tri_dead(g.st[a], g.st[b], qdead) || za == extra_dead)
+                acc.add(tri_w(dx, dy, g.deg[za]));
             ++a;
             ++b;
         }
     }
-    const double u = ulp_of(s0);
-    const double half = 0.5 * (double)nd * u * (1.0 + 0x1.0p-40) + 0x1.0p-1060;
-    double d, de;
-    two_sum(hi, -thr, d, de);
-    d += de + lo;
-    if (d >= half) return CERT_KEEP;
-    if (d < -half) return CERT_REMOVE;
-    if (nd > kMaxPerm) return CERT_UNDECIDED;
-    // every distinct order, with the reference's plain subtractions
-    for (int i = 1; i < nd; ++i)  // insertion sort ascending
-        for (int j = i; j > 0 && t8[j] < t8[j - 1]; --j) {
-            double t = t8[j];
-            t8[j] = t8[j - 1];
-            t8[j - 1] = t;
-        }
-    bool seen_lt = false, seen_ge = false;
-    do {
-        double v = s0;
-        for (int i = 0; i < nd; ++i) v -= t8[i];
-        if (v < thr) seen_lt = true;
-        else seen_ge = true;
-        if (seen_lt && seen_ge) return CERT_UNDECIDED;
-    } while (next_perm(t8, nd));
-    return seen_lt ? CERT_REMOVE : CERT_KEEP;
+    return acc.finish(s0, thr);
+}
+
+// Same, reading edge f's stored triangle list; the triangle shared with edge
+// extra_edge (being deleted right now) counts as dead.
+__device__ int certify_terms(const GraphDev& g, const TermBuf& tb, uint32_t f,
+                             double thr, bool qdead, uint32_t extra_edge) {
+    const double s0 = g.sup0[f];
+    const uint64_t k0 = tb.zoff[f], k1 = k0 + g.tcnt[f];
+    CertAcc acc(s0);
+    for (uint64_t k = k0; k < k1; ++k) {
+        const uint32_t a = tb.sa[k], b = tb.sb[k];
+        if (tri_dead(g.est[a], g.est[b], qdead) || a == extra_edge || b == extra_edge)
+            acc.add(tb.w[k]);
+    }
+    return acc.finish(s0, thr);
+}
+
+// Either form, by what the graph keeps.  The extra dead triangle is named by
+// its third vertex (intersection form) and by the edge being deleted (stored
+// lists).
+__device__ __forceinline__ int certify_edge(const GraphDev& g, const TermBuf& tb,
+                                            uint32_t f, double thr, bool qdead,
+                                            uint32_t extra_vertex, uint32_t extra_edge) {
+    if (tb.w) return certify_terms(g, tb, f, thr, qdead, extra_edge);
+    const uint32_t sl = g.eslot[f];
+    return certify(g, g.esrc[f], g.nb[sl], g.sup0[f], thr, qdead, extra_vertex);
 }
 
 }  // namespace rtrex

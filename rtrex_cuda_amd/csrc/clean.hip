@@ -34,18 +34,28 @@ constexpr int T = 256;
 // Initial frontier: alive edges below threshold (Decomposition.h:378-405).
 // The count-phase supports are bit-identical to the reference's, so these
 // decisions are the reference's exactly.
+__device__ __forceinline__ int work_bucket(uint32_t w) {
+    return w <= 4u ? 0 : w <= 16u ? 1 : w <= 96u ? 2 : 3;
+}
+
+// When bk is given (stored triangle lists), the frontier is also written
+// split by work bucket of tcnt (bk: 4 lists of capacity m, then 4 counts).
 __global__ void __launch_bounds__(T) k_seed_frontier(GraphDev g, double eps,
                                                      uint32_t* __restrict__ F,
-                                                     unsigned* nF) {
+                                                     unsigned* nF,
+                                                     uint32_t* __restrict__ bk) {
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m;
          e += gridDim.x * T) {
-        const uint32_t sl = g.eslot[e];
-        if (g.st[sl] != ST_ALIVE) continue;
-        const uint32_t x = g.esrc[e], y = g.nb[sl];
-        if (g.sup[e] < edge_thr(eps, g.deg[x], g.deg[y])) {
-            g.st[sl] = ST_QUEUED;
-            g.st[g.part[sl]] = ST_QUEUED;
-            F[atomicAdd(nF, 1u)] = e;
+        if (g.sup[e] < eps * g.ew[e]) {
+            if (g.est[e] != ST_ALIVE) continue;
+            set_status(g, e, ST_QUEUED);
+            F[wave_append(nF)] = e;
+            if (bk) {
+                const int b = work_bucket(g.tcnt[e]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (b == q) bk[(uint64_t)q * g.m + wave_append(bk + 4ull * g.m + q)] = e;
+            }
         }
     }
 }
@@ -69,12 +79,11 @@ __global__ void __launch_bounds__(T) k_seed_pairs(GraphDev g,
             continue;
         }
         const uint32_t e = g.s2e[sl];
-        if (g.st[sl] != ST_ALIVE || atomicCAS(&g.claim[e], 0u, 2u) != 0u) {
+        if (g.est[e] != ST_ALIVE || atomicCAS(&g.claim[e], 0u, 2u) != 0u) {
             atomicOr(err, ERR_UNSORTED_OR_DUP);
             continue;
         }
-        g.st[sl] = ST_QUEUED;
-        g.st[g.part[sl]] = ST_QUEUED;
+        set_status(g, e, ST_QUEUED);
         F[atomicAdd(nF, 1u)] = e;
     }
 }
@@ -82,10 +91,26 @@ __global__ void __launch_bounds__(T) k_seed_pairs(GraphDev g,
 __device__ __forceinline__ void claim_edge(const GraphDev& g, uint32_t f,
                                            uint32_t* Fn, unsigned* nFn) {
     if (atomicCAS(&g.claim[f], 0u, 1u) == 0u) {
-        const uint32_t sl = g.eslot[f];
-        g.st[sl] = ST_PENDING;
-        g.st[g.part[sl]] = ST_PENDING;
-        Fn[atomicAdd(nFn, 1u)] = f;
+        set_status(g, f, ST_PENDING);
+        Fn[wave_append(nFn)] = f;
+    }
+}
+
+// Splits a frontier into four work buckets (shorter-list length <= 4, 16,
+// 96, larger) so each edge gets 1, 4, 16 or 64 lanes.
+__global__ void __launch_bounds__(T) k_frontier_buckets(GraphDev g, bool terms,
+                                                        const uint32_t* __restrict__ F,
+                                                        uint32_t nF,
+                                                        uint32_t* __restrict__ out,
+                                                        unsigned* cur) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nF; i += gridDim.x * T) {
+        const uint32_t e = F[i];
+        const uint32_t w = terms ? g.tcnt[e]
+                                 : min(g.deg[g.esrc[e]], g.deg[g.nb[g.eslot[e]]]);
+        const int b = work_bucket(w);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (b == k) out[(uint64_t)k * nF + wave_append(&cur[k])] = e;
     }
 }
 
@@ -101,9 +126,117 @@ struct PeelArgs {
     uint32_t* nearst;
     uint32_t stamp;  // this round's near-list stamp
     Counters* ctr;
+    TermBuf tb;
 };
 
+// Shared tail of both peel kernels for one owned triangle {x, y, z}: the two
+// other edges f (slot sf_slot, status sf) lose t; an alive one that falls
+// below its threshold is claimed, or near-listed when within the cheap bound.
+__device__ __forceinline__ void peel_side(const PeelArgs& A, uint32_t f, uint8_t sf,
+                                          double t) {
+    const GraphDev& g = A.g;
+    if (sf != ST_ALIVE) return;  // queued or claimed edges die anyway
+    const double old = atomicAdd(&g.sup[f], -t);
+    g.touched[f] = 1;
+    const double nv = old - t;
+    const double thr = A.eps * g.ew[f];
+    const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
+    if (nv < thr - cb) {
+        claim_edge(g, f, A.Fn, A.nFn);
+    } else if (nv < thr + cb && atomicExch(&A.nearst[f], A.stamp) != A.stamp) {
+        A.near[wave_append(A.nNear)] = f;
+    }
+}
+
+// One "push" peel round over stored triangle lists, G lanes per frontier edge:
+// no intersections, each lane takes one (weight, edge, edge) record.  Vertex
+// supports and the live-triangle total are not touched here: both are pulled
+// once the clean converges.
+template <int G>
+__global__ void __launch_bounds__(T) k_peel_terms(PeelArgs A) {
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    const int gl = lane & (G - 1);
+    const uint32_t per_wave = WAVE / G;
+    const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
+    const uint32_t nwaves = gridDim.x * T / WAVE;
+    for (uint32_t base = wave * per_wave; base < A.nF; base += nwaves * per_wave) {
+        const uint32_t idx = base + lane / G;
+        const bool act = idx < A.nF;
+        const uint32_t e = act ? A.F[idx] : 0;
+        const uint64_t k0 = act ? tb.zoff[e] : 0;
+        const uint64_t k1 = act ? k0 + g.tcnt[e] : 0;
+        uint64_t k = k0 + gl;
+        while (__any(act && k - gl < k1)) {
+            const bool go = act && k - gl < k1;
+            const bool in = go && k < k1;
+            if (in) {
+                const uint32_t fj = tb.sa[k], fh = tb.sb[k];
+                const uint8_t sj = g.est[fj], sh = g.est[fh];
+                // Only a triangle with an alive edge needs work: that edge
+                // loses t, once, by the smallest frontier edge id.
+                if (sj != ST_DEAD && sh != ST_DEAD && (sj == ST_ALIVE || sh == ST_ALIVE) &&
+                    !(sj == ST_QUEUED && fj < e) && !(sh == ST_QUEUED && fh < e)) {
+                    const double t = tb.w[k];
+                    peel_side(A, fj, sj, t);
+                    peel_side(A, fh, sh, t);
+                }
+            }
+            k += G;
+        }
+    }
+}
+
 // One peel round with G lanes per frontier edge.
+// One "pull" peel round: every alive edge subtracts, in its stored order, the
+// triangles that die this round (one other edge in the frontier, none already
+// deleted) and decides for itself.  Chosen when the frontier is larger than
+// the alive set, e.g. the first round, where nearly every edge is below its
+// threshold.  No atomics on supports, no ownership rule: each triangle is seen
+// by each of its alive edges exactly once.
+__global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A) {
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        if (g.est[e] != ST_ALIVE) continue;
+        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+        double v = g.sup[e];
+        bool any = false;
+        for (uint64_t k = k0; k < k1; ++k) {
+            const uint8_t a = g.est[tb.sa[k]], b = g.est[tb.sb[k]];
+            if ((a == ST_QUEUED || b == ST_QUEUED) && a != ST_DEAD && b != ST_DEAD) {
+                v -= tb.w[k];
+                any = true;
+            }
+        }
+        if (!any) continue;
+        g.sup[e] = v;
+        g.touched[e] = 1;
+        const double thr = A.eps * g.ew[e];
+        const double cb = cheap_bound(g.tcnt[e], g.sup0[e]);
+        if (v < thr - cb) {
+            claim_edge(g, e, A.Fn, A.nFn);
+        } else if (v < thr + cb && atomicExch(&A.nearst[e], A.stamp) != A.stamp) {
+            A.near[wave_append(A.nNear)] = e;
+        }
+    }
+}
+
+// Triangles with all three edges alive, counted once per alive edge.
+__global__ void __launch_bounds__(T) k_live_incidences(GraphDev g, TermBuf tb,
+                                                       unsigned long long* out) {
+    unsigned long long c = 0;
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        if (g.est[e] != ST_ALIVE) continue;
+        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+        for (uint64_t k = k0; k < k1; ++k)
+            c += g.est[tb.sa[k]] == ST_ALIVE && g.est[tb.sb[k]] == ST_ALIVE;
+    }
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
 template <int G>
 __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
     const GraphDev& g = A.g;
@@ -139,27 +272,26 @@ __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
             bool tri = in && sj != ST_DEAD && pos < he && g.nb[pos] == w;
             uint8_t sh = tri ? g.st[pos] : ST_DEAD;
             tri = tri && sh != ST_DEAD;
+            double t = 0.0;
             if (tri) {
                 const uint32_t fj = g.s2e[j], fh = g.s2e[pos];
                 const bool owned = !(sj == ST_QUEUED && fj < e) &&
                                    !(sh == ST_QUEUED && fh < e);
                 if (owned) {
                     const uint32_t dw = g.deg[w];
-                    const double t = tri_w(dx, dy, dw);
-                    atomicAdd(&g.sup[e], -t);
-                    atomicAdd(&g.pv[x], -t);
-                    atomicAdd(&g.pv[y], -t);
+                    t = tri_w(dx, dy, dw);
                     atomicAdd(&g.pv[w], -t);
                     removed++;
                     // The two other edges: (lo, w) at slot j, (hi, w) at pos.
+                    // Edges already queued or claimed die anyway: their
+                    // supports are not maintained (set to 0 at deletion).
 #pragma unroll
                     for (int side = 0; side < 2; ++side) {
                         const uint32_t f = side ? fh : fj;
                         const uint8_t sf = side ? sh : sj;
-                        const double old = atomicAdd(&g.sup[f], -t);
-                        if (sf == ST_QUEUED) continue;
-                        g.touched[f] = 1;
                         if (sf != ST_ALIVE) continue;
+                        const double old = atomicAdd(&g.sup[f], -t);
+                        g.touched[f] = 1;
                         const double nv = old - t;
                         const double thr = edge_thr(A.eps, side ? dhi : dlo, dw);
                         const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
@@ -167,10 +299,19 @@ __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
                             claim_edge(g, f, A.Fn, A.nFn);
                         } else if (nv < thr + cb &&
                                    atomicExch(&A.nearst[f], A.stamp) != A.stamp) {
-                            A.near[atomicAdd(A.nNear, 1u)] = f;
+                            A.near[wave_append(A.nNear)] = f;
                         }
                     }
                 }
+            }
+            // x and y lose every owned triangle of the chunk: one subtraction
+            // of the group's sum each.
+            double ts = t;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) ts += __shfl_xor(ts, d, 64);
+            if (go && gl == 0 && ts != 0.0) {
+                atomicAdd(&g.pv[x], -ts);
+                atomicAdd(&g.pv[y], -ts);
             }
             if (go) {
                 hlo = nh;
@@ -189,11 +330,9 @@ __global__ void __launch_bounds__(T) k_near_check(PeelArgs A, uint32_t nNear) {
     unsigned long long amb = 0;
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nNear; i += gridDim.x * T) {
         const uint32_t f = A.near[i];
-        const uint32_t sl = g.eslot[f];
-        if (g.st[sl] != ST_ALIVE) continue;  // already claimed
-        const uint32_t x = g.esrc[f], y = g.nb[sl];
-        const double thr = edge_thr(A.eps, g.deg[x], g.deg[y]);
-        int c = certify(g, x, y, g.sup0[f], thr, true, 0xffffffffu);
+        if (g.est[f] != ST_ALIVE) continue;  // already claimed
+        const double thr = A.eps * g.ew[f];
+        int c = certify_edge(g, A.tb, f, thr, true, 0xffffffffu, 0xffffffffu);
         if (c == CERT_UNDECIDED) {
             amb++;
             c = g.sup[f] < thr ? CERT_REMOVE : CERT_KEEP;
@@ -213,15 +352,11 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < max(nF, nFn);
          i += gridDim.x * T) {
         if (i < nF) {
-            const uint32_t sl = g.eslot[F[i]];
-            g.st[sl] = ST_DEAD;
-            g.st[g.part[sl]] = ST_DEAD;
+            const uint32_t f = F[i];
+            set_status(g, f, ST_DEAD);
+            g.sup[f] = 0.0;
         }
-        if (i < nFn) {
-            const uint32_t sl = g.eslot[Fn[i]];
-            g.st[sl] = ST_QUEUED;
-            g.st[g.part[sl]] = ST_QUEUED;
-        }
+        if (i < nFn) set_status(g, Fn[i], ST_QUEUED);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&ctr->edges_removed, (unsigned long long)nF);
@@ -229,25 +364,37 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
 
 // After convergence, a safety net: every alive edge that lost triangles and
 // is not clearly above its threshold is certified once more.
-__global__ void __launch_bounds__(T) k_audit_survivors(GraphDev g, double eps,
-                                                       Counters* ctr) {
+__global__ void __launch_bounds__(T) k_audit_survivors(GraphDev g, TermBuf tb,
+                                                       double eps, Counters* ctr) {
     unsigned long long amb = 0;
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m;
          e += gridDim.x * T) {
-        if (!g.touched[e]) continue;
-        const uint32_t sl = g.eslot[e];
-        if (g.st[sl] != ST_ALIVE) continue;
-        const uint32_t x = g.esrc[e], y = g.nb[sl];
-        const double thr = edge_thr(eps, g.deg[x], g.deg[y]);
+        if (!g.touched[e] || g.est[e] != ST_ALIVE) continue;
+        const double thr = eps * g.ew[e];
         if (g.sup[e] >= thr + cheap_bound(g.tcnt[e], g.sup0[e])) continue;
-        if (certify(g, x, y, g.sup0[e], thr, false, 0xffffffffu) != CERT_KEEP) amb++;
+        if (certify_edge(g, tb, e, thr, false, 0xffffffffu, 0xffffffffu) != CERT_KEEP) amb++;
     }
     for (int d = 32; d >= 1; d >>= 1) amb += __shfl_xor(amb, d, 64);
     if (lane_id() == 0 && amb) atomicAdd(&ctr->ambiguous_clean, amb);
 }
 
+// Packed survivor bitmap (bit e = edge e alive), kept after the initial clean.
+__global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits) {
+    const uint32_t nw = (g.m + 31) / 32;
+    for (uint32_t w = blockIdx.x * T + threadIdx.x; w < nw; w += gridDim.x * T) {
+        uint32_t b = 0;
+        for (uint32_t i = 0; i < 32; ++i) {
+            const uint32_t e = w * 32 + i;
+            if (e < g.m && g.est[e] == ST_ALIVE) b |= 1u << i;
+        }
+        bits[w] = b;
+    }
+}
+
+// Frontier rounds to convergence.  first_bk (optional) holds the first
+// frontier already split by work bucket (4 lists of capacity m + 4 counts).
 rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
-                  uint32_t nF) {
+                  uint32_t nF, const uint32_t* first_bk = nullptr) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
@@ -266,16 +413,65 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     A.nNear = &h->ctr->changed;
     A.nearst = h->nearst;
     A.ctr = h->ctr;
+    A.tb = h->terms();
+    uint32_t* bk;  // bucketed frontier: 4 x m ids + 4 counters
+    RT_TRY(sc.get(&bk, 4ull * g.m + 4));
+    unsigned* bcur = bk + 4ull * g.m;
     while (nF) {
         RT_HIP(hipMemsetAsync(A.nFn, 0, sizeof(unsigned), s));
         RT_HIP(hipMemsetAsync(A.nNear, 0, sizeof(unsigned), s));
-        A.F = F;
-        A.nF = nF;
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
-        uint64_t waves = ((uint64_t)nF * 16 + WAVE - 1) / WAVE;
-        unsigned grid = (unsigned)std::min<uint64_t>((waves * WAVE + T - 1) / T, 8192);
-        k_peel<16><<<grid, T, 0, s>>>(A);
+        uint32_t bc[4] = {nF, 0, 0, 0};
+        const uint32_t* lists[4] = {F, nullptr, nullptr, nullptr};
+        const bool pull_round = h->zw != nullptr && (uint64_t)nF > h->n_alive;
+        if (pull_round) {
+            first_bk = nullptr;  // pull rounds scan the alive edges instead
+        } else if (first_bk) {  // first round, bucketed while it was seeded
+            RT_HIP(hipMemcpyAsync(bc, first_bk + 4ull * g.m, sizeof(bc),
+                                  hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < 4; ++b) lists[b] = first_bk + (uint64_t)b * g.m;
+            first_bk = nullptr;
+        } else if (nF > 4096) {  // large frontiers: one launch per work bucket
+            RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
+            k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.w != nullptr,
+                                                                     F, nF, bk, bcur);
+            RT_HIP(hipMemcpyAsync(bc, bcur, sizeof(bc), hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < 4; ++b) lists[b] = bk + (uint64_t)b * nF;
+        } else {
+            bc[0] = 0;
+            bc[2] = nF;
+            lists[2] = F;
+        }
+        const bool pull = A.tb.w != nullptr && (uint64_t)nF > h->n_alive;
+        if (pull) {
+            A.F = F;
+            A.nF = nF;
+            k_peel_pull<<<blocks_for(g.m, T, 8192), T, 0, s>>>(A);
+        }
+        for (int b = 0; b < 4 && !pull; ++b) {
+            if (!bc[b]) continue;
+            const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
+            A.F = lists[b];
+            A.nF = bc[b];
+            uint64_t waves = ((uint64_t)bc[b] * G + WAVE - 1) / WAVE;
+            unsigned grid = (unsigned)std::min<uint64_t>((waves * WAVE + T - 1) / T, 8192);
+            if (A.tb.w) {
+                if (G == 1) k_peel_terms<1><<<grid, T, 0, s>>>(A);
+                else if (G == 4) k_peel_terms<4><<<grid, T, 0, s>>>(A);
+                else if (G == 16) k_peel_terms<16><<<grid, T, 0, s>>>(A);
+                else k_peel_terms<64><<<grid, T, 0, s>>>(A);
+            } else {
+                if (G == 1) k_peel<1><<<grid, T, 0, s>>>(A);
+                else if (G == 4) k_peel<4><<<grid, T, 0, s>>>(A);
+                else if (G == 16) k_peel<16><<<grid, T, 0, s>>>(A);
+                else k_peel<64><<<grid, T, 0, s>>>(A);
+            }
+        }
+        A.F = F;
+        A.nF = nF;
         uint32_t nNear = 0;
         RT_HIP(hipMemcpyAsync(&nNear, A.nNear, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
@@ -288,11 +484,27 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
             g, F, nF, Fn, nFn, h->ctr);
         RT_HIP(hipGetLastError());
         h->audit.clean_rounds++;
+        h->n_alive -= nFn;
         std::swap(F, Fn);
         nF = nFn;
     }
-    k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, eps, h->ctr);
+    if (A.tb.w) {  // live-triangle total (the peel kernels do not count)
+        Scratch sc2;
+        unsigned long long* inc;
+        RT_TRY(sc2.get(&inc, 1));
+        RT_HIP(hipMemsetAsync(inc, 0, 8, s));
+        k_live_incidences<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, inc);
+        unsigned long long c[2] = {0, 0};
+        RT_HIP(hipMemcpyAsync(&c[0], inc, 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipMemcpyAsync(&c[1], &h->ctr->tri_total, 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        unsigned long long removed = c[1] / 3 - c[0] / 3;
+        RT_HIP(hipMemcpyAsync(&h->ctr->tri_removed, &removed, 8, hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr);
     RT_HIP(hipGetLastError());
+    RT_TRY(pull_vertex_supports(h));
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }
@@ -307,18 +519,25 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
-    uint32_t *F, *Fn;
+    uint32_t *F, *Fn, *bk = nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
+    const bool terms = h->zw != nullptr;
+    if (terms) RT_TRY(sc.get(&bk, 4ull * g.m + 4));
     RT_HIP(hipEventRecord(h->ev[5], s));
     unsigned* cnt = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
-    k_seed_frontier<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, eps, F, cnt);
+    if (terms) RT_HIP(hipMemsetAsync(bk + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
+    k_seed_frontier<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, eps, F, cnt, bk);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
     RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    RT_TRY(peel(h, eps, F, Fn, nF));
+    h->n_alive = g.m - nF;
+    RT_TRY(peel(h, eps, F, Fn, nF, bk));
+    if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
+    k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
+    RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[6], s));
     RT_HIP(hipEventSynchronize(h->ev[6]));
     float ms = 0;
@@ -366,6 +585,7 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
         return fail(RTREX_EINVAL,
                     "delete_and_clean: a pair is not an alive edge of the "
                     "graph, or is listed twice (state is undefined now)");
+    h->n_alive -= nF;
     RT_TRY(peel(h, eps, F, Fn, nF));
     return RTREX_OK;
 }

@@ -15,8 +15,15 @@
 //       streamed in chunks of G ascending ids, each lane probes the longer list
 //       (binary search, window narrowed chunk by chunk), and the matches of a
 //       chunk are folded in lane order (ballot + lane shuffle).
-//   k_count_vertices<G>: G lanes per vertex walking y in N(v) in order, each
-//       step the same chunked intersection restricted to ids > y.
+//   The matched weights of every edge are also stored, ascending z, in a term
+//   buffer (capacity min(d_x, d_y) - 1 per edge), with the number of matches
+//   below x and below y.  Vertex v's fold is then the concatenation, over y in
+//   N(v) ascending, of the stored terms of edge {v, y} with z > y -- a
+//   streaming read folded serially (k_vfold_lane / k_vfold_wave), instead of
+//   a second round of intersections.
+//   k_count_vertices<G>: fallback when the term buffer does not fit in HBM:
+//       G lanes per vertex walking y in N(v) in order, each step a chunked
+//       intersection restricted to ids > y.
 // Edges and vertices are bucketed by work (shorter-list length, degree) so
 // that G matches the work: 1, 4, 16 or 64 lanes.
 #include <algorithm>
@@ -91,7 +98,7 @@ template <int G>
 __device__ __forceinline__ bool probe_chunk(const uint32_t* __restrict__ nb,
                                             bool act, uint32_t p, uint32_t le,
                                             uint32_t& hlo, uint32_t he,
-                                            uint32_t& z) {
+                                            uint32_t& z, uint32_t* hitpos = nullptr) {
     const int lane = lane_id();
     const int gl = lane & (G - 1);
     const int gbase = lane & ~(G - 1);
@@ -99,6 +106,7 @@ __device__ __forceinline__ bool probe_chunk(const uint32_t* __restrict__ nb,
     z = in ? nb[p + gl] : 0;
     uint32_t pos = in ? lower_bound(nb, hlo, he, z) : hlo;
     bool hit = in && pos < he && nb[pos] == z;
+    if (hitpos) *hitpos = pos;
     // The group's last in-range lane holds the largest id of the chunk.
     uint32_t lastl = act ? min((uint32_t)(G - 1), le > p ? le - p - 1 : 0u) : 0u;
     uint32_t nh = __shfl(pos, gbase + lastl, 64);
@@ -106,13 +114,17 @@ __device__ __forceinline__ bool probe_chunk(const uint32_t* __restrict__ nb,
     return hit;
 }
 
+
 template <int G>
 __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
                                                    const uint32_t* __restrict__ list,
                                                    uint32_t cnt,
-                                                   unsigned long long* tri3) {
+                                                   unsigned long long* tri3,
+                                                   TermBuf tb) {
     const int lane = lane_id();
     const int gl = lane & (G - 1);
+    const int gbase = lane & ~(G - 1);
+    const uint64_t below = ((1ull << gl) - 1ull);  // lanes of the group before me
     const uint32_t per_wave = WAVE / G;
     const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nwaves = gridDim.x * T / WAVE;
@@ -127,13 +139,30 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
         uint32_t lo = dx <= dy ? x : y, hi = dx <= dy ? y : x;
         uint32_t p = act ? g.off[lo] : 0, le = act ? g.off[lo + 1] : 0;
         uint32_t hlo = act ? g.off[hi] : 0, he = act ? g.off[hi + 1] : 0;
+        const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
         double S = 0.0;
-        uint32_t c = 0;
+        uint32_t c = 0, ca = 0, cb = 0;
         while (__any(act && p < le)) {
             bool go = act && p < le;
-            uint32_t z;
-            bool hit = probe_chunk<G>(g.nb, go, p, le, hlo, he, z);
+            uint32_t z, hp;
+            bool hit = probe_chunk<G>(g.nb, go, p, le, hlo, he, z, &hp);
             double t = hit ? tri_w(dx, dy, g.deg[z]) : 0.0;
+            if (tb.w) {
+                const uint64_t hm = G == 64 ? ballot64(hit)
+                                            : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
+                if (hit) {
+                    const uint64_t k = wbase + c + __popcll(hm & below);
+                    tb.w[k] = t;
+                    tb.sa[k] = g.s2e[p + gl];  // edge {lo, z}
+                    tb.sb[k] = g.s2e[hp];      // edge {hi, z}
+                }
+                const uint64_t am = G == 64 ? ballot64(hit && z < x)
+                                            : (ballot64(hit && z < x) >> gbase) & ((1ull << G) - 1);
+                const uint64_t bm = G == 64 ? ballot64(hit && z < y)
+                                            : (ballot64(hit && z < y) >> gbase) & ((1ull << G) - 1);
+                ca += (uint32_t)__popcll(am);
+                cb += (uint32_t)__popcll(bm);
+            }
             group_fold<G>(hit, t, S, c);
             if (go) p += G;
         }
@@ -141,12 +170,156 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
             g.sup[e] = S;
             g.sup0[e] = S;
             g.tcnt[e] = c;
+            g.ew[e] = 1.0 / (double)((uint64_t)dx * (uint64_t)dy);
+            if (tb.w) {
+                tb.za[e] = ca;
+                tb.zb[e] = cb;
+            }
             mytri += c;
         }
     }
     // Wave-reduce the triangle incidences, one atomic per wave.
     for (int d = 32; d >= 1; d >>= 1) mytri += __shfl_xor(mytri, d, 64);
     if (lane == 0 && mytri) atomicAdd(tri3, mytri);
+}
+
+// Segment of vertex v's fold contributed by its neighbour slot j: the stored
+// terms of edge {v, y} with z > y.
+__device__ __forceinline__ void vseg(const GraphDev& g, const TermBuf& tb,
+                                     uint32_t v, uint32_t j, uint64_t& start,
+                                     uint32_t& len) {
+    const uint32_t y = g.nb[j];
+    const uint32_t e = g.s2e[j];
+    const uint32_t skip = v < y ? tb.zb[e] : tb.za[e];
+    start = tb.zoff[e] + skip;
+    len = g.tcnt[e] - skip;
+}
+
+// Dead-triangle test of term k of the segment of slot j (PULL mode): the
+// triangle is gone once any of its three edges is deleted.
+__device__ __forceinline__ bool term_dead(const GraphDev& g, const TermBuf& tb,
+                                          uint32_t j, uint64_t k) {
+    return g.st[j] == ST_DEAD || g.est[tb.sa[k]] == ST_DEAD || g.est[tb.sb[k]] == ST_DEAD;
+}
+
+// One lane per (low-degree) vertex: a plain serial fold over its segments.
+// PULL=false: the count-phase fold.  PULL=true: after a clean, the vertex
+// support is the count-phase value minus its dead triangles, subtracted in the
+// same fold order.
+template <bool PULL>
+__global__ void __launch_bounds__(T) k_vfold_lane(GraphDev g, TermBuf tb,
+                                                  const uint32_t* __restrict__ list,
+                                                  uint32_t cnt) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < cnt; i += gridDim.x * T) {
+        const uint32_t v = list[i];
+        double S = PULL ? g.pv0[v] : 0.0;
+        uint32_t c = 0;
+        for (uint32_t j = g.off[v]; j < g.off[v + 1]; ++j) {
+            uint64_t st;
+            uint32_t len;
+            vseg(g, tb, v, j, st, len);
+            for (uint32_t k = 0; k < len; ++k) {
+                if (PULL) {
+                    if (term_dead(g, tb, j, st + k)) S -= tb.w[st + k];
+                } else {
+                    S += tb.w[st + k];
+                }
+            }
+            c += len;
+        }
+        g.pv[v] = S;
+        if (!PULL) {
+            g.pv0[v] = S;
+            g.vtcnt[v] = c;
+        }
+    }
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One wave per vertex: 64 neighbour segments at a time are laid end to end
+// (lane scan of their lengths), the wave gathers 64 consecutive terms of that
+// stream per step and folds them serially, uniform across lanes.
+constexpr int VT = 256;
+template <bool PULL>
+__global__ void __launch_bounds__(VT) k_vfold_wave(GraphDev g, TermBuf tb,
+                                                   const uint32_t* __restrict__ list,
+                                                   uint32_t cnt) {
+    __shared__ uint64_t s_start[VT];
+    __shared__ uint32_t s_pre[VT];
+    __shared__ uint32_t s_slot[VT];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    uint64_t* sst = s_start + wid * WAVE;
+    uint32_t* spre = s_pre + wid * WAVE;
+    uint32_t* sslot = s_slot + wid * WAVE;
+    const uint32_t wave = (blockIdx.x * VT + threadIdx.x) / WAVE;
+    const uint32_t nwaves = gridDim.x * VT / WAVE;
+    for (uint32_t i = wave; i < cnt; i += nwaves) {
+        const uint32_t v = list[i];
+        const uint32_t jb = g.off[v], je = g.off[v + 1];
+        double S = PULL ? g.pv0[v] : 0.0;
+        uint32_t c = 0;
+        for (uint32_t j0 = jb; j0 < je; j0 += WAVE) {
+            const uint32_t j = j0 + lane;
+            uint64_t st = 0;
+            uint32_t len = 0;
+            if (j < je) vseg(g, tb, v, j, st, len);
+            uint32_t inc = len;  // inclusive scan of segment lengths
+            for (int d = 1; d < WAVE; d <<= 1) {
+                uint32_t o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += o;
+            }
+            sst[lane] = st;
+            spre[lane] = inc - len;
+            sslot[lane] = j;
+            const uint32_t K = __shfl(inc, WAVE - 1, 64);
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t k0 = 0; k0 < K; k0 += WAVE) {
+                const uint32_t k = k0 + lane;
+                double val = 0.0;
+                if (k < K) {
+                    int lo = 0, hi = WAVE - 1;  // last segment with pre <= k
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (spre[mid] <= k) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint64_t kk = sst[lo] + (k - spre[lo]);
+                    val = tb.w[kk];
+                    if (PULL && !term_dead(g, tb, sslot[lo], kk)) val = 0.0;
+                }
+                const uint32_t nk = min((uint32_t)WAVE, K - k0);
+                if (PULL) {
+                    for (uint32_t b2 = 0; b2 < nk; ++b2) S -= readlane_f64(val, (int)b2);
+                } else {
+                    for (uint32_t b2 = 0; b2 < nk; ++b2) S += readlane_f64(val, (int)b2);
+                }
+            }
+            c += K;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            g.pv[v] = S;
+            if (!PULL) {
+                g.pv0[v] = S;
+                g.vtcnt[v] = c;
+            }
+        }
+    }
+}
+
+// Per edge: capacity of its term list, min(d_x, d_y) - 1.
+__global__ void __launch_bounds__(T) k_term_caps(GraphDev g, uint32_t* __restrict__ cap) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint32_t x = g.esrc[e], y = g.nb[g.eslot[e]];
+        cap[e] = min(g.deg[x], g.deg[y]) - 1;
+    }
 }
 
 template <int G>
@@ -218,40 +391,126 @@ unsigned grid_for(uint32_t items, int G) {
 
 }  // namespace
 
+// Work buckets and the term buffer depend only on the graph: built on the
+// first count and kept with the handle.
+rtrex_status prepare_count(rtrex_graph h) {
+    if (h->cnt_lists) return RTREX_OK;
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    RT_TRY(dalloc(h, &h->cnt_lists, (uint64_t)NB * (g.m + g.n) + 2 * NB));
+    uint32_t* elist = h->cnt_lists;
+    uint32_t* vlist = elist + (uint64_t)NB * g.m;
+    uint32_t* cur = vlist + (uint64_t)NB * g.n;
+    RT_HIP(hipMemsetAsync(cur, 0, 2 * NB * sizeof(uint32_t), s));
+    if (g.m) k_bucket<true><<<blocks_for(g.m, T), T, 0, s>>>(g, g.m, elist, cur);
+    if (g.n) k_bucket<false><<<blocks_for(g.n, T), T, 0, s>>>(g, g.n, vlist, cur + NB);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipMemcpyAsync(h->cnt_counts, cur, sizeof(h->cnt_counts), hipMemcpyDeviceToHost, s));
+    // term buffer: capacity min(d_x, d_y) - 1 per edge
+    RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));
+    RT_TRY(dalloc(h, &h->za, g.m));
+    RT_TRY(dalloc(h, &h->zb, g.m));
+    {
+        Scratch sc;
+        uint32_t* cap;
+        RT_TRY(sc.get(&cap, g.m));
+        if (g.m) k_term_caps<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, cap);
+        RT_TRY(exclusive_scan_u32(cap, h->zoff, g.m, s));
+    }
+    uint64_t total = 0;
+    RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    size_t freeb = 0, totalb = 0;
+    RT_HIP(hipMemGetInfo(&freeb, &totalb));
+    h->zw = nullptr;
+    if ((total + 1) * 16 < freeb / 3) {  // f64 weight + two u32 slots per term
+        void *q = nullptr, *qa = nullptr, *qb = nullptr;
+        if (hipMalloc(&q, (total + 1) * 8) == hipSuccess &&
+            hipMalloc(&qa, (total + 1) * 4) == hipSuccess &&
+            hipMalloc(&qb, (total + 1) * 4) == hipSuccess) {
+            h->allocs.push_back(q);
+            h->allocs.push_back(qa);
+            h->allocs.push_back(qb);
+            h->zw = static_cast<double*>(q);
+            h->zsa = static_cast<uint32_t*>(qa);
+            h->zsb = static_cast<uint32_t*>(qb);
+        } else {
+            (void)hipGetLastError();
+            for (void* r : {q, qa, qb})
+                if (r) (void)hipFree(r);
+        }
+    }
+    return RTREX_OK;
+}
+
+// Launches the per-vertex folds over the stored triangle lists: the count
+// fold (pull = false) or the post-clean pull (pull = true).
+rtrex_status vertex_folds(rtrex_graph h, bool pull) {
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    const TermBuf tb = h->terms();
+    const uint32_t* vlist = h->cnt_lists + (uint64_t)NB * g.m;
+    const uint32_t* vc = h->cnt_counts + NB;
+    for (int b = 0; b < 4; ++b) {
+        if (!vc[b]) continue;
+        const uint32_t* lst = vlist + (uint64_t)b * g.n;
+        if (b < 2) {
+            const unsigned grid = blocks_for(vc[b], T, 8192);
+            if (pull) k_vfold_lane<true><<<grid, T, 0, s>>>(g, tb, lst, vc[b]);
+            else k_vfold_lane<false><<<grid, T, 0, s>>>(g, tb, lst, vc[b]);
+        } else {
+            const unsigned grid =
+                (unsigned)std::min<uint64_t>(((uint64_t)vc[b] * WAVE + VT - 1) / VT, 8192);
+            if (pull) k_vfold_wave<true><<<grid, VT, 0, s>>>(g, tb, lst, vc[b]);
+            else k_vfold_wave<false><<<grid, VT, 0, s>>>(g, tb, lst, vc[b]);
+        }
+    }
+    RT_HIP(hipGetLastError());
+    return RTREX_OK;
+}
+
+rtrex_status pull_vertex_supports(rtrex_graph h) {
+    if (!h->zw) return RTREX_OK;  // the intersection peel keeps them itself
+    return vertex_folds(h, true);
+}
+
 rtrex_status run_count(rtrex_graph h) {
     if (h->phase < PH_BUILT) return fail(RTREX_ESTATE, "graph not built");
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
-    Scratch sc;
-    uint32_t *elist, *vlist, *cur;
-    RT_TRY(sc.get(&elist, (uint64_t)NB * g.m));
-    RT_TRY(sc.get(&vlist, (uint64_t)NB * g.n));
-    RT_TRY(sc.get(&cur, 2 * NB));
-    RT_HIP(hipMemsetAsync(cur, 0, 2 * NB * sizeof(uint32_t), s));
+    RT_TRY(prepare_count(h));
+    const uint32_t* counts = h->cnt_counts;
+    uint32_t* elist = h->cnt_lists;
+    uint32_t* vlist = elist + (uint64_t)NB * g.m;
     RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
-    if (g.m) k_bucket<true><<<blocks_for(g.m, T), T, 0, s>>>(g, g.m, elist, cur);
-    if (g.n) k_bucket<false><<<blocks_for(g.n, T), T, 0, s>>>(g, g.n, vlist, cur + NB);
-    RT_HIP(hipGetLastError());
-    uint32_t counts[2 * NB];
-    RT_HIP(hipMemcpyAsync(counts, cur, sizeof(counts), hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
+    TermBuf tb{h->zw, h->zsa, h->zsb, h->zoff, h->za, h->zb};
 
     RT_HIP(hipEventRecord(h->ev[2], s));
     unsigned long long* tri3 = &h->ctr->tri_total;
     auto ep = [&](int b) { return elist + (uint64_t)b * g.m; };
-    if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tri3);
-    if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tri3);
-    if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tri3);
-    if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tri3);
+    if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tri3, tb);
+    if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tri3, tb);
+    if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tri3, tb);
+    if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tri3, tb);
     RT_HIP(hipEventRecord(h->ev[3], s));
     auto vp = [&](int b) { return vlist + (uint64_t)b * g.n; };
     const uint32_t* vc = counts + NB;
-    if (vc[0]) k_count_vertices<1><<<grid_for(vc[0], 1), T, 0, s>>>(g, vp(0), vc[0]);
-    if (vc[1]) k_count_vertices<4><<<grid_for(vc[1], 4), T, 0, s>>>(g, vp(1), vc[1]);
-    if (vc[2]) k_count_vertices<16><<<grid_for(vc[2], 16), T, 0, s>>>(g, vp(2), vc[2]);
-    if (vc[3]) k_count_vertices<64><<<grid_for(vc[3], 64), T, 0, s>>>(g, vp(3), vc[3]);
+    if (tb.w) {
+        RT_TRY(vertex_folds(h, false));
+    } else {
+        if (vc[0]) k_count_vertices<1><<<grid_for(vc[0], 1), T, 0, s>>>(g, vp(0), vc[0]);
+        if (vc[1]) k_count_vertices<4><<<grid_for(vc[1], 4), T, 0, s>>>(g, vp(1), vc[1]);
+        if (vc[2]) k_count_vertices<16><<<grid_for(vc[2], 16), T, 0, s>>>(g, vp(2), vc[2]);
+        if (vc[3]) k_count_vertices<64><<<grid_for(vc[3], 64), T, 0, s>>>(g, vp(3), vc[3]);
+    }
     RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[4], s));
+    // Fresh clean state for the supports just computed.
+    RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, g.nnz, s));
+    RT_HIP(hipMemsetAsync(g.est, ST_ALIVE, g.m, s));
+    RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
+    RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
+    RT_HIP(hipMemsetAsync(&h->ctr->tri_removed, 0, 5 * sizeof(unsigned long long), s));
     RT_HIP(hipEventSynchronize(h->ev[4]));
     float a = 0, b = 0;
     RT_HIP(hipEventElapsedTime(&a, h->ev[2], h->ev[3]));
@@ -259,13 +518,9 @@ rtrex_status run_count(rtrex_graph h) {
     h->tm.count_edge_ms = a;
     h->tm.count_vertex_ms = b;
     h->tm.count_ms = a + b;
-    // Fresh clean state for the supports just computed.
-    RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, g.nnz, s));
-    RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
-    RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
-    RT_HIP(hipMemsetAsync(&h->ctr->tri_removed, 0, 5 * sizeof(unsigned long long), s));
     RT_HIP(hipStreamSynchronize(s));
     h->audit = AuditHost();
+    h->n_alive = g.m;
     h->phase = PH_COUNTED;
     return RTREX_OK;
 }

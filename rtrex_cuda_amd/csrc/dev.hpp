@@ -54,4 +54,18 @@ __device__ __forceinline__ uint64_t ballot64(bool p) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Slot for one item per calling lane in a list whose length is *ctr: one
+// atomic per wave (the lowest active lane reserves for all active lanes).
+// Callable from divergent code; every active lane must call it.
+__device__ __forceinline__ uint32_t wave_append(unsigned* ctr) {
+    const uint64_t act = __ballot(1);
+    const int lane = lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(act));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(act & below);
+}
+
 }  // namespace rtrex

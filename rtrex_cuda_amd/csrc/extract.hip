@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(T) k_comp_keys(uint32_t n,
         if (csize[r] < 2) {
             cseed[v] = v;
         } else {
-            keys[atomicAdd(nkeys, 1u)] = ((uint64_t)r << 32) | rank[v];
+            keys[wave_append(nkeys)] = ((uint64_t)r << 32) | rank[v];
         }
     }
 }
@@ -166,6 +166,34 @@ __global__ void __launch_bounds__(T) k_comp_starts(const uint32_t* __restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) cstart[fpos[nk]] = nk;
 }
 
+// Ascending sort of up to SB_MAX u64 keys by one workgroup (bitonic in LDS);
+// small component layouts skip the multi-pass radix sort.
+constexpr int SB_T = 1024;
+constexpr uint32_t SB_MAX = 8192;
+__global__ void __launch_bounds__(SB_T) k_sort_small(uint64_t* keys, uint32_t n) {
+    __shared__ uint64_t a[SB_MAX];
+    uint32_t np = 1;
+    while (np < n) np <<= 1;
+    for (uint32_t i = threadIdx.x; i < np; i += SB_T) a[i] = i < n ? keys[i] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < np; i += SB_T) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t ai = a[i], al = a[l];
+                    if ((ai > al) == ((i & k) == 0)) {
+                        a[i] = al;
+                        a[l] = ai;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += SB_T) keys[i] = a[i];
+}
+
 // ---------------------------------------------------------------- extraction
 struct XArgs {
     GraphDev g;
@@ -183,6 +211,7 @@ struct XArgs {
     Rec* grec;               // [start, ...): candidate records (large case)
     double* gpv;             // [start, ...): prefetched supports (large case)
     uint32_t* wl;            // [dpre[start], ...): deletion worklist (LIFO)
+    TermBuf tb;              // stored triangle lists (tb.w null: intersect)
     Counters* ctr;
 };
 
@@ -190,13 +219,13 @@ struct XArgs {
 // thr); `third` is the third vertex of f's triangle with the edge being
 // deleted.  Certain outside the cheap bound, otherwise certified against every
 // subtraction order (audit.hpp).
-__device__ bool decide(const GraphDev& g, uint32_t f, double v, double thr,
-                       uint32_t third, unsigned long long& amb) {
+__device__ bool decide(const GraphDev& g, const TermBuf& tb, uint32_t f, double v,
+                       double thr, uint32_t third, uint32_t deleting,
+                       unsigned long long& amb) {
     const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
     if (v < thr - cb) return true;
     if (v >= thr + cb) return false;
-    const uint32_t sl = g.eslot[f];
-    int c = certify(g, g.esrc[f], g.nb[sl], g.sup0[f], thr, false, third);
+    int c = certify_edge(g, tb, f, thr, false, third, deleting);
     if (c == CERT_UNDECIDED) {
         amb++;
         return v < thr;
@@ -363,7 +392,7 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             for (uint32_t i = lane; i < nc; i += XT) {
                 const uint32_t x = rec[i].vertex;
                 pvv[i] = g.pv[x];
-                if (small) bdd[i] = cheap_bound(g.vtcnt[x], g.pv0[x]);
+                if (small) bdd[i] = cheap_bound(2 * g.vtcnt[x], g.pv0[x]);
             }
             __syncthreads();
             if (lane == 0) {
@@ -377,7 +406,7 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     num += rec[i].wgt;
                     den += pvv[i];
                     const double bi = small ? bdd[i]
-                                            : cheap_bound(g.vtcnt[rec[i].vertex],
+                                            : cheap_bound(2 * g.vtcnt[rec[i].vertex],
                                                            g.pv0[rec[i].vertex]);
                     eden += bi + 0x1.0p-52 * fabs(den);
                     const double D = D0 + den;
@@ -411,9 +440,9 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     const bool al = j < ze && g.st[j] == ST_ALIVE;
                     const uint64_t bal = ballot64(al);
                     if (al) {
-                        A.wl[ebase + top + __popcll(bal & lanes_below())] = g.s2e[j];
-                        g.st[j] = ST_QUEUED;
-                        g.st[g.part[j]] = ST_QUEUED;
+                        const uint32_t f = g.s2e[j];
+                        A.wl[ebase + top + __popcll(bal & lanes_below())] = f;
+                        set_status(g, f, ST_QUEUED);
                     }
                     top += (uint32_t)__popcll(bal);
                 }
@@ -428,52 +457,80 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 const uint32_t x = g.esrc[e], y = g.nb[sl];
                 const uint32_t dx = g.deg[x], dy = g.deg[y];
                 const uint32_t lo = dx < dy ? x : y, hi = dx < dy ? y : x;
-                const uint32_t dlo = dx < dy ? dx : dy, dhi = dx < dy ? dy : dx;
-                uint32_t p = g.off[lo];
-                const uint32_t le = g.off[lo + 1];
+                // Triangles of e: from its stored list (ascending third
+                // vertex, the order the reference visits them), or by
+                // intersecting N(lo) with N(hi).  Either way, per chunk of 64:
+                // slot ja of (lo, w) and jb of (hi, w), both not deleted.
+                const bool terms = A.tb.w != nullptr;
+                const uint32_t tlo = terms ? (dx <= dy ? x : y) : lo;
+                const uint32_t thi = terms ? (dx <= dy ? y : x) : hi;
+                const uint64_t c0 = terms ? A.tb.zoff[e] : g.off[lo];
+                const uint64_t c1 = terms ? c0 + g.tcnt[e] : g.off[lo + 1];
                 uint32_t hlo = g.off[hi];
                 const uint32_t he = g.off[hi + 1];
-                for (; p < le; p += XT) {
-                    const uint32_t j = p + lane;
-                    const bool in = j < le;
-                    const uint8_t sj = in ? g.st[j] : ST_DEAD;
-                    const uint32_t w = in ? g.nb[j] : 0;
-                    const uint32_t pos = in ? lower_bound(g.nb, hlo, he, w) : hlo;
-                    const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
-                    bool hit = in && sj != ST_DEAD && pos < he && g.nb[pos] == w;
-                    const uint8_t sh = hit ? g.st[pos] : ST_DEAD;
-                    hit = hit && sh != ST_DEAD;
-                    const uint32_t dw = hit ? g.deg[w] : 1;
-                    const double t = hit ? tri_w(dx, dy, dw) : 0.0;
+                for (uint64_t p = c0; p < c1; p += XT) {
+                    const uint64_t kk = p + lane;
+                    const bool in = kk < c1;
+                    uint32_t fj = 0, fh = 0, w = 0;
+                    bool hit = false;
+                    double t = 0.0;
+                    if (terms) {
+                        if (in) {
+                            fj = A.tb.sa[kk];
+                            fh = A.tb.sb[kk];
+                            hit = g.est[fj] != ST_DEAD && g.est[fh] != ST_DEAD;
+                            if (hit) {
+                                const uint32_t a0 = g.esrc[fj];  // third vertex
+                                w = a0 == tlo ? g.nb[g.eslot[fj]] : a0;
+                                t = A.tb.w[kk];
+                            }
+                        }
+                    } else {
+                        const uint32_t ja = (uint32_t)kk;
+                        w = in ? g.nb[ja] : 0;
+                        const uint32_t jb = in ? lower_bound(g.nb, hlo, he, w) : hlo;
+                        const uint32_t nh = __shfl(jb, (int)min<uint64_t>(63, c1 - p - 1), 64);
+                        hit = in && g.st[ja] != ST_DEAD && jb < he && g.nb[jb] == w &&
+                              g.st[jb] != ST_DEAD;
+                        if (hit) {
+                            fj = g.s2e[ja];
+                            fh = g.s2e[jb];
+                            t = tri_w(dx, dy, g.deg[w]);
+                        }
+                        hlo = nh;
+                    }
+                    const uint8_t sj = hit ? g.est[fj] : ST_DEAD;
+                    const uint8_t sh = hit ? g.est[fh] : ST_DEAD;
                     uint64_t hb = ballot64(hit);
                     n_tri += (lane == 0) ? (unsigned long long)__popcll(hb) : 0ull;
-                    while (hb) {  // the popped edge's own accumulators, in w order
+                    while (hb) {  // the popped edge's endpoints, in w order
                         const int bl = __ffsll((unsigned long long)hb) - 1;
                         const double tv = __shfl(t, bl, 64);
                         if (lane == 0) {
-                            g.sup[e] -= tv;
                             g.pv[x] -= tv;
                             g.pv[y] -= tv;
                         }
                         hb &= hb - 1;
                     }
                     bool push1 = false, push2 = false;
-                    uint32_t fj = 0, fh = 0;
                     if (hit) {
-                        fj = g.s2e[j];
-                        fh = g.s2e[pos];
-                        const double v1 = g.sup[fj] - t;
-                        const double v2 = g.sup[fh] - t;
-                        g.sup[fj] = v1;
-                        g.sup[fh] = v2;
                         g.pv[w] -= t;
-                        g.touched[fj] = 1;
-                        g.touched[fh] = 1;
                         // (lo, w) then (hi, w), as Decomposition.h:238-264.
-                        if (sj == ST_ALIVE)
-                            push1 = decide(g, fj, v1, edge_thr(A.eps, dlo, dw), hi, amb_c);
-                        if (sh == ST_ALIVE)
-                            push2 = decide(g, fh, v2, edge_thr(A.eps, dhi, dw), lo, amb_c);
+                        // Queued edges die anyway; their supports are not kept.
+                        if (sj == ST_ALIVE) {
+                            const double v1 = g.sup[fj] - t;
+                            g.sup[fj] = v1;
+                            g.touched[fj] = 1;
+                            push1 = decide(g, A.tb, fj, v1, A.eps * g.ew[fj],
+                                           terms ? thi : hi, e, amb_c);
+                        }
+                        if (sh == ST_ALIVE) {
+                            const double v2 = g.sup[fh] - t;
+                            g.sup[fh] = v2;
+                            g.touched[fh] = 1;
+                            push2 = decide(g, A.tb, fh, v2, A.eps * g.ew[fh],
+                                           terms ? tlo : lo, e, amb_c);
+                        }
                     }
                     const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
                     // exclusive prefix of pushes over lanes
@@ -489,21 +546,18 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     const uint32_t tot = __shfl(pre + np2, 63, 64);
                     if (push1) {
                         A.wl[ebase + top + pre] = fj;
-                        g.st[j] = ST_QUEUED;
-                        g.st[g.part[j]] = ST_QUEUED;
+                        set_status(g, fj, ST_QUEUED);
                     }
                     if (push2) {
                         A.wl[ebase + top + pre + (uint32_t)push1] = fh;
-                        g.st[pos] = ST_QUEUED;
-                        g.st[g.part[pos]] = ST_QUEUED;
+                        set_status(g, fh, ST_QUEUED);
                     }
                     top += tot;
-                    hlo = nh;
                     __syncthreads();
                 }
                 if (lane == 0) {
-                    g.st[sl] = ST_DEAD;
-                    g.st[g.part[sl]] = ST_DEAD;
+                    set_status(g, e, ST_DEAD);
+                    g.sup[e] = 0.0;
                 }
                 n_edges++;
                 __syncthreads();
@@ -599,16 +653,14 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     RT_HIP(hipEventRecord(h->ev[5], s));
     Scratch sc;
     uint64_t *keys, *tmp;
-    uint32_t *order, *rank, *parent, *csize, *vseg = nullptr;
+    uint32_t *parent, *csize, *vseg = nullptr;
     RT_TRY(sc.get(&keys, n));
     RT_TRY(sc.get(&tmp, n));
-    RT_TRY(sc.get(&order, n));
-    RT_TRY(sc.get(&rank, n));
     RT_TRY(sc.get(&parent, n));
     RT_TRY(sc.get(&csize, n));
     const int Bi = bits_for((uint64_t)n + 1);
     int Bs = 0;
-    if (!h->seg_vtx.empty()) {
+    if (!h->seg_vtx.empty() && !h->seed_order) {
         const size_t ns = h->seg_vtx.size() - 1;
         Bs = bits_for(ns + 1);
         std::vector<uint32_t> seg(n);
@@ -620,11 +672,18 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     }
     if (Bs + 2 * Bi > 64) return fail(RTREX_EINVAL, "too many vertices/segments");
     const unsigned bn = blocks_for(n, T, 8192);
-    // seed order: (segment, degree, id) -- pairCompareSecond (Graph.cpp:40-45)
-    k_seed_keys<<<bn, T, 0, s>>>(g, vseg, Bi, Bi, keys);
-    RT_HIP(hipGetLastError());
-    RT_TRY(radix_sort_u64(keys, tmp, n, Bs + 2 * Bi, s));
-    k_ranks<<<bn, T, 0, s>>>(keys, n, Bi, order, rank);
+    // seed order: (segment, degree, id) -- pairCompareSecond (Graph.cpp:40-45);
+    // static for the graph, so computed once per handle.
+    if (!h->seed_order) {
+        RT_TRY(dalloc(h, &h->seed_order, n));
+        RT_TRY(dalloc(h, &h->seed_rank, n));
+        k_seed_keys<<<bn, T, 0, s>>>(g, vseg, Bi, Bi, keys);
+        RT_HIP(hipGetLastError());
+        RT_TRY(radix_sort_u64(keys, tmp, n, Bs + 2 * Bi, s));
+        k_ranks<<<bn, T, 0, s>>>(keys, n, Bi, h->seed_order, h->seed_rank);
+    }
+    uint32_t* order = h->seed_order;
+    uint32_t* rank = h->seed_rank;
     // alive components
     k_uf_init<<<bn, T, 0, s>>>(n, parent, csize, h->cseed);
     k_uf_hook<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, parent);
@@ -639,7 +698,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     h->audit.components = 0;
     h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
     if (nk) {
-        RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
+        if (nk <= SB_MAX) k_sort_small<<<1, SB_T, 0, s>>>(keys, nk);
+        else RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
         uint32_t *cv, *vdeg, *flag, *fpos, *cstart;
         uint64_t* dpre;
         RT_TRY(sc.get(&cv, nk));
@@ -699,6 +759,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.grec = grec;
         a.gpv = gpv;
         a.wl = wl;
+        a.tb = h->terms();
         a.ctr = h->ctr;
         const unsigned grid = std::min<uint32_t>(ncomp, 8192);
         k_extract<<<grid, XT, 0, s>>>(a);

@@ -173,6 +173,7 @@ rtrex_status alloc_state(rtrex_graph h, uint32_t n, uint64_t nnz) {
     RT_TRY(dalloc(h, &g.esrc, m));
     RT_TRY(dalloc(h, &g.eslot, m));
     RT_TRY(dalloc(h, &g.st, nnz));
+    RT_TRY(dalloc(h, &g.est, m));
     RT_TRY(dalloc(h, &g.sup, m));
     RT_TRY(dalloc(h, &g.sup0, m));
     RT_TRY(dalloc(h, &g.tcnt, m));
@@ -181,6 +182,7 @@ rtrex_status alloc_state(rtrex_graph h, uint32_t n, uint64_t nnz) {
     RT_TRY(dalloc(h, &g.vtcnt, n));
     RT_TRY(dalloc(h, &g.touched, m));
     RT_TRY(dalloc(h, &g.claim, m));
+    RT_TRY(dalloc(h, &g.ew, m));
     RT_TRY(dalloc(h, &h->cseed, n));
     RT_TRY(dalloc(h, &h->ctr, 1));
     RT_HIP(hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream));
@@ -217,6 +219,7 @@ rtrex_status finish(rtrex_graph h, uint32_t* src) {
     k_edge_ids_mirror<<<bs, T, 0, s>>>(g.part, canon, rank, nnz, g.s2e);
     RT_HIP(hipGetLastError());
     RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, nnz, s));
+    RT_HIP(hipMemsetAsync(g.est, ST_ALIVE, g.m, s));
     RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
     RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
     RT_HIP(hipStreamSynchronize(s));

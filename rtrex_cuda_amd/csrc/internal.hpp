@@ -10,11 +10,13 @@
 //                   endpoint's list), so edge order = lexicographic (x<y)
 //   esrc[m], eslot[m]  u32  lower-id endpoint and canonical slot per edge
 //   st[nnz]    u8   per-slot status, both copies always equal
+//   est[m]     u8   the same status per undirected edge
 //   sup[m]     f64  current support of each undirected edge
 //   sup0[m]    f64  support after the count phase (audit bound)
 //   tcnt[m]    u32  triangles on the edge after the count phase
 //   pv[n]      f64  current per-vertex support; pv0[n] after count
 //   touched[m] u8   edge lost at least one triangle since the count phase
+//   ew[m]      f64  1/(d_x d_y) of each edge (threshold = eps * ew)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,6 +42,7 @@ struct GraphDev {
     uint32_t* esrc = nullptr;
     uint32_t* eslot = nullptr;
     uint8_t* st = nullptr;
+    uint8_t* est = nullptr;  // per-edge copy of the status (coalesced scans)
     double* sup = nullptr;
     double* sup0 = nullptr;
     uint32_t* tcnt = nullptr;
@@ -48,6 +51,7 @@ struct GraphDev {
     uint32_t* vtcnt = nullptr;
     uint8_t* touched = nullptr;
     uint32_t* claim = nullptr;
+    double* ew = nullptr;  // per edge: 1 / (d_x d_y), the reference's edgeWt
     uint32_t n = 0, m = 0;
     uint64_t nnz = 0;
 };
@@ -73,6 +77,20 @@ enum : uint32_t {
     ERR_RANGE = 4u,
     ERR_ASYMMETRIC = 8u,
     ERR_SCRATCH = 16u,
+};
+
+// Triangle list of every edge, written by the count phase: for edge e with
+// lower-degree endpoint lo (ties: the lower id) and the other endpoint hi,
+// terms [zoff[e], zoff[e] + tcnt[e]) hold, ascending in the third vertex z,
+// the weight 1/(d_x d_y d_z) and the edge ids of {lo, z} (sa) and {hi, z} (sb).
+// za/zb count the terms with z below the lower/higher id endpoint.
+struct TermBuf {
+    double* w;  // null when the buffer did not fit (intersection fallbacks)
+    uint32_t* sa;
+    uint32_t* sb;
+    const uint64_t* zoff;
+    uint32_t* za;
+    uint32_t* zb;
 };
 
 struct Timings {
@@ -109,13 +127,33 @@ struct rtrex_graph_s {
     // Batch layout (one graph = one segment): vertex segment boundaries.
     std::vector<int64_t> seg_vtx;  // empty for a single graph
     uint32_t* nearst = nullptr;  // per edge: last peel round it was near-threshold
+    uint32_t* surv = nullptr;    // packed survivors of the last initial clean
+    uint64_t n_alive = 0;        // alive edges (host bookkeeping of the peel)
+    // Count-phase plan (static per graph): work-bucket lists, their sizes and
+    // the term buffer (per-edge matched weights, ascending z).
+    uint32_t* cnt_lists = nullptr;
+    uint32_t *seed_order = nullptr, *seed_rank = nullptr;  // (segment, degree, id)
+    uint32_t cnt_counts[8] = {};
+    uint64_t* zoff = nullptr;
+    uint32_t *za = nullptr, *zb = nullptr;
+    double* zw = nullptr;
+    uint32_t *zsa = nullptr, *zsb = nullptr;
     uint32_t round_stamp = 0;
     rtrex::Timings tm;
     rtrex::AuditHost audit;
+    rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, za, zb}; }
     hipEvent_t ev[8] = {};
 };
 
 namespace rtrex {
+
+// Sets the status of undirected edge e in its per-edge byte and both slots.
+__device__ __forceinline__ void set_status(const GraphDev& g, uint32_t e, uint8_t c) {
+    g.est[e] = c;
+    const uint32_t sl = g.eslot[e];
+    g.st[sl] = c;
+    g.st[g.part[sl]] = c;
+}
 
 // ---- error reporting (api.cpp) -------------------------------------------
 void set_error(const std::string& msg);
@@ -197,6 +235,9 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
 rtrex_status run_extract(rtrex_graph h, double eps);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
+// After a clean over stored triangle lists: per-vertex supports = count-phase
+// value minus dead triangles, in the count fold's order (count.hip).
+rtrex_status pull_vertex_supports(rtrex_graph h);
 
 inline unsigned blocks_for(uint64_t work, unsigned per_block,
                            unsigned cap = 1u << 20) {

@@ -1,0 +1,190 @@
+"""Pins the CPU oracle (oracle/rtrex_oracle.cpp).
+
+The oracle is the checker of every GPU parity test, so it is itself pinned
+here, on the CPU, by:
+  * the reference's own outputs (tests/golden, written by make_golden.py from
+    the reference sources compiled in the build container): CSR, triangle
+    totals, FP64 supports bit for bit, survivors, cluster ids and stats;
+  * brute force (trace(A^3)/6 triangle counts) and the K_n closed form;
+  * the order-independent fixpoint that defines the clean phase, checked in
+    exact rational arithmetic;
+  * planted mistakes in a Python copy of the fold order, which the golden
+    hashes must reject (so the pins can tell orders apart).
+"""
+from __future__ import annotations
+
+import hashlib
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import (GOLDEN_TAGS, golden_graph, golden_meta, golden_result,
+                      survivors_of)
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _csr(name):
+    n, u, v = golden_graph(name)
+    return O.csr_from_undirected(n, u, v)
+
+
+@pytest.mark.parametrize("tag", GOLDEN_TAGS)
+def test_oracle_matches_reference_outputs(tag):
+    meta = golden_meta()[tag]
+    ref = golden_result(tag)
+    name = tag.rsplit("_eps", 1)[0]
+    off, nb = _csr(name)
+    assert sha(np.concatenate([off, nb])) == meta["csr_sha"]  # makeCSR
+    cnt = O.count(off, nb)
+    assert cnt.total == meta["count_total"]
+    assert sha(cnt.perEdge) == meta["count_perEdge_sha"]
+    assert sha(cnt.perVertex) == meta["count_perVertex_sha"]
+    cl = O.initial_clean(off, nb, meta["eps"], cnt)
+    assert cl.total == meta["clean_total"]
+    assert sha(cl.perEdge) == meta["clean_perEdge_sha"]
+    assert sha(cl.perVertex) == meta["clean_perVertex_sha"]
+    assert np.array_equal(survivors_of(off, nb, cl.status), ref["survivors"])
+    d = O.decompose(off, nb, meta["eps"])
+    assert d.n_clusters == meta["n_clusters"]
+    assert np.array_equal(d.cluster_map, ref["cluster_map"])
+    stats = np.array([[c[0], c[1], c[2]] for c in d.clusters], np.int64).reshape(-1, 3)
+    assert np.array_equal(stats, ref["stats"])
+
+
+def _random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    a = np.triu(rng.random((n, n)) < p, 1)
+    u, v = np.nonzero(a)
+    return n, u.astype(np.uint32), v.astype(np.uint32)
+
+
+@pytest.mark.parametrize("n,p,seed", [(30, 0.3, 1), (60, 0.15, 2), (45, 0.5, 3), (80, 0.08, 4)])
+def test_triangle_total_brute_force(n, p, seed):
+    n, u, v = _random_graph(n, p, seed)
+    a = np.zeros((n, n), np.int64)
+    a[u, v] = a[v, u] = 1
+    off, nb = O.csr_from_undirected(n, u, v)
+    assert O.count(off, nb).total == int(np.trace(a @ a @ a)) // 6
+
+
+@pytest.mark.parametrize("k", [4, 5, 7, 10, 13])
+def test_clique_closed_form(k):
+    iu = np.array([(a, b) for a in range(k) for b in range(a + 1, k)], np.uint32)
+    off, nb = O.csr_from_undirected(k, iu[:, 0], iu[:, 1])
+    c = O.count(off, nb)
+    t = 1.0 / float((k - 1) ** 3)
+    s = 0.0
+    for _ in range(k - 2):  # each edge: k-2 triangles of equal weight
+        s += t
+    assert np.all(c.perEdge == s)
+    s = 0.0
+    for _ in range((k - 1) * (k - 2) // 2):
+        s += t
+    assert np.all(c.perVertex == s)
+    assert c.total == k * (k - 1) * (k - 2) // 6
+
+
+def test_planted_fold_order_is_detected():
+    """A descending-z fold (a plausible mistake) must not reproduce the
+    reference's perEdge bits on clean_email."""
+    off, nb = _csr("clean_email")
+    meta = golden_meta()["clean_email_eps0.3"]
+    deg = np.diff(off)
+    pe = np.zeros(len(nb))
+    nbl = [nb[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+    for x in range(len(off) - 1):
+        for s in range(off[x], off[x + 1]):
+            y = nb[s]
+            if y < x:
+                continue
+            z = np.intersect1d(nbl[x], nbl[y])[::-1]  # wrong order
+            acc = 0.0
+            for w in z:
+                acc += 1.0 / float(int(deg[x]) * int(deg[y]) * int(deg[w]))
+            pe[s] = acc
+            pe[off[y] + np.searchsorted(nbl[y], x)] = acc
+    assert sha(pe) != meta["count_perEdge_sha"]
+    assert np.allclose(pe, O.count(off, nb).perEdge, rtol=1e-12, atol=0)
+
+
+def _exact_peel(off, nb, eps, status0):
+    """Least fixpoint of 'delete edges below threshold' in exact rational
+    arithmetic, starting from the reference's initial decisions."""
+    n = len(off) - 1
+    deg = np.diff(off)
+    nbs = [set(nb[off[i]:off[i + 1]].tolist()) for i in range(n)]
+    alive = {(x, y) for x in range(n) for y in nbs[x] if x < y}
+    src = np.repeat(np.arange(n), np.diff(off))
+    for i in range(len(nb)):
+        if status0[i] and src[i] < nb[i]:
+            alive.discard((int(src[i]), int(nb[i])))
+    thr = {e: Fraction(eps) * Fraction(1.0 / float(int(deg[e[0]]) * int(deg[e[1]])))
+           for e in alive}
+
+    def sup(e):
+        x, y = e
+        s = Fraction(0)
+        for z in nbs[x] & nbs[y]:
+            if (min(x, z), max(x, z)) in alive and (min(y, z), max(y, z)) in alive:
+                s += Fraction(1.0 / float(int(deg[x]) * int(deg[y]) * int(deg[z])))
+        return s
+
+    changed = True
+    while changed:
+        changed = False
+        for e in sorted(alive):
+            if sup(e) < thr[e]:
+                alive.discard(e)
+                changed = True
+    return alive
+
+
+@pytest.mark.parametrize("seed,eps", [(11, 0.2937), (12, 0.1713), (13, 0.4111)])
+def test_clean_is_the_exact_fixpoint(seed, eps):
+    n, u, v = _random_graph(40, 0.3, seed)
+    off, nb = O.csr_from_undirected(n, u, v)
+    cnt = O.count(off, nb)
+    deg = np.diff(off)
+    src = np.repeat(np.arange(n), np.diff(off))
+    thr = eps * (1.0 / (deg[src].astype(np.float64) * deg[nb]))
+    initial = cnt.perEdge < thr  # identical FP values to the reference's
+    cl = O.initial_clean(off, nb, eps, cnt)
+    exact = _exact_peel(off, nb, eps, initial)
+    got = {tuple(map(int, e)) for e in survivors_of(off, nb, cl.status)}
+    assert got == exact
+
+
+def test_small_test_known_clusters():
+    off, nb = _csr("small_test")
+    d = O.decompose(off, nb, 0.3)
+    sets = sorted(sorted(map(int, c[3])) for c in d.clusters)
+    assert [1, 3, 4, 6] in sets
+    assert sum(1 for s in sets if len(s) == 1) == 6
+
+
+def test_disjoint_cliques_one_cluster_each():
+    edges = []
+    base = 0
+    for k in (4, 5, 6, 7):
+        edges += [(base + a, base + b) for a in range(k) for b in range(a + 1, k)]
+        base += k
+    e = np.array(edges, np.uint32)
+    off, nb = O.csr_from_undirected(base, e[:, 0], e[:, 1])
+    d = O.decompose(off, nb, 0.3)
+    sizes = sorted(c[0] for c in d.clusters)
+    assert sizes == [4, 5, 6, 7]
+
+
+def test_delete_and_clean_removes_given_edge():
+    off, nb = _csr("clique_k6")
+    cnt = O.count(off, nb)
+    cl = O.initial_clean(off, nb, 0.3, cnt)
+    st = O.delete_and_clean(off, nb, 0.3, np.array([[0, 1]]), cl)
+    s = off[0] + np.searchsorted(nb[off[0]:off[1]], 1)
+    assert st.status[s] == ord("N")
+    assert st.total == cl.total - 4  # K6: edge {0,1} lies in 4 triangles
