@@ -266,6 +266,7 @@ rtrex_status rtrex_get_supports(rtrex_graph g, double* perEdge,
                                 double* perVertex, int64_t* total) {
     RT_TRY(check(g));
     if (g->phase < PH_COUNTED) return fail(RTREX_ESTATE, "count has not run");
+    if (perVertex) RT_TRY(ensure_vertex_supports(g, nullptr, 0));
     hipStream_t s = g->stream;
     if (perEdge && g->g.nnz) {
         std::vector<double> sup(g->g.m);

@@ -504,7 +504,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     }
     k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr);
     RT_HIP(hipGetLastError());
-    RT_TRY(pull_vertex_supports(h));
+    h->pv_full = false;  // vertex supports are pulled again on demand
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }

@@ -391,6 +391,33 @@ unsigned grid_for(uint32_t items, int G) {
 
 }  // namespace
 
+// Term buffer of `total` records (f64 weight + two u32 edge ids) when it
+// fits in a third of the free HBM.
+bool try_alloc_terms(rtrex_graph h, uint64_t total) {
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if ((total + 1) * 16 >= freeb / 3) return false;
+    void *q = nullptr, *qa = nullptr, *qb = nullptr;
+    if (hipMalloc(&q, (total + 1) * 8) == hipSuccess &&
+        hipMalloc(&qa, (total + 1) * 4) == hipSuccess &&
+        hipMalloc(&qb, (total + 1) * 4) == hipSuccess) {
+        h->allocs.push_back(q);
+        h->allocs.push_back(qa);
+        h->allocs.push_back(qb);
+        h->zw = static_cast<double*>(q);
+        h->zsa = static_cast<uint32_t*>(qa);
+        h->zsb = static_cast<uint32_t*>(qb);
+        return true;
+    }
+    (void)hipGetLastError();
+    for (void* r : {q, qa, qb})
+        if (r) (void)hipFree(r);
+    return false;
+}
+
 // Work buckets and the term buffer depend only on the graph: built on the
 // first count and kept with the handle.
 rtrex_status prepare_count(rtrex_graph h) {
@@ -420,26 +447,11 @@ rtrex_status prepare_count(rtrex_graph h) {
     uint64_t total = 0;
     RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    size_t freeb = 0, totalb = 0;
-    RT_HIP(hipMemGetInfo(&freeb, &totalb));
     h->zw = nullptr;
-    if ((total + 1) * 16 < freeb / 3) {  // f64 weight + two u32 slots per term
-        void *q = nullptr, *qa = nullptr, *qb = nullptr;
-        if (hipMalloc(&q, (total + 1) * 8) == hipSuccess &&
-            hipMalloc(&qa, (total + 1) * 4) == hipSuccess &&
-            hipMalloc(&qb, (total + 1) * 4) == hipSuccess) {
-            h->allocs.push_back(q);
-            h->allocs.push_back(qa);
-            h->allocs.push_back(qb);
-            h->zw = static_cast<double*>(q);
-            h->zsa = static_cast<uint32_t*>(qa);
-            h->zsb = static_cast<uint32_t*>(qb);
-        } else {
-            (void)hipGetLastError();
-            for (void* r : {q, qa, qb})
-                if (r) (void)hipFree(r);
-        }
-    }
+    h->terms_exact_pending = false;
+    if (try_alloc_terms(h, total)) return RTREX_OK;
+    // The upper bound does not fit: size exactly after a counting pass.
+    h->terms_exact_pending = true;
     return RTREX_OK;
 }
 
@@ -469,9 +481,29 @@ rtrex_status vertex_folds(rtrex_graph h, bool pull) {
     return RTREX_OK;
 }
 
-rtrex_status pull_vertex_supports(rtrex_graph h) {
-    if (!h->zw) return RTREX_OK;  // the intersection peel keeps them itself
-    return vertex_folds(h, true);
+// Per-vertex supports over stored triangle lists are computed on demand: the
+// reference only reads them for extraction candidates (Decomposition.h:856),
+// so extraction asks for its component vertices and rtrex_get_supports for
+// all.  The value is the count-phase fold, then (after a clean) minus the dead
+// triangles in the same order -- the same as computing it eagerly.
+rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_t cnt) {
+    if (!h->zw || h->pv_full) return RTREX_OK;  // intersection paths keep them
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    const TermBuf tb = h->terms();
+    const bool cleaned = h->phase >= PH_CLEANED;
+    if (!list) {
+        RT_TRY(vertex_folds(h, false));
+        if (cleaned) RT_TRY(vertex_folds(h, true));
+        h->pv_full = true;
+    } else if (cnt) {
+        const unsigned grid =
+            (unsigned)std::min<uint64_t>(((uint64_t)cnt * WAVE + VT - 1) / VT, 8192);
+        k_vfold_wave<false><<<grid, VT, 0, s>>>(g, tb, list, cnt);
+        if (cleaned) k_vfold_wave<true><<<grid, VT, 0, s>>>(g, tb, list, cnt);
+    }
+    RT_HIP(hipGetLastError());
+    return RTREX_OK;
 }
 
 rtrex_status run_count(rtrex_graph h) {
@@ -483,20 +515,37 @@ rtrex_status run_count(rtrex_graph h) {
     uint32_t* elist = h->cnt_lists;
     uint32_t* vlist = elist + (uint64_t)NB * g.m;
     RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
+    unsigned long long* tri3 = &h->ctr->tri_total;
+    auto ep = [&](int b) { return elist + (uint64_t)b * g.m; };
+    auto edge_pass = [&](const TermBuf& tb) {
+        if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tri3, tb);
+        if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tri3, tb);
+        if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tri3, tb);
+        if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tri3, tb);
+    };
+    if (h->terms_exact_pending) {
+        // Counting pass, exact offsets from the triangle counts, then the
+        // buffer is filled by the timed pass below (once per graph).
+        edge_pass(TermBuf{nullptr, nullptr, nullptr, h->zoff, h->za, h->zb});
+        RT_HIP(hipGetLastError());
+        RT_TRY(exclusive_scan_u32(g.tcnt, h->zoff, g.m, s));
+        uint64_t total = 0;
+        RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        h->terms_exact_pending = false;
+        try_alloc_terms(h, total);
+        RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
+    }
     TermBuf tb{h->zw, h->zsa, h->zsb, h->zoff, h->za, h->zb};
 
     RT_HIP(hipEventRecord(h->ev[2], s));
-    unsigned long long* tri3 = &h->ctr->tri_total;
-    auto ep = [&](int b) { return elist + (uint64_t)b * g.m; };
-    if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tri3, tb);
-    if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tri3, tb);
-    if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tri3, tb);
-    if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tri3, tb);
+    edge_pass(tb);
     RT_HIP(hipEventRecord(h->ev[3], s));
     auto vp = [&](int b) { return vlist + (uint64_t)b * g.n; };
     const uint32_t* vc = counts + NB;
+    h->pv_full = false;
     if (tb.w) {
-        RT_TRY(vertex_folds(h, false));
+        // on demand (ensure_vertex_supports)
     } else {
         if (vc[0]) k_count_vertices<1><<<grid_for(vc[0], 1), T, 0, s>>>(g, vp(0), vc[0]);
         if (vc[1]) k_count_vertices<4><<<grid_for(vc[1], 4), T, 0, s>>>(g, vp(1), vc[1]);

@@ -718,6 +718,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         k_comp_starts<<<bk, T, 0, s>>>(flag, fpos, nk, cstart);
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
+        RT_TRY(ensure_vertex_supports(h, cv, nk));  // candidates' supports
         {
             std::vector<uint32_t> cs(ncomp + 1);
             RT_HIP(hipMemcpyAsync(cs.data(), cstart, (ncomp + 1) * 4ull,

@@ -138,6 +138,8 @@ struct rtrex_graph_s {
     uint32_t *za = nullptr, *zb = nullptr;
     double* zw = nullptr;
     uint32_t *zsa = nullptr, *zsb = nullptr;
+    bool terms_exact_pending = false;  // size the buffer after a counting pass
+    bool pv_full = false;              // per-vertex supports valid everywhere
     uint32_t round_stamp = 0;
     rtrex::Timings tm;
     rtrex::AuditHost audit;
@@ -235,9 +237,9 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
 rtrex_status run_extract(rtrex_graph h, double eps);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
-// After a clean over stored triangle lists: per-vertex supports = count-phase
-// value minus dead triangles, in the count fold's order (count.hip).
-rtrex_status pull_vertex_supports(rtrex_graph h);
+// Per-vertex supports of the listed vertices (all when list is null) on
+// demand, when stored triangle lists are kept (count.hip).
+rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_t cnt);
 
 inline unsigned blocks_for(uint64_t work, unsigned per_block,
                            unsigned cap = 1u << 20) {
