@@ -53,6 +53,62 @@ def algorithmic_count_bytes(n, u, v) -> tuple[int, int]:
     return 8 * s_min + 28 * m + 16 * n, s_min
 
 
+def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
+    """C4: `count` planted-community graphs resident at once as one batch vs
+    the same graphs through the plain path one after another (each graph
+    already resident), same eps."""
+    import rtrex_cuda_amd as R
+    from rtrex_cuda_amd import graphs as G
+    t0 = time.perf_counter()
+    graphs = G.config4(count)
+    gen_s = time.perf_counter() - t0
+    m = int(sum(len(g[1]) for g in graphs))
+    n = int(sum(g[0] for g in graphs))
+    B_count = sum(algorithmic_count_bytes(*g)[0] for g in graphs)
+    b = R.Batch(graphs)
+    b.decompose(eps)  # warmup (plans, triangle lists)
+    tms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        maps, nc = b.decompose(eps)
+        tms.append(b.timings())
+    bt = (time.perf_counter() - t0) / steps
+    audit = b.audit()
+    b.close()
+    # Plain path on every 8th graph: each graph resident (built, warmed up),
+    # then one timed decomposition; throughput = sampled edges / summed time.
+    stride = 8
+    pt = 0.0
+    pm = 0
+    same = 0
+    sample = list(range(0, count, stride))
+    for i in sample:
+        with R.DeviceGraph.from_edges(*graphs[i]) as p:
+            p.decompose(eps)
+            t0 = time.perf_counter()
+            c2, _ = p.decompose(eps)
+            pt += time.perf_counter() - t0
+        pm += len(graphs[i][1])
+        same += int(np.array_equal(c2, maps[i]))
+    cnt_ms = float(np.mean([t["count_ms"] for t in tms]))
+    ach = B_count / (cnt_ms / 1e3) / 1e9
+    return {"workload": f"C4 {count} planted-community graphs, n log-uniform [50, 20000], "
+                        "communities 5-40 p=0.4, background mean degree 6, seed 23",
+            "graphs": count, "n": n, "m": m, "generation_s": round(gen_s, 1),
+            "batched_value": round(m / bt / 1e6, 3), "batched_ms": round(bt * 1e3, 3),
+            "plain_value": round(pm / pt / 1e6, 3),
+            "plain_sample": f"{len(sample)} of {count} graphs (every {stride}th), "
+                            f"{pm} edges in {pt:.3f} s",
+            "speedup": round((m / bt) / (pm / pt), 2),
+            "per_graph_identical": f"{same}/{len(sample)} sampled graphs equal alone vs in batch",
+            "phases_ms": {k: round(float(np.mean([t[k] for t in tms])), 3)
+                          for k in ("count_ms", "clean_ms", "extract_ms")},
+            "count_roofline": {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(ach / HBM_PEAK_GBS, 4),
+                               "algorithmic_bytes": int(B_count)},
+            "audit": audit}
+
+
 def cpu_baseline(n, u, v, eps) -> dict:
     import oracle as O
     if O.ref_available():
@@ -85,6 +141,8 @@ def main() -> int:
     ap.add_argument("--eps", type=float, default=0.3)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=4096,
+                    help="C4 batched-vs-plain leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of "
@@ -145,6 +203,8 @@ def main() -> int:
                      "k_count_vertices)", "algorithmic_bytes": B_count, "s_min": s_min},
         "audit": audit,
     }
+    if args.batch_graphs > 0:
+        out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, u, v, args.eps)
     print(json.dumps(out), flush=True)

@@ -173,10 +173,23 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 /* Batched decomposition (no reference counterpart: the reference runs one   */
 /* graph per process).  n_graphs independent graphs given as concatenated    */
 /* edge lists with LOCAL ids: graph i has vtx[i] vertices and edges          */
-/* [eoff[i], eoff[i+1]) of u/v.  All graphs are resident at once and every   */
-/* phase runs over all of them per launch.  cluster_map (sum of vtx) gets    */
-/* per-graph cluster ids; n_clusters[i] per graph.  Results equal running    */
-/* each graph alone.                                                         */
+/* [eoff[i], eoff[i+1]) of u/v.  All graphs are resident at once as one      */
+/* disjoint union and every phase runs over all of them per launch.          */
+/* Results equal running each graph alone.                                   */
+
+/* Builds the resident batch.  The handle works with every phase call above;
+ * rtrex_decompose / rtrex_extract then return PER-GRAPH cluster ids (graph i
+ * occupies cluster_map[sum(vtx[0..i)) ...]). */
+rtrex_status rtrex_batch_create(int64_t n_graphs, const int64_t* vtx,
+                                const int64_t* eoff, const uint32_t* u,
+                                const uint32_t* v, int device,
+                                rtrex_graph* out);
+
+/* Clusters per graph of the last extraction of a batch handle. */
+rtrex_status rtrex_batch_cluster_counts(rtrex_graph g, int64_t* n_clusters,
+                                        int64_t n_graphs);
+
+/* One-shot: rtrex_batch_create + rtrex_decompose + counts + destroy. */
 rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
                                    const int64_t* eoff, const uint32_t* u,
                                    const uint32_t* v, double eps, int device,

@@ -41,7 +41,8 @@ EXPORTS = [
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
     "rtrex_get_survivors",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
-    "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_decompose",
+    "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
+    "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
 ]
 
 
@@ -104,6 +105,8 @@ def lib() -> C.CDLL:
             "rtrex_get_timings": [G, C.POINTER(Timings)],
             "rtrex_batch_decompose": [C.c_int64, P64, P64, PU32, PU32, C.c_double, C.c_int,
                                       P64, P64, C.POINTER(Timings), C.POINTER(Audit)],
+            "rtrex_batch_create": [C.c_int64, P64, P64, PU32, PU32, C.c_int, C.POINTER(G)],
+            "rtrex_batch_cluster_counts": [G, P64, C.c_int64],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -259,6 +262,38 @@ class DeviceGraph:
         t = Timings()
         _check(lib().rtrex_get_timings(self._h, C.byref(t)))
         return t.as_dict()
+
+
+def _batch_arrays(graphs):
+    vtx = np.array([g[0] for g in graphs], np.int64)
+    sizes = np.array([len(g[1]) for g in graphs], np.int64)
+    eoff = np.zeros(len(graphs) + 1, np.int64)
+    np.cumsum(sizes, out=eoff[1:])
+    u = np.ascontiguousarray(np.concatenate([np.asarray(g[1], np.uint32) for g in graphs]))
+    v = np.ascontiguousarray(np.concatenate([np.asarray(g[2], np.uint32) for g in graphs]))
+    return vtx, eoff, u, v
+
+
+class Batch(DeviceGraph):
+    """Many independent graphs resident as one disjoint union (rtrex_batch_*).
+    decompose() returns one cluster map per graph with per-graph ids."""
+
+    def __init__(self, graphs, device: int = 0):
+        vtx, eoff, u, v = _batch_arrays(graphs)
+        h = C.c_void_p()
+        _check(lib().rtrex_batch_create(len(graphs), _p(vtx, P64), _p(eoff, P64),
+                                        _p(u, PU32), _p(v, PU32), device, C.byref(h)))
+        super().__init__(h, int(vtx.sum()), 2 * len(u))
+        self.vtx = vtx
+        self.vbase = np.zeros(len(graphs) + 1, np.int64)
+        np.cumsum(vtx, out=self.vbase[1:])
+
+    def decompose(self, eps: float):
+        cm, _ = super().decompose(eps)
+        nc = np.empty(len(self.vtx), np.int64)
+        _check(lib().rtrex_batch_cluster_counts(self._h, _p(nc, P64), len(nc)))
+        maps = [cm[self.vbase[i]:self.vbase[i + 1]] for i in range(len(self.vtx))]
+        return maps, nc
 
 
 def batch_decompose(graphs, eps: float, device: int = 0):

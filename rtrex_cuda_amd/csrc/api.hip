@@ -376,12 +376,11 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out) {
     return RTREX_OK;
 }
 
-rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
-                                   const int64_t* eoff, const uint32_t* u,
-                                   const uint32_t* v, double eps, int device,
-                                   int64_t* cluster_map, int64_t* n_clusters,
-                                   rtrex_timings* timings, rtrex_audit* audit) {
-    if (n_graphs <= 0 || !vtx || !eoff) return fail(RTREX_EINVAL, "bad batch description");
+rtrex_status rtrex_batch_create(int64_t n_graphs, const int64_t* vtx,
+                                const int64_t* eoff, const uint32_t* u,
+                                const uint32_t* v, int device, rtrex_graph* out) {
+    if (n_graphs <= 0 || !vtx || !eoff || !out)
+        return fail(RTREX_EINVAL, "bad batch description");
     // One disjoint union: graph i's vertices are shifted by the vertices of
     // the graphs before it.  Triangles, components and seeds never cross
     // graphs, so every phase runs over all graphs per launch.
@@ -402,6 +401,36 @@ rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
         }
     rtrex_graph h = nullptr;
     RT_TRY(new_handle(device, &h));
+    rtrex_status st = build_from_edges(h, U.data(), V.data(), N, M);
+    if (st != RTREX_OK) {
+        std::string msg = g_last_error;
+        rtrex_graph_destroy(h);
+        set_error(msg);
+        return st;
+    }
+    h->seg_vtx = vbase;
+    *out = h;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_batch_cluster_counts(rtrex_graph g, int64_t* n_clusters,
+                                        int64_t n_graphs) {
+    RT_TRY(check(g));
+    if (g->seg_vtx.empty()) return fail(RTREX_EINVAL, "not a batch handle");
+    if (g->phase < PH_EXTRACTED) return fail(RTREX_ESTATE, "extract has not run");
+    if (!n_clusters || n_graphs != (int64_t)g->seg_clusters.size())
+        return fail(RTREX_EINVAL, "n_graphs does not match the batch");
+    std::copy(g->seg_clusters.begin(), g->seg_clusters.end(), n_clusters);
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
+                                   const int64_t* eoff, const uint32_t* u,
+                                   const uint32_t* v, double eps, int device,
+                                   int64_t* cluster_map, int64_t* n_clusters,
+                                   rtrex_timings* timings, rtrex_audit* audit) {
+    rtrex_graph h = nullptr;
+    RT_TRY(rtrex_batch_create(n_graphs, vtx, eoff, u, v, device, &h));
     struct Guard {
         rtrex_graph h;
         ~Guard() {
@@ -410,13 +439,8 @@ rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
             set_error(msg);
         }
     } guard{h};
-    RT_TRY(build_from_edges(h, U.data(), V.data(), N, M));
-    h->seg_vtx = vbase;
-    RT_TRY(run_count(h));
-    RT_TRY(run_initial_clean(h, eps));
-    RT_TRY(run_extract(h, eps));
-    if (cluster_map) std::copy(h->cluster_map_host.begin(), h->cluster_map_host.end(), cluster_map);
-    if (n_clusters) std::copy(h->seg_clusters.begin(), h->seg_clusters.end(), n_clusters);
+    RT_TRY(rtrex_decompose(h, eps, cluster_map, nullptr));
+    if (n_clusters) RT_TRY(rtrex_batch_cluster_counts(h, n_clusters, n_graphs));
     if (timings) RT_TRY(rtrex_get_timings(h, timings));
     if (audit) RT_TRY(rtrex_get_audit(h, audit));
     return RTREX_OK;

@@ -497,10 +497,9 @@ rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_
         if (cleaned) RT_TRY(vertex_folds(h, true));
         h->pv_full = true;
     } else if (cnt) {
-        const unsigned grid =
-            (unsigned)std::min<uint64_t>(((uint64_t)cnt * WAVE + VT - 1) / VT, 8192);
-        k_vfold_wave<false><<<grid, VT, 0, s>>>(g, tb, list, cnt);
-        if (cleaned) k_vfold_wave<true><<<grid, VT, 0, s>>>(g, tb, list, cnt);
+        const unsigned grid = blocks_for(cnt, T, 8192);
+        k_vfold_lane<false><<<grid, T, 0, s>>>(g, tb, list, cnt);
+        if (cleaned) k_vfold_lane<true><<<grid, T, 0, s>>>(g, tb, list, cnt);
     }
     RT_HIP(hipGetLastError());
     return RTREX_OK;

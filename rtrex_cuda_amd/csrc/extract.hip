@@ -114,10 +114,18 @@ __global__ void __launch_bounds__(T) k_uf_hook(const GraphDev g, uint32_t* paren
     }
 }
 
+// Every parent[] entry set to its root.  The walk up does not shorten paths:
+// a halving write (parent[x] = grandparent) racing with another thread's
+// parent[x] = root could leave x pointing at a non-root, and the component
+// keys below read parent[] as the root.  Only roots are written here.
 __global__ void __launch_bounds__(T) k_uf_compress(uint32_t n, uint32_t* parent,
                                                    uint32_t* csize) {
     for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
-        uint32_t r = uf_find(parent, v);
+        uint32_t r = v, p = parent[v];
+        while (p != r) {
+            r = p;
+            p = parent[r];
+        }
         parent[v] = r;
         atomicAdd(&csize[r], 1u);
     }

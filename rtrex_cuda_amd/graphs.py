@@ -109,5 +109,9 @@ def config4_graph(i: int, seed: int = 23):
                    seed=seed * 1_000_003 + i)
 
 
-def config4(count: int = 4096, seed: int = 23):
-    return [config4_graph(i, seed) for i in range(count)]
+def config4(count: int = 4096, seed: int = 23, threads: int = 16):
+    """The 4,096-graph batch (generated in parallel; each graph depends only on
+    its own seed, so the result does not depend on the thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda i: config4_graph(i, seed), range(count)))

@@ -156,6 +156,36 @@ def test_deterministic_repeat():
     assert np.array_equal(a["pe"], b["pe"]) and np.array_equal(a["pv"], b["pv"])
 
 
+@pytest.mark.parametrize("tag", ["clean_astro_eps0.3", "clean_email_eps0.3"])
+def test_components_match_oracle(tag):
+    """The alive components extraction works on (union-find over survivors)
+    equal the oracle's connected components of the same survivor graph, and
+    the cluster map is right on every repeat of the plain path (no supports
+    read back between phases, so per-vertex supports are folded for the
+    component vertices only)."""
+    name = tag.rsplit("_eps", 1)[0]
+    eps = golden_meta()[tag]["eps"]
+    n, u, v = golden_graph(name)
+    exp = golden_result(tag)
+    for rep in range(3):
+        print(f"[gpu] components {tag} repeat {rep}", flush=True)
+        with R.DeviceGraph.from_edges(n, u, v) as g:
+            off, nb = g.csr()
+            g.gpu_common_nbr()
+            g.gpu_initial_clean(eps)
+            st = g.sync_edge_status()
+            cm, k = g.disjoint_extract(eps)
+            a = g.audit()
+        sv = survivors_of(off, nb, st)
+        soff, snb = O.csr_from_undirected(n, sv[:, 0], sv[:, 1])
+        comp, _ = O.connected_components(soff, snb)
+        sizes = np.bincount(comp)
+        assert a["components"] == int((sizes >= 2).sum())
+        assert a["largest_component"] == int(sizes.max())
+        assert np.array_equal(cm, exp["cluster_map"])
+        assert k == golden_meta()[tag]["n_clusters"]
+
+
 def test_invalid_graphs_rejected():
     with pytest.raises(R.RtrexError) as e:  # self-loop
         R.DeviceGraph.from_edges(3, np.array([0, 1], np.uint32), np.array([0, 2], np.uint32))
