@@ -109,6 +109,22 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
             "audit": audit}
 
 
+def copy_calibration(nbytes: int, iters: int) -> int:
+    import ctypes as C
+
+    from rtrex_cuda_amd import build as B
+    L = C.CDLL(B.BENCH_LIB)
+    L.rtrex_bench_copy.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.rtrex_bench_copy.restype = C.c_int
+    ms = C.c_double()
+    if L.rtrex_bench_copy(0, nbytes, iters, C.byref(ms)):
+        print("bench.py: calibration copy failed", file=sys.stderr)
+        return 1
+    print(json.dumps({"copy_bytes": nbytes, "launches": iters + 1, "ms_per_launch": ms.value,
+                      "GB/s (read+write)": 2 * nbytes / (ms.value / 1e3) / 1e9}), flush=True)
+    return 0
+
+
 def cpu_baseline(n, u, v, eps) -> dict:
     import oracle as O
     if O.ref_available():
@@ -143,7 +159,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="C4 batched-vs-plain leg (0 skips it)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the calibration copy kernel over BYTES and print "
+                         "its time (scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, max(1, args.steps))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of "
               "it (each graph / job fits one GPU; see DESIGN.md for how it would shard)",
@@ -168,6 +189,15 @@ def main() -> int:
     audit = g.audit()
     g.close()
     ms = wall * 1e3 / args.steps
+    # Same work including the host->device copy of the edge list and the
+    # device CSR build (not the headline value: inputs are not resident).
+    nb_steps = max(1, args.steps // 3)
+    t0 = time.perf_counter()
+    for _ in range(nb_steps):
+        with R.DeviceGraph.from_edges(n, u, v) as g2:
+            g2.decompose(args.eps)
+            g2.survivors()
+    ms_build = (time.perf_counter() - t0) * 1e3 / nb_steps
     value = m / (ms / 1e3) / 1e6
 
     cnt_ms = float(np.mean([p["count_ms"] for p in phases]))
@@ -202,6 +232,10 @@ def main() -> int:
                      "traffic": traffic, "kernel": "count phase (k_count_edges + "
                      "k_count_vertices)", "algorithmic_bytes": B_count, "s_min": s_min},
         "audit": audit,
+        "with_upload_and_build": {"value": round(m / (ms_build / 1e3) / 1e6, 3),
+                                  "ms_per_step": round(ms_build, 3),
+                                  "note": "host edge list -> device CSR build -> "
+                                          "decomposition -> results on the host"},
     }
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))

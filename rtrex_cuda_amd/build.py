@@ -3,6 +3,7 @@
     librtrex_hip.so   HIP kernels for gfx950 + the C ABI of include/rtrex.h
     librtrex_gen.so   seeded synthetic graph generators (host C++)
     librtrex_hostcheck.so  host builds of device routines, for CPU tests
+    librtrex_bench.so calibration copy kernel for the HBM byte counters
 
 Run ``python -m rtrex_cuda_amd.build`` (or ``__graft_entry__.build()``).
 hipcc cross-compiles for gfx950 without a GPU present.
@@ -18,6 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 HIP_LIB = os.path.join(PKG, "librtrex_hip.so")
 GEN_LIB = os.path.join(PKG, "librtrex_gen.so")
 CHECK_LIB = os.path.join(PKG, "librtrex_hostcheck.so")
+BENCH_LIB = os.path.join(PKG, "librtrex_bench.so")
 
 HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip"]
 
@@ -45,6 +47,14 @@ def build_hip(out: str = HIP_LIB) -> str:
     return out
 
 
+def build_bench(out: str = BENCH_LIB) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = out + ".tmp"
+    _run([hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "bench_copy.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_gen(out: str = GEN_LIB) -> str:
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", tmp,
@@ -65,6 +75,7 @@ def build_all() -> None:
     build_gen()
     build_hostcheck()
     build_hip()
+    build_bench()
 
 
 if __name__ == "__main__":
@@ -73,5 +84,7 @@ if __name__ == "__main__":
         build_gen()
     elif "hip" in what:
         build_hip()
+    elif "bench" in what:
+        build_bench()
     else:
         build_all()

@@ -40,23 +40,46 @@ __device__ __forceinline__ int work_bucket(uint32_t w) {
 
 // When bk is given (stored triangle lists), the frontier is also written
 // split by work bucket of tcnt (bk: 4 lists of capacity m, then 4 counts).
+// At eps = 0.3 nearly every edge starts in the frontier, so appends are
+// aggregated per workgroup (LDS counters, one global atomic per list per
+// tile of SEED_K * T edges) rather than per wave.
+constexpr int SEED_K = 8;
 __global__ void __launch_bounds__(T) k_seed_frontier(GraphDev g, double eps,
                                                      uint32_t* __restrict__ F,
                                                      unsigned* nF,
                                                      uint32_t* __restrict__ bk) {
-    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m;
-         e += gridDim.x * T) {
-        if (g.sup[e] < eps * g.ew[e]) {
-            if (g.est[e] != ST_ALIVE) continue;
-            set_status(g, e, ST_QUEUED);
-            F[wave_append(nF)] = e;
-            if (bk) {
-                const int b = work_bucket(g.tcnt[e]);
+    __shared__ uint32_t lcnt[5], lbase[5];
+    for (uint64_t tile = (uint64_t)blockIdx.x * SEED_K * T; tile < g.m;
+         tile += (uint64_t)gridDim.x * SEED_K * T) {
+        if (threadIdx.x < 5) lcnt[threadIdx.x] = 0;
+        __syncthreads();
+        int qb[SEED_K];
+        uint32_t rf[SEED_K], rb[SEED_K];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (b == q) bk[(uint64_t)q * g.m + wave_append(bk + 4ull * g.m + q)] = e;
+        for (int k = 0; k < SEED_K; ++k) {
+            const uint64_t e = tile + (uint64_t)k * T + threadIdx.x;
+            qb[k] = -1;
+            if (e < g.m && g.sup[e] < eps * g.ew[e] && g.est[e] == ST_ALIVE) {
+                set_status(g, (uint32_t)e, ST_QUEUED);
+                qb[k] = bk ? work_bucket(g.tcnt[e]) : 0;
+                rf[k] = atomicAdd(&lcnt[4], 1u);
+                if (bk) rb[k] = atomicAdd(&lcnt[qb[k]], 1u);
             }
         }
+        __syncthreads();
+        if (threadIdx.x < 5 && lcnt[threadIdx.x]) {
+            unsigned* c = threadIdx.x == 4 ? nF : bk + 4ull * g.m + threadIdx.x;
+            if (threadIdx.x == 4 || bk) lbase[threadIdx.x] = atomicAdd(c, lcnt[threadIdx.x]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k) {
+            if (qb[k] < 0) continue;
+            const uint32_t e = (uint32_t)(tile + (uint64_t)k * T + threadIdx.x);
+            F[lbase[4] + rf[k]] = e;
+            if (bk) bk[(uint64_t)qb[k] * g.m + lbase[qb[k]] + rb[k]] = e;
+        }
+        __syncthreads();  // lcnt / lbase reused by the next tile
     }
 }
 
@@ -528,7 +551,7 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     unsigned* cnt = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
     if (terms) RT_HIP(hipMemsetAsync(bk + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
-    k_seed_frontier<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, eps, F, cnt, bk);
+    k_seed_frontier<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, eps, F, cnt, bk);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
     RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
