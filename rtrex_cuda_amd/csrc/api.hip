@@ -20,6 +20,37 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+thread_local Arena* tl_arena = nullptr;
+
+namespace {
+// Makes the handle's scratch arena current for one API call, growing it
+// first (no scratch is outstanding between calls) when the last call asked
+// for more than it holds.  If the larger buffer cannot be had, calls keep
+// allocating per request.
+struct ArenaScope {
+    Arena* prev = tl_arena;
+    explicit ArenaScope(rtrex_graph h) {
+        Arena& a = h->arena;
+        if (a.top == 0 && a.demand > a.cap) {
+            (void)hipStreamSynchronize(h->stream);
+            if (a.base) (void)hipFree(a.base);
+            a.base = nullptr;
+            a.cap = 0;
+            const uint64_t want = a.demand + a.demand / 4;
+            void* q = nullptr;
+            if (hipMalloc(&q, want) == hipSuccess) {
+                a.base = static_cast<char*>(q);
+                a.cap = want;
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+        tl_arena = &a;
+    }
+    ~ArenaScope() { tl_arena = prev; }
+};
+}  // namespace
+
 rtrex_status fail(rtrex_status code, const std::string& msg) {
     set_error(msg);
     return code;
@@ -214,6 +245,7 @@ void rtrex_graph_destroy(rtrex_graph g) {
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (void* p : g->allocs) (void)hipFree(p);
+    if (g->arena.base) (void)hipFree(g->arena.base);
     for (auto& ev : g->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -254,6 +286,7 @@ int64_t rtrex_graph_footprint_bytes(int64_t n, int64_t nnz) {
 
 rtrex_status rtrex_count(rtrex_graph g, int64_t* total) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     RT_TRY(run_count(g));
     unsigned long long t3 = 0;
     RT_HIP(hipMemcpyAsync(&t3, &g->ctr->tri_total, 8, hipMemcpyDeviceToHost, g->stream));
@@ -265,6 +298,7 @@ rtrex_status rtrex_count(rtrex_graph g, int64_t* total) {
 rtrex_status rtrex_get_supports(rtrex_graph g, double* perEdge,
                                 double* perVertex, int64_t* total) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     if (g->phase < PH_COUNTED) return fail(RTREX_ESTATE, "count has not run");
     if (perVertex) RT_TRY(ensure_vertex_supports(g, nullptr, 0));
     hipStream_t s = g->stream;
@@ -287,6 +321,7 @@ rtrex_status rtrex_get_supports(rtrex_graph g, double* perEdge,
 
 rtrex_status rtrex_initial_clean(rtrex_graph g, double eps) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     return run_initial_clean(g, eps);
 }
 
@@ -316,12 +351,14 @@ rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords) 
 rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
                                     int64_t k, double eps) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     return run_delete_and_clean(g, pairs, k, eps);
 }
 
 rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
                            int64_t* n_clusters) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     RT_TRY(run_extract(g, eps));
     if (cluster_map)
         std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
@@ -332,6 +369,7 @@ rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
 rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
                              int64_t* n_clusters) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     RT_TRY(run_count(g));
     RT_TRY(run_initial_clean(g, eps));
     RT_TRY(run_extract(g, eps));
@@ -344,6 +382,7 @@ rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
 rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
                                  int64_t* n_edges, int64_t* cut) {
     RT_TRY(check(g));
+    ArenaScope scope_(g);
     return run_cluster_stats(g, n_vertices, n_edges, cut);
 }
 

@@ -41,6 +41,24 @@ __device__ __forceinline__ int bucket_of(uint32_t w) {
     return w <= 4u ? 0 : w <= 16u ? 1 : w <= 96u ? 2 : 3;
 }
 
+// Hub classes.  Edge {x, y} streams the list of lo = (d_x <= d_y ? x : y)
+// (x < y) and probes the list of the other endpoint, its hub.  Edges whose
+// hub degree is in (HUB_MIN, HUB_MAX] are counted per hub (k_count_hub), the
+// hub's list staged once in LDS; the rest per edge (k_count_edges).
+constexpr uint32_t HUB_MIN = 16, HUB_MAX = 4096;
+constexpr int HUB_NL = 3, HUB_NG = 5;  // staging sizes 64/512/4096; G = 4..64
+__host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub) {
+    return dhub > HUB_MIN && dhub <= HUB_MAX;
+}
+__host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub) {
+    return dhub <= 64 ? 0 : dhub <= 512 ? 1 : 2;
+}
+// Lanes per edge from the mean shorter-list length of the hub's edges.
+__host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges) {
+    const uint64_t mean = (work + edges - 1) / edges;
+    return mean <= 6 ? 0 : mean <= 12 ? 1 : mean <= 48 ? 2 : mean <= 160 ? 3 : 4;
+}
+
 // Appends ids to per-bucket lists: work(e) = length of the shorter list for
 // edges, the degree for vertices.
 template <bool EDGES>
@@ -58,11 +76,14 @@ __global__ void __launch_bounds__(T) k_bucket(GraphDev g, uint32_t count,
         if (EDGES) {
             uint32_t x = g.esrc[i], y = g.nb[g.eslot[i]];
             w = min(g.deg[x], g.deg[y]);
+            if (hub_edge(max(g.deg[x], g.deg[y]))) w = 0xffffffffu;  // k_count_hub
         } else {
             w = g.deg[i];
         }
-        b = bucket_of(w);
-        r = atomicAdd(&cnt[b], 1u);
+        if (w != 0xffffffffu) {
+            b = bucket_of(w);
+            r = atomicAdd(&cnt[b], 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x < NB)
@@ -119,7 +140,6 @@ template <int G>
 __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
                                                    const uint32_t* __restrict__ list,
                                                    uint32_t cnt,
-                                                   unsigned long long* tri3,
                                                    TermBuf tb) {
     const int lane = lane_id();
     const int gl = lane & (G - 1);
@@ -128,8 +148,7 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
     const uint32_t per_wave = WAVE / G;
     const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nwaves = gridDim.x * T / WAVE;
-    unsigned long long mytri = 0;
-    for (uint32_t base = wave * per_wave; base < cnt; base += nwaves * per_wave) {
+        for (uint32_t base = wave * per_wave; base < cnt; base += nwaves * per_wave) {
         const uint32_t idx = base + lane / G;
         bool act = idx < cnt;
         uint32_t e = act ? list[idx] : 0;
@@ -175,12 +194,235 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
                 tb.za[e] = ca;
                 tb.zb[e] = cb;
             }
-            mytri += c;
         }
     }
-    // Wave-reduce the triangle incidences, one atomic per wave.
-    for (int d = 32; d >= 1; d >>= 1) mytri += __shfl_xor(mytri, d, 64);
-    if (lane == 0 && mytri) atomicAdd(tri3, mytri);
+}
+
+// Triangle incidences sum_e tcnt(e) (three per triangle): per-block partial
+// sums, then one block adds them (no contended atomics).
+constexpr int RED_BLOCKS = 1024;
+__global__ void __launch_bounds__(T) k_sum_tcnt(const uint32_t* __restrict__ tcnt, uint32_t m,
+                                                unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long sw[T / WAVE];
+    unsigned long long c = 0;
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) c += tcnt[e];
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane_id() == 0) sw[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < T / WAVE; ++w) t += sw[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_sum_final(const unsigned long long* __restrict__ part,
+                                                 unsigned long long* out) {
+    __shared__ unsigned long long sw[T / WAVE];
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += T) c += part[i];
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane_id() == 0) sw[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < T / WAVE; ++w) t += sw[w];
+        *out = t;
+    }
+}
+
+// ---------------------------------------------------------------- hub path
+// Ownership flags: the hub-side slot of every hub-path edge; hwork[h] = the
+// streamed work of hub h (sum of its owned edges' shorter lists, <= 4096^2).
+__global__ void __launch_bounds__(T) k_own_flags(GraphDev g, uint32_t* __restrict__ flag,
+                                                 uint32_t* __restrict__ hwork) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint32_t sx = g.eslot[e], x = g.esrc[e], y = g.nb[sx];
+        const uint32_t dx = g.deg[x], dy = g.deg[y];
+        if (!hub_edge(max(dx, dy))) continue;
+        flag[dx <= dy ? g.part[sx] : sx] = 1u;  // hub y (slot in N(y)) or hub x
+        atomicAdd(&hwork[dx <= dy ? y : x], min(dx, dy));
+    }
+}
+
+// own_idx[pos[j]] = local index of hub-side slot j in its hub's list;
+// own_off[h] = pos[off[h]].
+__global__ void __launch_bounds__(T) k_own_scatter(GraphDev g, const uint32_t* __restrict__ pos,
+                                                   uint32_t* __restrict__ own_off,
+                                                   uint32_t* __restrict__ own_idx) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint32_t sx = g.eslot[e], x = g.esrc[e], y = g.nb[sx];
+        const uint32_t dx = g.deg[x], dy = g.deg[y];
+        if (!hub_edge(max(dx, dy))) continue;
+        const uint32_t h = dx <= dy ? y : x;
+        const uint32_t j = dx <= dy ? g.part[sx] : sx;
+        own_idx[pos[j]] = j - g.off[h];
+    }
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v <= g.n; v += gridDim.x * T)
+        own_off[v] = pos[g.off[v]];
+}
+
+// Work units of every hub: P parts of its owned edges, P from the streamed
+// work (sum of the shorter lists), each part worth at least max(1024, d_h)
+// elements so that staging the hub list is amortised.  COUNT: per-class
+// totals; otherwise the units are written (cursor per class).
+template <bool COUNT>
+__global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __restrict__ own_off,
+                                                 const uint32_t* __restrict__ hwork,
+                                                 uint32_t* __restrict__ cls_cnt,
+                                                 uint64_t* __restrict__ units,
+                                                 const uint32_t* __restrict__ cls_base) {
+    // One thread per hub; class totals aggregated per block (LDS counters,
+    // one global atomic per class per block).
+    constexpr int NC = HUB_NL * HUB_NG;
+    __shared__ uint32_t lc[NC], lb[NC];
+    if (threadIdx.x < NC) lc[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t h = blockIdx.x * T + threadIdx.x;
+    int c = -1;
+    uint64_t P = 0;
+    uint32_t at = 0;
+    if (h < g.n) {
+        const uint32_t oc = own_off[h + 1] - own_off[h];
+        if (oc) {
+            const uint32_t dh = g.deg[h];
+            const uint64_t work = hwork[h];
+            const uint64_t uw = max<uint64_t>(1024, dh);
+            P = (work + uw - 1) / uw;
+            P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
+            c = hub_lclass(dh) * HUB_NG + hub_gclass(work, oc);
+            at = atomicAdd(&lc[c], (uint32_t)P);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NC && lc[threadIdx.x])
+        lb[threadIdx.x] = atomicAdd(&cls_cnt[threadIdx.x], lc[threadIdx.x]);
+    __syncthreads();
+    if (!COUNT && c >= 0) {
+        for (uint64_t p = 0; p < P; ++p)
+            units[cls_base[c] + lb[c] + at + p] = (uint64_t)h | (p << 32) | (P << 48);
+    }
+}
+
+// Position of the first element >= x in the LDS-staged sorted list s[lo, hi).
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* s, uint32_t lo,
+                                                    uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One workgroup (HT lanes) per work unit: the hub's list -- neighbour ids,
+// their degrees and the edge ids of the hub's slots -- is staged in LDS once,
+// then its owned edges (part p of P) are counted G lanes per edge exactly as
+// in k_count_edges: the shorter list (with its slots' edge ids) is streamed
+// ascending in chunks of G, every probe is a binary search in LDS, and the
+// matches of a chunk are folded in lane order.  A match needs no further
+// gather: d_z and the edge id of {hub, z} come from LDS.
+constexpr int HT = 256;
+template <int G, int L>
+__global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint64_t* __restrict__ units,
+                                                  uint32_t nunits,
+                                                  const uint32_t* __restrict__ own_off,
+                                                  const uint32_t* __restrict__ own_idx,
+                                                  TermBuf tb) {
+    // R chunks of the shorter list are loaded at once (one memory round
+    // trip), then probed and folded one after the other.
+    constexpr int R = G <= 8 ? 4 : 2;
+    __shared__ uint32_t snb[L], sed[L], sdg[L], sown[L];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    const int gl = lane & (G - 1);
+    const int gbase = lane & ~(G - 1);
+    const uint64_t below = ((1ull << gl) - 1ull);
+    constexpr uint32_t per_wave = WAVE / G;
+    constexpr uint32_t per_blk = per_wave * (HT / WAVE);
+    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+        const uint64_t U = units[un];
+        const uint32_t h = (uint32_t)U;
+        const uint32_t part = (uint32_t)(U >> 32) & 0xffffu, parts = (uint32_t)(U >> 48);
+        const uint32_t oh = g.off[h], dh = g.deg[h];
+        const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
+        const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * part / parts);
+        const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (part + 1) / parts);
+        for (uint32_t i = threadIdx.x; i < dh; i += HT) {
+            const uint32_t z = g.nb[oh + i];
+            snb[i] = z;
+            sed[i] = g.s2e[oh + i];
+            sdg[i] = g.deg[z];
+        }
+        for (uint32_t i = threadIdx.x; i < k1 - k0; i += HT) sown[i] = own_idx[k0 + i];
+        __syncthreads();
+        for (uint32_t base = wid * per_wave; base < k1 - k0; base += per_blk) {
+            const uint32_t idx = base + lane / G;
+            const bool act = idx < k1 - k0;
+            const uint32_t li = act ? sown[idx] : 0;
+            const uint32_t y = snb[li];  // lo endpoint
+            const uint32_t e = sed[li];
+            const uint32_t dy = sdg[li];
+            const uint32_t ex = min(h, y), ey = max(h, y);
+            uint32_t p = act ? g.off[y] : 0;
+            const uint32_t le = act ? g.off[y + 1] : 0;
+            uint32_t hlo = 0;
+            const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+            double S = 0.0;
+            uint32_t c = 0, ca = 0, cb = 0;
+            while (__any(act && p < le)) {
+                uint32_t zr[R], er[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t q = p + r * G + gl;
+                    const bool in = act && q < le;
+                    zr[r] = in ? g.nb[q] : 0;
+                    er[r] = (in && tb.w) ? g.s2e[q] : 0;  // edge {lo, z}
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const bool go = act && p < le;
+                    const bool in = go && p + gl < le;
+                    const uint32_t z = zr[r];
+                    const uint32_t pos = in ? lds_lower_bound(snb, hlo, dh, z) : hlo;
+                    const bool hit = in && pos < dh && snb[pos] == z;
+                    const uint32_t lastl = go ? min((uint32_t)(G - 1), le - p - 1) : 0u;
+                    const uint32_t nh = __shfl(pos, gbase + lastl, 64);
+                    const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
+                    const uint64_t hm = G == 64 ? ballot64(hit)
+                                                : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
+                    if (tb.w && hit) {
+                        const uint64_t k = wbase + c + __popcll(hm & below);
+                        tb.w[k] = t;
+                        tb.sa[k] = er[r];
+                        tb.sb[k] = sed[pos];  // edge {hub, z}
+                    }
+                    const uint64_t am = G == 64 ? ballot64(hit && z < ex)
+                                                : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
+                    const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
+                                                : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
+                    ca += (uint32_t)__popcll(am);
+                    cb += (uint32_t)__popcll(bm);
+                    group_fold<G>(hit, t, S, c);
+                    if (go) {
+                        hlo = nh;
+                        p += G;
+                    }
+                }
+            }
+            if (act && gl == 0) {
+                g.sup[e] = S;
+                g.sup0[e] = S;
+                g.tcnt[e] = c;
+                g.ew[e] = 1.0 / (double)((uint64_t)dh * (uint64_t)dy);
+                if (tb.w) {
+                    tb.za[e] = ca;
+                    tb.zb[e] = cb;
+                }
+            }
+        }
+        __syncthreads();  // the next unit restages the hub
+    }
 }
 
 // Segment of vertex v's fold contributed by its neighbour slot j: the stored
@@ -389,6 +631,88 @@ unsigned grid_for(uint32_t items, int G) {
     return (unsigned)blocks;
 }
 
+template <int G, int L>
+void launch_hub_gl(const GraphDev& g, const uint64_t* units, uint32_t nu, const uint32_t* own_off,
+                   const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
+    k_count_hub<G, L><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_off,
+                                                                 own_idx, tb);
+}
+
+template <int L>
+void launch_hub_l(int gc, const GraphDev& g, const uint64_t* units, uint32_t nu,
+                  const uint32_t* own_off, const uint32_t* own_idx, const TermBuf& tb,
+                  hipStream_t s) {
+    switch (gc) {
+        case 0: launch_hub_gl<4, L>(g, units, nu, own_off, own_idx, tb, s); break;
+        case 1: launch_hub_gl<8, L>(g, units, nu, own_off, own_idx, tb, s); break;
+        case 2: launch_hub_gl<16, L>(g, units, nu, own_off, own_idx, tb, s); break;
+        case 3: launch_hub_gl<32, L>(g, units, nu, own_off, own_idx, tb, s); break;
+        default: launch_hub_gl<64, L>(g, units, nu, own_off, own_idx, tb, s); break;
+    }
+}
+
+// Every hub-path unit class, one launch each.
+void hub_pass(rtrex_graph h, const TermBuf& tb) {
+    uint64_t at = 0;
+    for (int c = 0; c < HUB_NL * HUB_NG; ++c) {
+        const uint32_t nu = h->hub_ucnt[c];
+        if (!nu) continue;
+        const uint64_t* u = h->hub_units + at;
+        const int lc = c / HUB_NG, gc = c % HUB_NG;
+        if (lc == 0) launch_hub_l<64>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
+        else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
+        else launch_hub_l<4096>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
+        at += nu;
+    }
+}
+
+// Hub plan (static per graph): ownership lists and work units.
+rtrex_status prepare_hubs(rtrex_graph h) {
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    Scratch sc;
+    uint32_t *flag, *pos, *ccnt, *cbase, *hwork;
+    RT_TRY(sc.get(&flag, g.nnz));
+    RT_TRY(sc.get(&hwork, g.n));
+    RT_HIP(hipMemsetAsync(hwork, 0, (uint64_t)g.n * 4, s));
+    RT_TRY(sc.get(&pos, g.nnz + 1));
+    RT_TRY(sc.get(&ccnt, HUB_NL * HUB_NG));
+    RT_TRY(sc.get(&cbase, HUB_NL * HUB_NG));
+    RT_HIP(hipMemsetAsync(flag, 0, g.nnz * 4, s));
+    const unsigned be = blocks_for(g.m, T, 8192);
+    if (g.m) k_own_flags<<<be, T, 0, s>>>(g, flag, hwork);
+    RT_TRY(exclusive_scan_u32_32(flag, pos, g.nnz, s));
+    uint32_t nown = 0;
+    RT_HIP(hipMemcpyAsync(&nown, pos + g.nnz, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    RT_TRY(dalloc(h, &h->own_off, (uint64_t)g.n + 1));
+    RT_TRY(dalloc(h, &h->own_idx, std::max<uint32_t>(nown, 1)));
+    k_own_scatter<<<blocks_for(std::max<uint64_t>(g.m, g.n + 1ull), T, 8192), T, 0, s>>>(
+        g, pos, h->own_off, h->own_idx);
+    RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
+    const unsigned bv = (unsigned)std::max<uint64_t>(1, ((uint64_t)g.n + T - 1) / T);  // one thread per hub
+    if (nown)
+        k_hub_units<true><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, nullptr, nullptr);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipMemcpyAsync(h->hub_ucnt, ccnt, sizeof(h->hub_ucnt), hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    uint32_t base[HUB_NL * HUB_NG];
+    uint64_t total = 0;
+    for (int c = 0; c < HUB_NL * HUB_NG; ++c) {
+        base[c] = (uint32_t)total;
+        total += h->hub_ucnt[c];
+    }
+    RT_TRY(dalloc(h, &h->hub_units, std::max<uint64_t>(total, 1)));
+    if (total) {
+        RT_HIP(hipMemcpyAsync(cbase, base, sizeof(base), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
+        k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase);
+        RT_HIP(hipGetLastError());
+    }
+    RT_HIP(hipStreamSynchronize(s));
+    return RTREX_OK;
+}
+
 }  // namespace
 
 // Term buffer of `total` records (f64 weight + two u32 edge ids) when it
@@ -433,6 +757,8 @@ rtrex_status prepare_count(rtrex_graph h) {
     if (g.n) k_bucket<false><<<blocks_for(g.n, T), T, 0, s>>>(g, g.n, vlist, cur + NB);
     RT_HIP(hipGetLastError());
     RT_HIP(hipMemcpyAsync(h->cnt_counts, cur, sizeof(h->cnt_counts), hipMemcpyDeviceToHost, s));
+    RT_TRY(prepare_hubs(h));
+    RT_TRY(dalloc(h, &h->red, RED_BLOCKS));
     // term buffer: capacity min(d_x, d_y) - 1 per edge
     RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));
     RT_TRY(dalloc(h, &h->za, g.m));
@@ -514,13 +840,15 @@ rtrex_status run_count(rtrex_graph h) {
     uint32_t* elist = h->cnt_lists;
     uint32_t* vlist = elist + (uint64_t)NB * g.m;
     RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
-    unsigned long long* tri3 = &h->ctr->tri_total;
     auto ep = [&](int b) { return elist + (uint64_t)b * g.m; };
     auto edge_pass = [&](const TermBuf& tb) {
-        if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tri3, tb);
-        if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tri3, tb);
-        if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tri3, tb);
-        if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tri3, tb);
+        if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tb);
+        if (counts[1]) k_count_edges<4><<<grid_for(counts[1], 4), T, 0, s>>>(g, ep(1), counts[1], tb);
+        if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tb);
+        if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tb);
+        hub_pass(h, tb);
+        k_sum_tcnt<<<RED_BLOCKS, T, 0, s>>>(g.tcnt, g.m, h->red);
+        k_sum_final<<<1, T, 0, s>>>(h->red, &h->ctr->tri_total);
     };
     if (h->terms_exact_pending) {
         // Counting pass, exact offsets from the triangle counts, then the

@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/rtrex.h"
@@ -104,6 +105,12 @@ struct AuditHost {
     int64_t components = 0, largest_component = 0;
 };
 
+// Per-handle scratch arena (see Scratch below).
+struct Arena {
+    char* base = nullptr;
+    uint64_t cap = 0, top = 0, vtop = 0, demand = 0;
+};
+
 enum Phase { PH_NONE = 0, PH_BUILT = 1, PH_COUNTED = 2, PH_CLEANED = 3, PH_EXTRACTED = 4 };
 
 }  // namespace rtrex
@@ -134,6 +141,15 @@ struct rtrex_graph_s {
     uint32_t* cnt_lists = nullptr;
     uint32_t *seed_order = nullptr, *seed_rank = nullptr;  // (segment, degree, id)
     uint32_t cnt_counts[8] = {};
+    // Hub plan: edges owned by their longer-list endpoint ("hub"), listed per
+    // hub (own_off[n+1], own_idx = local slot index in the hub's list), and
+    // work units (hub | part << 32 | parts << 48) per (staging size, lanes)
+    // class, hub_ucnt[] units each, concatenated in hub_units.
+    uint32_t* own_off = nullptr;
+    uint32_t* own_idx = nullptr;
+    uint64_t* hub_units = nullptr;
+    uint32_t hub_ucnt[15] = {};
+    unsigned long long* red = nullptr;  // per-block partial sums (count phase)
     uint64_t* zoff = nullptr;
     uint32_t *za = nullptr, *zb = nullptr;
     double* zw = nullptr;
@@ -143,6 +159,7 @@ struct rtrex_graph_s {
     uint32_t round_stamp = 0;
     rtrex::Timings tm;
     rtrex::AuditHost audit;
+    rtrex::Arena arena;  // phase scratch
     rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, za, zb}; }
     hipEvent_t ev[8] = {};
 };
@@ -192,16 +209,41 @@ rtrex_status dalloc(rtrex_graph h, T** p, uint64_t count) {
     return RTREX_OK;
 }
 
+// Per-handle scratch arena: one device buffer reused by every phase call
+// (stack discipline: a Scratch releases what it took when it goes out of
+// scope).  Work on a handle is ordered on its one stream, so memory released
+// by a finished phase can be handed to the next one without a sync.  A
+// request that does not fit is served by hipMalloc for that call and raises
+// `demand`; the arena grows to the demand at the start of the next call.
+// Arena of the handle whose API call is running on this thread (or null).
+extern thread_local Arena* tl_arena;
+
 // Scratch that lives for one call.
 struct Scratch {
     std::vector<void*> ptrs;
+    Arena* a = tl_arena;
+    uint64_t mark = a ? a->top : 0, vmark = a ? a->vtop : 0;
     ~Scratch() {
         for (void* p : ptrs) (void)hipFree(p);
+        if (a) {
+            a->top = mark;
+            a->vtop = vmark;
+        }
     }
     template <class T>
     rtrex_status get(T** p, uint64_t count) {
         void* q = nullptr;
         uint64_t bytes = count ? count * sizeof(T) : sizeof(T);
+        if (a) {
+            bytes = (bytes + 255) & ~255ull;
+            a->vtop += bytes;
+            a->demand = std::max(a->demand, a->vtop);
+            if (a->top + bytes <= a->cap) {
+                *p = reinterpret_cast<T*>(a->base + a->top);
+                a->top += bytes;
+                return RTREX_OK;
+            }
+        }
         hipError_t e = hipMalloc(&q, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();

@@ -118,7 +118,9 @@ rtrex_status exclusive_scan_impl(const TI* in, TO* out, uint64_t n,
     RT_TRY(sc.get(&bsum, scan_blocks(n) + 1));
     scan_enqueue<TI, TO>(in, out, n, bsum, s);
     RT_HIP(hipGetLastError());
-    RT_HIP(hipStreamSynchronize(s));  // scratch is freed on return
+    // Arena scratch is reused in stream order; hipMalloc'ed scratch is freed
+    // on return, after the work that reads it.
+    if (!tl_arena) RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }
 
@@ -243,7 +245,7 @@ rtrex_status radix_sort_u64(uint64_t* keys, uint64_t* tmp, uint64_t n,
     if (a != keys)
         RT_HIP(hipMemcpyAsync(keys, a, n * sizeof(uint64_t),
                               hipMemcpyDeviceToDevice, s));
-    RT_HIP(hipStreamSynchronize(s));
+    if (!tl_arena) RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }
 
