@@ -211,29 +211,99 @@ __global__ void __launch_bounds__(T) k_peel_terms(PeelArgs A) {
     }
 }
 
-// One peel round with G lanes per frontier edge.
-// One "pull" peel round: every alive edge subtracts, in its stored order, the
-// triangles that die this round (one other edge in the frontier, none already
-// deleted) and decides for itself.  Chosen when the frontier is larger than
-// the alive set, e.g. the first round, where nearly every edge is below its
-// threshold.  No atomics on supports, no ownership rule: each triangle is seen
-// by each of its alive edges exactly once.
-__global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A) {
-    const GraphDev& g = A.g;
-    const TermBuf& tb = A.tb;
-    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
-        if (g.est[e] != ST_ALIVE) continue;
-        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
-        double v = g.sup[e];
-        bool any = false;
-        for (uint64_t k = k0; k < k1; ++k) {
-            const uint8_t a = g.est[tb.sa[k]], b = g.est[tb.sb[k]];
-            if ((a == ST_QUEUED || b == ST_QUEUED) && a != ST_DEAD && b != ST_DEAD) {
-                v -= tb.w[k];
-                any = true;
+// Alive edges split by work bucket of tcnt (lists of capacity m, counts in
+// cur[0..3]), block-aggregated appends.
+__global__ void __launch_bounds__(T) k_alive_buckets(GraphDev g, uint32_t* __restrict__ bk,
+                                                     unsigned* cur) {
+    __shared__ uint32_t lcnt[4], lbase[4];
+    for (uint64_t tile = (uint64_t)blockIdx.x * SEED_K * T; tile < g.m;
+         tile += (uint64_t)gridDim.x * SEED_K * T) {
+        if (threadIdx.x < 4) lcnt[threadIdx.x] = 0;
+        __syncthreads();
+        int qb[SEED_K];
+        uint32_t rb[SEED_K];
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k) {
+            const uint64_t e = tile + (uint64_t)k * T + threadIdx.x;
+            qb[k] = -1;
+            if (e < g.m && g.est[e] == ST_ALIVE && g.tcnt[e]) {
+                qb[k] = work_bucket(g.tcnt[e]);
+                rb[k] = atomicAdd(&lcnt[qb[k]], 1u);
             }
         }
-        if (!any) continue;
+        __syncthreads();
+        if (threadIdx.x < 4 && lcnt[threadIdx.x])
+            lbase[threadIdx.x] = atomicAdd(&cur[threadIdx.x], lcnt[threadIdx.x]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k)
+            if (qb[k] >= 0)
+                bk[(uint64_t)qb[k] * g.m + lbase[qb[k]] + rb[k]] =
+                    (uint32_t)(tile + (uint64_t)k * T + threadIdx.x);
+        __syncthreads();
+    }
+}
+
+// One "pull" peel round: every alive edge subtracts the triangles that die
+// this round (one other edge in the frontier, none already deleted) and
+// decides for itself.  Chosen when the frontier is larger than the alive
+// set, e.g. the first round, where nearly every edge is below its threshold.
+// No atomics on supports and no ownership rule: each triangle is seen by each
+// of its alive edges exactly once.  G lanes per edge (bucketed by tcnt) sum
+// the dying weights, in any order: the result is S0 minus the dead terms with
+// at most tcnt roundings of size <= ulp(S0)/2 in all, inside the cheap bound,
+// and every decision within that bound is certified from the terms.
+template <int G>
+__global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __restrict__ list,
+                                                 uint32_t cnt) {
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    const int gl = lane & (G - 1);
+    const uint32_t per_wave = WAVE / G;
+    const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
+    const uint32_t nwaves = gridDim.x * T / WAVE;
+    for (uint32_t base = wave * per_wave; base < cnt; base += nwaves * per_wave) {
+        const uint32_t idx = base + lane / G;
+        const bool act = idx < cnt;
+        const uint32_t e = act ? list[idx] : 0;
+        const uint64_t k0 = act ? tb.zoff[e] : 0;
+        const uint64_t k1 = act ? k0 + g.tcnt[e] : 0;
+        double d = 0.0;
+        bool any = false;
+        // U terms per lane per step: all their edge ids, then all their
+        // statuses, are loaded before any is used (two round trips per step).
+        constexpr int U = 4;
+        for (uint64_t k = k0 + gl; k < k1; k += (uint64_t)U * G) {
+            uint32_t fa[U], fb[U];
+            uint8_t sa_[U], sb_[U];
+#pragma unroll
+            for (int r = 0; r < U; ++r) {
+                const uint64_t q = k + (uint64_t)r * G;
+                fa[r] = q < k1 ? tb.sa[q] : 0;
+                fb[r] = q < k1 ? tb.sb[q] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < U; ++r) {
+                const uint64_t q = k + (uint64_t)r * G;
+                sa_[r] = q < k1 ? g.est[fa[r]] : (uint8_t)ST_DEAD;
+                sb_[r] = q < k1 ? g.est[fb[r]] : (uint8_t)ST_DEAD;
+            }
+#pragma unroll
+            for (int r = 0; r < U; ++r) {
+                const uint8_t a = sa_[r], b = sb_[r];
+                if ((a == ST_QUEUED || b == ST_QUEUED) && a != ST_DEAD && b != ST_DEAD) {
+                    d += tb.w[k + (uint64_t)r * G];
+                    any = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) d += __shfl_xor(d, o, 64);
+        const uint64_t ab = ballot64(any);
+        const bool gany = G == 64 ? ab != 0 : ((ab >> (lane & ~(G - 1))) & ((1ull << G) - 1)) != 0;
+        if (!act || gl != 0 || !gany) continue;
+        const double v = g.sup[e] - d;
         g.sup[e] = v;
         g.touched[e] = 1;
         const double thr = A.eps * g.ew[e];
@@ -414,6 +484,12 @@ __global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits)
     }
 }
 
+// Blocks for `items` work items at G lanes each (capped; kernels stride).
+unsigned grid_items(uint32_t items, int G) {
+    const uint64_t waves = ((uint64_t)items * G + WAVE - 1) / WAVE;
+    return (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (waves * WAVE + T - 1) / T), 8192);
+}
+
 // Frontier rounds to convergence.  first_bk (optional) holds the first
 // frontier already split by work bucket (4 lists of capacity m + 4 counts).
 rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
@@ -472,15 +548,28 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         if (pull) {
             A.F = F;
             A.nF = nF;
-            k_peel_pull<<<blocks_for(g.m, T, 8192), T, 0, s>>>(A);
+            RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
+            k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, bk, bcur);
+            uint32_t ac[4] = {0, 0, 0, 0};
+            RT_HIP(hipMemcpyAsync(ac, bcur, sizeof(ac), hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < 4; ++b) {
+                if (!ac[b]) continue;
+                const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
+                const uint32_t* L = bk + (uint64_t)b * g.m;
+                const unsigned grid = grid_items(ac[b], G);
+                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, ac[b]);
+                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, ac[b]);
+                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, ac[b]);
+                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, ac[b]);
+            }
         }
         for (int b = 0; b < 4 && !pull; ++b) {
             if (!bc[b]) continue;
             const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
             A.F = lists[b];
             A.nF = bc[b];
-            uint64_t waves = ((uint64_t)bc[b] * G + WAVE - 1) / WAVE;
-            unsigned grid = (unsigned)std::min<uint64_t>((waves * WAVE + T - 1) / T, 8192);
+            const unsigned grid = grid_items(bc[b], G);
             if (A.tb.w) {
                 if (G == 1) k_peel_terms<1><<<grid, T, 0, s>>>(A);
                 else if (G == 4) k_peel_terms<4><<<grid, T, 0, s>>>(A);
