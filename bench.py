@@ -125,6 +125,80 @@ def copy_calibration(nbytes: int, iters: int) -> int:
     return 0
 
 
+def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb: float) -> dict:
+    """C5: Poisson request stream through the server (continuous batching,
+    HBM-footprint admission) vs the same requests one at a time through the
+    plain path.  Latency = result completion - scheduled arrival."""
+    import queue
+    import threading
+
+    import rtrex_cuda_amd as R
+    from rtrex_cuda_amd import graphs as G
+    sizes = (64, 32, 3)
+    t0 = time.perf_counter()
+    pool = G.serve_pool(*sizes)
+    gen_s = time.perf_counter() - t0
+    trace = G.serve_trace(seconds, rate, pool_sizes=sizes)
+    graph = lambda kind, i: pool[kind][i]  # noqa: E731
+    done = [None] * len(trace)
+    with R.Server(workers=workers, hbm_budget_bytes=int(budget_gb * (1 << 30))) as srv:
+        for kind in ("cl", "planted", "rmat"):  # warm the code paths once
+            srv.wait(srv.submit(*graph(kind, 0), eps))
+        q: "queue.Queue" = queue.Queue()
+
+        def waiter():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                i, j = item
+                cm, k, info = srv.wait(j)
+                done[i] = (k, info)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = srv.clock() + 0.05
+        for i, (t, kind, idx) in enumerate(trace):
+            while srv.clock() < start + t:
+                time.sleep(min(0.0005, max(0.0, start + t - srv.clock())))
+            q.put((i, srv.submit(*graph(kind, idx), eps)))
+        q.put(None)
+        th.join()
+    lat = np.array([done[i][1]["end_s"] - (start + t) for i, (t, _, _) in enumerate(trace)])
+    edges = np.array([len(graph(k, i)[1]) for _, k, i in trace], np.float64)
+    makespan = max(d[1]["end_s"] for d in done) - start
+    batched = np.array([d[1]["batch_graphs"] for d in done])
+    # Serial plain runs on a sample (every 10th request), inputs in host memory
+    # as the server gets them: upload + build + decomposition per request.
+    sample = list(range(0, len(trace), 10))
+    st, sm, same = 0.0, 0.0, 0
+    for i in sample:
+        _, kind, idx = trace[i]
+        n, u, v = graph(kind, idx)
+        t1 = time.perf_counter()
+        with R.DeviceGraph.from_edges(n, u, v) as g:
+            _, k = g.decompose(eps)
+        st += time.perf_counter() - t1
+        sm += len(u)
+        same += int(k == done[i][0])
+    return {"workload": f"C5 Poisson {rate:g} req/s for {seconds:g} s, mix 70% Chung-Lu "
+                        "m in [1k,50k], 25% planted m in [50k,2M], 5% R-MAT scale 18-20; "
+                        f"graphs from a seeded pool of {sum(sizes)} (seed 42)",
+            "requests": len(trace), "pool_generation_s": round(gen_s, 1),
+            "offered_Medges_per_s": round(edges.sum() / seconds / 1e6, 2),
+            "served_Medges_per_s": round(edges.sum() / makespan / 1e6, 2),
+            "makespan_s": round(makespan, 3),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                           "max": round(float(lat.max()) * 1e3, 2)},
+            "mean_graphs_per_launch": round(float(batched.mean()), 2),
+            "workers": workers, "hbm_budget_GB": budget_gb,
+            "serial_plain_Medges_per_s": round(sm / st / 1e6, 2),
+            "serial_plain_sample": f"{len(sample)} requests (every 10th), {st:.2f} s",
+            "served_vs_serial": round((edges.sum() / makespan) / (sm / st), 2),
+            "same_clusters_count_as_alone": f"{same}/{len(sample)} sampled requests"}
+
+
 def cpu_baseline(n, u, v, eps) -> dict:
     import oracle as O
     if O.ref_available():
@@ -159,6 +233,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="C4 batched-vs-plain leg (0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="C5 serving leg: length of the request trace (0 skips it)")
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy kernel over BYTES and print "
                          "its time (scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -239,6 +316,8 @@ def main() -> int:
     }
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))
+    if args.serve_seconds > 0:
+        out["serving"] = serving_leg(args.eps, args.serve_seconds, 200.0, args.serve_workers, 64.0)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, u, v, args.eps)
     print(json.dumps(out), flush=True)

@@ -196,6 +196,51 @@ rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
                                    int64_t* cluster_map, int64_t* n_clusters,
                                    rtrex_timings* timings, rtrex_audit* audit);
 
+/* ------------------------------------------------------------------------ */
+/* Serving (no reference counterpart).  A request loop: jobs are queued,      */
+/* admitted by HBM footprint into one device pool of hbm_budget_bytes, run    */
+/* by `workers` host threads on their own HIP streams (launches overlap on    */
+/* the GPU), and small jobs waiting behind each other are batched into one    */
+/* launch (continuous batching; per-graph results equal running alone).       */
+typedef struct rtrex_server_s* rtrex_server;
+
+typedef struct {
+    int device;
+    int workers;               /* concurrent launches, 1..16                  */
+    int64_t hbm_budget_bytes;  /* device pool shared by admitted launches     */
+    int64_t small_job_edges;   /* jobs up to this many edges may be batched   */
+    int64_t batch_max_edges;   /* edge cap of one batched launch              */
+    int64_t batch_max_graphs;  /* graph cap of one batched launch             */
+} rtrex_server_config;         /* zero caps select defaults (64Ki, 4Mi, 1024) */
+
+typedef struct {
+    double submit_s, start_s, end_s; /* server clock (rtrex_server_clock)     */
+    int64_t batch_graphs;            /* graphs in the launch that ran the job */
+    int64_t bytes;                   /* HBM reserved for that launch          */
+    int32_t worker;
+    int32_t pooled;                  /* 0: ran outside the pool (too large)   */
+} rtrex_job_info;
+
+rtrex_status rtrex_server_start(const rtrex_server_config* cfg, rtrex_server* out);
+
+/* Queues one decomposition of an undirected edge list (each edge once, ids <
+ * n); the edges are copied.  *job receives the job id. */
+rtrex_status rtrex_server_submit(rtrex_server s, const uint32_t* u,
+                                 const uint32_t* v, int64_t n, int64_t m,
+                                 double eps, int64_t* job);
+
+/* Blocks until the job is done, then returns its status (the job's own error
+ * in rtrex_last_error) and, on success, the cluster ids (cluster_map[n],
+ * nullable) and count.  The job id is released. */
+rtrex_status rtrex_server_wait(rtrex_server s, int64_t job, int64_t* cluster_map,
+                               int64_t* n_clusters, rtrex_job_info* info);
+
+/* Seconds since rtrex_server_start (the clock of rtrex_job_info). */
+double rtrex_server_clock(rtrex_server s);
+
+/* Runs every queued job, joins the workers and frees the pool. */
+rtrex_status rtrex_server_stop(rtrex_server s);
+
 #ifdef __cplusplus
 }
 #endif

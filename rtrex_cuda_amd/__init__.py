@@ -43,6 +43,8 @@ EXPORTS = [
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
+    "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
+    "rtrex_server_stop",
 ]
 
 
@@ -68,6 +70,21 @@ class Timings(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("workers", C.c_int), ("hbm_budget_bytes", C.c_int64),
+                ("small_job_edges", C.c_int64), ("batch_max_edges", C.c_int64),
+                ("batch_max_graphs", C.c_int64)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("start_s", C.c_double), ("end_s", C.c_double),
+                ("batch_graphs", C.c_int64), ("bytes", C.c_int64), ("worker", C.c_int32),
+                ("pooled", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: (float if t is C.c_double else int)(getattr(self, k)) for k, t in self._fields_}
 
 
 _lib = None
@@ -107,6 +124,10 @@ def lib() -> C.CDLL:
                                       P64, P64, C.POINTER(Timings), C.POINTER(Audit)],
             "rtrex_batch_create": [C.c_int64, P64, P64, PU32, PU32, C.c_int, C.POINTER(G)],
             "rtrex_batch_cluster_counts": [G, P64, C.c_int64],
+            "rtrex_server_start": [C.POINTER(ServerConfig), C.POINTER(G)],
+            "rtrex_server_submit": [G, PU32, PU32, C.c_int64, C.c_int64, C.c_double, P64],
+            "rtrex_server_wait": [G, C.c_int64, P64, P64, C.POINTER(JobInfo)],
+            "rtrex_server_stop": [G],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -116,6 +137,8 @@ def lib() -> C.CDLL:
         L.rtrex_graph_destroy.restype = None
         L.rtrex_graph_footprint_bytes.argtypes = [C.c_int64, C.c_int64]
         L.rtrex_graph_footprint_bytes.restype = C.c_int64
+        L.rtrex_server_clock.argtypes = [G]
+        L.rtrex_server_clock.restype = C.c_double
         _lib = L
     return _lib
 
@@ -316,3 +339,48 @@ def batch_decompose(graphs, eps: float, device: int = 0):
     np.cumsum(vtx, out=vb[1:])
     maps = [cm[vb[i]:vb[i + 1]] for i in range(len(graphs))]
     return maps, [int(x) for x in nc], t.as_dict(), a.as_dict()
+
+
+class Server:
+    """Request loop with HBM-footprint admission and continuous batching
+    (rtrex_server_*): submit() queues a job (the edges are copied) and returns
+    its id; wait(id) blocks for (cluster_map, n_clusters, info)."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget_bytes: int = 16 << 30,
+                 small_job_edges: int = 1 << 16, batch_max_edges: int = 1 << 22,
+                 batch_max_graphs: int = 1024):
+        cfg = ServerConfig(device, workers, hbm_budget_bytes, small_job_edges,
+                           batch_max_edges, batch_max_graphs)
+        self._h = C.c_void_p()
+        _check(lib().rtrex_server_start(C.byref(cfg), C.byref(self._h)))
+        self._n = {}
+
+    def clock(self) -> float:
+        return float(lib().rtrex_server_clock(self._h))
+
+    def submit(self, n: int, u, v, eps: float) -> int:
+        u = np.ascontiguousarray(u, np.uint32)
+        v = np.ascontiguousarray(v, np.uint32)
+        j = C.c_int64()
+        _check(lib().rtrex_server_submit(self._h, _p(u, PU32), _p(v, PU32), int(n), len(u),
+                                         float(eps), C.byref(j)))
+        self._n[j.value] = int(n)
+        return int(j.value)
+
+    def wait(self, job: int):
+        cm = np.empty(self._n.pop(job), np.int64)
+        k = C.c_int64()
+        info = JobInfo()
+        _check(lib().rtrex_server_wait(self._h, int(job), _p(cm, P64), C.byref(k), C.byref(info)))
+        return cm, int(k.value), info.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            lib().rtrex_server_stop(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

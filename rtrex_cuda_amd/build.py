@@ -21,7 +21,8 @@ GEN_LIB = os.path.join(PKG, "librtrex_gen.so")
 CHECK_LIB = os.path.join(PKG, "librtrex_hostcheck.so")
 BENCH_LIB = os.path.join(PKG, "librtrex_bench.so")
 
-HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip"]
+HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip",
+               "serve.hip"]
 
 # IEEE FP64 throughout (no contraction into FMA, no fast-math): the supports
 # must round exactly like the reference's separate multiply/add/divide.

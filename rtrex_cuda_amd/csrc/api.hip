@@ -33,9 +33,10 @@ struct ArenaScope {
         Arena& a = h->arena;
         if (a.top == 0 && a.demand > a.cap) {
             (void)hipStreamSynchronize(h->stream);
-            if (a.base) (void)hipFree(a.base);
+            if (a.base && a.owned) (void)hipFree(a.base);
             a.base = nullptr;
             a.cap = 0;
+            a.owned = true;
             const uint64_t want = a.demand + a.demand / 4;
             void* q = nullptr;
             if (hipMalloc(&q, want) == hipSuccess) {
@@ -83,13 +84,19 @@ rtrex_status open_device(int device) {
     return RTREX_OK;
 }
 
-rtrex_status new_handle(int device, rtrex_graph* out) {
+rtrex_status new_handle(int device, rtrex_graph* out, hipStream_t stream = nullptr) {
     if (!out) return fail(RTREX_EINVAL, "null output handle");
     *out = nullptr;
     RT_TRY(open_device(device));
     rtrex_graph h = new rtrex_graph_s();
     h->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    if (stream) {
+        h->stream = stream;
+        h->own_stream = false;
+    } else {
+        e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    }
     for (auto& ev : h->ev)
         if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e != hipSuccess) {
@@ -158,6 +165,91 @@ rtrex_status read_file(const char* path, EscapeFile* f) {
 }
 
 }  // namespace
+}  // namespace rtrex
+
+namespace rtrex {
+// Every array a resident graph allocates (alloc_state, the count plans, the
+// clean's near stamps and survivor bits, the seed order and cluster ids),
+// each rounded up to 256 bytes.
+uint64_t resident_bytes(uint64_t n, uint64_t m) {
+    const uint64_t nnz = 2 * m;
+    return 13 * nnz + 91 * m + 64 * n + 96 * 256 + 8192 + 64;
+}
+// The largest phase scratch: the build (edge upload, radix keys, partner
+// ranks), the hub plan, the clean (frontiers and buckets) or the extraction
+// (component layout, candidates, worklist), plus sort/scan block sums.
+uint64_t scratch_bytes(uint64_t n, uint64_t m) {
+    const uint64_t nnz = 2 * m;
+    const uint64_t build = 28 * nnz + 8 * m;
+    const uint64_t hubs = 8 * nnz + 4 * n;
+    const uint64_t clean = 44 * m;
+    const uint64_t extract = 120 * n + 4 * nnz;
+    return std::max(std::max(build, hubs), std::max(clean, extract)) + nnz / 16 +
+           (4ull << 20) + 64 * 256;
+}
+}  // namespace rtrex
+
+namespace rtrex {
+
+rtrex_status create_edges_in(const uint32_t* u, const uint32_t* v, int64_t n, int64_t m,
+                             int device, hipStream_t stream, char* slab, uint64_t slab_cap,
+                             rtrex_graph* out) {
+    rtrex_graph h = nullptr;
+    RT_TRY(new_handle(device, &h, stream));
+    rtrex_status st = RTREX_OK;
+    if (slab) {
+        const uint64_t pad = (256 - ((uintptr_t)slab & 255)) & 255;  // keep 256-byte alignment
+        h->slab = slab + pad;
+        h->slab_cap = slab_cap > pad ? slab_cap - pad : 0;
+        // The phase scratch (the build's included) is carved up front.
+        const uint64_t sb = (n >= 0 && m >= 0) ? scratch_bytes((uint64_t)n, (uint64_t)m) : 0;
+        if (void* q = slab_take(h, sb)) {
+            h->arena.base = static_cast<char*>(q);
+            h->arena.cap = (sb + 255) & ~255ull;
+            h->arena.owned = false;
+        }
+        ArenaScope scope_(h);
+        st = build_from_edges(h, u, v, n, m);
+    } else {
+        st = build_from_edges(h, u, v, n, m);
+    }
+    if (st != RTREX_OK) {
+        std::string msg = g_last_error;
+        rtrex_graph_destroy(h);
+        set_error(msg);
+        return st;
+    }
+    *out = h;
+    return RTREX_OK;
+}
+
+// One disjoint union: graph i's vertices are shifted by the vertices of the
+// graphs before it.  Triangles, components and seeds never cross graphs, so
+// every phase runs over all graphs per launch.
+rtrex_status union_edges(int64_t n_graphs, const int64_t* vtx, const int64_t* eoff,
+                         const uint32_t* u, const uint32_t* v, std::vector<int64_t>& vbase,
+                         std::vector<uint32_t>& U, std::vector<uint32_t>& V) {
+    if (n_graphs <= 0 || !vtx || !eoff) return fail(RTREX_EINVAL, "bad batch description");
+    vbase.assign(n_graphs + 1, 0);
+    for (int64_t i = 0; i < n_graphs; ++i) {
+        if (vtx[i] < 0 || eoff[i + 1] < eoff[i]) return fail(RTREX_EINVAL, "bad batch sizes");
+        vbase[i + 1] = vbase[i] + vtx[i];
+    }
+    const int64_t N = vbase[n_graphs], M = eoff[n_graphs] - eoff[0];
+    if (N >= (int64_t)UINT32_MAX) return fail(RTREX_EINVAL, "batch too large");
+    if (M && (!u || !v)) return fail(RTREX_EINVAL, "null edge array");
+    U.resize(M);
+    V.resize(M);
+    for (int64_t i = 0; i < n_graphs; ++i)
+        for (int64_t k = eoff[i]; k < eoff[i + 1]; ++k) {
+            if ((int64_t)u[k] >= vtx[i] || (int64_t)v[k] >= vtx[i])
+                return fail(RTREX_EINVAL, "edge endpoint outside its graph");
+            U[k - eoff[0]] = (uint32_t)(u[k] + vbase[i]);
+            V[k - eoff[0]] = (uint32_t)(v[k] + vbase[i]);
+        }
+    return RTREX_OK;
+}
+
 }  // namespace rtrex
 
 using namespace rtrex;
@@ -245,10 +337,10 @@ void rtrex_graph_destroy(rtrex_graph g) {
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (void* p : g->allocs) (void)hipFree(p);
-    if (g->arena.base) (void)hipFree(g->arena.base);
+    if (g->arena.base && g->arena.owned) (void)hipFree(g->arena.base);
     for (auto& ev : g->ev)
         if (ev) (void)hipEventDestroy(ev);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
 
@@ -272,16 +364,11 @@ rtrex_status rtrex_graph_get_csr(rtrex_graph g, int64_t* offsets,
     return RTREX_OK;
 }
 
+
 int64_t rtrex_graph_footprint_bytes(int64_t n, int64_t nnz) {
     if (n < 0 || nnz < 0) return -1;
-    const int64_t m = nnz / 2;
-    // resident: off, nb, deg, part, s2e, st + per-edge / per-vertex state
-    int64_t resident = 4 * (n + 1) + nnz * (4 + 4 + 4 + 1) + m * (4 + 4 + 8 + 8 + 4 + 1 + 4) +
-                       n * (4 + 8 + 8 + 4 + 4 + 4) + 256;
-    // peak scratch: build (keys + tmp + src + canon + rank) or extraction
-    int64_t build = nnz * (8 + 8 + 4 + 4 + 4);
-    int64_t extract = n * (8 + 8 + 4 * 8 + 8 + 8 + 16) + nnz * 8 + m * 4;
-    return resident + std::max(build, extract);
+    return (int64_t)(resident_bytes((uint64_t)n, (uint64_t)nnz / 2) +
+                     scratch_bytes((uint64_t)n, (uint64_t)nnz / 2));
 }
 
 rtrex_status rtrex_count(rtrex_graph g, int64_t* total) {
@@ -415,38 +502,17 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out) {
     return RTREX_OK;
 }
 
+
 rtrex_status rtrex_batch_create(int64_t n_graphs, const int64_t* vtx,
                                 const int64_t* eoff, const uint32_t* u,
                                 const uint32_t* v, int device, rtrex_graph* out) {
-    if (n_graphs <= 0 || !vtx || !eoff || !out)
-        return fail(RTREX_EINVAL, "bad batch description");
-    // One disjoint union: graph i's vertices are shifted by the vertices of
-    // the graphs before it.  Triangles, components and seeds never cross
-    // graphs, so every phase runs over all graphs per launch.
-    std::vector<int64_t> vbase(n_graphs + 1, 0);
-    for (int64_t i = 0; i < n_graphs; ++i) {
-        if (vtx[i] < 0 || eoff[i + 1] < eoff[i]) return fail(RTREX_EINVAL, "bad batch sizes");
-        vbase[i + 1] = vbase[i] + vtx[i];
-    }
-    const int64_t N = vbase[n_graphs], M = eoff[n_graphs] - eoff[0];
-    if (N >= (int64_t)UINT32_MAX) return fail(RTREX_EINVAL, "batch too large");
-    std::vector<uint32_t> U(M), V(M);
-    for (int64_t i = 0; i < n_graphs; ++i)
-        for (int64_t k = eoff[i]; k < eoff[i + 1]; ++k) {
-            if ((int64_t)u[k] >= vtx[i] || (int64_t)v[k] >= vtx[i])
-                return fail(RTREX_EINVAL, "edge endpoint outside its graph");
-            U[k - eoff[0]] = (uint32_t)(u[k] + vbase[i]);
-            V[k - eoff[0]] = (uint32_t)(v[k] + vbase[i]);
-        }
+    if (!out) return fail(RTREX_EINVAL, "null output handle");
+    std::vector<int64_t> vbase;
+    std::vector<uint32_t> U, V;
+    RT_TRY(union_edges(n_graphs, vtx, eoff, u, v, vbase, U, V));
     rtrex_graph h = nullptr;
-    RT_TRY(new_handle(device, &h));
-    rtrex_status st = build_from_edges(h, U.data(), V.data(), N, M);
-    if (st != RTREX_OK) {
-        std::string msg = g_last_error;
-        rtrex_graph_destroy(h);
-        set_error(msg);
-        return st;
-    }
+    RT_TRY(create_edges_in(U.data(), V.data(), vbase.back(), (int64_t)U.size(), device, nullptr,
+                           nullptr, 0, &h));
     h->seg_vtx = vbase;
     *out = h;
     return RTREX_OK;

@@ -718,6 +718,12 @@ rtrex_status prepare_hubs(rtrex_graph h) {
 // Term buffer of `total` records (f64 weight + two u32 edge ids) when it
 // fits in a third of the free HBM.
 bool try_alloc_terms(rtrex_graph h, uint64_t total) {
+    if (h->slab && h->slab_top + (total + 1) * 16 + 768 <= h->slab_cap) {
+        h->zw = static_cast<double*>(slab_take(h, (total + 1) * 8));
+        h->zsa = static_cast<uint32_t*>(slab_take(h, (total + 1) * 4));
+        h->zsb = static_cast<uint32_t*>(slab_take(h, (total + 1) * 4));
+        return true;
+    }
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
         (void)hipGetLastError();

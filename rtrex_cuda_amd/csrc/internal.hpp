@@ -109,6 +109,7 @@ struct AuditHost {
 struct Arena {
     char* base = nullptr;
     uint64_t cap = 0, top = 0, vtop = 0, demand = 0;
+    bool owned = true;  // false: carved from the handle's slab
 };
 
 enum Phase { PH_NONE = 0, PH_BUILT = 1, PH_COUNTED = 2, PH_CLEANED = 3, PH_EXTRACTED = 4 };
@@ -160,11 +161,44 @@ struct rtrex_graph_s {
     rtrex::Timings tm;
     rtrex::AuditHost audit;
     rtrex::Arena arena;  // phase scratch
+    // Slab mode (serving): every device allocation of the handle is carved
+    // from [slab, slab + slab_cap) of a pool the caller owns; nothing is freed
+    // per allocation.  The stream may belong to the caller as well.
+    char* slab = nullptr;
+    uint64_t slab_cap = 0, slab_top = 0;
+    bool own_stream = true;
     rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, za, zb}; }
     hipEvent_t ev[8] = {};
 };
 
 namespace rtrex {
+
+// Bytes taken from the handle's slab, 256-byte aligned; null if it is full.
+inline void* slab_take(rtrex_graph h, uint64_t bytes) {
+    if (!h->slab) return nullptr;
+    bytes = (bytes + 255) & ~255ull;
+    if (h->slab_top + bytes > h->slab_cap) return nullptr;
+    void* q = h->slab + h->slab_top;
+    h->slab_top += bytes;
+    return q;
+}
+
+// Device bytes a graph of n vertices and m undirected edges holds while
+// resident (every per-graph array, count plans included) and the largest
+// phase scratch; the stored triangle lists are 16 bytes per (term + 1) on top.
+uint64_t resident_bytes(uint64_t n, uint64_t m);
+uint64_t scratch_bytes(uint64_t n, uint64_t m);
+
+// Graph from an undirected edge list on `device`.  With `stream` the handle
+// uses (and does not own) that stream; with `slab` every device allocation
+// is carved from [slab, slab + slab_cap) (hipMalloc only past its end).
+rtrex_status create_edges_in(const uint32_t* u, const uint32_t* v, int64_t n, int64_t m,
+                             int device, hipStream_t stream, char* slab, uint64_t slab_cap,
+                             rtrex_graph* out);
+// Concatenated per-graph edge lists (local ids) -> one disjoint union.
+rtrex_status union_edges(int64_t n_graphs, const int64_t* vtx, const int64_t* eoff,
+                         const uint32_t* u, const uint32_t* v, std::vector<int64_t>& vbase,
+                         std::vector<uint32_t>& U, std::vector<uint32_t>& V);
 
 // Sets the status of undirected edge e in its per-edge byte and both slots.
 __device__ __forceinline__ void set_status(const GraphDev& g, uint32_t e, uint8_t c) {
@@ -198,6 +232,10 @@ template <class T>
 rtrex_status dalloc(rtrex_graph h, T** p, uint64_t count) {
     void* q = nullptr;
     uint64_t bytes = count ? count * sizeof(T) : sizeof(T);
+    if ((q = slab_take(h, bytes))) {
+        *p = static_cast<T*>(q);
+        return RTREX_OK;
+    }
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();

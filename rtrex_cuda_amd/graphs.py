@@ -16,6 +16,11 @@ Workloads (seeds and parameters of the benchmark configurations):
   C4  4,096 planted graphs, n log-uniform in [50, 20000], communities of
       5..40 with p=0.4 covering all vertices, background mean degree 6
       (Chung-Lu gamma 2.5), seed 23 (graph i uses seed 23 * 1_000_003 + i).
+  C5  request mix, seed 42: 70% Chung-Lu with m log-uniform in [1k, 50k]
+      (n = m / 5, gamma 2.5, max degree 4 sqrt(n)), 25% planted-community
+      with m log-uniform in [50k, 2M] (the C4 generator at n = m / 8.3), 5%
+      R-MAT of scale 18, 19 or 20 (edge factor 16, a,b,c = 0.57,0.19,0.19).
+      Requests draw graphs from a seeded pool generated once (serve_pool).
 """
 from __future__ import annotations
 
@@ -115,3 +120,48 @@ def config4(count: int = 4096, seed: int = 23, threads: int = 16):
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
         return list(ex.map(lambda i: config4_graph(i, seed), range(count)))
+
+
+def serve_graph(kind: str, i: int, seed: int = 42):
+    """Graph i of one kind of the serving mix ('cl', 'planted' or 'rmat')."""
+    rng = np.random.default_rng(seed * 7_919 + i * 31 + {"cl": 1, "planted": 2, "rmat": 3}[kind])
+    gseed = seed * 1_000_003 + i * 10 + {"cl": 1, "planted": 2, "rmat": 3}[kind]
+    if kind == "cl":
+        m = int(np.exp(rng.uniform(np.log(1_000), np.log(50_000))))
+        n = max(100, m // 5)
+        return chung_lu(n, m, 2.5, 4.0 * n ** 0.5, seed=gseed)
+    if kind == "planted":
+        m = int(np.exp(rng.uniform(np.log(50_000), np.log(2_000_000))))
+        n = max(100, int(m / 8.3))
+        return planted(n, 0, 5, 40, 0.4, 2.5, 3 * n, max(8.0, n ** 0.5), seed=gseed)
+    scale = 18 + i % 3
+    return rmat(scale, 16, 0.57, 0.19, 0.19, seed=gseed)
+
+
+def serve_pool(n_cl: int = 64, n_planted: int = 32, n_rmat: int = 3, seed: int = 42,
+               threads: int = 16):
+    """The serving pool: {kind: [graphs]} (generated in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = [("cl", i) for i in range(n_cl)] + [("planted", i) for i in range(n_planted)] + \
+           [("rmat", i) for i in range(n_rmat)]
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        out = list(ex.map(lambda kj: serve_graph(kj[0], kj[1], seed), jobs))
+    pool = {"cl": [], "planted": [], "rmat": []}
+    for (k, _), g in zip(jobs, out):
+        pool[k].append(g)
+    return pool
+
+
+def serve_trace(seconds: float, rate: float = 200.0, seed: int = 42, pool_sizes=(64, 32, 3)):
+    """Poisson arrivals (rate per second) over `seconds`: list of
+    (arrival_s, kind, pool index), kinds drawn 70/25/5."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        r = rng.random()
+        kind = "cl" if r < 0.70 else "planted" if r < 0.95 else "rmat"
+        idx = int(rng.integers(0, pool_sizes[("cl", "planted", "rmat").index(kind)]))
+        out.append((t, kind, idx))
