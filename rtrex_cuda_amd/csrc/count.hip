@@ -94,9 +94,28 @@ __global__ void __launch_bounds__(T) k_bucket(GraphDev g, uint32_t count,
 
 // Folds, in lane order, the values t of the lanes of this lane's group whose
 // hit flag is set.  All 64 lanes must call it.
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 template <int G>
 __device__ __forceinline__ void group_fold(bool hit, double t, double& S,
                                            uint32_t& c) {
+    if (G == 64) {
+        // One group: the hit mask is wave-uniform, so each hit's value is
+        // read with a scalar lane index (no LDS permute).
+        uint64_t mask = ballot64(hit);
+        c += (uint32_t)__popcll(mask);
+        while (mask) {
+            const int b = __ffsll((unsigned long long)mask) - 1;
+            S += readlane_f64(t, b);
+            mask &= mask - 1;
+        }
+        return;
+    }
     const int lane = lane_id();
     const int gbase = lane & ~(G - 1);
     uint64_t bal = ballot64(hit);
@@ -287,8 +306,12 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         if (oc) {
             const uint32_t dh = g.deg[h];
             const uint64_t work = hwork[h];
-            const uint64_t uw = max<uint64_t>(1024, dh);
+            // Units of at least max(1024, d_h) elements; the hash class
+            // (d_h > 512) rebuilds a table per unit: 8 d_h elements, but at
+            // most 32 owned edges apart so that hubs still spread out.
+            const uint64_t uw = dh > 512 ? 8ull * dh : max<uint64_t>(1024, dh);
             P = (work + uw - 1) / uw;
+            if (dh > 512) P = max<uint64_t>(P, (oc + 31) / 32);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = hub_lclass(dh) * HUB_NG + hub_gclass(work, oc);
             at = atomicAdd(&lc[c], (uint32_t)P);
@@ -425,6 +448,124 @@ __global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint64_t* __
     }
 }
 
+// Large hubs (staging class 4096): the hub list is held in an LDS hash table
+// (id -> position, open addressing, 2x capacity), so a probe is ~1 LDS read
+// instead of a 12-step binary search; d_z and the edge id of {hub, z} are
+// gathered for matches only.  Everything else is as in k_count_hub.
+constexpr uint32_t HH = 8192;  // hash slots (>= 2 * 4096)
+constexpr uint32_t HEMPTY = 0xffffffffu;
+__device__ __forceinline__ uint32_t hub_hash(uint32_t z) { return (z * 2654435761u) >> 19; }
+
+template <int G>
+__global__ void __launch_bounds__(HT) k_count_hub_hash(GraphDev g,
+                                                       const uint64_t* __restrict__ units,
+                                                       uint32_t nunits,
+                                                       const uint32_t* __restrict__ own_off,
+                                                       const uint32_t* __restrict__ own_idx,
+                                                       TermBuf tb) {
+    constexpr int R = 2;
+    __shared__ uint32_t hk[HH];
+    __shared__ uint16_t hv[HH];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    const int gl = lane & (G - 1);
+    const int gbase = lane & ~(G - 1);
+    const uint64_t below = ((1ull << gl) - 1ull);
+    constexpr uint32_t per_wave = WAVE / G;
+    constexpr uint32_t per_blk = per_wave * (HT / WAVE);
+    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+        const uint64_t U = units[un];
+        const uint32_t h = (uint32_t)U;
+        const uint32_t part = (uint32_t)(U >> 32) & 0xffffu, parts = (uint32_t)(U >> 48);
+        const uint32_t oh = g.off[h], dh = g.deg[h];
+        for (uint32_t i = threadIdx.x; i < HH; i += HT) hk[i] = HEMPTY;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < dh; i += HT) {
+            const uint32_t z = g.nb[oh + i];
+            uint32_t q = hub_hash(z);
+            while (atomicCAS(&hk[q], HEMPTY, z) != HEMPTY) q = (q + 1) & (HH - 1);
+            hv[q] = (uint16_t)i;
+        }
+        __syncthreads();
+        const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
+        const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * part / parts);
+        const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (part + 1) / parts);
+        for (uint32_t base = k0 + wid * per_wave; base < k1; base += per_blk) {
+            const uint32_t idx = base + lane / G;
+            const bool act = idx < k1;
+            const uint32_t li = act ? own_idx[idx] : 0;
+            const uint32_t y = act ? g.nb[oh + li] : 0;  // lo endpoint
+            const uint32_t e = act ? g.s2e[oh + li] : 0;
+            const uint32_t dy = act ? g.deg[y] : 0;
+            const uint32_t ex = min(h, y), ey = max(h, y);
+            uint32_t p = act ? g.off[y] : 0;
+            const uint32_t le = act ? g.off[y + 1] : 0;
+            const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+            double S = 0.0;
+            uint32_t c = 0, ca = 0, cb = 0;
+            while (__any(act && p < le)) {
+                uint32_t zr[R], er[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t q = p + r * G + gl;
+                    const bool in = act && q < le;
+                    zr[r] = in ? g.nb[q] : 0;
+                    er[r] = (in && tb.w) ? g.s2e[q] : 0;  // edge {lo, z}
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const bool go = act && p < le;
+                    const bool in = go && p + gl < le;
+                    const uint32_t z = zr[r];
+                    uint32_t pos = 0;
+                    bool hit = false;
+                    if (in) {
+                        uint32_t q = hub_hash(z);
+                        for (;;) {
+                            const uint32_t kz = hk[q];
+                            if (kz == z) {
+                                pos = hv[q];
+                                hit = true;
+                                break;
+                            }
+                            if (kz == HEMPTY) break;
+                            q = (q + 1) & (HH - 1);
+                        }
+                    }
+                    const double t = hit ? tri_w(dh, dy, g.deg[z]) : 0.0;
+                    const uint64_t hm = G == 64 ? ballot64(hit)
+                                                : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
+                    if (tb.w && hit) {
+                        const uint64_t k = wbase + c + __popcll(hm & below);
+                        tb.w[k] = t;
+                        tb.sa[k] = er[r];
+                        tb.sb[k] = g.s2e[oh + pos];  // edge {hub, z}
+                    }
+                    const uint64_t am = G == 64 ? ballot64(hit && z < ex)
+                                                : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
+                    const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
+                                                : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
+                    ca += (uint32_t)__popcll(am);
+                    cb += (uint32_t)__popcll(bm);
+                    group_fold<G>(hit, t, S, c);
+                    if (go) p += G;
+                }
+            }
+            if (act && gl == 0) {
+                g.sup[e] = S;
+                g.sup0[e] = S;
+                g.tcnt[e] = c;
+                g.ew[e] = 1.0 / (double)((uint64_t)dh * (uint64_t)dy);
+                if (tb.w) {
+                    tb.za[e] = ca;
+                    tb.zb[e] = cb;
+                }
+            }
+        }
+        __syncthreads();  // the next unit rebuilds the table
+    }
+}
+
 // Segment of vertex v's fold contributed by its neighbour slot j: the stored
 // terms of edge {v, y} with z > y.
 __device__ __forceinline__ void vseg(const GraphDev& g, const TermBuf& tb,
@@ -477,12 +618,6 @@ __global__ void __launch_bounds__(T) k_vfold_lane(GraphDev g, TermBuf tb,
     }
 }
 
-__device__ __forceinline__ double readlane_f64(double x, int l) {
-    const uint64_t b = __double_as_longlong(x);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 
 // One wave per vertex: 64 neighbour segments at a time are laid end to end
 // (lane scan of their lengths), the wave gathers 64 consecutive terms of that
@@ -638,6 +773,13 @@ void launch_hub_gl(const GraphDev& g, const uint64_t* units, uint32_t nu, const 
                                                                  own_idx, tb);
 }
 
+template <int G>
+void launch_hub_hash(const GraphDev& g, const uint64_t* units, uint32_t nu, const uint32_t* own_off,
+                     const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
+    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_off,
+                                                                    own_idx, tb);
+}
+
 template <int L>
 void launch_hub_l(int gc, const GraphDev& g, const uint64_t* units, uint32_t nu,
                   const uint32_t* own_off, const uint32_t* own_idx, const TermBuf& tb,
@@ -661,7 +803,15 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         const int lc = c / HUB_NG, gc = c % HUB_NG;
         if (lc == 0) launch_hub_l<64>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
-        else launch_hub_l<4096>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
+        else {
+            switch (gc) {
+                case 0: launch_hub_hash<4>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+                case 1: launch_hub_hash<8>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+                case 2: launch_hub_hash<16>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+                case 3: launch_hub_hash<32>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+                default: launch_hub_hash<64>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+            }
+        }
         at += nu;
     }
 }
