@@ -453,11 +453,12 @@ __global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint64_t* __
 // instead of a 12-step binary search; d_z and the edge id of {hub, z} are
 // gathered for matches only.  Everything else is as in k_count_hub.
 constexpr uint32_t HH = 8192;  // hash slots (>= 2 * 4096)
+constexpr int HTH = 512;       // 8 waves share one table (occupancy: 3 x 48 KB per CU)
 constexpr uint32_t HEMPTY = 0xffffffffu;
 __device__ __forceinline__ uint32_t hub_hash(uint32_t z) { return (z * 2654435761u) >> 19; }
 
 template <int G>
-__global__ void __launch_bounds__(HT) k_count_hub_hash(GraphDev g,
+__global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
                                                        const uint64_t* __restrict__ units,
                                                        uint32_t nunits,
                                                        const uint32_t* __restrict__ own_off,
@@ -472,15 +473,15 @@ __global__ void __launch_bounds__(HT) k_count_hub_hash(GraphDev g,
     const int gbase = lane & ~(G - 1);
     const uint64_t below = ((1ull << gl) - 1ull);
     constexpr uint32_t per_wave = WAVE / G;
-    constexpr uint32_t per_blk = per_wave * (HT / WAVE);
+    constexpr uint32_t per_blk = per_wave * (HTH / WAVE);
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         const uint64_t U = units[un];
         const uint32_t h = (uint32_t)U;
         const uint32_t part = (uint32_t)(U >> 32) & 0xffffu, parts = (uint32_t)(U >> 48);
         const uint32_t oh = g.off[h], dh = g.deg[h];
-        for (uint32_t i = threadIdx.x; i < HH; i += HT) hk[i] = HEMPTY;
+        for (uint32_t i = threadIdx.x; i < HH; i += HTH) hk[i] = HEMPTY;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < dh; i += HT) {
+        for (uint32_t i = threadIdx.x; i < dh; i += HTH) {
             const uint32_t z = g.nb[oh + i];
             uint32_t q = hub_hash(z);
             while (atomicCAS(&hk[q], HEMPTY, z) != HEMPTY) q = (q + 1) & (HH - 1);
@@ -776,7 +777,7 @@ void launch_hub_gl(const GraphDev& g, const uint64_t* units, uint32_t nu, const 
 template <int G>
 void launch_hub_hash(const GraphDev& g, const uint64_t* units, uint32_t nu, const uint32_t* own_off,
                      const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
-    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_off,
+    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HTH, 0, s>>>(g, units, nu, own_off,
                                                                     own_idx, tb);
 }
 
