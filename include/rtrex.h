@@ -140,6 +140,12 @@ rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
 rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
                              int64_t* n_clusters);
 
+/* Position at which every vertex joined its cluster in the last extraction:
+ * 0 for the seed, then its alive neighbours in slot order, then the chosen
+ * candidates in sorted order (Decomposition.h:610-631,869-875).  With
+ * rtrex_write_decomposition this reproduces the reference's file order. */
+rtrex_status rtrex_get_member_order(rtrex_graph g, int64_t* rank);
+
 /* Per-cluster statistics over the original graph, populateStats
  * (Decomposition.h:473-491): arrays of length n_clusters, indexed by id. */
 rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
@@ -195,6 +201,33 @@ rtrex_status rtrex_batch_decompose(int64_t n_graphs, const int64_t* vtx,
                                    const uint32_t* v, double eps, int device,
                                    int64_t* cluster_map, int64_t* n_clusters,
                                    rtrex_timings* timings, rtrex_audit* audit);
+
+/* ------------------------------------------------------------------------ */
+/* Output files, byte-identical to the reference driver's (host code; no GPU  */
+/* needed).  CSR in the reference layout.                                      */
+
+/* <name>-decomposition.txt, -stats.txt, -cut.txt, -freq.txt as
+ * printDecomposition (RTRex.cpp:19-118) writes them for the clusters given by
+ * cluster_of[n] (ids 0..n_clusters-1) with member_rank[n] the order in which
+ * vertices joined each cluster (rtrex_get_member_order). */
+rtrex_status rtrex_write_decomposition(const char* name, int64_t n,
+                                       const int64_t* offsets,
+                                       const int64_t* nbors,
+                                       const int64_t* cluster_of,
+                                       const int64_t* member_rank,
+                                       int64_t n_clusters);
+
+/* The same four files for the connected components, connectedComp
+ * (Decomposition.h:1023-1068), as RTRex.cpp:203-204 writes them. */
+rtrex_status rtrex_write_components(const char* name, int64_t n,
+                                    const int64_t* offsets,
+                                    const int64_t* nbors);
+
+/* The survivor dump of initialClean (Decomposition.h:419-452): "u v" per
+ * surviving edge, u < v, sorted; status[nnz] as rtrex_get_edge_status. */
+rtrex_status rtrex_write_survivors(const char* path, int64_t n,
+                                   const int64_t* offsets, const int64_t* nbors,
+                                   const uint8_t* status);
 
 /* ------------------------------------------------------------------------ */
 /* Serving (no reference counterpart).  A request loop: jobs are queued,      */

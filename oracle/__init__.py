@@ -45,7 +45,7 @@ def lib() -> C.CDLL:
         L.oracle_delete_and_clean.argtypes = [C.c_int64, p64, p64, C.c_double, p64,
                                               C.c_int64, pd, pd, p64, pu8]
         L.oracle_decompose.argtypes = [C.c_int64, p64, p64, C.c_double, p64, p64, pu8,
-                                       p64, C.c_int64, p64, pd]
+                                       p64, C.c_int64, p64, pd, p64]
         L.oracle_connected_components.argtypes = [C.c_int64, p64, p64, p64, p64]
         for f in (L.oracle_make_csr, L.oracle_count, L.oracle_initial_clean,
                   L.oracle_delete_and_clean, L.oracle_decompose,
@@ -151,6 +151,7 @@ class Decomposition:
     status_after_clean: np.ndarray
     clusters: list  # list of (nVertices, nEdges, cut, vertices-in-reference-order)
     times: tuple  # (count, clean, extract) seconds
+    member_rank: np.ndarray  # position at which each vertex joined its cluster
 
 
 def decompose(off, nb, eps: float, with_clusters: bool = True) -> Decomposition:
@@ -164,10 +165,11 @@ def decompose(off, nb, eps: float, with_clusters: bool = True) -> Decomposition:
     flat = np.zeros(max(cap, 1), np.int64)
     flen = np.zeros(1, np.int64)
     t = np.zeros(3, np.float64)
+    mr = np.zeros(n, np.int64)
     rc = lib().oracle_decompose(n, _p(off, C.c_int64), _p(nb, C.c_int64), eps,
                                 _p(cm, C.c_int64), _p(nc, C.c_int64), _p(st, C.c_uint8),
                                 _p(flat, C.c_int64) if with_clusters else None, cap,
-                                _p(flen, C.c_int64), _p(t, C.c_double))
+                                _p(flen, C.c_int64), _p(t, C.c_double), _p(mr, C.c_int64))
     if rc:
         raise ValueError(f"oracle_decompose failed rc={rc}")
     clusters = []
@@ -178,7 +180,7 @@ def decompose(off, nb, eps: float, with_clusters: bool = True) -> Decomposition:
             nv, ne, cut = int(fl[p]), int(fl[p + 1]), int(fl[p + 2])
             clusters.append((nv, ne, cut, fl[p + 3 : p + 3 + nv].copy()))
             p += 3 + nv
-    return Decomposition(cm, int(nc[0]), st, clusters, tuple(float(x) for x in t))
+    return Decomposition(cm, int(nc[0]), st, clusters, tuple(float(x) for x in t), mr)
 
 
 def connected_components(off, nb):

@@ -222,7 +222,7 @@ struct Cand {
 
 // disjointExtract, sequential inner-loop branch (Decomposition.h:538-940),
 // entered after count and initial clean.
-std::vector<Cluster> extract(State& s, i64* cluster_map) {
+std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
     const Graph& g = s.g;
     const i64 n = g.n;
     std::vector<std::pair<i64, i64>> order(n);  // (vertex, degree)
@@ -246,6 +246,8 @@ std::vector<Cluster> extract(State& s, i64* cluster_map) {
         c.vertices.insert(u);
         cluster_map[u] = nclus;
         clustered[u] = 1;
+        i64 joined = 0;  // order in which vertices join the cluster
+        if (member_rank) member_rank[u] = joined;
         std::set<i64> two_hop;
         double internal = 0;
 
@@ -253,6 +255,7 @@ std::vector<Cluster> extract(State& s, i64* cluster_map) {
             if (s.st[j] == 'N') continue;
             const i64 v = g.nb[j];
             c.vertices.insert(v);
+            if (member_rank) member_rank[v] = ++joined;
             nbd[v] = 1;
             cluster_map[v] = nclus;
             clustered[v] = 1;
@@ -319,6 +322,7 @@ std::vector<Cluster> extract(State& s, i64* cluster_map) {
         for (i64 idx = 0; idx <= maxind; ++idx) {  // :870-876
             const i64 x = cand[idx].vertex;
             c.vertices.insert(x);
+            if (member_rank) member_rank[x] = ++joined;
             cluster_map[x] = nclus;
             clustered[x] = 1;
         }
@@ -440,7 +444,8 @@ int oracle_delete_and_clean(int64_t n, const int64_t* offsets,
 int oracle_decompose(int64_t n, const int64_t* offsets, const int64_t* nbors,
                      double eps, int64_t* cluster_map, int64_t* n_clusters,
                      uint8_t* status_after_clean, int64_t* flat,
-                     int64_t flat_cap, int64_t* flat_len, double* t) {
+                     int64_t flat_cap, int64_t* flat_len, double* t,
+                     int64_t* member_rank) {
     if (!valid(n, offsets, nbors)) return 1;
     Graph g{n, offsets, nbors};
     State s(g);
@@ -461,7 +466,7 @@ int oracle_decompose(int64_t n, const int64_t* offsets, const int64_t* nbors,
     if (status_after_clean)
         for (i64 i = 0; i < nnz; ++i) status_after_clean[i] = (uint8_t)s.st[i];
     auto t2 = std::chrono::steady_clock::now();
-    std::vector<Cluster> dec = extract(s, cluster_map);
+    std::vector<Cluster> dec = extract(s, cluster_map, member_rank);
     double te = since(t2);
     *n_clusters = (i64)dec.size();
     if (t) {

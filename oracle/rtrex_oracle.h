@@ -53,11 +53,15 @@ int oracle_delete_and_clean(int64_t n, const int64_t* offsets,
  * clean (may be NULL), and phase wall times in seconds t[3] (may be NULL).
  * If flat != NULL it receives, per cluster in id order: nVertices, nEdges,
  * cut, then the vertices in the reference's iteration order; flat_cap bounds
- * it (n_clusters*3 + n suffices) and *flat_len receives the length used. */
+ * it (n_clusters*3 + n suffices) and *flat_len receives the length used.
+ * member_rank[n] (may be NULL) receives the position at which each vertex
+ * joined its cluster (seed 0, alive neighbours, then chosen candidates,
+ * Decomposition.h:610-631,869-875). */
 int oracle_decompose(int64_t n, const int64_t* offsets, const int64_t* nbors,
                      double eps, int64_t* cluster_map, int64_t* n_clusters,
                      uint8_t* status_after_clean, int64_t* flat,
-                     int64_t flat_cap, int64_t* flat_len, double* t);
+                     int64_t flat_cap, int64_t* flat_len, double* t,
+                     int64_t* member_rank);
 
 /* connectedComp (Decomposition.h:1023-1068): component id per vertex in BFS
  * discovery order; returns the number of components in *n_comp. */

@@ -44,7 +44,8 @@ EXPORTS = [
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
-    "rtrex_server_stop",
+    "rtrex_server_stop", "rtrex_get_member_order", "rtrex_write_decomposition",
+    "rtrex_write_components", "rtrex_write_survivors",
 ]
 
 
@@ -128,6 +129,10 @@ def lib() -> C.CDLL:
             "rtrex_server_submit": [G, PU32, PU32, C.c_int64, C.c_int64, C.c_double, P64],
             "rtrex_server_wait": [G, C.c_int64, P64, P64, C.POINTER(JobInfo)],
             "rtrex_server_stop": [G],
+            "rtrex_get_member_order": [G, P64],
+            "rtrex_write_decomposition": [C.c_char_p, C.c_int64, P64, P64, P64, P64, C.c_int64],
+            "rtrex_write_components": [C.c_char_p, C.c_int64, P64, P64],
+            "rtrex_write_survivors": [C.c_char_p, C.c_int64, P64, P64, PU8],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -269,6 +274,12 @@ class DeviceGraph:
         _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64), C.byref(k)))
         return cm, int(k.value)
 
+    def member_order(self) -> np.ndarray:
+        """Position at which each vertex joined its cluster (last extraction)."""
+        r = np.empty(self.n, np.int64)
+        _check(lib().rtrex_get_member_order(self._h, _p(r, P64)))
+        return r
+
     def populate_stats(self, n_clusters: int):
         nv = np.empty(n_clusters, np.int64)
         ne = np.empty(n_clusters, np.int64)
@@ -384,3 +395,32 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+# ---------------------------------------------------------------- output files
+def _csr64(offsets, nbors):
+    return (np.ascontiguousarray(offsets, np.int64), np.ascontiguousarray(nbors, np.int64))
+
+
+def write_decomposition(name: str, offsets, nbors, cluster_of, member_rank, n_clusters: int):
+    """<name>-decomposition/-stats/-cut/-freq.txt exactly as the reference's
+    printDecomposition writes them (RTRex.cpp:19-118)."""
+    off, nb = _csr64(offsets, nbors)
+    co = np.ascontiguousarray(cluster_of, np.int64)
+    mr = np.ascontiguousarray(member_rank, np.int64)
+    _check(lib().rtrex_write_decomposition(name.encode(), len(off) - 1, _p(off, P64), _p(nb, P64),
+                                           _p(co, P64), _p(mr, P64), int(n_clusters)))
+
+
+def write_components(name: str, offsets, nbors):
+    """The connected-components files of RTRex.cpp:203-204."""
+    off, nb = _csr64(offsets, nbors)
+    _check(lib().rtrex_write_components(name.encode(), len(off) - 1, _p(off, P64), _p(nb, P64)))
+
+
+def write_survivors(path: str, offsets, nbors, status):
+    """The survivor dump of initialClean (Decomposition.h:419-452)."""
+    off, nb = _csr64(offsets, nbors)
+    st = np.ascontiguousarray(status, np.uint8)
+    _check(lib().rtrex_write_survivors(path.encode(), len(off) - 1, _p(off, P64), _p(nb, P64),
+                                       _p(st, PU8)))

@@ -4,6 +4,7 @@
     librtrex_gen.so   seeded synthetic graph generators (host C++)
     librtrex_hostcheck.so  host builds of device routines, for CPU tests
     librtrex_bench.so calibration copy kernel for the HBM byte counters
+    rtrex             the command-line driver (host C++ on the C ABI)
 
 Run ``python -m rtrex_cuda_amd.build`` (or ``__graft_entry__.build()``).
 hipcc cross-compiles for gfx950 without a GPU present.
@@ -20,9 +21,10 @@ HIP_LIB = os.path.join(PKG, "librtrex_hip.so")
 GEN_LIB = os.path.join(PKG, "librtrex_gen.so")
 CHECK_LIB = os.path.join(PKG, "librtrex_hostcheck.so")
 BENCH_LIB = os.path.join(PKG, "librtrex_bench.so")
+CLI = os.path.join(PKG, "rtrex")
 
 HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip",
-               "serve.hip"]
+               "serve.hip", "writers.hip"]
 
 # IEEE FP64 throughout (no contraction into FMA, no fast-math): the supports
 # must round exactly like the reference's separate multiply/add/divide.
@@ -56,6 +58,14 @@ def build_bench(out: str = BENCH_LIB) -> str:
     return out
 
 
+def build_cli(out: str = CLI) -> str:
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(CSRC, "cli.cpp"),
+          "-L", PKG, "-lrtrex_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_gen(out: str = GEN_LIB) -> str:
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", tmp,
@@ -77,6 +87,7 @@ def build_all() -> None:
     build_hostcheck()
     build_hip()
     build_bench()
+    build_cli()
 
 
 if __name__ == "__main__":
@@ -87,5 +98,7 @@ if __name__ == "__main__":
         build_hip()
     elif "bench" in what:
         build_bench()
+    elif "cli" in what:
+        build_cli()
     else:
         build_all()

@@ -173,7 +173,7 @@ namespace rtrex {
 // each rounded up to 256 bytes.
 uint64_t resident_bytes(uint64_t n, uint64_t m) {
     const uint64_t nnz = 2 * m;
-    return 13 * nnz + 91 * m + 64 * n + 96 * 256 + 8192 + 64;
+    return 13 * nnz + 91 * m + 68 * n + 96 * 256 + 8192 + 64;
 }
 // The largest phase scratch: the build (edge upload, radix keys, partner
 // ranks), the hub plan, the clean (frontiers and buckets) or the extraction
@@ -463,6 +463,19 @@ rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
     if (cluster_map)
         std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
     if (n_clusters) *n_clusters = g->n_clusters;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_get_member_order(rtrex_graph g, int64_t* rank) {
+    RT_TRY(check(g));
+    if (g->phase < PH_EXTRACTED || !g->mrank)
+        return fail(RTREX_ESTATE, "extract has not run");
+    if (!rank) return fail(RTREX_EINVAL, "null rank array");
+    std::vector<uint32_t> r(g->g.n);
+    RT_HIP(hipMemcpyAsync(r.data(), g->mrank, (uint64_t)g->g.n * 4, hipMemcpyDeviceToHost,
+                          g->stream));
+    RT_HIP(hipStreamSynchronize(g->stream));
+    for (uint32_t v = 0; v < g->g.n; ++v) rank[v] = r[v];
     return RTREX_OK;
 }
 

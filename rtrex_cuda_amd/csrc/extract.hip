@@ -138,12 +138,14 @@ __global__ void __launch_bounds__(T) k_comp_keys(uint32_t n,
                                                  const uint32_t* __restrict__ csize,
                                                  const uint32_t* __restrict__ rank,
                                                  uint32_t* __restrict__ cseed,
+                                                 uint32_t* __restrict__ mrank,
                                                  uint64_t* __restrict__ keys,
                                                  unsigned* nkeys) {
     for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
         uint32_t r = parent[v];
         if (csize[r] < 2) {
             cseed[v] = v;
+            mrank[v] = 0;
         } else {
             keys[wave_append(nkeys)] = ((uint64_t)r << 32) | rank[v];
         }
@@ -211,6 +213,7 @@ struct XArgs {
     const uint64_t* dpre;    // prefix of degrees over cv (scratch placement)
     uint32_t ncomp;
     uint32_t* cseed;         // seed of each vertex's cluster, NONE if not yet
+    uint32_t* mrank;         // position at which the vertex joined its cluster
     uint32_t* mk;            // neighbour stamp (seed + 1)
     uint32_t* cm;            // candidate stamp (seed + 1)
     double* tw;              // two-hop weight per candidate vertex
@@ -302,15 +305,18 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 const uint32_t v = j < ue ? g.nb[j] : 0;
                 const uint64_t bal = ballot64(alive);
                 if (alive) {
-                    A.mem[start + 1 + nA + __popcll(bal & lanes_below())] = v;
+                    const uint32_t r = nA + (uint32_t)__popcll(bal & lanes_below());
+                    A.mem[start + 1 + r] = v;
                     A.mk[v] = stamp;
                     A.cseed[v] = u;
+                    A.mrank[v] = 1 + r;
                 }
                 nA += (uint32_t)__popcll(bal);
             }
             if (lane == 0) {
                 A.mem[start] = u;
                 A.cseed[u] = u;
+                A.mrank[u] = 0;
             }
             __syncthreads();
             if (nA == 0) continue;  // singleton cluster {u}
@@ -434,7 +440,10 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             }
             __syncthreads();
             const int maxind = s_maxind;
-            for (int i = lane; i <= maxind; i += XT) A.cseed[rec[i].vertex] = u;
+            for (int i = lane; i <= maxind; i += XT) {
+                A.cseed[rec[i].vertex] = u;
+                A.mrank[rec[i].vertex] = 1 + nA + (uint32_t)i;
+            }
             __syncthreads();
 
             // 4. queue every alive edge of the cluster (Decomposition.h:897-914)
@@ -698,7 +707,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     k_uf_compress<<<bn, T, 0, s>>>(n, parent, csize);
     unsigned* nk_d = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(nk_d, 0, sizeof(unsigned), s));
-    k_comp_keys<<<bn, T, 0, s>>>(n, parent, csize, rank, h->cseed, keys, nk_d);
+    if (!h->mrank) RT_TRY(dalloc(h, &h->mrank, n));
+    k_comp_keys<<<bn, T, 0, s>>>(n, parent, csize, rank, h->cseed, h->mrank, keys, nk_d);
     RT_HIP(hipGetLastError());
     uint32_t nk = 0;
     RT_HIP(hipMemcpyAsync(&nk, nk_d, 4, hipMemcpyDeviceToHost, s));
@@ -760,6 +770,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.dpre = dpre;
         a.ncomp = ncomp;
         a.cseed = h->cseed;
+        a.mrank = h->mrank;
         a.mk = mk;
         a.cm = cm;
         a.tw = tw;

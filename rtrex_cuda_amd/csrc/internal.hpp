@@ -129,6 +129,7 @@ struct rtrex_graph_s {
     // and the cluster id of every vertex.
     uint32_t* cseed = nullptr;
     uint32_t* cid = nullptr;  // cluster id per vertex (global numbering)
+    uint32_t* mrank = nullptr;  // position at which each vertex joined its cluster
     int64_t n_clusters = -1;
     std::vector<int64_t> seg_clusters;  // per segment (batch)
     std::vector<int64_t> cluster_map_host;

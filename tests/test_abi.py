@@ -84,3 +84,19 @@ def test_device_calls_fail_loudly_without_gpu():
         R.DeviceGraph.from_edges(3, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32))
     assert e.value.status == 2
     assert "gfx950" in str(e.value) or "device" in str(e.value)
+
+
+def test_cli_usage_and_no_device_error(tmp_path):
+    """The command-line driver is built, prints its usage, and fails loudly
+    (non-zero exit) where no gfx950 device is present."""
+    import subprocess
+    cli = os.path.join(os.path.dirname(R.HIP_LIB), "rtrex")
+    r = subprocess.run([cli], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Usage" in r.stdout
+    if _gpu_present():
+        return
+    g = tmp_path / "g.txt"
+    g.write_text("3 3\n0 1\n1 2\n0 2\n")
+    r = subprocess.run([cli, str(g), "x", "0.3"], capture_output=True, text=True, timeout=60,
+                       cwd=tmp_path)
+    assert r.returncode == 1 and "gfx950" in r.stderr
