@@ -67,6 +67,8 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
     B_count = sum(algorithmic_count_bytes(*g)[0] for g in graphs)
     b = R.Batch(graphs)
     b.decompose(eps)  # warmup (plans, triangle lists)
+    params, tbest, tstart = b.autotune(repeats=2)
+    b.decompose(eps)
     tms = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -101,6 +103,8 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
                             f"{pm} edges in {pt:.3f} s",
             "speedup": round((m / bt) / (pm / pt), 2),
             "per_graph_identical": f"{same}/{len(sample)} sampled graphs equal alone vs in batch",
+            "autotune": {"params": params, "count_ms_default": round(tstart, 3),
+                         "count_ms_tuned": round(tbest, 3)},
             "phases_ms": {k: round(float(np.mean([t[k] for t in tms])), 3)
                           for k in ("count_ms", "clean_ms", "extract_ms")},
             "count_roofline": {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -231,6 +235,8 @@ def main() -> int:
     ap.add_argument("--eps", type=float, default=0.3)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the count-plan autotuner (default parameters)")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="C4 batched-vs-plain leg (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
@@ -254,6 +260,12 @@ def main() -> int:
     B_count, s_min = algorithmic_count_bytes(n, u, v)
 
     g = R.DeviceGraph.from_edges(n, u, v)  # resident in HBM before timing
+    g.decompose(args.eps)  # count plan and term buffer
+    tuned = {}
+    if not args.no_tune:
+        params, best, start = g.autotune(repeats=2)
+        tuned = {"params": params, "count_ms_default": round(start, 3),
+                 "count_ms_tuned": round(best, 3)}
     for _ in range(args.warmup):
         g.decompose(args.eps)
     phases = []
@@ -309,6 +321,7 @@ def main() -> int:
                      "traffic": traffic, "kernel": "count phase (k_count_edges + "
                      "k_count_vertices)", "algorithmic_bytes": B_count, "s_min": s_min},
         "audit": audit,
+        "autotune": tuned,
         "with_upload_and_build": {"value": round(m / (ms_build / 1e3) / 1e6, 3),
                                   "ms_per_step": round(ms_build, 3),
                                   "note": "host edge list -> device CSR build -> "

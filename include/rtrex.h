@@ -176,6 +176,36 @@ typedef struct {
 rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 
 /* ------------------------------------------------------------------------ */
+/* Count-plan parameters and the autotuner (no reference counterpart).        */
+/* An edge whose longer-list endpoint (its hub) has degree in                 */
+/* (hub_min, hub_max] is counted per hub with the hub's list staged in LDS:   */
+/* a sorted list of up to 64 entries (degree <= l0_max), of 512 (<= l1_max),  */
+/* or a hash table (above, hub_max <= 4096); other edges are counted per edge. */
+/* A hub's edges get 4, 8, 16, 32 or 64 lanes each by their mean shorter-list */
+/* length against g_thr[0..3].  Units of work hold at least unit_min streamed  */
+/* elements (hash class: unit_hash_mult * hub degree).  Results never depend   */
+/* on the parameters, only the time does.                                      */
+typedef struct {
+    int32_t hub_min, hub_max, l0_max, l1_max;
+    int32_t g_thr[4];
+    int32_t unit_min, unit_hash_mult;
+} rtrex_count_params;
+
+rtrex_status rtrex_get_count_params(rtrex_graph g, rtrex_count_params* out);
+
+/* Validated; the plan is rebuilt at the next count. */
+rtrex_status rtrex_set_count_params(rtrex_graph g, const rtrex_count_params* p);
+
+/* Times the count phase (device time of its kernels, best of `repeats`) over
+ * a coordinate search of the parameters above, starting from the current
+ * ones, and keeps the fastest setting on the handle.  *best receives it,
+ * *best_ms its time and *start_ms the starting setting's time (nullable).
+ * Leaves the handle counted (as after rtrex_count). */
+rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats,
+                                  rtrex_count_params* best, double* best_ms,
+                                  double* start_ms);
+
+/* ------------------------------------------------------------------------ */
 /* Batched decomposition (no reference counterpart: the reference runs one   */
 /* graph per process).  n_graphs independent graphs given as concatenated    */
 /* edge lists with LOCAL ids: graph i has vtx[i] vertices and edges          */

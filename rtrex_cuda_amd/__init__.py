@@ -45,7 +45,8 @@ EXPORTS = [
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
     "rtrex_server_stop", "rtrex_get_member_order", "rtrex_write_decomposition",
-    "rtrex_write_components", "rtrex_write_survivors",
+    "rtrex_write_components", "rtrex_write_survivors", "rtrex_get_count_params",
+    "rtrex_set_count_params", "rtrex_autotune_count",
 ]
 
 
@@ -71,6 +72,28 @@ class Timings(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+class CountParams(C.Structure):
+    _fields_ = [("hub_min", C.c_int32), ("hub_max", C.c_int32), ("l0_max", C.c_int32),
+                ("l1_max", C.c_int32), ("g_thr", C.c_int32 * 4), ("unit_min", C.c_int32),
+                ("unit_hash_mult", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "g_thr"}
+        d["g_thr"] = [int(x) for x in self.g_thr]
+        return d
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "CountParams":
+        p = cls()
+        for k, v in d.items():
+            if k == "g_thr":
+                for i, x in enumerate(v):
+                    p.g_thr[i] = int(x)
+            else:
+                setattr(p, k, int(v))
+        return p
 
 
 class ServerConfig(C.Structure):
@@ -133,6 +156,9 @@ def lib() -> C.CDLL:
             "rtrex_write_decomposition": [C.c_char_p, C.c_int64, P64, P64, P64, P64, C.c_int64],
             "rtrex_write_components": [C.c_char_p, C.c_int64, P64, P64],
             "rtrex_write_survivors": [C.c_char_p, C.c_int64, P64, P64, PU8],
+            "rtrex_get_count_params": [G, C.POINTER(CountParams)],
+            "rtrex_set_count_params": [G, C.POINTER(CountParams)],
+            "rtrex_autotune_count": [G, C.c_int, C.POINTER(CountParams), PD, PD],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -273,6 +299,28 @@ class DeviceGraph:
         k = C.c_int64()
         _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64), C.byref(k)))
         return cm, int(k.value)
+
+    def count_params(self) -> dict:
+        p = CountParams()
+        _check(lib().rtrex_get_count_params(self._h, C.byref(p)))
+        return p.as_dict()
+
+    def set_count_params(self, params: dict) -> None:
+        p = CountParams()
+        _check(lib().rtrex_get_count_params(self._h, C.byref(p)))
+        cur = p.as_dict()
+        cur.update(params)
+        _check(lib().rtrex_set_count_params(self._h, C.byref(CountParams.from_dict(cur))))
+
+    def autotune(self, repeats: int = 2):
+        """Coordinate search of the count-plan parameters; keeps the fastest.
+        Returns (params, best_ms, start_ms)."""
+        p = CountParams()
+        b = C.c_double()
+        s0 = C.c_double()
+        _check(lib().rtrex_autotune_count(self._h, int(repeats), C.byref(p), C.byref(b),
+                                          C.byref(s0)))
+        return p.as_dict(), float(b.value), float(s0.value)
 
     def member_order(self) -> np.ndarray:
         """Position at which each vertex joined its cluster (last extraction)."""

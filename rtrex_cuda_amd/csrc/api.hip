@@ -466,6 +466,138 @@ rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
     return RTREX_OK;
 }
 
+namespace {
+rtrex_count_params to_abi(const CountParams& c) {
+    rtrex_count_params p;
+    p.hub_min = (int32_t)c.hub_min;
+    p.hub_max = (int32_t)c.hub_max;
+    p.l0_max = (int32_t)c.l0_max;
+    p.l1_max = (int32_t)c.l1_max;
+    for (int i = 0; i < 4; ++i) p.g_thr[i] = (int32_t)c.g_thr[i];
+    p.unit_min = (int32_t)c.unit_min;
+    p.unit_hash_mult = (int32_t)c.unit_hash_mult;
+    return p;
+}
+
+rtrex_status from_abi(const rtrex_count_params& p, CountParams* c) {
+    if (p.hub_min < 0 || p.hub_min >= p.hub_max || p.hub_max > 4096)
+        return fail(RTREX_EINVAL, "need 0 <= hub_min < hub_max <= 4096");
+    if (p.l0_max < 0 || p.l0_max > 64 || p.l1_max < p.l0_max || p.l1_max > 512)
+        return fail(RTREX_EINVAL, "need 0 <= l0_max <= 64 and l0_max <= l1_max <= 512");
+    for (int i = 0; i < 4; ++i)
+        if (p.g_thr[i] < 0 || (i && p.g_thr[i] < p.g_thr[i - 1]))
+            return fail(RTREX_EINVAL, "g_thr must be non-negative and non-decreasing");
+    if (p.unit_min < 1 || p.unit_hash_mult < 1)
+        return fail(RTREX_EINVAL, "unit_min and unit_hash_mult must be >= 1");
+    c->hub_min = (uint32_t)p.hub_min;
+    c->hub_max = (uint32_t)p.hub_max;
+    c->l0_max = (uint32_t)p.l0_max;
+    c->l1_max = (uint32_t)p.l1_max;
+    for (int i = 0; i < 4; ++i) c->g_thr[i] = (uint32_t)p.g_thr[i];
+    c->unit_min = (uint32_t)p.unit_min;
+    c->unit_hash_mult = (uint32_t)p.unit_hash_mult;
+    return RTREX_OK;
+}
+
+// Best (smallest) count-phase device time of `repeats` runs with the
+// handle's current parameters.
+rtrex_status time_count(rtrex_graph g, int repeats, double* ms) {
+    double best = 1e300;
+    for (int r = 0; r < repeats; ++r) {
+        RT_TRY(run_count(g));
+        best = std::min(best, (double)g->tm.count_edge_ms);
+    }
+    *ms = best;
+    return RTREX_OK;
+}
+}  // namespace
+
+rtrex_status rtrex_get_count_params(rtrex_graph g, rtrex_count_params* out) {
+    RT_TRY(check(g));
+    if (!out) return fail(RTREX_EINVAL, "null output");
+    *out = to_abi(g->cparams);
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_set_count_params(rtrex_graph g, const rtrex_count_params* p) {
+    RT_TRY(check(g));
+    if (!p) return fail(RTREX_EINVAL, "null parameters");
+    CountParams c;
+    RT_TRY(from_abi(*p, &c));
+    g->cparams = c;
+    g->plan_valid = false;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params* best,
+                                  double* best_ms, double* start_ms) {
+    RT_TRY(check(g));
+    ArenaScope scope_(g);
+    if (g->phase < PH_BUILT) return fail(RTREX_ESTATE, "graph not built");
+    if (repeats < 1) repeats = 1;
+    CountParams cur = g->cparams;
+    g->plan_valid = false;
+    double t_cur = 0;
+    RT_TRY(time_count(g, repeats, &t_cur));
+    const double t_start = t_cur;
+    auto trial = [&](const CountParams& c, double* t) -> rtrex_status {
+        g->cparams = c;
+        g->plan_valid = false;
+        return time_count(g, repeats, t);
+    };
+    // Coordinate search: each parameter over its candidates, the others held.
+    for (int coord = 0; coord < 6; ++coord) {
+        std::vector<CountParams> cand;
+        CountParams c = cur;
+        switch (coord) {
+            case 0:
+                for (uint32_t v : {8u, 16u, 32u, 64u}) { c.hub_min = v; cand.push_back(c); }
+                break;
+            case 1:
+                for (auto lv : {std::pair<uint32_t, uint32_t>{64, 512}, {32, 512}, {64, 256},
+                                {32, 256}}) {
+                    c.l0_max = lv.first;
+                    c.l1_max = lv.second;
+                    cand.push_back(c);
+                }
+                break;
+            case 2:
+                for (uint32_t num : {1u, 2u, 4u}) {  // thresholds x 0.5, 1, 2
+                    const uint32_t base[4] = {6, 12, 48, 160};
+                    for (int i = 0; i < 4; ++i) c.g_thr[i] = base[i] * num / 2;
+                    cand.push_back(c);
+                }
+                break;
+            case 3:
+                for (uint32_t v : {512u, 1024u, 4096u}) { c.unit_min = v; cand.push_back(c); }
+                break;
+            case 4:
+                for (uint32_t v : {4u, 8u, 16u}) { c.unit_hash_mult = v; cand.push_back(c); }
+                break;
+            default:
+                for (uint32_t v : {1024u, 2048u, 4096u}) {
+                    c.hub_max = v;
+                    if (c.hub_min < v) cand.push_back(c);
+                }
+                break;
+        }
+        for (const CountParams& k : cand) {
+            double t = 0;
+            RT_TRY(trial(k, &t));
+            if (t < t_cur) {
+                t_cur = t;
+                cur = k;
+            }
+        }
+    }
+    g->cparams = cur;
+    g->plan_valid = false;
+    if (best) *best = to_abi(cur);
+    if (best_ms) *best_ms = t_cur;
+    if (start_ms) *start_ms = t_start;
+    return RTREX_OK;
+}
+
 rtrex_status rtrex_get_member_order(rtrex_graph g, int64_t* rank) {
     RT_TRY(check(g));
     if (g->phase < PH_EXTRACTED || !g->mrank)

@@ -41,22 +41,24 @@ __device__ __forceinline__ int bucket_of(uint32_t w) {
     return w <= 4u ? 0 : w <= 16u ? 1 : w <= 96u ? 2 : 3;
 }
 
-// Hub classes.  Edge {x, y} streams the list of lo = (d_x <= d_y ? x : y)
-// (x < y) and probes the list of the other endpoint, its hub.  Edges whose
-// hub degree is in (HUB_MIN, HUB_MAX] are counted per hub (k_count_hub), the
-// hub's list staged once in LDS; the rest per edge (k_count_edges).
-constexpr uint32_t HUB_MIN = 16, HUB_MAX = 4096;
-constexpr int HUB_NL = 3, HUB_NG = 5;  // staging sizes 64/512/4096; G = 4..64
-__host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub) {
-    return dhub > HUB_MIN && dhub <= HUB_MAX;
+// Hub classes (CountParams).  Edge {x, y} streams the list of
+// lo = (d_x <= d_y ? x : y) (x < y) and probes the list of the other
+// endpoint, its hub.  Edges whose hub degree is in (hub_min, hub_max] are
+// counted per hub (k_count_hub*), the hub's list staged once in LDS; the rest
+// per edge (k_count_edges).
+constexpr int HUB_NL = 3, HUB_NG = 5;  // staging classes 64/512/hash; G = 4..64
+__host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
+    return dhub > P.hub_min && dhub <= P.hub_max;
 }
-__host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub) {
-    return dhub <= 64 ? 0 : dhub <= 512 ? 1 : 2;
+__host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub, const CountParams& P) {
+    return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : 2;
 }
 // Lanes per edge from the mean shorter-list length of the hub's edges.
-__host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges) {
+__host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges,
+                                                   const CountParams& P) {
     const uint64_t mean = (work + edges - 1) / edges;
-    return mean <= 6 ? 0 : mean <= 12 ? 1 : mean <= 48 ? 2 : mean <= 160 ? 3 : 4;
+    return mean <= P.g_thr[0] ? 0 : mean <= P.g_thr[1] ? 1 : mean <= P.g_thr[2] ? 2
+           : mean <= P.g_thr[3] ? 3 : 4;
 }
 
 // Appends ids to per-bucket lists: work(e) = length of the shorter list for
@@ -64,7 +66,7 @@ __host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges
 template <bool EDGES>
 __global__ void __launch_bounds__(T) k_bucket(GraphDev g, uint32_t count,
                                               uint32_t* __restrict__ lists,
-                                              uint32_t* __restrict__ cursor) {
+                                              uint32_t* __restrict__ cursor, CountParams P) {
     __shared__ uint32_t cnt[NB], base[NB];
     if (threadIdx.x < NB) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -76,7 +78,7 @@ __global__ void __launch_bounds__(T) k_bucket(GraphDev g, uint32_t count,
         if (EDGES) {
             uint32_t x = g.esrc[i], y = g.nb[g.eslot[i]];
             w = min(g.deg[x], g.deg[y]);
-            if (hub_edge(max(g.deg[x], g.deg[y]))) w = 0xffffffffu;  // k_count_hub
+            if (hub_edge(max(g.deg[x], g.deg[y]), P)) w = 0xffffffffu;  // k_count_hub
         } else {
             w = g.deg[i];
         }
@@ -254,11 +256,11 @@ __global__ void __launch_bounds__(T) k_sum_final(const unsigned long long* __res
 // Ownership flags: the hub-side slot of every hub-path edge; hwork[h] = the
 // streamed work of hub h (sum of its owned edges' shorter lists, <= 4096^2).
 __global__ void __launch_bounds__(T) k_own_flags(GraphDev g, uint32_t* __restrict__ flag,
-                                                 uint32_t* __restrict__ hwork) {
+                                                 uint32_t* __restrict__ hwork, CountParams P) {
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
         const uint32_t sx = g.eslot[e], x = g.esrc[e], y = g.nb[sx];
         const uint32_t dx = g.deg[x], dy = g.deg[y];
-        if (!hub_edge(max(dx, dy))) continue;
+        if (!hub_edge(max(dx, dy), P)) continue;
         flag[dx <= dy ? g.part[sx] : sx] = 1u;  // hub y (slot in N(y)) or hub x
         atomicAdd(&hwork[dx <= dy ? y : x], min(dx, dy));
     }
@@ -268,11 +270,11 @@ __global__ void __launch_bounds__(T) k_own_flags(GraphDev g, uint32_t* __restric
 // own_off[h] = pos[off[h]].
 __global__ void __launch_bounds__(T) k_own_scatter(GraphDev g, const uint32_t* __restrict__ pos,
                                                    uint32_t* __restrict__ own_off,
-                                                   uint32_t* __restrict__ own_idx) {
+                                                   uint32_t* __restrict__ own_idx, CountParams P) {
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
         const uint32_t sx = g.eslot[e], x = g.esrc[e], y = g.nb[sx];
         const uint32_t dx = g.deg[x], dy = g.deg[y];
-        if (!hub_edge(max(dx, dy))) continue;
+        if (!hub_edge(max(dx, dy), P)) continue;
         const uint32_t h = dx <= dy ? y : x;
         const uint32_t j = dx <= dy ? g.part[sx] : sx;
         own_idx[pos[j]] = j - g.off[h];
@@ -290,7 +292,8 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
                                                  const uint32_t* __restrict__ hwork,
                                                  uint32_t* __restrict__ cls_cnt,
                                                  uint64_t* __restrict__ units,
-                                                 const uint32_t* __restrict__ cls_base) {
+                                                 const uint32_t* __restrict__ cls_base,
+                                                 CountParams Pm) {
     // One thread per hub; class totals aggregated per block (LDS counters,
     // one global atomic per class per block).
     constexpr int NC = HUB_NL * HUB_NG;
@@ -306,14 +309,16 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         if (oc) {
             const uint32_t dh = g.deg[h];
             const uint64_t work = hwork[h];
-            // Units of at least max(1024, d_h) elements; the hash class
-            // (d_h > 512) rebuilds a table per unit: 8 d_h elements, but at
-            // most 32 owned edges apart so that hubs still spread out.
-            const uint64_t uw = dh > 512 ? 8ull * dh : max<uint64_t>(1024, dh);
+            // Units of at least max(unit_min, d_h) elements; the hash class
+            // rebuilds a table per unit: unit_hash_mult * d_h elements, but
+            // at most 32 owned edges apart so that hubs still spread out.
+            const int lc_ = hub_lclass(dh, Pm);
+            const uint64_t uw = lc_ == 2 ? (uint64_t)Pm.unit_hash_mult * dh
+                                         : max<uint64_t>(Pm.unit_min, dh);
             P = (work + uw - 1) / uw;
-            if (dh > 512) P = max<uint64_t>(P, (oc + 31) / 32);
+            if (lc_ == 2) P = max<uint64_t>(P, (oc + 31) / 32);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
-            c = hub_lclass(dh) * HUB_NG + hub_gclass(work, oc);
+            c = lc_ * HUB_NG + hub_gclass(work, oc, Pm);
             at = atomicAdd(&lc[c], (uint32_t)P);
         }
     }
@@ -817,8 +822,11 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
     }
 }
 
-// Hub plan (static per graph): ownership lists and work units.
+// Hub plan (per graph and parameters): ownership lists and work units.  The
+// arrays are sized for any parameters (own_idx, hub_units: at most m
+// entries, every unit has an owned edge) and refilled when they change.
 rtrex_status prepare_hubs(rtrex_graph h) {
+    const CountParams P = h->cparams;
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
@@ -831,19 +839,22 @@ rtrex_status prepare_hubs(rtrex_graph h) {
     RT_TRY(sc.get(&cbase, HUB_NL * HUB_NG));
     RT_HIP(hipMemsetAsync(flag, 0, g.nnz * 4, s));
     const unsigned be = blocks_for(g.m, T, 8192);
-    if (g.m) k_own_flags<<<be, T, 0, s>>>(g, flag, hwork);
+    if (g.m) k_own_flags<<<be, T, 0, s>>>(g, flag, hwork, P);
     RT_TRY(exclusive_scan_u32_32(flag, pos, g.nnz, s));
     uint32_t nown = 0;
     RT_HIP(hipMemcpyAsync(&nown, pos + g.nnz, 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    RT_TRY(dalloc(h, &h->own_off, (uint64_t)g.n + 1));
-    RT_TRY(dalloc(h, &h->own_idx, std::max<uint32_t>(nown, 1)));
+    if (!h->own_off) {
+        RT_TRY(dalloc(h, &h->own_off, (uint64_t)g.n + 1));
+        RT_TRY(dalloc(h, &h->own_idx, std::max<uint64_t>(g.m, 1)));
+        RT_TRY(dalloc(h, &h->hub_units, std::max<uint64_t>(g.m, 1)));
+    }
     k_own_scatter<<<blocks_for(std::max<uint64_t>(g.m, g.n + 1ull), T, 8192), T, 0, s>>>(
-        g, pos, h->own_off, h->own_idx);
+        g, pos, h->own_off, h->own_idx, P);
     RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
     const unsigned bv = (unsigned)std::max<uint64_t>(1, ((uint64_t)g.n + T - 1) / T);  // one thread per hub
     if (nown)
-        k_hub_units<true><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, nullptr, nullptr);
+        k_hub_units<true><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, nullptr, nullptr, P);
     RT_HIP(hipGetLastError());
     RT_HIP(hipMemcpyAsync(h->hub_ucnt, ccnt, sizeof(h->hub_ucnt), hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
@@ -853,11 +864,10 @@ rtrex_status prepare_hubs(rtrex_graph h) {
         base[c] = (uint32_t)total;
         total += h->hub_ucnt[c];
     }
-    RT_TRY(dalloc(h, &h->hub_units, std::max<uint64_t>(total, 1)));
     if (total) {
         RT_HIP(hipMemcpyAsync(cbase, base, sizeof(base), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
-        k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase);
+        k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase, P);
         RT_HIP(hipGetLastError());
     }
     RT_HIP(hipStreamSynchronize(s));
@@ -899,22 +909,27 @@ bool try_alloc_terms(rtrex_graph h, uint64_t total) {
     return false;
 }
 
-// Work buckets and the term buffer depend only on the graph: built on the
-// first count and kept with the handle.
+// The count plan: work buckets and hub units (rebuilt when the parameters
+// change) and the term buffer (depends on the graph only, built once).
 rtrex_status prepare_count(rtrex_graph h) {
-    if (h->cnt_lists) return RTREX_OK;
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
-    RT_TRY(dalloc(h, &h->cnt_lists, (uint64_t)NB * (g.m + g.n) + 2 * NB));
-    uint32_t* elist = h->cnt_lists;
-    uint32_t* vlist = elist + (uint64_t)NB * g.m;
-    uint32_t* cur = vlist + (uint64_t)NB * g.n;
-    RT_HIP(hipMemsetAsync(cur, 0, 2 * NB * sizeof(uint32_t), s));
-    if (g.m) k_bucket<true><<<blocks_for(g.m, T), T, 0, s>>>(g, g.m, elist, cur);
-    if (g.n) k_bucket<false><<<blocks_for(g.n, T), T, 0, s>>>(g, g.n, vlist, cur + NB);
-    RT_HIP(hipGetLastError());
-    RT_HIP(hipMemcpyAsync(h->cnt_counts, cur, sizeof(h->cnt_counts), hipMemcpyDeviceToHost, s));
-    RT_TRY(prepare_hubs(h));
+    if (!h->plan_valid) {
+        if (!h->cnt_lists)
+            RT_TRY(dalloc(h, &h->cnt_lists, (uint64_t)NB * (g.m + g.n) + 2 * NB));
+        uint32_t* elist = h->cnt_lists;
+        uint32_t* vlist = elist + (uint64_t)NB * g.m;
+        uint32_t* cur = vlist + (uint64_t)NB * g.n;
+        RT_HIP(hipMemsetAsync(cur, 0, 2 * NB * sizeof(uint32_t), s));
+        if (g.m) k_bucket<true><<<blocks_for(g.m, T), T, 0, s>>>(g, g.m, elist, cur, h->cparams);
+        if (g.n)
+            k_bucket<false><<<blocks_for(g.n, T), T, 0, s>>>(g, g.n, vlist, cur + NB, h->cparams);
+        RT_HIP(hipGetLastError());
+        RT_HIP(hipMemcpyAsync(h->cnt_counts, cur, sizeof(h->cnt_counts), hipMemcpyDeviceToHost, s));
+        RT_TRY(prepare_hubs(h));
+        h->plan_valid = true;
+    }
+    if (h->zoff) return RTREX_OK;
     RT_TRY(dalloc(h, &h->red, RED_BLOCKS));
     // term buffer: capacity min(d_x, d_y) - 1 per edge
     RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));

@@ -105,6 +105,20 @@ struct AuditHost {
     int64_t components = 0, largest_component = 0;
 };
 
+// Count-plan parameters (the autotuner's search space).  An edge whose
+// longer-list endpoint (its hub) has degree in (hub_min, hub_max] is counted
+// per hub with the hub list staged in LDS: as a sorted list of 64 entries
+// (degree <= l0_max), of 512 (<= l1_max), or in a hash table (above); the rest
+// per edge.  A hub's edges get G = 4, 8, 16, 32 or 64 lanes each by their
+// mean shorter-list length against g_thr[0..3].  Work units carry at least
+// unit_min streamed elements (hash class: unit_hash_mult * d_h).
+struct CountParams {
+    uint32_t hub_min = 16, hub_max = 4096;
+    uint32_t l0_max = 64, l1_max = 512;
+    uint32_t g_thr[4] = {6, 12, 48, 160};
+    uint32_t unit_min = 1024, unit_hash_mult = 8;
+};
+
 // Per-handle scratch arena (see Scratch below).
 struct Arena {
     char* base = nullptr;
@@ -152,6 +166,8 @@ struct rtrex_graph_s {
     uint64_t* hub_units = nullptr;
     uint32_t hub_ucnt[15] = {};
     unsigned long long* red = nullptr;  // per-block partial sums (count phase)
+    rtrex::CountParams cparams;
+    bool plan_valid = false;  // buckets and hub units built for cparams
     uint64_t* zoff = nullptr;
     uint32_t *za = nullptr, *zb = nullptr;
     double* zw = nullptr;
