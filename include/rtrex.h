@@ -154,7 +154,11 @@ rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
 /* Decision audit.  Every threshold or ratio decision whose margin is within
  * the FP64 error bound of the reference's own summation is counted as
  * ambiguous; with zero ambiguous decisions the survivor and cluster sets are
- * provably those of the sequential reference. */
+ * provably those of the sequential reference.  A cleaning decision that the
+ * bound cannot settle makes the clean re-run the triangle-connected regions
+ * holding such edges in the reference's own stack order (exact FP64, so those
+ * decisions are the reference's): they are counted in replayed_edges, not in
+ * ambiguous_clean. */
 typedef struct {
     int64_t clean_rounds;         /* frontier rounds, all clean calls        */
     int64_t edges_removed;        /* undirected edges deleted by cleaning    */
@@ -164,6 +168,8 @@ typedef struct {
     int64_t seeds;                /* clusters formed                          */
     int64_t components;           /* alive components at extraction start    */
     int64_t largest_component;    /* vertices in the largest one             */
+    int64_t replayed_edges;       /* edges of the regions the clean re-ran   */
+                                  /* in the reference's exact stack order    */
 } rtrex_audit;
 rtrex_status rtrex_get_audit(rtrex_graph g, rtrex_audit* out);
 

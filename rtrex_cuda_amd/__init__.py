@@ -59,7 +59,7 @@ class RtrexError(RuntimeError):
 class Audit(C.Structure):
     _fields_ = [(k, C.c_int64) for k in (
         "clean_rounds", "edges_removed", "triangles_removed", "ambiguous_clean",
-        "ambiguous_ratio", "seeds", "components", "largest_component")]
+        "ambiguous_ratio", "seeds", "components", "largest_component", "replayed_edges")]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

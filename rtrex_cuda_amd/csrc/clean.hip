@@ -150,7 +150,18 @@ struct PeelArgs {
     uint32_t stamp;  // this round's near-list stamp
     Counters* ctr;
     TermBuf tb;
+    uint32_t* undec;     // edges no bound could decide (initial clean), or null
+    uint32_t undec_cap;  // capacity; ctr->n_undecided counts every attempt
 };
+
+// Records an edge whose decision no error bound settles (for the exact
+// replay); past the capacity only the count grows.
+__device__ __forceinline__ void note_undecided(uint32_t* list, uint32_t cap, Counters* ctr,
+                                               uint32_t e) {
+    if (!list) return;
+    const unsigned i = atomicAdd(&ctr->n_undecided, 1u);
+    if (i < cap) list[i] = e;
+}
 
 // Shared tail of both peel kernels for one owned triangle {x, y, z}: the two
 // other edges f (slot sf_slot, status sf) lose t; an alive one that falls
@@ -428,6 +439,7 @@ __global__ void __launch_bounds__(T) k_near_check(PeelArgs A, uint32_t nNear) {
         int c = certify_edge(g, A.tb, f, thr, true, 0xffffffffu, 0xffffffffu);
         if (c == CERT_UNDECIDED) {
             amb++;
+            note_undecided(A.undec, A.undec_cap, A.ctr, f);
             c = g.sup[f] < thr ? CERT_REMOVE : CERT_KEEP;
         }
         if (c == CERT_REMOVE) claim_edge(g, f, A.Fn, A.nFn);
@@ -458,14 +470,18 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
 // After convergence, a safety net: every alive edge that lost triangles and
 // is not clearly above its threshold is certified once more.
 __global__ void __launch_bounds__(T) k_audit_survivors(GraphDev g, TermBuf tb,
-                                                       double eps, Counters* ctr) {
+                                                       double eps, Counters* ctr,
+                                                       uint32_t* undec, uint32_t undec_cap) {
     unsigned long long amb = 0;
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m;
          e += gridDim.x * T) {
         if (!g.touched[e] || g.est[e] != ST_ALIVE) continue;
         const double thr = eps * g.ew[e];
         if (g.sup[e] >= thr + cheap_bound(g.tcnt[e], g.sup0[e])) continue;
-        if (certify_edge(g, tb, e, thr, false, 0xffffffffu, 0xffffffffu) != CERT_KEEP) amb++;
+        if (certify_edge(g, tb, e, thr, false, 0xffffffffu, 0xffffffffu) != CERT_KEEP) {
+            amb++;
+            note_undecided(undec, undec_cap, ctr, e);
+        }
     }
     for (int d = 32; d >= 1; d >>= 1) amb += __shfl_xor(amb, d, 64);
     if (lane_id() == 0 && amb) atomicAdd(&ctr->ambiguous_clean, amb);
@@ -484,6 +500,304 @@ __global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits)
     }
 }
 
+// ------------------------------------------------------------ exact replay
+// When a cleaning decision is undecided (its value lies within the error
+// bound of the reference's own subtraction order), the clean re-runs, from
+// the count-phase supports, every triangle-connected region holding such an
+// edge in the reference's own order: the initial pushes of the region's
+// below-threshold edges in ascending edge id (initialClean,
+// Decomposition.h:378-405), then LIFO pops (deleteAndClean,
+// Decomposition.h:147-283) that visit the popped edge's triangles in
+// ascending third vertex, subtract the weight from both other edges and push
+// (lower, w) before (higher, w), lower being the popped pair's endpoint of
+// smaller degree, its second on a tie.  Edges of different regions share no
+// triangle, so a region's subtraction sequence -- the global one restricted
+// to it -- and thus its FP64 values and decisions are the reference's.  One
+// wave per region; the terms of a pop are handled 64 at a time (they touch
+// distinct edges), pushes placed in the reference's order.
+__device__ __forceinline__ uint32_t rg_root(uint32_t* parent, uint32_t x) {
+    uint32_t p = parent[x];
+    while (p != x) {
+        const uint32_t gp = parent[p];
+        if (gp != p) parent[x] = gp;
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void rg_union(uint32_t* parent, uint32_t a, uint32_t b) {
+    a = rg_root(parent, a);
+    b = rg_root(parent, b);
+    while (a != b) {
+        if (a > b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        const uint32_t old = atomicCAS(&parent[b], b, a);
+        if (old == b) return;
+        b = rg_root(parent, old);
+        a = rg_root(parent, a);
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rg_init(uint32_t m, uint32_t* parent) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) parent[e] = e;
+}
+
+// Unions the three edges of every triangle (once, from its smallest edge id).
+__global__ void __launch_bounds__(T) k_rg_hook(GraphDev g, TermBuf tb, uint32_t* parent) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+        for (uint64_t k = k0; k < k1; ++k) {
+            const uint32_t a = tb.sa[k], b = tb.sb[k];
+            if (e < a && e < b) {
+                rg_union(parent, e, a);
+                rg_union(parent, e, b);
+            }
+        }
+    }
+}
+
+// parent[] := root, writing roots only (no path-halving races).
+__global__ void __launch_bounds__(T) k_rg_compress(uint32_t m, uint32_t* parent) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) {
+        uint32_t r = e, p = parent[e];
+        while (p != r) {
+            r = p;
+            p = parent[r];
+        }
+        parent[e] = r;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rg_flag(const uint32_t* __restrict__ undec, uint32_t n,
+                                               const uint32_t* __restrict__ parent,
+                                               uint8_t* __restrict__ flag, bool all, uint32_t m) {
+    if (all) {
+        for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) flag[e] = 1;
+        return;
+    }
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < n; i += gridDim.x * T)
+        flag[parent[undec[i]]] = 1;
+}
+
+// Keys (root, edge id) of the edges of flagged regions; region edges are
+// reset to their state before the clean.
+__global__ void __launch_bounds__(T) k_rg_keys(GraphDev g, const uint32_t* __restrict__ parent,
+                                               const uint8_t* __restrict__ flag, int Bi,
+                                               uint64_t* __restrict__ keys, unsigned* nk) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        const uint32_t r = parent[e];
+        if (!flag[r]) continue;
+        keys[wave_append(nk)] = ((uint64_t)r << Bi) | e;
+        set_status(g, e, ST_ALIVE);
+        g.sup[e] = g.sup0[e];
+        g.touched[e] = 0;
+        g.claim[e] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rg_layout(const uint64_t* __restrict__ keys, uint32_t nk,
+                                                 int Bi, uint32_t* __restrict__ redge,
+                                                 uint32_t* __restrict__ rflag) {
+    const uint64_t mask = (1ull << Bi) - 1;
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nk; i += gridDim.x * T) {
+        redge[i] = (uint32_t)(keys[i] & mask);
+        rflag[i] = (i == 0 || (keys[i] >> Bi) != (keys[i - 1] >> Bi)) ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rg_starts(const uint32_t* __restrict__ rflag,
+                                                 const uint32_t* __restrict__ rpos, uint32_t nk,
+                                                 uint32_t* __restrict__ rstart) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nk; i += gridDim.x * T)
+        if (rflag[i]) rstart[rpos[i]] = i;
+    if (blockIdx.x == 0 && threadIdx.x == 0) rstart[rpos[nk]] = nk;
+}
+
+__device__ __forceinline__ uint64_t lanes_lt() {
+    const int l = lane_id();
+    return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+struct ReplayArgs {
+    GraphDev g;
+    TermBuf tb;
+    double eps;
+    const uint32_t* redge;   // region edges, ascending id within a region
+    const uint32_t* rstart;  // nreg + 1 boundaries
+    uint32_t nreg;
+    uint64_t* stk;           // per region: (edge << 32) | first endpoint of the pair
+    unsigned* next;          // work queue head
+};
+
+constexpr int RT = 64;  // one wave per workgroup
+__global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    for (;;) {
+        uint32_t r = 0;
+        if (lane == 0) r = atomicAdd(A.next, 1u);
+        r = __shfl(r, 0, 64);
+        if (r >= A.nreg) break;
+        const uint32_t b0 = A.rstart[r], b1 = A.rstart[r + 1];
+        uint64_t* stk = A.stk + b0;
+        uint32_t top = 0;
+        // initial pushes: (u, v), u < v, in ascending edge id
+        for (uint32_t p = b0; p < b1; p += 64) {
+            const uint32_t i = p + lane;
+            const uint32_t e = i < b1 ? A.redge[i] : 0;
+            const bool push = i < b1 && g.sup0[e] < A.eps * g.ew[e];
+            const uint64_t bal = ballot64(push);
+            if (push) {
+                stk[top + __popcll(bal & lanes_lt())] = ((uint64_t)e << 32) | g.esrc[e];
+                set_status(g, e, ST_QUEUED);
+            }
+            top += (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        while (top > 0) {
+            --top;
+            const uint64_t ent = stk[top];
+            const uint32_t e = (uint32_t)(ent >> 32), a = (uint32_t)ent;
+            const uint32_t x = g.esrc[e], y = g.nb[g.eslot[e]];
+            const uint32_t b = a == x ? y : x;
+            const uint32_t da = g.deg[a], db = g.deg[b];
+            const uint32_t lower = da < db ? a : b, higher = da < db ? b : a;
+            // stored terms: sa = {lo, z}, sb = {hi, z}, lo = d_x <= d_y ? x : y
+            const uint32_t lo = g.deg[x] <= g.deg[y] ? x : y;
+            const bool swap = lower != lo;
+            const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+            for (uint64_t p = k0; p < k1; p += 64) {
+                const uint64_t k = p + lane;
+                const bool in = k < k1;
+                const uint32_t fa = in ? tb.sa[k] : 0, fb = in ? tb.sb[k] : 0;
+                const uint32_t jl = swap ? fb : fa, jh = swap ? fa : fb;
+                const uint8_t sl = in ? g.est[jl] : (uint8_t)ST_DEAD;
+                const uint8_t sh = in ? g.est[jh] : (uint8_t)ST_DEAD;
+                const bool alive = in && sl != ST_DEAD && sh != ST_DEAD;
+                bool pl = false, ph = false;
+                if (alive) {
+                    const double t = tb.w[k];
+                    const double vl = g.sup[jl] - t, vh = g.sup[jh] - t;
+                    g.sup[jl] = vl;
+                    g.sup[jh] = vh;
+                    g.touched[jl] = 1;
+                    g.touched[jh] = 1;
+                    pl = sl != ST_QUEUED && vl < A.eps * g.ew[jl];
+                    ph = sh != ST_QUEUED && vh < A.eps * g.ew[jh];
+                }
+                const uint32_t np = (uint32_t)pl + (uint32_t)ph;
+                uint32_t pre = np;  // exclusive prefix of pushes over lanes
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t o = __shfl_up(pre, d, 64);
+                    if (lane >= d) pre += o;
+                }
+                const uint32_t tot = __shfl(pre, 63, 64);
+                pre -= np;
+                if (pl) {
+                    stk[top + pre] = ((uint64_t)jl << 32) | lower;
+                    set_status(g, jl, ST_QUEUED);
+                }
+                if (ph) {
+                    stk[top + pre + (uint32_t)pl] = ((uint64_t)jh << 32) | higher;
+                    set_status(g, jh, ST_QUEUED);
+                }
+                top += tot;
+                __syncthreads();
+            }
+            if (lane == 0) {
+                set_status(g, e, ST_DEAD);
+                g.sup[e] = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_count_alive(GraphDev g, unsigned long long* out) {
+    unsigned long long c = 0;
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T)
+        c += g.est[e] == ST_ALIVE;
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane_id() == 0 && c) atomicAdd(out, c);
+}
+
+int bits_of(uint64_t n) {
+    int b = 1;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+// Re-runs the regions of the undecided edges exactly (see above).
+rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint32_t cap) {
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    unsigned nund = 0;
+    RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (!nund || !h->zw) return RTREX_OK;
+    Scratch sc;
+    uint32_t *parent, *redge, *rflag, *rpos, *rstart;
+    uint64_t *keys, *tmp, *stk;
+    uint8_t* flag;
+    unsigned long long* cnt;
+    RT_TRY(sc.get(&parent, g.m));
+    RT_TRY(sc.get(&flag, g.m));
+    RT_TRY(sc.get(&cnt, 2));
+    const unsigned be = blocks_for(g.m, T, 8192);
+    k_rg_init<<<be, T, 0, s>>>(g.m, parent);
+    k_rg_hook<<<be, T, 0, s>>>(g, h->terms(), parent);
+    k_rg_compress<<<be, T, 0, s>>>(g.m, parent);
+    RT_HIP(hipMemsetAsync(flag, 0, g.m, s));
+    const bool all = nund > cap;
+    k_rg_flag<<<blocks_for(all ? g.m : nund, T, 4096), T, 0, s>>>(undec, nund, parent, flag, all,
+                                                                  g.m);
+    RT_HIP(hipMemsetAsync(cnt, 0, 16, s));
+    const int Bi = bits_of((uint64_t)g.m + 1);
+    RT_TRY(sc.get(&keys, g.m));
+    k_rg_keys<<<be, T, 0, s>>>(g, parent, flag, Bi, keys, (unsigned*)cnt);
+    RT_HIP(hipGetLastError());
+    uint32_t nk = 0;
+    RT_HIP(hipMemcpyAsync(&nk, cnt, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (nk) {
+        RT_TRY(sc.get(&tmp, nk));
+        RT_TRY(radix_sort_u64(keys, tmp, nk, 2 * Bi, s));
+        RT_TRY(sc.get(&redge, nk));
+        RT_TRY(sc.get(&rflag, nk));
+        RT_TRY(sc.get(&rpos, (uint64_t)nk + 1));
+        const unsigned bk = blocks_for(nk, T, 8192);
+        k_rg_layout<<<bk, T, 0, s>>>(keys, nk, Bi, redge, rflag);
+        RT_TRY(exclusive_scan_u32_32(rflag, rpos, nk, s));
+        uint32_t nreg = 0;
+        RT_HIP(hipMemcpyAsync(&nreg, rpos + nk, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        RT_TRY(sc.get(&rstart, (uint64_t)nreg + 1));
+        k_rg_starts<<<bk, T, 0, s>>>(rflag, rpos, nk, rstart);
+        RT_TRY(sc.get(&stk, nk));
+        RT_HIP(hipMemsetAsync(&h->ctr->work_next, 0, 4, s));
+        ReplayArgs A{g, h->terms(), eps, redge, rstart, nreg, stk, &h->ctr->work_next};
+        k_replay<<<std::min<uint32_t>(nreg, 8192), RT, 0, s>>>(A);
+        RT_HIP(hipGetLastError());
+    }
+    // Every undecided decision now follows the reference's order.
+    k_count_alive<<<be, T, 0, s>>>(g, cnt + 1);
+    unsigned long long alive = 0;
+    RT_HIP(hipMemcpyAsync(&alive, cnt + 1, 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    const unsigned long long removed = g.m - alive, replayed = nk, zero = 0;
+    RT_HIP(hipMemcpyAsync(&h->ctr->edges_removed, &removed, 8, hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(&h->ctr->replayed_edges, &replayed, 8, hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(&h->ctr->ambiguous_clean, &zero, 8, hipMemcpyHostToDevice, s));
+    RT_HIP(hipStreamSynchronize(s));
+    h->n_alive = alive;
+    return RTREX_OK;
+}
+
 // Blocks for `items` work items at G lanes each (capped; kernels stride).
 unsigned grid_items(uint32_t items, int G) {
     const uint64_t waves = ((uint64_t)items * G + WAVE - 1) / WAVE;
@@ -493,7 +807,8 @@ unsigned grid_items(uint32_t items, int G) {
 // Frontier rounds to convergence.  first_bk (optional) holds the first
 // frontier already split by work bucket (4 lists of capacity m + 4 counts).
 rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
-                  uint32_t nF, const uint32_t* first_bk = nullptr) {
+                  uint32_t nF, const uint32_t* first_bk = nullptr,
+                  uint32_t* undec = nullptr, uint32_t undec_cap = 0) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
@@ -513,6 +828,8 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     A.nearst = h->nearst;
     A.ctr = h->ctr;
     A.tb = h->terms();
+    A.undec = undec;
+    A.undec_cap = undec_cap;
     uint32_t* bk;  // bucketed frontier: 4 x m ids + 4 counters
     RT_TRY(sc.get(&bk, 4ull * g.m + 4));
     unsigned* bcur = bk + 4ull * g.m;
@@ -600,6 +917,10 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         std::swap(F, Fn);
         nF = nFn;
     }
+    k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr, undec,
+                                                               undec_cap);
+    RT_HIP(hipGetLastError());
+    if (undec) RT_TRY(exact_replay(h, eps, undec, undec_cap));
     if (A.tb.w) {  // live-triangle total (the peel kernels do not count)
         Scratch sc2;
         unsigned long long* inc;
@@ -614,8 +935,6 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_HIP(hipMemcpyAsync(&h->ctr->tri_removed, &removed, 8, hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
     }
-    k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr);
-    RT_HIP(hipGetLastError());
     h->pv_full = false;  // vertex supports are pulled again on demand
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
@@ -646,7 +965,9 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
     h->n_alive = g.m - nF;
-    RT_TRY(peel(h, eps, F, Fn, nF, bk));
+    uint32_t* undec = nullptr;
+    RT_TRY(sc.get(&undec, g.m));
+    RT_TRY(peel(h, eps, F, Fn, nF, bk, terms ? undec : nullptr, g.m));
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
     k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
     RT_HIP(hipGetLastError());

@@ -1059,6 +1059,8 @@ rtrex_status run_count(rtrex_graph h) {
     RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
     RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
     RT_HIP(hipMemsetAsync(&h->ctr->tri_removed, 0, 5 * sizeof(unsigned long long), s));
+    RT_HIP(hipMemsetAsync(&h->ctr->n_undecided, 0, sizeof(unsigned), s));
+    RT_HIP(hipMemsetAsync(&h->ctr->replayed_edges, 0, sizeof(unsigned long long), s));
     RT_HIP(hipEventSynchronize(h->ev[4]));
     float a = 0, b = 0;
     RT_HIP(hipEventElapsedTime(&a, h->ev[2], h->ev[3]));

@@ -69,7 +69,9 @@ struct Counters {
     unsigned int error;                // validation / capacity flags
     unsigned int work_next;            // extraction work queue head
     unsigned int changed;              // union-find progress flag
-    unsigned int pad[2];
+    unsigned int n_undecided;          // edges the clean could not decide
+    unsigned int pad;
+    unsigned long long replayed_edges; // edges re-run by the exact replay
 };
 
 enum : uint32_t {

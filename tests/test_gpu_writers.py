@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 # Decisions that depend on the reference's own LIFO order (reported as
 # ambiguous by the audit; see test_gpu_parity).
-ORDER_DEPENDENT = {"clean_email_eps0.1", "clean_astro_eps0.1"}
+ORDER_DEPENDENT = {"clean_email_eps0.1"}
 CLI = os.path.join(os.path.dirname(R.HIP_LIB), "rtrex")
 
 
