@@ -442,11 +442,42 @@ rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
     return run_delete_and_clean(g, pairs, k, eps);
 }
 
+namespace {
+// Extraction whose every decision is the reference's.  A cascade decision the
+// certified post-clean supports cannot settle is recorded; the clean then
+// runs again with those edges' triangle-connected regions replayed in the
+// reference's order (their supports become the reference's bits) and the
+// extraction runs again -- at most three times.  The regions are remembered
+// for later decompositions at the same eps.
+rtrex_status extract_exact(rtrex_graph g, double eps) {
+    if (g->force_eps != eps) {
+        g->force_replay.clear();
+        g->force_all = false;
+        g->force_eps = eps;
+    }
+    RT_TRY(run_extract(g, eps));
+    for (int tries = 0; tries < 3 && g->zw && (!g->x_undec.empty() || g->x_undec_all); ++tries) {
+        if (g->x_undec_all) g->force_all = true;
+        g->force_replay.insert(g->force_replay.end(), g->x_undec.begin(), g->x_undec.end());
+        std::sort(g->force_replay.begin(), g->force_replay.end());
+        g->force_replay.erase(std::unique(g->force_replay.begin(), g->force_replay.end()),
+                              g->force_replay.end());
+        if (g->force_replay.size() > g->g.m) g->force_all = true;
+        RT_TRY(run_count(g));
+        RT_TRY(run_initial_clean(g, eps));
+        RT_TRY(run_extract(g, eps));
+    }
+    return RTREX_OK;
+}
+}  // namespace
+
 rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
                            int64_t* n_clusters) {
     RT_TRY(check(g));
     ArenaScope scope_(g);
-    RT_TRY(run_extract(g, eps));
+    if (g->phase >= PH_CLEANED && g->eps != eps)
+        return fail(RTREX_EINVAL, "extract needs the eps of the initial clean");
+    RT_TRY(extract_exact(g, eps));
     if (cluster_map)
         std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
     if (n_clusters) *n_clusters = g->n_clusters;
@@ -457,9 +488,14 @@ rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
                              int64_t* n_clusters) {
     RT_TRY(check(g));
     ArenaScope scope_(g);
+    if (g->force_eps != eps) {  // regions remembered for another eps
+        g->force_replay.clear();
+        g->force_all = false;
+        g->force_eps = eps;
+    }
     RT_TRY(run_count(g));
     RT_TRY(run_initial_clean(g, eps));
-    RT_TRY(run_extract(g, eps));
+    RT_TRY(extract_exact(g, eps));
     if (cluster_map)
         std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
     if (n_clusters) *n_clusters = g->n_clusters;

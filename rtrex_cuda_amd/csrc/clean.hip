@@ -587,11 +587,13 @@ __global__ void __launch_bounds__(T) k_rg_flag(const uint32_t* __restrict__ unde
 // reset to their state before the clean.
 __global__ void __launch_bounds__(T) k_rg_keys(GraphDev g, const uint32_t* __restrict__ parent,
                                                const uint8_t* __restrict__ flag, int Bi,
-                                               uint64_t* __restrict__ keys, unsigned* nk) {
+                                               uint64_t* __restrict__ keys, unsigned* nk,
+                                               uint8_t* __restrict__ xact) {
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
         const uint32_t r = parent[e];
         if (!flag[r]) continue;
         keys[wave_append(nk)] = ((uint64_t)r << Bi) | e;
+        if (xact) xact[e] = 1;  // replayed: the reference's own values
         set_status(g, e, ST_ALIVE);
         g.sup[e] = g.sup0[e];
         g.touched[e] = 0;
@@ -726,6 +728,13 @@ __global__ void __launch_bounds__(T) k_count_alive(GraphDev g, unsigned long lon
     if (lane_id() == 0 && c) atomicAdd(out, c);
 }
 
+// A support the fast peel never decremented is the count-phase value, which
+// is the reference's (no triangle of the edge died there either).
+__global__ void __launch_bounds__(T) k_xact_init(GraphDev g, uint8_t* __restrict__ xact) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T)
+        xact[e] = g.touched[e] ? 0 : 1;
+}
+
 int bits_of(uint64_t n) {
     int b = 1;
     while ((1ull << b) < n) ++b;
@@ -759,7 +768,7 @@ rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint
     RT_HIP(hipMemsetAsync(cnt, 0, 16, s));
     const int Bi = bits_of((uint64_t)g.m + 1);
     RT_TRY(sc.get(&keys, g.m));
-    k_rg_keys<<<be, T, 0, s>>>(g, parent, flag, Bi, keys, (unsigned*)cnt);
+    k_rg_keys<<<be, T, 0, s>>>(g, parent, flag, Bi, keys, (unsigned*)cnt, h->xact);
     RT_HIP(hipGetLastError());
     uint32_t nk = 0;
     RT_HIP(hipMemcpyAsync(&nk, cnt, 4, hipMemcpyDeviceToHost, s));
@@ -920,7 +929,11 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr, undec,
                                                                undec_cap);
     RT_HIP(hipGetLastError());
-    if (undec) RT_TRY(exact_replay(h, eps, undec, undec_cap));
+    if (undec) {
+        if (!h->xact) RT_TRY(dalloc(h, &h->xact, g.m));
+        k_xact_init<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, h->xact);
+        RT_TRY(exact_replay(h, eps, undec, undec_cap));
+    }
     if (A.tb.w) {  // live-triangle total (the peel kernels do not count)
         Scratch sc2;
         unsigned long long* inc;
@@ -967,6 +980,15 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     h->n_alive = g.m - nF;
     uint32_t* undec = nullptr;
     RT_TRY(sc.get(&undec, g.m));
+    // Regions an earlier extraction could not decide are re-run as well.
+    if (terms && (h->force_all || !h->force_replay.empty())) {
+        const uint32_t nf = h->force_all ? g.m + 1 : (uint32_t)h->force_replay.size();
+        if (!h->force_all)
+            RT_HIP(hipMemcpyAsync(undec, h->force_replay.data(), nf * 4ull, hipMemcpyHostToDevice,
+                                  s));
+        RT_HIP(hipMemcpyAsync(&h->ctr->n_undecided, &nf, 4, hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
     RT_TRY(peel(h, eps, F, Fn, nF, bk, terms ? undec : nullptr, g.m));
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
     k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
@@ -1018,6 +1040,7 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
         return fail(RTREX_EINVAL,
                     "delete_and_clean: a pair is not an alive edge of the "
                     "graph, or is listed twice (state is undefined now)");
+    if (h->xact) RT_HIP(hipMemsetAsync(h->xact, 0, g.m, s));  // supports no longer the reference's
     h->n_alive -= nF;
     RT_TRY(peel(h, eps, F, Fn, nF));
     return RTREX_OK;

@@ -34,11 +34,13 @@
 // subtractions, so they may differ from the reference's by a few ulps; each
 // ratio decision is audited against that bound (ambiguous_ratio).
 #include <algorithm>
+#include <unordered_set>
 
 #include "audit.hpp"
 #include "dev.hpp"
 #include "internal.hpp"
 #include "introsort.hpp"
+#include "uset.hpp"
 
 namespace rtrex {
 namespace {
@@ -221,24 +223,40 @@ struct XArgs {
     uint32_t* cand;          // [2 start, ...): candidates, padded for sorting
     Rec* grec;               // [start, ...): candidate records (large case)
     double* gpv;             // [start, ...): prefetched supports (large case)
-    uint32_t* wl;            // [dpre[start], ...): deletion worklist (LIFO)
+    uint64_t* wl;            // [dpre[start], ...): deletion worklist (LIFO):
+                             // (edge << 32) | first endpoint of the pair
     TermBuf tb;              // stored triangle lists (tb.w null: intersect)
     Counters* ctr;
+    const uint8_t* xact;     // per edge: post-clean support is the reference's bits
+    const uint32_t* bcnt;    // unordered_set bucket count after k insertions
+    const uint32_t* hoff;    // per component: start of its bucket scratch
+    uint32_t* hnext;         // [start, ...): set node links
+    uint32_t* hbkt;          // [hoff[c], ...): set buckets
+    uint32_t* undec;         // edges no bound decided (for an exact re-run)
+    uint32_t undec_cap;
 };
 
 // Cascade decision for alive edge f whose support just became v (threshold
 // thr); `third` is the third vertex of f's triangle with the edge being
 // deleted.  Certain outside the cheap bound, otherwise certified against every
 // subtraction order (audit.hpp).
-__device__ bool decide(const GraphDev& g, const TermBuf& tb, uint32_t f, double v,
-                       double thr, uint32_t third, uint32_t deleting,
-                       unsigned long long& amb) {
+// An edge whose post-clean support is the reference's own value (xact) has,
+// in the reference's cascade order, the reference's value here too: its
+// comparison is taken as is.
+__device__ bool decide(const XArgs& A, uint32_t f, double v, double thr, uint32_t third,
+                       uint32_t deleting, unsigned long long& amb) {
+    const GraphDev& g = A.g;
+    if (A.xact && A.xact[f]) return v < thr;
     const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
     if (v < thr - cb) return true;
     if (v >= thr + cb) return false;
-    int c = certify_edge(g, tb, f, thr, false, third, deleting);
+    int c = certify_edge(g, A.tb, f, thr, false, third, deleting);
     if (c == CERT_UNDECIDED) {
         amb++;
+        if (A.undec) {
+            const unsigned i = atomicAdd(&A.ctr->n_undecided, 1u);
+            if (i < A.undec_cap) A.undec[i] = f;
+        }
         return v < thr;
     }
     return c == CERT_REMOVE;
@@ -446,11 +464,21 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             }
             __syncthreads();
 
-            // 4. queue every alive edge of the cluster (Decomposition.h:897-914)
+            // 4. queue every alive edge of the cluster, its vertices in the
+            // iteration order of the reference's unordered_set (filled as the
+            // cluster grew: seed, alive neighbours, chosen candidates), their
+            // slots in order (Decomposition.h:897-914)
             uint32_t top = 0;
             const uint32_t nmem = 1 + nA + (uint32_t)(maxind + 1);
+            uint32_t* ins = A.cand + cb;   // members in insertion order
+            uint32_t* ord = ins + len;     // ... in iteration order
+            for (uint32_t q = lane; q < nmem; q += XT)
+                ins[q] = q < 1 + nA ? A.mem[start + q] : rec[q - 1 - nA].vertex;
+            __syncthreads();
+            if (lane == 0) uset_order(ins, nmem, A.bcnt, A.hnext + start, A.hbkt + A.hoff[c], ord);
+            __syncthreads();
             for (uint32_t q = 0; q < nmem; ++q) {
-                const uint32_t z = q < 1 + nA ? A.mem[start + q] : rec[q - 1 - nA].vertex;
+                const uint32_t z = ord[q];
                 const uint32_t zb = g.off[z], ze = g.off[z + 1];
                 for (uint32_t p = zb; p < ze; p += XT) {
                     const uint32_t j = p + lane;
@@ -458,7 +486,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     const uint64_t bal = ballot64(al);
                     if (al) {
                         const uint32_t f = g.s2e[j];
-                        A.wl[ebase + top + __popcll(bal & lanes_below())] = f;
+                        A.wl[ebase + top + __popcll(bal & lanes_below())] =
+                            ((uint64_t)f << 32) | z;  // the pair (z, nbor)
                         set_status(g, f, ST_QUEUED);
                     }
                     top += (uint32_t)__popcll(bal);
@@ -469,11 +498,18 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             // ... and cascade (deleteAndClean, Decomposition.h:147-283)
             while (top > 0) {
                 --top;
-                const uint32_t e = A.wl[ebase + top];
+                const uint64_t ent = A.wl[ebase + top];
+                const uint32_t e = (uint32_t)(ent >> 32), pa = (uint32_t)ent;
                 const uint32_t sl = g.eslot[e];
                 const uint32_t x = g.esrc[e], y = g.nb[sl];
                 const uint32_t dx = g.deg[x], dy = g.deg[y];
                 const uint32_t lo = dx < dy ? x : y, hi = dx < dy ? y : x;
+                // The reference's lower endpoint of the popped pair (pa, pb):
+                // the one of smaller degree, pb on a tie.  (lower, w) is
+                // checked and pushed before (higher, w).
+                const uint32_t pb = pa == x ? y : x;
+                const uint32_t rlo = g.deg[pa] < g.deg[pb] ? pa : pb;
+                const uint32_t rhi = rlo == pa ? pb : pa;
                 // Triangles of e: from its stored list (ascending third
                 // vertex, the order the reference visits them), or by
                 // intersecting N(lo) with N(hi).  Either way, per chunk of 64:
@@ -529,24 +565,28 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                         }
                         hb &= hb - 1;
                     }
+                    // fj is the edge (L, w) of our lower side L, fh (H, w);
+                    // the reference's (lower, w) is fj unless its lower
+                    // endpoint is H (degree tie resolved by the pair order).
+                    const bool swp = rlo != (terms ? tlo : lo);
+                    const uint32_t jl = swp ? fh : fj, jh = swp ? fj : fh;
+                    const uint8_t sl = swp ? sh : sj, shh = swp ? sj : sh;
                     bool push1 = false, push2 = false;
                     if (hit) {
                         g.pv[w] -= t;
-                        // (lo, w) then (hi, w), as Decomposition.h:238-264.
+                        // (lower, w) then (higher, w), as Decomposition.h:238-264.
                         // Queued edges die anyway; their supports are not kept.
-                        if (sj == ST_ALIVE) {
-                            const double v1 = g.sup[fj] - t;
-                            g.sup[fj] = v1;
-                            g.touched[fj] = 1;
-                            push1 = decide(g, A.tb, fj, v1, A.eps * g.ew[fj],
-                                           terms ? thi : hi, e, amb_c);
+                        if (sl == ST_ALIVE) {
+                            const double v1 = g.sup[jl] - t;
+                            g.sup[jl] = v1;
+                            g.touched[jl] = 1;
+                            push1 = decide(A, jl, v1, A.eps * g.ew[jl], rhi, e, amb_c);
                         }
-                        if (sh == ST_ALIVE) {
-                            const double v2 = g.sup[fh] - t;
-                            g.sup[fh] = v2;
-                            g.touched[fh] = 1;
-                            push2 = decide(g, A.tb, fh, v2, A.eps * g.ew[fh],
-                                           terms ? tlo : lo, e, amb_c);
+                        if (shh == ST_ALIVE) {
+                            const double v2 = g.sup[jh] - t;
+                            g.sup[jh] = v2;
+                            g.touched[jh] = 1;
+                            push2 = decide(A, jh, v2, A.eps * g.ew[jh], rlo, e, amb_c);
                         }
                     }
                     const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
@@ -562,12 +602,12 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     }
                     const uint32_t tot = __shfl(pre + np2, 63, 64);
                     if (push1) {
-                        A.wl[ebase + top + pre] = fj;
-                        set_status(g, fj, ST_QUEUED);
+                        A.wl[ebase + top + pre] = ((uint64_t)jl << 32) | rlo;
+                        set_status(g, jl, ST_QUEUED);
                     }
                     if (push2) {
-                        A.wl[ebase + top + pre + (uint32_t)push1] = fh;
-                        set_status(g, fh, ST_QUEUED);
+                        A.wl[ebase + top + pre + (uint32_t)push1] = ((uint64_t)jh << 32) | rhi;
+                        set_status(g, jh, ST_QUEUED);
                     }
                     top += tot;
                     __syncthreads();
@@ -737,6 +777,10 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
         RT_TRY(ensure_vertex_supports(h, cv, nk));  // candidates' supports
+        // Bucket counts of the reference's unordered_set (measured from the
+        // library) up to the largest component, and each component's share
+        // of the set scratch.
+        uint32_t *bcnt_d, *hoff, *hnext, *hbkt;
         {
             std::vector<uint32_t> cs(ncomp + 1);
             RT_HIP(hipMemcpyAsync(cs.data(), cstart, (ncomp + 1) * 4ull,
@@ -747,8 +791,28 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                 big = std::max<int64_t>(big, cs[c + 1] - cs[c]);
             h->audit.components = ncomp;
             h->audit.largest_component = big;
+            if ((int64_t)h->bcnt.size() < big + 1) {
+                std::unordered_set<int64_t> probe;
+                h->bcnt.assign(1, (uint32_t)probe.bucket_count());
+                for (int64_t k = 1; k <= big; ++k) {
+                    probe.insert(k);
+                    h->bcnt.push_back((uint32_t)probe.bucket_count());
+                }
+            }
+            std::vector<uint32_t> ho(ncomp + 1, 0);
+            for (uint32_t c = 0; c < ncomp; ++c) ho[c + 1] = ho[c] + h->bcnt[cs[c + 1] - cs[c]];
+            RT_TRY(sc.get(&bcnt_d, (uint64_t)big + 1));
+            RT_TRY(sc.get(&hoff, (uint64_t)ncomp + 1));
+            RT_TRY(sc.get(&hnext, nk));
+            RT_TRY(sc.get(&hbkt, std::max<uint32_t>(ho[ncomp], 1)));
+            RT_HIP(hipMemcpyAsync(bcnt_d, h->bcnt.data(), ((uint64_t)big + 1) * 4,
+                                  hipMemcpyHostToDevice, s));
+            RT_HIP(hipMemcpyAsync(hoff, ho.data(), ((uint64_t)ncomp + 1) * 4,
+                                  hipMemcpyHostToDevice, s));
+            RT_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
         }
-        uint32_t *mk, *cm, *mem, *cand, *wl;
+        uint32_t *mk, *cm, *mem, *cand, *undec;
+        uint64_t* wl;
         double *tw, *gpv;
         Rec* grec;
         RT_TRY(sc.get(&mk, n));
@@ -759,6 +823,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         RT_TRY(sc.get(&gpv, n));
         RT_TRY(sc.get(&grec, n));
         RT_TRY(sc.get(&wl, g.nnz));
+        RT_TRY(sc.get(&undec, g.m));
+        RT_HIP(hipMemsetAsync(&h->ctr->n_undecided, 0, sizeof(unsigned), s));
         RT_HIP(hipMemsetAsync(mk, 0, (uint64_t)n * 4, s));
         RT_HIP(hipMemsetAsync(cm, 0, (uint64_t)n * 4, s));
         RT_HIP(hipMemsetAsync(&h->ctr->work_next, 0, sizeof(unsigned), s));
@@ -781,9 +847,26 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.wl = wl;
         a.tb = h->terms();
         a.ctr = h->ctr;
+        a.xact = h->xact;
+        a.bcnt = bcnt_d;
+        a.hoff = hoff;
+        a.hnext = hnext;
+        a.hbkt = hbkt;
+        a.undec = undec;
+        a.undec_cap = g.m;
         const unsigned grid = std::min<uint32_t>(ncomp, 8192);
         k_extract<<<grid, XT, 0, s>>>(a);
         RT_HIP(hipGetLastError());
+        // Undecided cascade decisions (inexact post-clean supports): kept for
+        // an exact re-run of their regions (rtrex_extract).
+        unsigned nu = 0;
+        RT_HIP(hipMemcpyAsync(&nu, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        h->x_undec.resize(std::min<uint32_t>(nu, g.m));
+        if (nu)
+            RT_HIP(hipMemcpyAsync(h->x_undec.data(), undec, h->x_undec.size() * 4ull,
+                                  hipMemcpyDeviceToHost, s));
+        h->x_undec_all = nu > g.m;
     }
     // cluster ids in seed order
     uint32_t *flag2, *cidr;

@@ -146,6 +146,13 @@ struct rtrex_graph_s {
     uint32_t* cseed = nullptr;
     uint32_t* cid = nullptr;  // cluster id per vertex (global numbering)
     uint32_t* mrank = nullptr;  // position at which each vertex joined its cluster
+    uint8_t* xact = nullptr;    // per edge: post-clean support is the reference's bits
+    std::vector<uint32_t> bcnt;     // unordered_set bucket count after k insertions
+    std::vector<uint32_t> x_undec;  // edges the last extraction could not decide
+    bool x_undec_all = false;       // ... more than the list holds
+    std::vector<uint32_t> force_replay;  // edges whose regions the clean re-runs
+    bool force_all = false;
+    double force_eps = -1;  // the eps force_replay was found at
     int64_t n_clusters = -1;
     std::vector<int64_t> seg_clusters;  // per segment (batch)
     std::vector<int64_t> cluster_map_host;

@@ -4,15 +4,15 @@ supports after the count phase are compared bit for bit (their order is fixed);
 after cascades FP64 values may differ by a few ulps, so only the decisions
 (survivors, clusters) are compared -- exactly.
 
-Some cases contain cleaning decisions that depend on the reference's LIFO
+Some cases contain decisions that depend on the reference's LIFO
 subtraction order itself (an exact rational tie whose FP rounding differs
-between orders).  The clean detects every such decision and re-runs its
-triangle-connected region in the reference's exact order, so survivors are
-the reference's in every case (audit: replayed_edges).  The extraction's
-cascades still decide such ties from certified, not exact, post-clean values:
-where its audit reports ambiguity the cluster comparison is xfail
-(clean_email at eps 0.1, config2 at eps 0.1), and test_no_silent_mismatch
-checks that a mismatch never occurs without such a report.
+between orders): clean_email / clean_astro / config2 at eps 0.1.  The clean
+detects every such decision and re-runs its triangle-connected region in the
+reference's exact order; the extraction queues cluster edges in the
+reference's unordered_set order and, where a cascade decision is still
+undecided, has the clean re-run that edge's region too.  Every case is
+therefore compared exactly (audit: replayed_edges counts the re-run edges,
+ambiguous_clean must end at 0).
 """
 from __future__ import annotations
 
@@ -28,7 +28,7 @@ from rtrex_cuda_amd import graphs as G
 
 pytestmark = pytest.mark.gpu
 
-ORDER_DEPENDENT = {"clean_email_eps0.1"}  # extraction ties (clusters only)
+ORDER_DEPENDENT: set = set()
 
 
 def sha(a) -> str:
@@ -104,6 +104,7 @@ def test_no_silent_mismatch(tag):
             and np.array_equal(r["cm"], ref["cluster_map"]))
     a = r["audit"]
     assert same or (a["ambiguous_clean"] + a["ambiguous_ratio"]) > 0
+    assert a["ambiguous_clean"] == 0  # every cleaning decision is the reference's
 
 
 _c2 = {}
@@ -129,8 +130,7 @@ def test_config2_count_and_clean_against_oracle(eps):
     assert np.array_equal(r["st"] == ord("Y"), d.status_after_clean == ord("Y"))
 
 
-@pytest.mark.parametrize("eps", [0.3, pytest.param(0.1, marks=pytest.mark.xfail(
-    reason="extraction ties (reported as ambiguous)", strict=False))])
+@pytest.mark.parametrize("eps", [0.3, 0.1])
 def test_config2_extract_against_oracle(eps):
     off, nb, d, r = config2_run(eps)
     assert np.array_equal(r["cm"], d.cluster_map)

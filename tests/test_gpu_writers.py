@@ -18,9 +18,8 @@ from test_writers import FILES, check_files, sha_file
 
 pytestmark = pytest.mark.gpu
 
-# Decisions that depend on the reference's own LIFO order (reported as
-# ambiguous by the audit; see test_gpu_parity).
-ORDER_DEPENDENT = {"clean_email_eps0.1"}
+# Every golden case is decided exactly (see test_gpu_parity).
+ORDER_DEPENDENT: set = set()
 CLI = os.path.join(os.path.dirname(R.HIP_LIB), "rtrex")
 
 
