@@ -208,25 +208,29 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
         }
         if (act && gl == 0) {
             g.sup[e] = S;
-            g.sup0[e] = S;
             g.tcnt[e] = c;
-            g.ew[e] = 1.0 / (double)((uint64_t)dx * (uint64_t)dy);
             if (tb.w) {
-                tb.za[e] = ca;
-                tb.zb[e] = cb;
+                tb.zab[e] = make_uint2(ca, cb);
             }
         }
     }
 }
 
 // Triangle incidences sum_e tcnt(e) (three per triangle): per-block partial
-// sums, then one block adds them (no contended atomics).
+// sums, then one block adds them (no contended atomics).  The same stream
+// copies the supports into sup0, the count-phase value the clean starts from
+// (the counting kernels write sup only: one scattered store per edge fewer).
 constexpr int RED_BLOCKS = 1024;
-__global__ void __launch_bounds__(T) k_sum_tcnt(const uint32_t* __restrict__ tcnt, uint32_t m,
+__global__ void __launch_bounds__(T) k_sum_tcnt(const uint32_t* __restrict__ tcnt,
+                                                const double* __restrict__ sup,
+                                                double* __restrict__ sup0, uint32_t m,
                                                 unsigned long long* __restrict__ part) {
     __shared__ unsigned long long sw[T / WAVE];
     unsigned long long c = 0;
-    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) c += tcnt[e];
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) {
+        c += tcnt[e];
+        sup0[e] = sup[e];
+    }
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
     if (lane_id() == 0) sw[threadIdx.x / WAVE] = c;
     __syncthreads();
@@ -291,7 +295,7 @@ template <bool COUNT>
 __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __restrict__ own_off,
                                                  const uint32_t* __restrict__ hwork,
                                                  uint32_t* __restrict__ cls_cnt,
-                                                 uint64_t* __restrict__ units,
+                                                 uint4* __restrict__ units,
                                                  const uint32_t* __restrict__ cls_base,
                                                  CountParams Pm) {
     // One thread per hub; class totals aggregated per block (LDS counters,
@@ -327,8 +331,13 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         lb[threadIdx.x] = atomicAdd(&cls_cnt[threadIdx.x], lc[threadIdx.x]);
     __syncthreads();
     if (!COUNT && c >= 0) {
-        for (uint64_t p = 0; p < P; ++p)
-            units[cls_base[c] + lb[c] + at + p] = (uint64_t)h | (p << 32) | (P << 48);
+        const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
+        const uint32_t oh = g.off[h], dh = g.deg[h];
+        for (uint64_t p = 0; p < P; ++p) {
+            const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * p / P);
+            const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (p + 1) / P);
+            units[cls_base[c] + lb[c] + at + p] = make_uint4(h, oh, k0, dh | ((k1 - k0) << 16));
+        }
     }
 }
 
@@ -351,104 +360,198 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* s, uint32_t 
 // matches of a chunk are folded in lane order.  A match needs no further
 // gather: d_z and the edge id of {hub, z} come from LDS.
 constexpr int HT = 256;
+
+// Number of entries of the LDS-staged sorted list s[0, n) smaller than z
+// (n <= L): log2(L) fixed steps, no divergent loop; every read is in s[0, L).
+template <int L>
+__device__ __forceinline__ uint32_t lds_rank(const uint32_t* s, uint32_t n, uint32_t z) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = L / 2; step >= 1; step >>= 1) {
+        const uint32_t q = pos + step;  // q - 1 <= L - 2
+        if (q <= n && s[q - 1] < z) pos = q;
+    }
+    if (pos < n && s[pos] < z) ++pos;  // pos <= L - 1 after the steps
+    return pos;
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (LDS operations of one wave complete in order; the fences keep the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Hub-path work unit: part of one hub's owned edges.  x = hub, y = offset of
+// its list, z = first own_idx entry, w = d_hub | (edge count << 16) (both
+// <= 4096: hub_max is at most 4096).
+__device__ __forceinline__ void unit_fields(const uint4& U, uint32_t& h, uint32_t& oh,
+                                            uint32_t& k0, uint32_t& dh, uint32_t& cnt) {
+    h = U.x;
+    oh = U.y;
+    k0 = U.z;
+    dh = U.w & 0xffffu;
+    cnt = U.w >> 16;
+}
+
+// The owned edges [first, cnt) of a unit, stride apart, G lanes per edge:
+// each group streams its edge's shorter list from HBM (R chunks of G ids per
+// load, neighbour and edge id in one 8-byte load) and looks every id up in
+// the hub list staged in LDS: snb (sorted neighbour ids), sed (edge ids),
+// sdg / sof (the neighbours' degrees and list offsets), sown (owned slots).
+// Matches go to the term buffer in ascending z; the support is folded in lane
+// order.  sfold is this wave's 64-entry LDS slice.
 template <int G, int L>
-__global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint64_t* __restrict__ units,
-                                                  uint32_t nunits,
-                                                  const uint32_t* __restrict__ own_off,
-                                                  const uint32_t* __restrict__ own_idx,
-                                                  TermBuf tb) {
-    // R chunks of the shorter list are loaded at once (one memory round
-    // trip), then probed and folded one after the other.
+__device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t h,
+                                          uint32_t dh, const uint32_t* snb, const uint32_t* sed,
+                                          const uint32_t* sdg, const uint32_t* sof,
+                                          const uint32_t* sown, double* sfold, uint32_t first,
+                                          uint32_t cnt, uint32_t stride) {
     constexpr int R = G <= 8 ? 4 : 2;
-    __shared__ uint32_t snb[L], sed[L], sdg[L], sown[L];
     const int lane = lane_id();
-    const int wid = threadIdx.x / WAVE;
     const int gl = lane & (G - 1);
     const int gbase = lane & ~(G - 1);
     const uint64_t below = ((1ull << gl) - 1ull);
+    for (uint32_t base = first; base < cnt; base += stride) {
+        const uint32_t idx = base + lane / G;
+        const bool act = idx < cnt;
+        const uint32_t li = act ? sown[idx] : 0;
+        const uint32_t y = snb[li];  // lo endpoint
+        const uint32_t e = sed[li];
+        const uint32_t dy = sdg[li];
+        const uint32_t ex = min(h, y), ey = max(h, y);
+        uint32_t p = sof[li];
+        const uint32_t le = act ? p + dy : p;
+        const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+        double S = 0.0;
+        uint32_t c = 0, ca = 0, cb = 0;
+        while (__any(p < le)) {
+            uint32_t zr[R], er[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t q = p + r * G + gl;
+                const uint2 a = q < le ? g.nbe[q] : make_uint2(0u, 0u);
+                zr[r] = a.x;
+                er[r] = a.y;  // edge {lo, z}
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool go = p < le;
+                const bool in = go && p + gl < le;
+                const uint32_t z = zr[r];
+                const uint32_t pos = lds_rank<L>(snb, dh, z);
+                const bool hit = in && pos < dh && snb[pos] == z;
+                const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
+                const uint64_t hm = G == 64 ? ballot64(hit)
+                                            : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
+                if (tb.w && hit) {
+                    const uint64_t k = wbase + c + __popcll(hm & below);
+                    tb.w[k] = t;
+                    tb.sa[k] = er[r];
+                    tb.sb[k] = sed[pos];  // edge {hub, z}
+                }
+                const uint64_t am = G == 64 ? ballot64(hit && z < ex)
+                                            : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
+                const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
+                                            : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
+                ca += (uint32_t)__popcll(am);
+                cb += (uint32_t)__popcll(bm);
+                c += (uint32_t)__popcll(hm);
+                // Ordered fold: the group's weights in lane order, a miss
+                // adding 0.0 (which leaves the sum's bits unchanged).
+                if (G == 64) {
+                    uint64_t mask = hm;
+                    while (mask) {
+                        const int b = __ffsll((unsigned long long)mask) - 1;
+                        S += readlane_f64(t, b);
+                        mask &= mask - 1;
+                    }
+                } else if (__any(hm != 0)) {
+                    sfold[lane] = t;
+                    wave_sync();
+                    if (gl == 0 && hm) {
+                        const double* f = sfold + gbase;
+#pragma unroll
+                        for (int i = 0; i < G; ++i) S += f[i];
+                    }
+                    wave_sync();
+                }
+                if (go) p += G;
+            }
+        }
+        if (act && gl == 0) {
+            g.sup[e] = S;
+            g.tcnt[e] = c;
+            if (tb.w) tb.zab[e] = make_uint2(ca, cb);
+        }
+    }
+}
+
+// Small hubs (d_hub <= 64): one wave per unit.  The wave stages its hub in
+// its own LDS slice, one slot per lane in a single load round trip (the
+// neighbours' degrees and offsets come precomputed per slot), so units of
+// a block never wait on each other.
+constexpr int HW = 256;
+template <int G>
+__global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __restrict__ units,
+                                                    uint32_t nunits,
+                                                    const uint32_t* __restrict__ own_idx,
+                                                    TermBuf tb) {
+    constexpr int NW = HW / WAVE;
+    __shared__ uint32_t s_nb[NW][WAVE], s_ed[NW][WAVE], s_dg[NW][WAVE], s_of[NW][WAVE],
+        s_own[NW][WAVE];
+    __shared__ double s_fold[NW][WAVE];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    uint32_t *snb = s_nb[wid], *sed = s_ed[wid], *sdg = s_dg[wid], *sof = s_of[wid],
+             *sown = s_own[wid];
+    for (uint32_t un = blockIdx.x * NW + wid; un < nunits; un += gridDim.x * NW) {
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);  // dh, cnt <= 64 in this class
+        if (lane < dh) {
+            const uint2 a = g.nbe[oh + lane];
+            const uint2 b = g.ndo[oh + lane];
+            snb[lane] = a.x;
+            sed[lane] = a.y;
+            sdg[lane] = b.x;
+            sof[lane] = b.y;
+        }
+        if (lane < cnt) sown[lane] = own_idx[k0 + lane];
+        wave_sync();
+        hub_edges<G, WAVE>(g, tb, h, dh, snb, sed, sdg, sof, sown, s_fold[wid], 0, cnt,
+                           WAVE / G);
+        wave_sync();  // the next unit restages the slice
+    }
+}
+
+// Medium hubs (d_hub <= 512): one workgroup per unit, the hub staged once in
+// LDS and shared by its four waves.
+template <int G, int L>
+__global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint4* __restrict__ units,
+                                                  uint32_t nunits,
+                                                  const uint32_t* __restrict__ own_idx,
+                                                  TermBuf tb) {
+    __shared__ uint32_t snb[L], sed[L], sdg[L], sof[L], sown[L];
+    __shared__ double sfold[HT];
+    const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
-    constexpr uint32_t per_blk = per_wave * (HT / WAVE);
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
-        const uint64_t U = units[un];
-        const uint32_t h = (uint32_t)U;
-        const uint32_t part = (uint32_t)(U >> 32) & 0xffffu, parts = (uint32_t)(U >> 48);
-        const uint32_t oh = g.off[h], dh = g.deg[h];
-        const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
-        const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * part / parts);
-        const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (part + 1) / parts);
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);
         for (uint32_t i = threadIdx.x; i < dh; i += HT) {
-            const uint32_t z = g.nb[oh + i];
-            snb[i] = z;
-            sed[i] = g.s2e[oh + i];
-            sdg[i] = g.deg[z];
+            const uint2 a = g.nbe[oh + i];
+            const uint2 b = g.ndo[oh + i];
+            snb[i] = a.x;
+            sed[i] = a.y;
+            sdg[i] = b.x;
+            sof[i] = b.y;
         }
-        for (uint32_t i = threadIdx.x; i < k1 - k0; i += HT) sown[i] = own_idx[k0 + i];
+        for (uint32_t i = threadIdx.x; i < cnt; i += HT) sown[i] = own_idx[k0 + i];
         __syncthreads();
-        for (uint32_t base = wid * per_wave; base < k1 - k0; base += per_blk) {
-            const uint32_t idx = base + lane / G;
-            const bool act = idx < k1 - k0;
-            const uint32_t li = act ? sown[idx] : 0;
-            const uint32_t y = snb[li];  // lo endpoint
-            const uint32_t e = sed[li];
-            const uint32_t dy = sdg[li];
-            const uint32_t ex = min(h, y), ey = max(h, y);
-            uint32_t p = act ? g.off[y] : 0;
-            const uint32_t le = act ? g.off[y + 1] : 0;
-            uint32_t hlo = 0;
-            const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
-            double S = 0.0;
-            uint32_t c = 0, ca = 0, cb = 0;
-            while (__any(act && p < le)) {
-                uint32_t zr[R], er[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint32_t q = p + r * G + gl;
-                    const bool in = act && q < le;
-                    zr[r] = in ? g.nb[q] : 0;
-                    er[r] = (in && tb.w) ? g.s2e[q] : 0;  // edge {lo, z}
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const bool go = act && p < le;
-                    const bool in = go && p + gl < le;
-                    const uint32_t z = zr[r];
-                    const uint32_t pos = in ? lds_lower_bound(snb, hlo, dh, z) : hlo;
-                    const bool hit = in && pos < dh && snb[pos] == z;
-                    const uint32_t lastl = go ? min((uint32_t)(G - 1), le - p - 1) : 0u;
-                    const uint32_t nh = __shfl(pos, gbase + lastl, 64);
-                    const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
-                    const uint64_t hm = G == 64 ? ballot64(hit)
-                                                : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
-                    if (tb.w && hit) {
-                        const uint64_t k = wbase + c + __popcll(hm & below);
-                        tb.w[k] = t;
-                        tb.sa[k] = er[r];
-                        tb.sb[k] = sed[pos];  // edge {hub, z}
-                    }
-                    const uint64_t am = G == 64 ? ballot64(hit && z < ex)
-                                                : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
-                    const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
-                                                : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
-                    ca += (uint32_t)__popcll(am);
-                    cb += (uint32_t)__popcll(bm);
-                    group_fold<G>(hit, t, S, c);
-                    if (go) {
-                        hlo = nh;
-                        p += G;
-                    }
-                }
-            }
-            if (act && gl == 0) {
-                g.sup[e] = S;
-                g.sup0[e] = S;
-                g.tcnt[e] = c;
-                g.ew[e] = 1.0 / (double)((uint64_t)dh * (uint64_t)dy);
-                if (tb.w) {
-                    tb.za[e] = ca;
-                    tb.zb[e] = cb;
-                }
-            }
-        }
+        hub_edges<G, L>(g, tb, h, dh, snb, sed, sdg, sof, sown, sfold + wid * WAVE,
+                        wid * per_wave, cnt, per_wave * (HT / WAVE));
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -464,9 +567,8 @@ __device__ __forceinline__ uint32_t hub_hash(uint32_t z) { return (z * 265443576
 
 template <int G>
 __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
-                                                       const uint64_t* __restrict__ units,
+                                                       const uint4* __restrict__ units,
                                                        uint32_t nunits,
-                                                       const uint32_t* __restrict__ own_off,
                                                        const uint32_t* __restrict__ own_idx,
                                                        TermBuf tb) {
     constexpr int R = 2;
@@ -480,10 +582,8 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
     constexpr uint32_t per_wave = WAVE / G;
     constexpr uint32_t per_blk = per_wave * (HTH / WAVE);
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
-        const uint64_t U = units[un];
-        const uint32_t h = (uint32_t)U;
-        const uint32_t part = (uint32_t)(U >> 32) & 0xffffu, parts = (uint32_t)(U >> 48);
-        const uint32_t oh = g.off[h], dh = g.deg[h];
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);
         for (uint32_t i = threadIdx.x; i < HH; i += HTH) hk[i] = HEMPTY;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < dh; i += HTH) {
@@ -493,34 +593,33 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
             hv[q] = (uint16_t)i;
         }
         __syncthreads();
-        const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
-        const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * part / parts);
-        const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (part + 1) / parts);
-        for (uint32_t base = k0 + wid * per_wave; base < k1; base += per_blk) {
+        for (uint32_t base = wid * per_wave; base < cnt; base += per_blk) {
             const uint32_t idx = base + lane / G;
-            const bool act = idx < k1;
-            const uint32_t li = act ? own_idx[idx] : 0;
-            const uint32_t y = act ? g.nb[oh + li] : 0;  // lo endpoint
-            const uint32_t e = act ? g.s2e[oh + li] : 0;
-            const uint32_t dy = act ? g.deg[y] : 0;
+            const bool act = idx < cnt;
+            const uint32_t li = act ? own_idx[k0 + idx] : 0;
+            const uint2 ye = act ? g.nbe[oh + li] : make_uint2(0u, 0u);
+            const uint2 yo = act ? g.ndo[oh + li] : make_uint2(0u, 0u);
+            const uint32_t y = ye.x;  // lo endpoint
+            const uint32_t e = ye.y;
+            const uint32_t dy = yo.x;
             const uint32_t ex = min(h, y), ey = max(h, y);
-            uint32_t p = act ? g.off[y] : 0;
-            const uint32_t le = act ? g.off[y + 1] : 0;
+            uint32_t p = yo.y;
+            const uint32_t le = p + dy;
             const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
             double S = 0.0;
             uint32_t c = 0, ca = 0, cb = 0;
-            while (__any(act && p < le)) {
+            while (__any(p < le)) {
                 uint32_t zr[R], er[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const uint32_t q = p + r * G + gl;
-                    const bool in = act && q < le;
-                    zr[r] = in ? g.nb[q] : 0;
-                    er[r] = (in && tb.w) ? g.s2e[q] : 0;  // edge {lo, z}
+                    const uint2 a = q < le ? g.nbe[q] : make_uint2(0u, 0u);
+                    zr[r] = a.x;
+                    er[r] = a.y;  // edge {lo, z}
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const bool go = act && p < le;
+                    const bool go = p < le;
                     const bool in = go && p + gl < le;
                     const uint32_t z = zr[r];
                     uint32_t pos = 0;
@@ -538,14 +637,15 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
                             q = (q + 1) & (HH - 1);
                         }
                     }
-                    const double t = hit ? tri_w(dh, dy, g.deg[z]) : 0.0;
+                    const uint2 zd = hit ? g.ndo[oh + pos] : make_uint2(0u, 0u);
+                    const double t = hit ? tri_w(dh, dy, zd.x) : 0.0;
                     const uint64_t hm = G == 64 ? ballot64(hit)
                                                 : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
                     if (tb.w && hit) {
                         const uint64_t k = wbase + c + __popcll(hm & below);
                         tb.w[k] = t;
                         tb.sa[k] = er[r];
-                        tb.sb[k] = g.s2e[oh + pos];  // edge {hub, z}
+                        tb.sb[k] = g.nbe[oh + pos].y;  // edge {hub, z}
                     }
                     const uint64_t am = G == 64 ? ballot64(hit && z < ex)
                                                 : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
@@ -559,13 +659,8 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
             }
             if (act && gl == 0) {
                 g.sup[e] = S;
-                g.sup0[e] = S;
                 g.tcnt[e] = c;
-                g.ew[e] = 1.0 / (double)((uint64_t)dh * (uint64_t)dy);
-                if (tb.w) {
-                    tb.za[e] = ca;
-                    tb.zb[e] = cb;
-                }
+                if (tb.w) tb.zab[e] = make_uint2(ca, cb);
             }
         }
         __syncthreads();  // the next unit rebuilds the table
@@ -579,7 +674,8 @@ __device__ __forceinline__ void vseg(const GraphDev& g, const TermBuf& tb,
                                      uint32_t& len) {
     const uint32_t y = g.nb[j];
     const uint32_t e = g.s2e[j];
-    const uint32_t skip = v < y ? tb.zb[e] : tb.za[e];
+    const uint2 ab = tb.zab[e];
+    const uint32_t skip = v < y ? ab.y : ab.x;
     start = tb.zoff[e] + skip;
     len = g.tcnt[e] - skip;
 }
@@ -773,29 +869,31 @@ unsigned grid_for(uint32_t items, int G) {
 }
 
 template <int G, int L>
-void launch_hub_gl(const GraphDev& g, const uint64_t* units, uint32_t nu, const uint32_t* own_off,
-                   const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
-    k_count_hub<G, L><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_off,
-                                                                 own_idx, tb);
+void launch_hub_gl(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
+                   const TermBuf& tb, hipStream_t s) {
+    if (L == WAVE) {
+        const uint32_t blocks = std::min<uint32_t>((nu + HW / WAVE - 1) / (HW / WAVE), 16384);
+        k_count_hub_w<G><<<blocks, HW, 0, s>>>(g, units, nu, own_idx, tb);
+    } else {
+        k_count_hub<G, L><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_idx, tb);
+    }
 }
 
 template <int G>
-void launch_hub_hash(const GraphDev& g, const uint64_t* units, uint32_t nu, const uint32_t* own_off,
-                     const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
-    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HTH, 0, s>>>(g, units, nu, own_off,
-                                                                    own_idx, tb);
+void launch_hub_hash(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
+                     const TermBuf& tb, hipStream_t s) {
+    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HTH, 0, s>>>(g, units, nu, own_idx, tb);
 }
 
 template <int L>
-void launch_hub_l(int gc, const GraphDev& g, const uint64_t* units, uint32_t nu,
-                  const uint32_t* own_off, const uint32_t* own_idx, const TermBuf& tb,
-                  hipStream_t s) {
+void launch_hub_l(int gc, const GraphDev& g, const uint4* units, uint32_t nu,
+                  const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
     switch (gc) {
-        case 0: launch_hub_gl<4, L>(g, units, nu, own_off, own_idx, tb, s); break;
-        case 1: launch_hub_gl<8, L>(g, units, nu, own_off, own_idx, tb, s); break;
-        case 2: launch_hub_gl<16, L>(g, units, nu, own_off, own_idx, tb, s); break;
-        case 3: launch_hub_gl<32, L>(g, units, nu, own_off, own_idx, tb, s); break;
-        default: launch_hub_gl<64, L>(g, units, nu, own_off, own_idx, tb, s); break;
+        case 0: launch_hub_gl<4, L>(g, units, nu, own_idx, tb, s); break;
+        case 1: launch_hub_gl<8, L>(g, units, nu, own_idx, tb, s); break;
+        case 2: launch_hub_gl<16, L>(g, units, nu, own_idx, tb, s); break;
+        case 3: launch_hub_gl<32, L>(g, units, nu, own_idx, tb, s); break;
+        default: launch_hub_gl<64, L>(g, units, nu, own_idx, tb, s); break;
     }
 }
 
@@ -805,17 +903,17 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
     for (int c = 0; c < HUB_NL * HUB_NG; ++c) {
         const uint32_t nu = h->hub_ucnt[c];
         if (!nu) continue;
-        const uint64_t* u = h->hub_units + at;
+        const uint4* u = h->hub_units + at;
         const int lc = c / HUB_NG, gc = c % HUB_NG;
-        if (lc == 0) launch_hub_l<64>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
-        else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_off, h->own_idx, tb, h->stream);
+        if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
+        else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else {
             switch (gc) {
-                case 0: launch_hub_hash<4>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_hash<8>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_hash<16>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_hash<32>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
-                default: launch_hub_hash<64>(h->g, u, nu, h->own_off, h->own_idx, tb, h->stream); break;
+                case 0: launch_hub_hash<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 1: launch_hub_hash<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 2: launch_hub_hash<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 3: launch_hub_hash<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                default: launch_hub_hash<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
             }
         }
         at += nu;
@@ -933,8 +1031,7 @@ rtrex_status prepare_count(rtrex_graph h) {
     RT_TRY(dalloc(h, &h->red, RED_BLOCKS));
     // term buffer: capacity min(d_x, d_y) - 1 per edge
     RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));
-    RT_TRY(dalloc(h, &h->za, g.m));
-    RT_TRY(dalloc(h, &h->zb, g.m));
+    RT_TRY(dalloc(h, &h->zab, g.m));
     {
         Scratch sc;
         uint32_t* cap;
@@ -1019,13 +1116,13 @@ rtrex_status run_count(rtrex_graph h) {
         if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tb);
         if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tb);
         hub_pass(h, tb);
-        k_sum_tcnt<<<RED_BLOCKS, T, 0, s>>>(g.tcnt, g.m, h->red);
+        k_sum_tcnt<<<RED_BLOCKS, T, 0, s>>>(g.tcnt, g.sup, g.sup0, g.m, h->red);
         k_sum_final<<<1, T, 0, s>>>(h->red, &h->ctr->tri_total);
     };
     if (h->terms_exact_pending) {
         // Counting pass, exact offsets from the triangle counts, then the
         // buffer is filled by the timed pass below (once per graph).
-        edge_pass(TermBuf{nullptr, nullptr, nullptr, h->zoff, h->za, h->zb});
+        edge_pass(TermBuf{nullptr, nullptr, nullptr, h->zoff, h->zab});
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(g.tcnt, h->zoff, g.m, s));
         uint64_t total = 0;
@@ -1035,7 +1132,7 @@ rtrex_status run_count(rtrex_graph h) {
         try_alloc_terms(h, total);
         RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
     }
-    TermBuf tb{h->zw, h->zsa, h->zsb, h->zoff, h->za, h->zb};
+    TermBuf tb{h->zw, h->zsa, h->zsb, h->zoff, h->zab};
 
     RT_HIP(hipEventRecord(h->ev[2], s));
     edge_pass(tb);

@@ -82,17 +82,23 @@ __global__ void __launch_bounds__(T) k_edge_ids(const uint32_t* __restrict__ nb,
                                                 const uint32_t* __restrict__ src,
                                                 const uint32_t* __restrict__ canon,
                                                 const uint32_t* __restrict__ rank,
+                                                const uint32_t* __restrict__ deg,
                                                 uint64_t nnz,
                                                 uint32_t* __restrict__ s2e,
                                                 uint32_t* __restrict__ esrc,
-                                                uint32_t* __restrict__ eslot) {
+                                                uint32_t* __restrict__ eslot,
+                                                double* __restrict__ ew) {
     for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
          i += (uint64_t)gridDim.x * T) {
         if (canon[i]) {
             uint32_t e = rank[i];
+            const uint32_t x = src[i], y = nb[i];
             s2e[i] = e;
-            esrc[e] = src[i];
+            esrc[e] = x;
             eslot[e] = (uint32_t)i;
+            // edgeWt = 1/double(d_u * d_v), the product in int64
+            // (Escape/Decomposition.h:395); depends on the degrees only.
+            ew[e] = 1.0 / (double)((uint64_t)deg[x] * (uint64_t)deg[y]);
         }
     }
 }
@@ -103,6 +109,25 @@ __global__ void __launch_bounds__(T) k_edge_ids_mirror(
     for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
          i += (uint64_t)gridDim.x * T)
         if (!canon[i]) s2e[i] = rank[part[i]];
+}
+
+// Per slot, side by side: the neighbour id and the edge id (nbe), the
+// neighbour's degree and list offset (ndo).  The counting kernels stream nbe
+// for every element of a shorter list (one 8-byte load touches half the cache
+// lines of two 4-byte loads from separate arrays) and stage a hub's nbe and
+// ndo in one load round trip, with no dependent gather of degrees/offsets.
+__global__ void __launch_bounds__(T) k_pack_slots(const uint32_t* __restrict__ nb,
+                                                  const uint32_t* __restrict__ s2e,
+                                                  const uint32_t* __restrict__ deg,
+                                                  const uint32_t* __restrict__ off, uint64_t nnz,
+                                                  uint2* __restrict__ nbe,
+                                                  uint2* __restrict__ ndo) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * T) {
+        const uint32_t z = nb[i];
+        nbe[i] = make_uint2(z, s2e[i]);
+        ndo[i] = make_uint2(deg[z], off[z]);
+    }
 }
 
 // ---- edge-list build --------------------------------------------------------
@@ -170,6 +195,8 @@ rtrex_status alloc_state(rtrex_graph h, uint32_t n, uint64_t nnz) {
     RT_TRY(dalloc(h, &g.deg, n));
     RT_TRY(dalloc(h, &g.part, nnz));
     RT_TRY(dalloc(h, &g.s2e, nnz));
+    RT_TRY(dalloc(h, &g.nbe, nnz));
+    RT_TRY(dalloc(h, &g.ndo, nnz));
     RT_TRY(dalloc(h, &g.esrc, m));
     RT_TRY(dalloc(h, &g.eslot, m));
     RT_TRY(dalloc(h, &g.st, nnz));
@@ -214,9 +241,10 @@ rtrex_status finish(rtrex_graph h, uint32_t* src) {
                                       describe(hostErr));
     if ((uint64_t)ncanon * 2 != nnz)
         return fail(RTREX_EINVAL, "graph is not symmetric");
-    k_edge_ids<<<bs, T, 0, s>>>(g.nb, src, canon, rank, nnz, g.s2e, g.esrc,
-                                g.eslot);
+    k_edge_ids<<<bs, T, 0, s>>>(g.nb, src, canon, rank, g.deg, nnz, g.s2e, g.esrc,
+                                g.eslot, g.ew);
     k_edge_ids_mirror<<<bs, T, 0, s>>>(g.part, canon, rank, nnz, g.s2e);
+    k_pack_slots<<<bs, T, 0, s>>>(g.nb, g.s2e, g.deg, g.off, nnz, g.nbe, g.ndo);
     RT_HIP(hipGetLastError());
     RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, nnz, s));
     RT_HIP(hipMemsetAsync(g.est, ST_ALIVE, g.m, s));

@@ -40,6 +40,8 @@ struct GraphDev {
     uint32_t* deg = nullptr;
     uint32_t* part = nullptr;
     uint32_t* s2e = nullptr;
+    uint2* nbe = nullptr;  // per slot: (nb, s2e) side by side, one 8-byte load
+    uint2* ndo = nullptr;  // per slot: (deg, off) of the neighbour nb
     uint32_t* esrc = nullptr;
     uint32_t* eslot = nullptr;
     uint8_t* st = nullptr;
@@ -86,14 +88,14 @@ enum : uint32_t {
 // lower-degree endpoint lo (ties: the lower id) and the other endpoint hi,
 // terms [zoff[e], zoff[e] + tcnt[e]) hold, ascending in the third vertex z,
 // the weight 1/(d_x d_y d_z) and the edge ids of {lo, z} (sa) and {hi, z} (sb).
-// za/zb count the terms with z below the lower/higher id endpoint.
+// zab[e] = (za, zb) counts the terms with z below the lower / higher id
+// endpoint (one 8-byte record, written with one store).
 struct TermBuf {
     double* w;  // null when the buffer did not fit (intersection fallbacks)
     uint32_t* sa;
     uint32_t* sb;
     const uint64_t* zoff;
-    uint32_t* za;
-    uint32_t* zb;
+    uint2* zab;
 };
 
 struct Timings {
@@ -172,13 +174,13 @@ struct rtrex_graph_s {
     // class, hub_ucnt[] units each, concatenated in hub_units.
     uint32_t* own_off = nullptr;
     uint32_t* own_idx = nullptr;
-    uint64_t* hub_units = nullptr;
+    uint4* hub_units = nullptr;
     uint32_t hub_ucnt[15] = {};
     unsigned long long* red = nullptr;  // per-block partial sums (count phase)
     rtrex::CountParams cparams;
     bool plan_valid = false;  // buckets and hub units built for cparams
     uint64_t* zoff = nullptr;
-    uint32_t *za = nullptr, *zb = nullptr;
+    uint2* zab = nullptr;
     double* zw = nullptr;
     uint32_t *zsa = nullptr, *zsb = nullptr;
     bool terms_exact_pending = false;  // size the buffer after a counting pass
@@ -193,7 +195,7 @@ struct rtrex_graph_s {
     char* slab = nullptr;
     uint64_t slab_cap = 0, slab_top = 0;
     bool own_stream = true;
-    rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, za, zb}; }
+    rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, zab}; }
     hipEvent_t ev[8] = {};
 };
 
