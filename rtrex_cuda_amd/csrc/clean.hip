@@ -291,8 +291,10 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
 #pragma unroll
             for (int r = 0; r < U; ++r) {
                 const uint64_t q = k + (uint64_t)r * G;
-                fa[r] = q < k1 ? tb.sa[q] : 0;
-                fb[r] = q < k1 ? tb.sb[q] : 0;
+                // Streamed once per round: non-temporal, so that the
+                // stream does not push the status bytes out of L2.
+                fa[r] = q < k1 ? __builtin_nontemporal_load(tb.sa + q) : 0;
+                fb[r] = q < k1 ? __builtin_nontemporal_load(tb.sb + q) : 0;
             }
 #pragma unroll
             for (int r = 0; r < U; ++r) {
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
             for (int r = 0; r < U; ++r) {
                 const uint8_t a = sa_[r], b = sb_[r];
                 if ((a == ST_QUEUED || b == ST_QUEUED) && a != ST_DEAD && b != ST_DEAD) {
-                    d += tb.w[k + (uint64_t)r * G];
+                    d += __builtin_nontemporal_load(tb.w + k + (uint64_t)r * G);
                     any = true;
                 }
             }
@@ -320,10 +322,46 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
         const double thr = A.eps * g.ew[e];
         const double cb = cheap_bound(g.tcnt[e], g.sup0[e]);
         if (v < thr - cb) {
-            claim_edge(g, e, A.Fn, A.nFn);
+            // This group is the only writer of claim[e] in a pull round;
+            // k_collect_claims appends the claimed edges to the next
+            // frontier (a per-wave append here would make one same-address
+            // atomic per wave, nearly every wave, in the first round).
+            g.claim[e] = 1u;
         } else if (v < thr + cb && atomicExch(&A.nearst[e], A.stamp) != A.stamp) {
             A.near[wave_append(A.nNear)] = e;
         }
+    }
+}
+
+// After the pull kernels of a round: the edges they claimed (claim set,
+// status still ALIVE) go to the next frontier as PENDING, one global atomic
+// per tile of SEED_K * T edges.
+__global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, uint32_t* __restrict__ Fn,
+                                                      unsigned* nFn) {
+    __shared__ uint32_t lcnt, lbase;
+    for (uint64_t tile = (uint64_t)blockIdx.x * SEED_K * T; tile < g.m;
+         tile += (uint64_t)gridDim.x * SEED_K * T) {
+        if (threadIdx.x == 0) lcnt = 0;
+        __syncthreads();
+        uint32_t r[SEED_K];
+        bool c[SEED_K];
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k) {
+            const uint64_t e = tile + (uint64_t)k * T + threadIdx.x;
+            c[k] = e < g.m && g.claim[e] && g.est[e] == ST_ALIVE;
+            if (c[k]) r[k] = atomicAdd(&lcnt, 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && lcnt) lbase = atomicAdd(nFn, lcnt);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k) {
+            if (!c[k]) continue;
+            const uint32_t e = (uint32_t)(tile + (uint64_t)k * T + threadIdx.x);
+            set_status(g, e, ST_PENDING);
+            Fn[lbase + r[k]] = e;
+        }
+        __syncthreads();  // lcnt is reset for the next tile
     }
 }
 
@@ -889,6 +927,8 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
                 else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, ac[b]);
                 else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, ac[b]);
             }
+            k_collect_claims<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, Fn,
+                                                                                         A.nFn);
         }
         for (int b = 0; b < 4 && !pull; ++b) {
             if (!bc[b]) continue;

@@ -1100,6 +1100,34 @@ rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_
     return RTREX_OK;
 }
 
+// Clean state after a count: every slot and edge ALIVE, no edge touched or
+// claimed, the clean's counters zero.  16 status bytes per thread step.
+__global__ void __launch_bounds__(T) k_reset_clean(GraphDev g, Counters* ctr) {
+    const uint64_t stride = (uint64_t)gridDim.x * T;
+    const uint64_t t0 = (uint64_t)blockIdx.x * T + threadIdx.x;
+    static_assert(ST_ALIVE == 0, "status bytes are cleared to zero");
+    for (uint64_t i = t0; i < (g.nnz + 15) / 16; i += stride) {
+        if (16 * i + 16 <= g.nnz && ((uintptr_t)g.st & 15) == 0)
+            reinterpret_cast<uint4*>(g.st)[i] = make_uint4(0u, 0u, 0u, 0u);
+        else
+            for (uint64_t j = 16 * i; j < min<uint64_t>(16 * i + 16, g.nnz); ++j) g.st[j] = 0;
+    }
+    for (uint64_t e = t0; e < g.m; e += stride) {
+        g.est[e] = ST_ALIVE;
+        g.touched[e] = 0;
+        g.claim[e] = 0;
+    }
+    if (t0 == 0) {
+        ctr->tri_removed = 0;
+        ctr->edges_removed = 0;
+        ctr->ambiguous_clean = 0;
+        ctr->ambiguous_ratio = 0;
+        ctr->seeds = 0;
+        ctr->n_undecided = 0;
+        ctr->replayed_edges = 0;
+    }
+}
+
 rtrex_status run_count(rtrex_graph h) {
     if (h->phase < PH_BUILT) return fail(RTREX_ESTATE, "graph not built");
     GraphDev& g = h->g;
@@ -1150,14 +1178,8 @@ rtrex_status run_count(rtrex_graph h) {
     }
     RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[4], s));
-    // Fresh clean state for the supports just computed.
-    RT_HIP(hipMemsetAsync(g.st, ST_ALIVE, g.nnz, s));
-    RT_HIP(hipMemsetAsync(g.est, ST_ALIVE, g.m, s));
-    RT_HIP(hipMemsetAsync(g.touched, 0, g.m, s));
-    RT_HIP(hipMemsetAsync(g.claim, 0, (uint64_t)g.m * 4, s));
-    RT_HIP(hipMemsetAsync(&h->ctr->tri_removed, 0, 5 * sizeof(unsigned long long), s));
-    RT_HIP(hipMemsetAsync(&h->ctr->n_undecided, 0, sizeof(unsigned), s));
-    RT_HIP(hipMemsetAsync(&h->ctr->replayed_edges, 0, sizeof(unsigned long long), s));
+    // Fresh clean state for the supports just computed (one launch).
+    k_reset_clean<<<blocks_for(std::max<uint64_t>(g.nnz / 16, 1), T, 4096), T, 0, s>>>(g, h->ctr);
     RT_HIP(hipEventSynchronize(h->ev[4]));
     float a = 0, b = 0;
     RT_HIP(hipEventElapsedTime(&a, h->ev[2], h->ev[3]));
