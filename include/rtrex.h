@@ -136,9 +136,17 @@ rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
 rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
                            int64_t* n_clusters);
 
-/* count + initial clean + extract in one call (RTRex.cpp:193). */
+/* count + initial clean + extract in one call (RTRex.cpp:193).  For both
+ * calls a null cluster_map leaves the map in device memory only (no host
+ * copy); rtrex_get_cluster_map fetches it later. */
 rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
                              int64_t* n_clusters);
+
+/* The cluster map of the last extraction (n entries, per-graph ids on a
+ * batch handle): the cluster_map output of rtrex_extract, fetched on demand.
+ * No reference counterpart (the reference holds its clusters in host memory,
+ * Decomposition.h:538-543). */
+rtrex_status rtrex_get_cluster_map(rtrex_graph g, int64_t* cluster_map);
 
 /* Position at which every vertex joined its cluster in the last extraction:
  * 0 for the seed, then its alive neighbours in slot order, then the chosen

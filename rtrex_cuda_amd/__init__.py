@@ -44,7 +44,7 @@ EXPORTS = [
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
-    "rtrex_server_stop", "rtrex_get_member_order", "rtrex_write_decomposition",
+    "rtrex_server_stop", "rtrex_get_member_order", "rtrex_get_cluster_map", "rtrex_write_decomposition",
     "rtrex_write_components", "rtrex_write_survivors", "rtrex_get_count_params",
     "rtrex_set_count_params", "rtrex_autotune_count",
 ]
@@ -153,6 +153,7 @@ def lib() -> C.CDLL:
             "rtrex_server_wait": [G, C.c_int64, P64, P64, C.POINTER(JobInfo)],
             "rtrex_server_stop": [G],
             "rtrex_get_member_order": [G, P64],
+            "rtrex_get_cluster_map": [G, P64],
             "rtrex_write_decomposition": [C.c_char_p, C.c_int64, P64, P64, P64, P64, C.c_int64],
             "rtrex_write_components": [C.c_char_p, C.c_int64, P64, P64],
             "rtrex_write_survivors": [C.c_char_p, C.c_int64, P64, P64, PU8],
@@ -294,11 +295,21 @@ class DeviceGraph:
         _check(lib().rtrex_extract(self._h, float(eps), _p(cm, P64), C.byref(k)))
         return cm, int(k.value)
 
-    def decompose(self, eps: float):
-        cm = np.empty(self.n, np.int64)
+    def decompose(self, eps: float, host_map: bool = True):
+        """count -> clean -> extract.  Returns (cluster map, cluster count);
+        with host_map=False the map stays in HBM (returned as None; fetch it
+        later with cluster_map())."""
+        cm = np.empty(self.n, np.int64) if host_map else None
         k = C.c_int64()
-        _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64), C.byref(k)))
+        _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64) if host_map else None,
+                                     C.byref(k)))
         return cm, int(k.value)
+
+    def cluster_map(self) -> np.ndarray:
+        """Cluster id per vertex from the last extraction (int64)."""
+        cm = np.empty(self.n, np.int64)
+        _check(lib().rtrex_get_cluster_map(self._h, _p(cm, P64)))
+        return cm
 
     def count_params(self) -> dict:
         p = CountParams()

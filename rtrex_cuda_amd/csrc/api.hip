@@ -338,6 +338,7 @@ void rtrex_graph_destroy(rtrex_graph g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (void* p : g->allocs) (void)hipFree(p);
     if (g->arena.base && g->arena.owned) (void)hipFree(g->arena.base);
+    if (g->cm_pin) (void)hipHostFree(g->cm_pin);
     for (auto& ev : g->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
@@ -478,8 +479,7 @@ rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
     if (g->phase >= PH_CLEANED && g->eps != eps)
         return fail(RTREX_EINVAL, "extract needs the eps of the initial clean");
     RT_TRY(extract_exact(g, eps));
-    if (cluster_map)
-        std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
+    if (cluster_map) RT_TRY(copy_cluster_map(g, cluster_map));
     if (n_clusters) *n_clusters = g->n_clusters;
     return RTREX_OK;
 }
@@ -496,8 +496,7 @@ rtrex_status rtrex_decompose(rtrex_graph g, double eps, int64_t* cluster_map,
     RT_TRY(run_count(g));
     RT_TRY(run_initial_clean(g, eps));
     RT_TRY(extract_exact(g, eps));
-    if (cluster_map)
-        std::copy(g->cluster_map_host.begin(), g->cluster_map_host.end(), cluster_map);
+    if (cluster_map) RT_TRY(copy_cluster_map(g, cluster_map));
     if (n_clusters) *n_clusters = g->n_clusters;
     return RTREX_OK;
 }
@@ -632,6 +631,12 @@ rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params
     if (best_ms) *best_ms = t_cur;
     if (start_ms) *start_ms = t_start;
     return RTREX_OK;
+}
+
+rtrex_status rtrex_get_cluster_map(rtrex_graph g, int64_t* cluster_map) {
+    RT_TRY(check(g));
+    if (!cluster_map) return fail(RTREX_EINVAL, "null cluster map");
+    return copy_cluster_map(g, cluster_map);
 }
 
 rtrex_status rtrex_get_member_order(rtrex_graph g, int64_t* rank) {

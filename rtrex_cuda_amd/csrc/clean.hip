@@ -141,7 +141,8 @@ struct PeelArgs {
     GraphDev g;
     double eps;
     const uint32_t* F;
-    uint32_t nF;
+    uint32_t nF;               // frontier length, or an upper bound when nFd is set
+    const unsigned* nFd;       // device-side length (no host round trip), or null
     uint32_t* Fn;
     unsigned* nFn;
     uint32_t* near;
@@ -195,9 +196,10 @@ __global__ void __launch_bounds__(T) k_peel_terms(PeelArgs A) {
     const uint32_t per_wave = WAVE / G;
     const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nwaves = gridDim.x * T / WAVE;
-    for (uint32_t base = wave * per_wave; base < A.nF; base += nwaves * per_wave) {
+    const uint32_t nF = A.nFd ? *A.nFd : A.nF;
+    for (uint32_t base = wave * per_wave; base < nF; base += nwaves * per_wave) {
         const uint32_t idx = base + lane / G;
-        const bool act = idx < A.nF;
+        const bool act = idx < nF;
         const uint32_t e = act ? A.F[idx] : 0;
         const uint64_t k0 = act ? tb.zoff[e] : 0;
         const uint64_t k1 = act ? k0 + g.tcnt[e] : 0;
@@ -266,7 +268,8 @@ __global__ void __launch_bounds__(T) k_alive_buckets(GraphDev g, uint32_t* __res
 // and every decision within that bound is certified from the terms.
 template <int G>
 __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __restrict__ list,
-                                                 uint32_t cnt) {
+                                                 const unsigned* __restrict__ cntp) {
+    const uint32_t cnt = *cntp;
     const GraphDev& g = A.g;
     const TermBuf& tb = A.tb;
     const int lane = lane_id();
@@ -389,9 +392,10 @@ __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
     const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nwaves = gridDim.x * T / WAVE;
     unsigned long long removed = 0;
-    for (uint32_t base = wave * per_wave; base < A.nF; base += nwaves * per_wave) {
+    const uint32_t nF = A.nFd ? *A.nFd : A.nF;
+    for (uint32_t base = wave * per_wave; base < nF; base += nwaves * per_wave) {
         const uint32_t idx = base + lane / G;
-        bool act = idx < A.nF;
+        bool act = idx < nF;
         const uint32_t e = act ? A.F[idx] : 0;
         const uint32_t x = act ? g.esrc[e] : 0;
         const uint32_t y = act ? g.nb[g.eslot[e]] : 0;
@@ -467,8 +471,9 @@ __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
 
 // Certifies the near-threshold edges of the round (statuses: the frontier is
 // QUEUED, so its triangles count as dead).
-__global__ void __launch_bounds__(T) k_near_check(PeelArgs A, uint32_t nNear) {
+__global__ void __launch_bounds__(T) k_near_check(PeelArgs A) {
     const GraphDev& g = A.g;
+    const uint32_t nNear = *A.nNear;
     unsigned long long amb = 0;
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nNear; i += gridDim.x * T) {
         const uint32_t f = A.near[i];
@@ -491,7 +496,9 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
                                                  const uint32_t* __restrict__ F,
                                                  uint32_t nF,
                                                  const uint32_t* __restrict__ Fn,
-                                                 uint32_t nFn, Counters* ctr) {
+                                                 const unsigned* __restrict__ nFnp,
+                                                 Counters* ctr) {
+    const uint32_t nFn = *nFnp;
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < max(nF, nFn);
          i += gridDim.x * T) {
         if (i < nF) {
@@ -846,6 +853,8 @@ rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint
 }
 
 // Blocks for `items` work items at G lanes each (capped; kernels stride).
+constexpr unsigned NEAR_BLOCKS = 64;  // k_near_check: grid-stride over the near list
+
 unsigned grid_items(uint32_t items, int G) {
     const uint64_t waves = ((uint64_t)items * G + WAVE - 1) / WAVE;
     return (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (waves * WAVE + T - 1) / T), 8192);
@@ -866,7 +875,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_HIP(hipMemsetAsync(h->nearst, 0, (uint64_t)g.m * 4, s));
     }
     unsigned* cnts = &h->ctr->frontier_next;  // [0] next frontier, [3] near
-    PeelArgs A;
+    PeelArgs A{};
     A.g = g;
     A.eps = eps;
     A.nFn = cnts;
@@ -880,87 +889,90 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     uint32_t* bk;  // bucketed frontier: 4 x m ids + 4 counters
     RT_TRY(sc.get(&bk, 4ull * g.m + 4));
     unsigned* bcur = bk + 4ull * g.m;
+    // One host round trip per round (the next frontier's length, read after
+    // the round ends); every launch in between takes its item count from
+    // device memory and is sized from a host-side upper bound.
     while (nF) {
         RT_HIP(hipMemsetAsync(A.nFn, 0, sizeof(unsigned), s));
         RT_HIP(hipMemsetAsync(A.nNear, 0, sizeof(unsigned), s));
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
-        uint32_t bc[4] = {nF, 0, 0, 0};
-        const uint32_t* lists[4] = {F, nullptr, nullptr, nullptr};
-        const bool pull_round = h->zw != nullptr && (uint64_t)nF > h->n_alive;
-        if (pull_round) {
-            first_bk = nullptr;  // pull rounds scan the alive edges instead
-        } else if (first_bk) {  // first round, bucketed while it was seeded
-            RT_HIP(hipMemcpyAsync(bc, first_bk + 4ull * g.m, sizeof(bc),
-                                  hipMemcpyDeviceToHost, s));
-            RT_HIP(hipStreamSynchronize(s));
-            for (int b = 0; b < 4; ++b) lists[b] = first_bk + (uint64_t)b * g.m;
-            first_bk = nullptr;
-        } else if (nF > 4096) {  // large frontiers: one launch per work bucket
-            RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
-            k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.w != nullptr,
-                                                                     F, nF, bk, bcur);
-            RT_HIP(hipMemcpyAsync(bc, bcur, sizeof(bc), hipMemcpyDeviceToHost, s));
-            RT_HIP(hipStreamSynchronize(s));
-            for (int b = 0; b < 4; ++b) lists[b] = bk + (uint64_t)b * nF;
-        } else {
-            bc[0] = 0;
-            bc[2] = nF;
-            lists[2] = F;
-        }
         const bool pull = A.tb.w != nullptr && (uint64_t)nF > h->n_alive;
         if (pull) {
+            first_bk = nullptr;  // pull rounds scan the alive edges instead
             A.F = F;
             A.nF = nF;
+            A.nFd = nullptr;
             RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
             k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, bk, bcur);
-            uint32_t ac[4] = {0, 0, 0, 0};
-            RT_HIP(hipMemcpyAsync(ac, bcur, sizeof(ac), hipMemcpyDeviceToHost, s));
-            RT_HIP(hipStreamSynchronize(s));
-            for (int b = 0; b < 4; ++b) {
-                if (!ac[b]) continue;
+            for (int b = 0; b < 4 && h->n_alive; ++b) {
                 const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
                 const uint32_t* L = bk + (uint64_t)b * g.m;
-                const unsigned grid = grid_items(ac[b], G);
-                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, ac[b]);
-                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, ac[b]);
-                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, ac[b]);
-                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, ac[b]);
+                const unsigned grid = grid_items((uint32_t)h->n_alive, G);
+                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, bcur + b);
+                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, bcur + b);
+                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, bcur + b);
+                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, bcur + b);
             }
             k_collect_claims<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, Fn,
                                                                                          A.nFn);
-        }
-        for (int b = 0; b < 4 && !pull; ++b) {
-            if (!bc[b]) continue;
-            const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
-            A.F = lists[b];
-            A.nF = bc[b];
-            const unsigned grid = grid_items(bc[b], G);
-            if (A.tb.w) {
-                if (G == 1) k_peel_terms<1><<<grid, T, 0, s>>>(A);
-                else if (G == 4) k_peel_terms<4><<<grid, T, 0, s>>>(A);
-                else if (G == 16) k_peel_terms<16><<<grid, T, 0, s>>>(A);
-                else k_peel_terms<64><<<grid, T, 0, s>>>(A);
+        } else {
+            // Work buckets of the frontier (1, 4, 16 or 64 lanes per edge):
+            // split while it was seeded (first round), split here when large,
+            // otherwise one list at 16 lanes.
+            const uint32_t* lists[4] = {nullptr, nullptr, nullptr, nullptr};
+            const unsigned* cnts[4] = {nullptr, nullptr, nullptr, nullptr};
+            if (first_bk) {
+                for (int b = 0; b < 4; ++b) {
+                    lists[b] = first_bk + (uint64_t)b * g.m;
+                    cnts[b] = first_bk + 4ull * g.m + b;
+                }
+                first_bk = nullptr;
+            } else if (nF > 4096) {
+                RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
+                k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.w != nullptr,
+                                                                         F, nF, bk, bcur);
+                for (int b = 0; b < 4; ++b) {
+                    lists[b] = bk + (uint64_t)b * nF;
+                    cnts[b] = bcur + b;
+                }
             } else {
-                if (G == 1) k_peel<1><<<grid, T, 0, s>>>(A);
-                else if (G == 4) k_peel<4><<<grid, T, 0, s>>>(A);
-                else if (G == 16) k_peel<16><<<grid, T, 0, s>>>(A);
-                else k_peel<64><<<grid, T, 0, s>>>(A);
+                lists[2] = F;
             }
+            for (int b = 0; b < 4; ++b) {
+                if (!lists[b]) continue;
+                const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
+                A.F = lists[b];
+                A.nF = nF;
+                A.nFd = cnts[b];
+                const unsigned grid = grid_items(nF, G);
+                if (A.tb.w) {
+                    if (G == 1) k_peel_terms<1><<<grid, T, 0, s>>>(A);
+                    else if (G == 4) k_peel_terms<4><<<grid, T, 0, s>>>(A);
+                    else if (G == 16) k_peel_terms<16><<<grid, T, 0, s>>>(A);
+                    else k_peel_terms<64><<<grid, T, 0, s>>>(A);
+                } else {
+                    if (G == 1) k_peel<1><<<grid, T, 0, s>>>(A);
+                    else if (G == 4) k_peel<4><<<grid, T, 0, s>>>(A);
+                    else if (G == 16) k_peel<16><<<grid, T, 0, s>>>(A);
+                    else k_peel<64><<<grid, T, 0, s>>>(A);
+                }
+            }
+            A.F = F;
+            A.nF = nF;
+            A.nFd = nullptr;
         }
-        A.F = F;
-        A.nF = nF;
-        uint32_t nNear = 0;
-        RT_HIP(hipMemcpyAsync(&nNear, A.nNear, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
-        if (nNear)
-            k_near_check<<<blocks_for(nNear, T, 4096), T, 0, s>>>(A, nNear);
+        // Near-threshold edges (a few hundred at most in practice): certified
+        // from their terms; count from device memory.
+        k_near_check<<<NEAR_BLOCKS, T, 0, s>>>(A);
+        // The frontier dies, the claimed edges are queued.  Both lists are
+        // bounded by the edges alive at the start of the round.
+        k_end_round<<<blocks_for(std::max<uint64_t>(nF, h->n_alive), T, 4096), T, 0, s>>>(
+            g, F, nF, Fn, A.nFn, h->ctr);
+        RT_HIP(hipGetLastError());
         uint32_t nFn = 0;
         RT_HIP(hipMemcpyAsync(&nFn, A.nFn, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
-        k_end_round<<<blocks_for(std::max(nF, nFn), T, 4096), T, 0, s>>>(
-            g, F, nF, Fn, nFn, h->ctr);
-        RT_HIP(hipGetLastError());
         h->audit.clean_rounds++;
         h->n_alive -= nFn;
         std::swap(F, Fn);

@@ -654,6 +654,12 @@ __global__ void __launch_bounds__(T) k_seed_flags(uint32_t n,
     }
 }
 
+__global__ void __launch_bounds__(T) k_gather_u32(const uint32_t* __restrict__ src,
+                                                  const uint32_t* __restrict__ idx, uint32_t k,
+                                                  uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < k; i += gridDim.x * T) out[i] = src[idx[i]];
+}
+
 __global__ void __launch_bounds__(T) k_cluster_ids(uint32_t n,
                                                    const uint32_t* __restrict__ cseed,
                                                    const uint32_t* __restrict__ rank,
@@ -868,7 +874,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                                   hipMemcpyDeviceToHost, s));
         h->x_undec_all = nu > g.m;
     }
-    // cluster ids in seed order
+    // cluster ids in seed order; they stay in HBM (copy_cluster_map)
     uint32_t *flag2, *cidr;
     RT_TRY(sc.get(&flag2, n));
     RT_TRY(sc.get(&cidr, (uint64_t)n + 1));
@@ -878,38 +884,60 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     RT_HIP(hipMemsetAsync(&h->ctr->error, 0, sizeof(unsigned), s));
     k_cluster_ids<<<bn, T, 0, s>>>(n, h->cseed, rank, cidr, h->cid, &h->ctr->error);
     RT_HIP(hipGetLastError());
-    unsigned cerr = 0;
-    RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
-    if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
-    std::vector<uint32_t> cid(n), cidr_h;
-    uint32_t total = 0;
-    RT_HIP(hipMemcpyAsync(cid.data(), h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipMemcpyAsync(&total, cidr + n, 4, hipMemcpyDeviceToHost, s));
-    if (!h->seg_vtx.empty()) {
-        cidr_h.resize((uint64_t)n + 1);
-        RT_HIP(hipMemcpyAsync(cidr_h.data(), cidr, ((uint64_t)n + 1) * 4,
-                              hipMemcpyDeviceToHost, s));
+    const size_t ns = h->seg_vtx.empty() ? 0 : h->seg_vtx.size() - 1;
+    uint32_t* bounds_d = nullptr;
+    if (ns) {
+        // Ranks of segment q start at seg_vtx[q]: its clusters are
+        // [cidr[seg_vtx[q]], cidr[seg_vtx[q + 1]]).
+        if (!h->seg_pos_d) {
+            std::vector<uint32_t> sp(h->seg_vtx.begin(), h->seg_vtx.end());
+            RT_TRY(dalloc(h, &h->seg_pos_d, ns + 1));
+            RT_HIP(hipMemcpyAsync(h->seg_pos_d, sp.data(), (ns + 1) * 4, hipMemcpyHostToDevice, s));
+            RT_HIP(hipStreamSynchronize(s));
+        }
+        RT_TRY(sc.get(&bounds_d, ns + 1));
+        k_gather_u32<<<blocks_for(ns + 1, T, 1024), T, 0, s>>>(cidr, h->seg_pos_d,
+                                                             (uint32_t)(ns + 1), bounds_d);
+        RT_HIP(hipGetLastError());
     }
+    unsigned cerr = 0;
+    uint32_t total = 0;
+    RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(&total, cidr + n, 4, hipMemcpyDeviceToHost, s));
+    h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
+    if (ns)
+        RT_HIP(hipMemcpyAsync(h->seg_bounds.data(), bounds_d, (ns + 1) * 4,
+                              hipMemcpyDeviceToHost, s));
     RT_HIP(hipEventRecord(h->ev[6], s));
     RT_HIP(hipEventSynchronize(h->ev[6]));
+    if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
     float ms = 0;
     RT_HIP(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
     h->tm.extract_ms = ms;
-    h->cluster_map_host.assign(cid.begin(), cid.end());
     h->n_clusters = total;
-    if (!h->seg_vtx.empty()) {
-        // Per-graph ids: ranks of segment q start at seg_vtx[q].
-        const size_t ns = h->seg_vtx.size() - 1;
-        h->seg_clusters.assign(ns, 0);
-        for (size_t q = 0; q < ns; ++q) {
-            const uint32_t b0 = cidr_h[h->seg_vtx[q]], b1 = cidr_h[h->seg_vtx[q + 1]];
-            h->seg_clusters[q] = (int64_t)b1 - b0;
-            for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v)
-                h->cluster_map_host[v] -= b0;
-        }
-    }
+    h->seg_clusters.assign(ns, 0);
+    for (size_t q = 0; q < ns; ++q)
+        h->seg_clusters[q] = (int64_t)h->seg_bounds[q + 1] - h->seg_bounds[q];
     h->phase = PH_EXTRACTED;
+    return RTREX_OK;
+}
+
+rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out) {
+    if (h->phase < PH_EXTRACTED || !h->cid) return fail(RTREX_ESTATE, "extract has not run");
+    const uint32_t n = h->g.n;
+    if (!n) return RTREX_OK;
+    if (!h->cm_pin) RT_HIP(hipHostMalloc((void**)&h->cm_pin, (uint64_t)n * 4, hipHostMallocDefault));
+    RT_HIP(hipMemcpyAsync(h->cm_pin, h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, h->stream));
+    RT_HIP(hipStreamSynchronize(h->stream));
+    const size_t ns = h->seg_bounds.empty() ? 0 : h->seg_bounds.size() - 1;
+    if (!ns) {
+        for (uint32_t v = 0; v < n; ++v) out[v] = h->cm_pin[v];
+        return RTREX_OK;
+    }
+    for (size_t q = 0; q < ns; ++q) {  // per-graph ids
+        const int64_t b0 = h->seg_bounds[q];
+        for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v) out[v] = h->cm_pin[v] - b0;
+    }
     return RTREX_OK;
 }
 

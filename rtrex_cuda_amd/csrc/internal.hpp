@@ -157,7 +157,11 @@ struct rtrex_graph_s {
     double force_eps = -1;  // the eps force_replay was found at
     int64_t n_clusters = -1;
     std::vector<int64_t> seg_clusters;  // per segment (batch)
-    std::vector<int64_t> cluster_map_host;
+    // Cluster ids stay in HBM (cid); copy_cluster_map brings them to the host
+    // on request through this pinned staging buffer (n entries, lazily).
+    uint32_t* cm_pin = nullptr;
+    std::vector<uint32_t> seg_bounds;  // batch: first cluster id of each segment, and the total
+    uint32_t* seg_pos_d = nullptr;     // batch: seg_vtx on the device (u32)
     // Batch layout (one graph = one segment): vertex segment boundaries.
     std::vector<int64_t> seg_vtx;  // empty for a single graph
     uint32_t* nearst = nullptr;  // per edge: last peel round it was near-threshold
@@ -343,6 +347,9 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps);
 rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                                   int64_t k, double eps);
 rtrex_status run_extract(rtrex_graph h, double eps);
+// The cluster map of the last extraction, as int64 per vertex (per-graph ids
+// on a batch handle).
+rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
 // Per-vertex supports of the listed vertices (all when list is null) on

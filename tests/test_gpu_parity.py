@@ -261,3 +261,23 @@ def test_batch_equals_alone():
         assert k == alone["k"]
         assert np.array_equal(cm, alone["cm"])
     assert au["ambiguous_clean"] == 0
+
+
+def test_cluster_map_kept_in_hbm():
+    """decompose without a host buffer leaves the map on the device; fetching
+    it afterwards gives the same map as the host form, also per graph on a
+    batch handle."""
+    n, u, v = golden_graph("clean_astro")
+    with R.DeviceGraph.from_edges(n, u, v) as g:
+        cm, k = g.decompose(0.3)
+        none, k2 = g.decompose(0.3, host_map=False)
+        assert none is None and k2 == k
+        assert np.array_equal(g.cluster_map(), cm)
+    graphs = [golden_graph(t) for t in ("small_test", "clique_k7")] + [G.config4_graph(1)]
+    b = R.Batch(graphs)
+    try:
+        maps, counts = b.decompose(0.3)
+        flat = b.cluster_map()
+        assert np.array_equal(flat, np.concatenate(maps))
+    finally:
+        b.close()
