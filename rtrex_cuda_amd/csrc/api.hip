@@ -173,7 +173,7 @@ namespace rtrex {
 // each rounded up to 256 bytes.
 uint64_t resident_bytes(uint64_t n, uint64_t m) {
     const uint64_t nnz = 2 * m;
-    return 29 * nnz + 99 * m + 68 * n + 96 * 256 + 8192 + 64;
+    return 29 * nnz + 91 * m + 68 * n + 96 * 256 + 8192 + 64;
 }
 // The largest phase scratch: the build (edge upload, radix keys, partner
 // ranks), the hub plan, the clean (frontiers and buckets) or the extraction
@@ -457,7 +457,7 @@ rtrex_status extract_exact(rtrex_graph g, double eps) {
         g->force_eps = eps;
     }
     RT_TRY(run_extract(g, eps));
-    for (int tries = 0; tries < 3 && g->zw && (!g->x_undec.empty() || g->x_undec_all); ++tries) {
+    for (int tries = 0; tries < 3 && g->zt && (!g->x_undec.empty() || g->x_undec_all); ++tries) {
         if (g->x_undec_all) g->force_all = true;
         g->force_replay.insert(g->force_replay.end(), g->x_undec.begin(), g->x_undec.end());
         std::sort(g->force_replay.begin(), g->force_replay.end());

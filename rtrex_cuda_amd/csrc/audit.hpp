@@ -149,9 +149,9 @@ __device__ int certify_terms(const GraphDev& g, const TermBuf& tb, uint32_t f,
     const uint64_t k0 = tb.zoff[f], k1 = k0 + g.tcnt[f];
     CertAcc acc(s0);
     for (uint64_t k = k0; k < k1; ++k) {
-        const uint32_t a = tb.sa[k], b = tb.sb[k];
-        if (tri_dead(g.est[a], g.est[b], qdead) || a == extra_edge || b == extra_edge)
-            acc.add(tb.w[k]);
+        const Term q = tb.t[k];
+        if (tri_dead(g.est[q.sa], g.est[q.sb], qdead) || q.sa == extra_edge || q.sb == extra_edge)
+            acc.add(q.w);
     }
     return acc.finish(s0, thr);
 }
@@ -162,7 +162,7 @@ __device__ int certify_terms(const GraphDev& g, const TermBuf& tb, uint32_t f,
 __device__ __forceinline__ int certify_edge(const GraphDev& g, const TermBuf& tb,
                                             uint32_t f, double thr, bool qdead,
                                             uint32_t extra_vertex, uint32_t extra_edge) {
-    if (tb.w) return certify_terms(g, tb, f, thr, qdead, extra_edge);
+    if (tb.t) return certify_terms(g, tb, f, thr, qdead, extra_edge);
     const uint32_t sl = g.eslot[f];
     return certify(g, g.esrc[f], g.nb[sl], g.sup0[f], thr, qdead, extra_vertex);
 }

@@ -208,13 +208,14 @@ __global__ void __launch_bounds__(T) k_peel_terms(PeelArgs A) {
             const bool go = act && k - gl < k1;
             const bool in = go && k < k1;
             if (in) {
-                const uint32_t fj = tb.sa[k], fh = tb.sb[k];
+                const Term q = tb.t[k];
+                const uint32_t fj = q.sa, fh = q.sb;
                 const uint8_t sj = g.est[fj], sh = g.est[fh];
                 // Only a triangle with an alive edge needs work: that edge
                 // loses t, once, by the smallest frontier edge id.
                 if (sj != ST_DEAD && sh != ST_DEAD && (sj == ST_ALIVE || sh == ST_ALIVE) &&
                     !(sj == ST_QUEUED && fj < e) && !(sh == ST_QUEUED && fh < e)) {
-                    const double t = tb.w[k];
+                    const double t = q.w;
                     peel_side(A, fj, sj, t);
                     peel_side(A, fh, sh, t);
                 }
@@ -290,14 +291,15 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
         constexpr int U = 4;
         for (uint64_t k = k0 + gl; k < k1; k += (uint64_t)U * G) {
             uint32_t fa[U], fb[U];
+            double tw[U];
             uint8_t sa_[U], sb_[U];
 #pragma unroll
             for (int r = 0; r < U; ++r) {
                 const uint64_t q = k + (uint64_t)r * G;
-                // Streamed once per round: non-temporal, so that the
-                // stream does not push the status bytes out of L2.
-                fa[r] = q < k1 ? __builtin_nontemporal_load(tb.sa + q) : 0;
-                fb[r] = q < k1 ? __builtin_nontemporal_load(tb.sb + q) : 0;
+                const Term rec = q < k1 ? tb.t[q] : Term{0.0, 0u, 0u};
+                fa[r] = rec.sa;
+                fb[r] = rec.sb;
+                tw[r] = rec.w;
             }
 #pragma unroll
             for (int r = 0; r < U; ++r) {
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
             for (int r = 0; r < U; ++r) {
                 const uint8_t a = sa_[r], b = sb_[r];
                 if ((a == ST_QUEUED || b == ST_QUEUED) && a != ST_DEAD && b != ST_DEAD) {
-                    d += __builtin_nontemporal_load(tb.w + k + (uint64_t)r * G);
+                    d += tw[r];
                     any = true;
                 }
             }
@@ -376,7 +378,7 @@ __global__ void __launch_bounds__(T) k_live_incidences(GraphDev g, TermBuf tb,
         if (g.est[e] != ST_ALIVE) continue;
         const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
         for (uint64_t k = k0; k < k1; ++k)
-            c += g.est[tb.sa[k]] == ST_ALIVE && g.est[tb.sb[k]] == ST_ALIVE;
+            c += g.est[tb.t[k].sa] == ST_ALIVE && g.est[tb.t[k].sb] == ST_ALIVE;
     }
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
     if (lane_id() == 0 && c) atomicAdd(out, c);
@@ -596,7 +598,7 @@ __global__ void __launch_bounds__(T) k_rg_hook(GraphDev g, TermBuf tb, uint32_t*
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
         const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
         for (uint64_t k = k0; k < k1; ++k) {
-            const uint32_t a = tb.sa[k], b = tb.sb[k];
+            const uint32_t a = tb.t[k].sa, b = tb.t[k].sb;
             if (e < a && e < b) {
                 rg_union(parent, e, a);
                 rg_union(parent, e, b);
@@ -721,14 +723,15 @@ __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
             for (uint64_t p = k0; p < k1; p += 64) {
                 const uint64_t k = p + lane;
                 const bool in = k < k1;
-                const uint32_t fa = in ? tb.sa[k] : 0, fb = in ? tb.sb[k] : 0;
+                const Term q = in ? tb.t[k] : Term{0.0, 0u, 0u};
+                const uint32_t fa = q.sa, fb = q.sb;
                 const uint32_t jl = swap ? fb : fa, jh = swap ? fa : fb;
                 const uint8_t sl = in ? g.est[jl] : (uint8_t)ST_DEAD;
                 const uint8_t sh = in ? g.est[jh] : (uint8_t)ST_DEAD;
                 const bool alive = in && sl != ST_DEAD && sh != ST_DEAD;
                 bool pl = false, ph = false;
                 if (alive) {
-                    const double t = tb.w[k];
+                    const double t = q.w;
                     const double vl = g.sup[jl] - t, vh = g.sup[jh] - t;
                     g.sup[jl] = vl;
                     g.sup[jh] = vh;
@@ -793,7 +796,7 @@ rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint
     unsigned nund = 0;
     RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    if (!nund || !h->zw) return RTREX_OK;
+    if (!nund || !h->zt) return RTREX_OK;
     Scratch sc;
     uint32_t *parent, *redge, *rflag, *rpos, *rstart;
     uint64_t *keys, *tmp, *stk;
@@ -897,7 +900,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_HIP(hipMemsetAsync(A.nNear, 0, sizeof(unsigned), s));
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
-        const bool pull = A.tb.w != nullptr && (uint64_t)nF > h->n_alive;
+        const bool pull = A.tb.t != nullptr && (uint64_t)nF > h->n_alive;
         if (pull) {
             first_bk = nullptr;  // pull rounds scan the alive edges instead
             A.F = F;
@@ -930,7 +933,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
                 first_bk = nullptr;
             } else if (nF > 4096) {
                 RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
-                k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.w != nullptr,
+                k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.t != nullptr,
                                                                          F, nF, bk, bcur);
                 for (int b = 0; b < 4; ++b) {
                     lists[b] = bk + (uint64_t)b * nF;
@@ -946,7 +949,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
                 A.nF = nF;
                 A.nFd = cnts[b];
                 const unsigned grid = grid_items(nF, G);
-                if (A.tb.w) {
+                if (A.tb.t) {
                     if (G == 1) k_peel_terms<1><<<grid, T, 0, s>>>(A);
                     else if (G == 4) k_peel_terms<4><<<grid, T, 0, s>>>(A);
                     else if (G == 16) k_peel_terms<16><<<grid, T, 0, s>>>(A);
@@ -986,7 +989,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         k_xact_init<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, h->xact);
         RT_TRY(exact_replay(h, eps, undec, undec_cap));
     }
-    if (A.tb.w) {  // live-triangle total (the peel kernels do not count)
+    if (A.tb.t) {  // live-triangle total (the peel kernels do not count)
         Scratch sc2;
         unsigned long long* inc;
         RT_TRY(sc2.get(&inc, 1));
@@ -1018,7 +1021,7 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     uint32_t *F, *Fn, *bk = nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
-    const bool terms = h->zw != nullptr;
+    const bool terms = h->zt != nullptr;
     if (terms) RT_TRY(sc.get(&bk, 4ull * g.m + 4));
     RT_HIP(hipEventRecord(h->ev[5], s));
     unsigned* cnt = &h->ctr->frontier_next;

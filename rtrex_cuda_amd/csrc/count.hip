@@ -179,29 +179,19 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
         uint32_t lo = dx <= dy ? x : y, hi = dx <= dy ? y : x;
         uint32_t p = act ? g.off[lo] : 0, le = act ? g.off[lo + 1] : 0;
         uint32_t hlo = act ? g.off[hi] : 0, he = act ? g.off[hi + 1] : 0;
-        const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+        const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
         double S = 0.0;
-        uint32_t c = 0, ca = 0, cb = 0;
+        uint32_t c = 0;
         while (__any(act && p < le)) {
             bool go = act && p < le;
             uint32_t z, hp;
             bool hit = probe_chunk<G>(g.nb, go, p, le, hlo, he, z, &hp);
             double t = hit ? tri_w(dx, dy, g.deg[z]) : 0.0;
-            if (tb.w) {
+            if (tb.t) {
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
-                if (hit) {
-                    const uint64_t k = wbase + c + __popcll(hm & below);
-                    tb.w[k] = t;
-                    tb.sa[k] = g.s2e[p + gl];  // edge {lo, z}
-                    tb.sb[k] = g.s2e[hp];      // edge {hi, z}
-                }
-                const uint64_t am = G == 64 ? ballot64(hit && z < x)
-                                            : (ballot64(hit && z < x) >> gbase) & ((1ull << G) - 1);
-                const uint64_t bm = G == 64 ? ballot64(hit && z < y)
-                                            : (ballot64(hit && z < y) >> gbase) & ((1ull << G) - 1);
-                ca += (uint32_t)__popcll(am);
-                cb += (uint32_t)__popcll(bm);
+                if (hit)  // edges {lo, z} and {hi, z}
+                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, g.s2e[p + gl], g.s2e[hp]};
             }
             group_fold<G>(hit, t, S, c);
             if (go) p += G;
@@ -209,9 +199,6 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
         if (act && gl == 0) {
             g.sup[e] = S;
             g.tcnt[e] = c;
-            if (tb.w) {
-                tb.zab[e] = make_uint2(ca, cb);
-            }
         }
     }
 }
@@ -418,15 +405,13 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
         const uint32_t idx = base + lane / G;
         const bool act = idx < cnt;
         const uint32_t li = act ? sown[idx] : 0;
-        const uint32_t y = snb[li];  // lo endpoint
-        const uint32_t e = sed[li];
+        const uint32_t e = sed[li];  // edge {hub, lo}
         const uint32_t dy = sdg[li];
-        const uint32_t ex = min(h, y), ey = max(h, y);
         uint32_t p = sof[li];
         const uint32_t le = act ? p + dy : p;
-        const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+        const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
         double S = 0.0;
-        uint32_t c = 0, ca = 0, cb = 0;
+        uint32_t c = 0;
         while (__any(p < le)) {
             uint32_t zr[R], er[R];
 #pragma unroll
@@ -446,18 +431,8 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
-                if (tb.w && hit) {
-                    const uint64_t k = wbase + c + __popcll(hm & below);
-                    tb.w[k] = t;
-                    tb.sa[k] = er[r];
-                    tb.sb[k] = sed[pos];  // edge {hub, z}
-                }
-                const uint64_t am = G == 64 ? ballot64(hit && z < ex)
-                                            : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
-                const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
-                                            : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
-                ca += (uint32_t)__popcll(am);
-                cb += (uint32_t)__popcll(bm);
+                if (tb.t && hit)  // edges {lo, z} and {hub, z}
+                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], sed[pos]};
                 c += (uint32_t)__popcll(hm);
                 // Ordered fold: the group's weights in lane order, a miss
                 // adding 0.0 (which leaves the sum's bits unchanged).
@@ -484,7 +459,6 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
         if (act && gl == 0) {
             g.sup[e] = S;
             g.tcnt[e] = c;
-            if (tb.w) tb.zab[e] = make_uint2(ca, cb);
         }
     }
 }
@@ -599,15 +573,13 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
             const uint32_t li = act ? own_idx[k0 + idx] : 0;
             const uint2 ye = act ? g.nbe[oh + li] : make_uint2(0u, 0u);
             const uint2 yo = act ? g.ndo[oh + li] : make_uint2(0u, 0u);
-            const uint32_t y = ye.x;  // lo endpoint
-            const uint32_t e = ye.y;
+            const uint32_t e = ye.y;  // edge {hub, lo}
             const uint32_t dy = yo.x;
-            const uint32_t ex = min(h, y), ey = max(h, y);
             uint32_t p = yo.y;
             const uint32_t le = p + dy;
-            const uint64_t wbase = (act && tb.w) ? tb.zoff[e] : 0;
+            const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
             double S = 0.0;
-            uint32_t c = 0, ca = 0, cb = 0;
+            uint32_t c = 0;
             while (__any(p < le)) {
                 uint32_t zr[R], er[R];
 #pragma unroll
@@ -641,18 +613,8 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
                     const double t = hit ? tri_w(dh, dy, zd.x) : 0.0;
                     const uint64_t hm = G == 64 ? ballot64(hit)
                                                 : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
-                    if (tb.w && hit) {
-                        const uint64_t k = wbase + c + __popcll(hm & below);
-                        tb.w[k] = t;
-                        tb.sa[k] = er[r];
-                        tb.sb[k] = g.nbe[oh + pos].y;  // edge {hub, z}
-                    }
-                    const uint64_t am = G == 64 ? ballot64(hit && z < ex)
-                                                : (ballot64(hit && z < ex) >> gbase) & ((1ull << G) - 1);
-                    const uint64_t bm = G == 64 ? ballot64(hit && z < ey)
-                                                : (ballot64(hit && z < ey) >> gbase) & ((1ull << G) - 1);
-                    ca += (uint32_t)__popcll(am);
-                    cb += (uint32_t)__popcll(bm);
+                    if (tb.t && hit)  // edges {lo, z} and {hub, z}
+                        tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], g.nbe[oh + pos].y};
                     group_fold<G>(hit, t, S, c);
                     if (go) p += G;
                 }
@@ -660,7 +622,6 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
             if (act && gl == 0) {
                 g.sup[e] = S;
                 g.tcnt[e] = c;
-                if (tb.w) tb.zab[e] = make_uint2(ca, cb);
             }
         }
         __syncthreads();  // the next unit rebuilds the table
@@ -668,23 +629,37 @@ __global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
 }
 
 // Segment of vertex v's fold contributed by its neighbour slot j: the stored
-// terms of edge {v, y} with z > y.
+// terms of edge {v, y} with z > y.  The list is ascending in z, so the segment
+// starts after the terms with z < y, found by binary search; the third vertex
+// of a term is the endpoint of its edge sa = {lo, z} other than lo.
 __device__ __forceinline__ void vseg(const GraphDev& g, const TermBuf& tb,
                                      uint32_t v, uint32_t j, uint64_t& start,
                                      uint32_t& len) {
     const uint32_t y = g.nb[j];
     const uint32_t e = g.s2e[j];
-    const uint2 ab = tb.zab[e];
-    const uint32_t skip = v < y ? ab.y : ab.x;
-    start = tb.zoff[e] + skip;
-    len = g.tcnt[e] - skip;
+    const uint32_t x0 = min(v, y), y0 = max(v, y);
+    const uint32_t lo = g.deg[x0] <= g.deg[y0] ? x0 : y0;
+    const uint64_t k0 = tb.zoff[e];
+    const uint32_t cnt = g.tcnt[e];
+    uint32_t a = 0, b = cnt;
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        const uint32_t f = tb.t[k0 + mid].sa;
+        const uint32_t fx = g.esrc[f];
+        const uint32_t z = fx == lo ? g.nb[g.eslot[f]] : fx;
+        if (z < y) a = mid + 1;
+        else b = mid;
+    }
+    start = k0 + a;
+    len = cnt - a;
 }
 
 // Dead-triangle test of term k of the segment of slot j (PULL mode): the
 // triangle is gone once any of its three edges is deleted.
 __device__ __forceinline__ bool term_dead(const GraphDev& g, const TermBuf& tb,
                                           uint32_t j, uint64_t k) {
-    return g.st[j] == ST_DEAD || g.est[tb.sa[k]] == ST_DEAD || g.est[tb.sb[k]] == ST_DEAD;
+    const Term q = tb.t[k];
+    return g.st[j] == ST_DEAD || g.est[q.sa] == ST_DEAD || g.est[q.sb] == ST_DEAD;
 }
 
 // One lane per (low-degree) vertex: a plain serial fold over its segments.
@@ -705,9 +680,9 @@ __global__ void __launch_bounds__(T) k_vfold_lane(GraphDev g, TermBuf tb,
             vseg(g, tb, v, j, st, len);
             for (uint32_t k = 0; k < len; ++k) {
                 if (PULL) {
-                    if (term_dead(g, tb, j, st + k)) S -= tb.w[st + k];
+                    if (term_dead(g, tb, j, st + k)) S -= tb.t[st + k].w;
                 } else {
-                    S += tb.w[st + k];
+                    S += tb.t[st + k].w;
                 }
             }
             c += len;
@@ -770,7 +745,7 @@ __global__ void __launch_bounds__(VT) k_vfold_wave(GraphDev g, TermBuf tb,
                         else hi = mid - 1;
                     }
                     const uint64_t kk = sst[lo] + (k - spre[lo]);
-                    val = tb.w[kk];
+                    val = tb.t[kk].w;
                     if (PULL && !term_dead(g, tb, sslot[lo], kk)) val = 0.0;
                 }
                 const uint32_t nk = min((uint32_t)WAVE, K - k0);
@@ -974,13 +949,12 @@ rtrex_status prepare_hubs(rtrex_graph h) {
 
 }  // namespace
 
-// Term buffer of `total` records (f64 weight + two u32 edge ids) when it
-// fits in a third of the free HBM.
+// Term buffer of `total` 16-byte records when it fits in a third of the free
+// HBM.
 bool try_alloc_terms(rtrex_graph h, uint64_t total) {
-    if (h->slab && h->slab_top + (total + 1) * 16 + 768 <= h->slab_cap) {
-        h->zw = static_cast<double*>(slab_take(h, (total + 1) * 8));
-        h->zsa = static_cast<uint32_t*>(slab_take(h, (total + 1) * 4));
-        h->zsb = static_cast<uint32_t*>(slab_take(h, (total + 1) * 4));
+    const uint64_t bytes = (total + 1) * sizeof(Term);
+    if (h->slab && h->slab_top + bytes + 256 <= h->slab_cap) {
+        h->zt = static_cast<Term*>(slab_take(h, bytes));
         return true;
     }
     size_t freeb = 0, totalb = 0;
@@ -988,22 +962,14 @@ bool try_alloc_terms(rtrex_graph h, uint64_t total) {
         (void)hipGetLastError();
         return false;
     }
-    if ((total + 1) * 16 >= freeb / 3) return false;
-    void *q = nullptr, *qa = nullptr, *qb = nullptr;
-    if (hipMalloc(&q, (total + 1) * 8) == hipSuccess &&
-        hipMalloc(&qa, (total + 1) * 4) == hipSuccess &&
-        hipMalloc(&qb, (total + 1) * 4) == hipSuccess) {
+    if (bytes >= freeb / 3) return false;
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) == hipSuccess) {
         h->allocs.push_back(q);
-        h->allocs.push_back(qa);
-        h->allocs.push_back(qb);
-        h->zw = static_cast<double*>(q);
-        h->zsa = static_cast<uint32_t*>(qa);
-        h->zsb = static_cast<uint32_t*>(qb);
+        h->zt = static_cast<Term*>(q);
         return true;
     }
     (void)hipGetLastError();
-    for (void* r : {q, qa, qb})
-        if (r) (void)hipFree(r);
     return false;
 }
 
@@ -1031,7 +997,6 @@ rtrex_status prepare_count(rtrex_graph h) {
     RT_TRY(dalloc(h, &h->red, RED_BLOCKS));
     // term buffer: capacity min(d_x, d_y) - 1 per edge
     RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));
-    RT_TRY(dalloc(h, &h->zab, g.m));
     {
         Scratch sc;
         uint32_t* cap;
@@ -1042,7 +1007,7 @@ rtrex_status prepare_count(rtrex_graph h) {
     uint64_t total = 0;
     RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    h->zw = nullptr;
+    h->zt = nullptr;
     h->terms_exact_pending = false;
     if (try_alloc_terms(h, total)) return RTREX_OK;
     // The upper bound does not fit: size exactly after a counting pass.
@@ -1082,7 +1047,7 @@ rtrex_status vertex_folds(rtrex_graph h, bool pull) {
 // all.  The value is the count-phase fold, then (after a clean) minus the dead
 // triangles in the same order -- the same as computing it eagerly.
 rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_t cnt) {
-    if (!h->zw || h->pv_full) return RTREX_OK;  // intersection paths keep them
+    if (!h->zt || h->pv_full) return RTREX_OK;  // intersection paths keep them
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     const TermBuf tb = h->terms();
@@ -1092,9 +1057,12 @@ rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_
         if (cleaned) RT_TRY(vertex_folds(h, true));
         h->pv_full = true;
     } else if (cnt) {
-        const unsigned grid = blocks_for(cnt, T, 8192);
-        k_vfold_lane<false><<<grid, T, 0, s>>>(g, tb, list, cnt);
-        if (cleaned) k_vfold_lane<true><<<grid, T, 0, s>>>(g, tb, list, cnt);
+        // One wave per candidate: its slots' segment bounds (binary
+        // searches) are found in parallel rather than one after another.
+        const unsigned grid =
+            (unsigned)std::min<uint64_t>(((uint64_t)cnt * WAVE + VT - 1) / VT, 8192);
+        k_vfold_wave<false><<<grid, VT, 0, s>>>(g, tb, list, cnt);
+        if (cleaned) k_vfold_wave<true><<<grid, VT, 0, s>>>(g, tb, list, cnt);
     }
     RT_HIP(hipGetLastError());
     return RTREX_OK;
@@ -1150,7 +1118,7 @@ rtrex_status run_count(rtrex_graph h) {
     if (h->terms_exact_pending) {
         // Counting pass, exact offsets from the triangle counts, then the
         // buffer is filled by the timed pass below (once per graph).
-        edge_pass(TermBuf{nullptr, nullptr, nullptr, h->zoff, h->zab});
+        edge_pass(TermBuf{nullptr, h->zoff});
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(g.tcnt, h->zoff, g.m, s));
         uint64_t total = 0;
@@ -1160,7 +1128,7 @@ rtrex_status run_count(rtrex_graph h) {
         try_alloc_terms(h, total);
         RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
     }
-    TermBuf tb{h->zw, h->zsa, h->zsb, h->zoff, h->zab};
+    TermBuf tb{h->zt, h->zoff};
 
     RT_HIP(hipEventRecord(h->ev[2], s));
     edge_pass(tb);
@@ -1168,7 +1136,7 @@ rtrex_status run_count(rtrex_graph h) {
     auto vp = [&](int b) { return vlist + (uint64_t)b * g.n; };
     const uint32_t* vc = counts + NB;
     h->pv_full = false;
-    if (tb.w) {
+    if (tb.t) {
         // on demand (ensure_vertex_supports)
     } else {
         if (vc[0]) k_count_vertices<1><<<grid_for(vc[0], 1), T, 0, s>>>(g, vp(0), vc[0]);

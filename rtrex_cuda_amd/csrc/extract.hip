@@ -225,7 +225,7 @@ struct XArgs {
     double* gpv;             // [start, ...): prefetched supports (large case)
     uint64_t* wl;            // [dpre[start], ...): deletion worklist (LIFO):
                              // (edge << 32) | first endpoint of the pair
-    TermBuf tb;              // stored triangle lists (tb.w null: intersect)
+    TermBuf tb;              // stored triangle lists (tb.t null: intersect)
     Counters* ctr;
     const uint8_t* xact;     // per edge: post-clean support is the reference's bits
     const uint32_t* bcnt;    // unordered_set bucket count after k insertions
@@ -514,9 +514,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 // vertex, the order the reference visits them), or by
                 // intersecting N(lo) with N(hi).  Either way, per chunk of 64:
                 // slot ja of (lo, w) and jb of (hi, w), both not deleted.
-                const bool terms = A.tb.w != nullptr;
+                const bool terms = A.tb.t != nullptr;
                 const uint32_t tlo = terms ? (dx <= dy ? x : y) : lo;
-                const uint32_t thi = terms ? (dx <= dy ? y : x) : hi;
                 const uint64_t c0 = terms ? A.tb.zoff[e] : g.off[lo];
                 const uint64_t c1 = terms ? c0 + g.tcnt[e] : g.off[lo + 1];
                 uint32_t hlo = g.off[hi];
@@ -529,13 +528,14 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     double t = 0.0;
                     if (terms) {
                         if (in) {
-                            fj = A.tb.sa[kk];
-                            fh = A.tb.sb[kk];
+                            const Term q = A.tb.t[kk];
+                            fj = q.sa;
+                            fh = q.sb;
                             hit = g.est[fj] != ST_DEAD && g.est[fh] != ST_DEAD;
                             if (hit) {
                                 const uint32_t a0 = g.esrc[fj];  // third vertex
                                 w = a0 == tlo ? g.nb[g.eslot[fj]] : a0;
-                                t = A.tb.w[kk];
+                                t = q.w;
                             }
                         }
                     } else {

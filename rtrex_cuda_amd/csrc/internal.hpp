@@ -87,15 +87,15 @@ enum : uint32_t {
 // Triangle list of every edge, written by the count phase: for edge e with
 // lower-degree endpoint lo (ties: the lower id) and the other endpoint hi,
 // terms [zoff[e], zoff[e] + tcnt[e]) hold, ascending in the third vertex z,
-// the weight 1/(d_x d_y d_z) and the edge ids of {lo, z} (sa) and {hi, z} (sb).
-// zab[e] = (za, zb) counts the terms with z below the lower / higher id
-// endpoint (one 8-byte record, written with one store).
+// one 16-byte record each (one vector load or store): the weight
+// 1/(d_x d_y d_z) and the edge ids of {lo, z} (sa) and {hi, z} (sb).
+struct __attribute__((aligned(16))) Term {
+    double w;
+    uint32_t sa, sb;
+};
 struct TermBuf {
-    double* w;  // null when the buffer did not fit (intersection fallbacks)
-    uint32_t* sa;
-    uint32_t* sb;
+    Term* t;  // null when the buffer did not fit (intersection fallbacks)
     const uint64_t* zoff;
-    uint2* zab;
 };
 
 struct Timings {
@@ -184,9 +184,7 @@ struct rtrex_graph_s {
     rtrex::CountParams cparams;
     bool plan_valid = false;  // buckets and hub units built for cparams
     uint64_t* zoff = nullptr;
-    uint2* zab = nullptr;
-    double* zw = nullptr;
-    uint32_t *zsa = nullptr, *zsb = nullptr;
+    rtrex::Term* zt = nullptr;  // term buffer (TermBuf::t)
     bool terms_exact_pending = false;  // size the buffer after a counting pass
     bool pv_full = false;              // per-vertex supports valid everywhere
     uint32_t round_stamp = 0;
@@ -199,7 +197,7 @@ struct rtrex_graph_s {
     char* slab = nullptr;
     uint64_t slab_cap = 0, slab_top = 0;
     bool own_stream = true;
-    rtrex::TermBuf terms() const { return {zw, zsa, zsb, zoff, zab}; }
+    rtrex::TermBuf terms() const { return {zt, zoff}; }
     hipEvent_t ev[8] = {};
 };
 
