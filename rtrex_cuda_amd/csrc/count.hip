@@ -383,19 +383,29 @@ __device__ __forceinline__ void unit_fields(const uint4& U, uint32_t& h, uint32_
     cnt = U.w >> 16;
 }
 
+// Wave-uniform probe index of a hub list of at most 64 ids staged in LDS,
+// padded to 64 with UINT32_MAX (not a vertex id): the last id of each block
+// of 8.  A probe counts the pivots below z in registers, then reads z's one
+// candidate block (two 16-byte LDS reads) -- one LDS round trip instead of
+// the six dependent reads of a binary search.
+struct Piv8 {
+    uint32_t p[8];
+};
+
 // The owned edges [first, cnt) of a unit, stride apart, G lanes per edge:
 // each group streams its edge's shorter list from HBM (R chunks of G ids per
 // load, neighbour and edge id in one 8-byte load) and looks every id up in
 // the hub list staged in LDS: snb (sorted neighbour ids), sed (edge ids),
 // sdg / sof (the neighbours' degrees and list offsets), sown (owned slots).
 // Matches go to the term buffer in ascending z; the support is folded in lane
-// order.  sfold is this wave's 64-entry LDS slice.
-template <int G, int L>
+// order.  sfold is this wave's 64-entry LDS slice.  PIV: snb is a 64-entry
+// padded list (16-byte aligned) probed through pv; otherwise a binary search.
+template <int G, int L, bool PIV>
 __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t h,
                                           uint32_t dh, const uint32_t* snb, const uint32_t* sed,
                                           const uint32_t* sdg, const uint32_t* sof,
                                           const uint32_t* sown, double* sfold, uint32_t first,
-                                          uint32_t cnt, uint32_t stride) {
+                                          uint32_t cnt, uint32_t stride, const Piv8& pv) {
     constexpr int R = G <= 8 ? 4 : 2;
     const int lane = lane_id();
     const int gl = lane & (G - 1);
@@ -426,8 +436,23 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 const bool go = p < le;
                 const bool in = go && p + gl < le;
                 const uint32_t z = zr[r];
-                const uint32_t pos = lds_rank<L>(snb, dh, z);
-                const bool hit = in && pos < dh && snb[pos] == z;
+                uint32_t pos;
+                bool hit;
+                if (PIV) {
+                    uint32_t blk = 0;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) blk += pv.p[i] < z ? 1u : 0u;
+                    const uint32_t bb = min(blk, 7u);
+                    const uint4 q0 = reinterpret_cast<const uint4*>(snb)[2 * bb];
+                    const uint4 q1 = reinterpret_cast<const uint4*>(snb)[2 * bb + 1];
+                    const int j = q0.x == z ? 0 : q0.y == z ? 1 : q0.z == z ? 2 : q0.w == z ? 3
+                                : q1.x == z ? 4 : q1.y == z ? 5 : q1.z == z ? 6 : q1.w == z ? 7 : -1;
+                    hit = in && blk < 8 && j >= 0;
+                    pos = hit ? 8 * bb + (uint32_t)j : 0u;
+                } else {
+                    pos = lds_rank<L>(snb, dh, z);
+                    hit = in && pos < dh && snb[pos] == z;
+                }
                 const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
@@ -474,8 +499,8 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
                                                     const uint32_t* __restrict__ own_idx,
                                                     TermBuf tb) {
     constexpr int NW = HW / WAVE;
-    __shared__ uint32_t s_nb[NW][WAVE], s_ed[NW][WAVE], s_dg[NW][WAVE], s_of[NW][WAVE],
-        s_own[NW][WAVE];
+    __shared__ __attribute__((aligned(16))) uint32_t s_nb[NW][WAVE];
+    __shared__ uint32_t s_ed[NW][WAVE], s_dg[NW][WAVE], s_of[NW][WAVE], s_own[NW][WAVE];
     __shared__ double s_fold[NW][WAVE];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
@@ -484,18 +509,23 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
     for (uint32_t un = blockIdx.x * NW + wid; un < nunits; un += gridDim.x * NW) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);  // dh, cnt <= 64 in this class
+        uint32_t z = 0xffffffffu;  // padding: never equal to, never below, a vertex id
         if (lane < dh) {
             const uint2 a = g.nbe[oh + lane];
             const uint2 b = g.ndo[oh + lane];
-            snb[lane] = a.x;
+            z = a.x;
             sed[lane] = a.y;
             sdg[lane] = b.x;
             sof[lane] = b.y;
         }
+        snb[lane] = z;
+        Piv8 pv;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) pv.p[i] = __builtin_amdgcn_readlane(z, 8 * i + 7);
         if (lane < cnt) sown[lane] = own_idx[k0 + lane];
         wave_sync();
-        hub_edges<G, WAVE>(g, tb, h, dh, snb, sed, sdg, sof, sown, s_fold[wid], 0, cnt,
-                           WAVE / G);
+        hub_edges<G, WAVE, true>(g, tb, h, dh, snb, sed, sdg, sof, sown, s_fold[wid], 0, cnt,
+                                 WAVE / G, pv);
         wave_sync();  // the next unit restages the slice
     }
 }
@@ -524,8 +554,8 @@ __global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint4* __res
         }
         for (uint32_t i = threadIdx.x; i < cnt; i += HT) sown[i] = own_idx[k0 + i];
         __syncthreads();
-        hub_edges<G, L>(g, tb, h, dh, snb, sed, sdg, sof, sown, sfold + wid * WAVE,
-                        wid * per_wave, cnt, per_wave * (HT / WAVE));
+        hub_edges<G, L, false>(g, tb, h, dh, snb, sed, sdg, sof, sown, sfold + wid * WAVE,
+                               wid * per_wave, cnt, per_wave * (HT / WAVE), Piv8{});
         __syncthreads();  // the next unit restages the hub
     }
 }
