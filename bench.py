@@ -335,7 +335,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "count phase (k_count_hub_w, "
-                     "k_count_hub, k_count_hub_hash, k_count_edges)",
+                     "k_count_hub, k_count_edges)",
                      "algorithmic_bytes": B_count, "s_min": s_min},
         "audit": audit,
         "autotune": tuned,

@@ -193,11 +193,12 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 /* Count-plan parameters and the autotuner (no reference counterpart).        */
 /* An edge whose longer-list endpoint (its hub) has degree in                 */
 /* (hub_min, hub_max] is counted per hub with the hub's list staged in LDS:   */
-/* a sorted list of up to 64 entries (degree <= l0_max), of 512 (<= l1_max),  */
-/* or a hash table (above, hub_max <= 4096); other edges are counted per edge. */
+/* a sorted list of up to 64 entries (degree <= l0_max), 512 (<= l1_max) or   */
+/* 4096 (above, hub_max <= 4096) with an 8-ary pivot tree; other edges are     */
+/* counted per edge.                                                           */
 /* A hub's edges get 4, 8, 16, 32 or 64 lanes each by their mean shorter-list */
 /* length against g_thr[0..3].  Units of work hold at least unit_min streamed  */
-/* elements (hash class: unit_hash_mult * hub degree).  Results never depend   */
+/* elements (large class: unit_hash_mult * hub degree).  Results never depend   */
 /* on the parameters, only the time does.                                      */
 typedef struct {
     int32_t hub_min, hub_max, l0_max, l1_max;

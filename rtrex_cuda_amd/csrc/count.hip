@@ -46,7 +46,7 @@ __device__ __forceinline__ int bucket_of(uint32_t w) {
 // endpoint, its hub.  Edges whose hub degree is in (hub_min, hub_max] are
 // counted per hub (k_count_hub*), the hub's list staged once in LDS; the rest
 // per edge (k_count_edges).
-constexpr int HUB_NL = 3, HUB_NG = 5;  // staging classes 64/512/hash; G = 4..64
+constexpr int HUB_NL = 3, HUB_NG = 5;  // staging classes 64/512/4096; G = 4..64
 __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
     return dhub > P.hub_min && dhub <= P.hub_max;
 }
@@ -300,8 +300,8 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         if (oc) {
             const uint32_t dh = g.deg[h];
             const uint64_t work = hwork[h];
-            // Units of at least max(unit_min, d_h) elements; the hash class
-            // rebuilds a table per unit: unit_hash_mult * d_h elements, but
+            // Units of at least max(unit_min, d_h) elements; the large class
+            // restages a 4096-entry tree per unit: unit_hash_mult * d_h, but
             // at most 32 owned edges apart so that hubs still spread out.
             const int lc_ = hub_lclass(dh, Pm);
             const uint64_t uw = lc_ == 2 ? (uint64_t)Pm.unit_hash_mult * dh
@@ -328,39 +328,13 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
     }
 }
 
-// Position of the first element >= x in the LDS-staged sorted list s[lo, hi).
-__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* s, uint32_t lo,
-                                                    uint32_t hi, uint32_t x) {
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s[mid] < x) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// One workgroup (HT lanes) per work unit: the hub's list -- neighbour ids,
-// their degrees and the edge ids of the hub's slots -- is staged in LDS once,
-// then its owned edges (part p of P) are counted G lanes per edge exactly as
-// in k_count_edges: the shorter list (with its slots' edge ids) is streamed
-// ascending in chunks of G, every probe is a binary search in LDS, and the
-// matches of a chunk are folded in lane order.  A match needs no further
-// gather: d_z and the edge id of {hub, z} come from LDS.
+// Hub kernels: one unit (part of one hub's owned edges) at a time, per wave
+// (small hubs) or per workgroup (larger ones).  The hub's sorted list is
+// staged in LDS once per unit; its owned edges are counted G lanes per edge:
+// the shorter list (with its slots' edge ids) is streamed ascending in chunks
+// of G, every id is looked up in the staged list, and the matches of a chunk
+// are folded in lane order.
 constexpr int HT = 256;
-
-// Number of entries of the LDS-staged sorted list s[0, n) smaller than z
-// (n <= L): log2(L) fixed steps, no divergent loop; every read is in s[0, L).
-template <int L>
-__device__ __forceinline__ uint32_t lds_rank(const uint32_t* s, uint32_t n, uint32_t z) {
-    uint32_t pos = 0;
-#pragma unroll
-    for (uint32_t step = L / 2; step >= 1; step >>= 1) {
-        const uint32_t q = pos + step;  // q - 1 <= L - 2
-        if (q <= n && s[q - 1] < z) pos = q;
-    }
-    if (pos < n && s[pos] < z) ++pos;  // pos <= L - 1 after the steps
-    return pos;
-}
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes
 // (LDS operations of one wave complete in order; the fences keep the
@@ -383,29 +357,90 @@ __device__ __forceinline__ void unit_fields(const uint4& U, uint32_t& h, uint32_
     cnt = U.w >> 16;
 }
 
-// Wave-uniform probe index of a hub list of at most 64 ids staged in LDS,
-// padded to 64 with UINT32_MAX (not a vertex id): the last id of each block
-// of 8.  A probe counts the pivots below z in registers, then reads z's one
-// candidate block (two 16-byte LDS reads) -- one LDS round trip instead of
-// the six dependent reads of a binary search.
+// Probe structure of a hub list staged in LDS: an 8-ary search tree.  The
+// list is padded with UINT32_MAX (never equal to, never below, a vertex id)
+// and stored whole (level 0, L = 8^(LV+1) entries); level l holds the last
+// entry of every block of 8 of level l - 1; the top 8 pivots (the last entry
+// of each eighth of the list) live in registers.  A probe counts the top
+// pivots below z, then reads one 8-entry block per level (two 16-byte LDS
+// reads) and ends on the only leaf block that can hold z: LV dependent LDS
+// reads instead of log2(L) for a binary search, and no hash table to build.
 struct Piv8 {
     uint32_t p[8];
+};
+constexpr uint32_t PAD_ID = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t below8(const uint4& a, const uint4& b, uint32_t z) {
+    return (a.x < z) + (a.y < z) + (a.z < z) + (a.w < z) + (b.x < z) + (b.y < z) + (b.z < z) +
+           (b.w < z);
+}
+
+// leaf: L entries, l1: L/8 pivots, l2: L/64 pivots; all 16-byte aligned.
+// True and pos = index of z when z is in the list.
+template <int LV>
+__device__ __forceinline__ bool tree_probe(const uint32_t* leaf, const uint32_t* l1,
+                                           const uint32_t* l2, const Piv8& top, uint32_t z,
+                                           uint32_t& pos) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b += top.p[i] < z ? 1u : 0u;
+    const bool ok = b < 8;  // b == 8: z is above every id
+    b = min(b, 7u);
+    if (LV >= 3) {
+        const uint4* q = reinterpret_cast<const uint4*>(l2) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+    }
+    if (LV >= 2) {
+        const uint4* q = reinterpret_cast<const uint4*>(l1) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+    }
+    const uint4* q = reinterpret_cast<const uint4*>(leaf) + 2 * b;
+    const uint4 a = q[0], c = q[1];
+    const int j = a.x == z ? 0 : a.y == z ? 1 : a.z == z ? 2 : a.w == z ? 3
+                : c.x == z ? 4 : c.y == z ? 5 : c.z == z ? 6 : c.w == z ? 7 : -1;
+    pos = 8 * b + (uint32_t)max(j, 0);
+    return ok && j >= 0;
+}
+
+// Entry i of the padded hub list (global loads; the pivots of the tree).
+__device__ __forceinline__ uint32_t hub_id(const GraphDev& g, uint32_t oh, uint32_t dh,
+                                           uint32_t i) {
+    return i < dh ? g.nb[oh + i] : PAD_ID;
+}
+
+// Per-slot data of the hub list for the probe and the matches: the edge id
+// of {hub, z}, d_z and z's list offset.  META: staged in LDS (sed, sdg, sof;
+// small and medium hubs); otherwise read from the per-slot arrays in HBM at
+// oh (large hubs, whose 4096-entry staging would leave no room for other
+// workgroups).
+template <bool META>
+struct HubMeta {
+    const uint32_t *sed, *sdg, *sof;
+    uint32_t oh;
+    __device__ __forceinline__ uint32_t ed(const GraphDev& g, uint32_t i) const {
+        return META ? sed[i] : g.nbe[oh + i].y;
+    }
+    __device__ __forceinline__ uint2 dego(const GraphDev& g, uint32_t i) const {
+        return META ? make_uint2(sdg[i], sof[i]) : g.ndo[oh + i];
+    }
+    __device__ __forceinline__ uint32_t deg(const GraphDev& g, uint32_t i) const {
+        return META ? sdg[i] : g.ndo[oh + i].x;
+    }
 };
 
 // The owned edges [first, cnt) of a unit, stride apart, G lanes per edge:
 // each group streams its edge's shorter list from HBM (R chunks of G ids per
 // load, neighbour and edge id in one 8-byte load) and looks every id up in
-// the hub list staged in LDS: snb (sorted neighbour ids), sed (edge ids),
-// sdg / sof (the neighbours' degrees and list offsets), sown (owned slots).
-// Matches go to the term buffer in ascending z; the support is folded in lane
-// order.  sfold is this wave's 64-entry LDS slice.  PIV: snb is a 64-entry
-// padded list (16-byte aligned) probed through pv; otherwise a binary search.
-template <int G, int L, bool PIV>
-__device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t h,
-                                          uint32_t dh, const uint32_t* snb, const uint32_t* sed,
-                                          const uint32_t* sdg, const uint32_t* sof,
-                                          const uint32_t* sown, double* sfold, uint32_t first,
-                                          uint32_t cnt, uint32_t stride, const Piv8& pv) {
+// the hub's tree; own[] holds the owned slots (LDS or HBM).  Matches go to
+// the term buffer in ascending z; the support is folded in lane order.
+// sfold is this wave's 64-entry LDS slice.
+template <int G, int LV, bool META>
+__device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t dh,
+                                          const uint32_t* leaf, const uint32_t* l1,
+                                          const uint32_t* l2, const Piv8& top,
+                                          const HubMeta<META>& hm_, const uint32_t* own,
+                                          double* sfold, uint32_t first, uint32_t cnt,
+                                          uint32_t stride) {
     constexpr int R = G <= 8 ? 4 : 2;
     const int lane = lane_id();
     const int gl = lane & (G - 1);
@@ -414,10 +449,11 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
     for (uint32_t base = first; base < cnt; base += stride) {
         const uint32_t idx = base + lane / G;
         const bool act = idx < cnt;
-        const uint32_t li = act ? sown[idx] : 0;
-        const uint32_t e = sed[li];  // edge {hub, lo}
-        const uint32_t dy = sdg[li];
-        uint32_t p = sof[li];
+        const uint32_t li = act ? own[idx] : 0;
+        const uint32_t e = hm_.ed(g, li);  // edge {hub, lo}
+        const uint2 yo = hm_.dego(g, li);
+        const uint32_t dy = yo.x;
+        uint32_t p = yo.y;
         const uint32_t le = act ? p + dy : p;
         const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
         double S = 0.0;
@@ -436,28 +472,13 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 const bool go = p < le;
                 const bool in = go && p + gl < le;
                 const uint32_t z = zr[r];
-                uint32_t pos;
-                bool hit;
-                if (PIV) {
-                    uint32_t blk = 0;
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) blk += pv.p[i] < z ? 1u : 0u;
-                    const uint32_t bb = min(blk, 7u);
-                    const uint4 q0 = reinterpret_cast<const uint4*>(snb)[2 * bb];
-                    const uint4 q1 = reinterpret_cast<const uint4*>(snb)[2 * bb + 1];
-                    const int j = q0.x == z ? 0 : q0.y == z ? 1 : q0.z == z ? 2 : q0.w == z ? 3
-                                : q1.x == z ? 4 : q1.y == z ? 5 : q1.z == z ? 6 : q1.w == z ? 7 : -1;
-                    hit = in && blk < 8 && j >= 0;
-                    pos = hit ? 8 * bb + (uint32_t)j : 0u;
-                } else {
-                    pos = lds_rank<L>(snb, dh, z);
-                    hit = in && pos < dh && snb[pos] == z;
-                }
-                const double t = hit ? tri_w(dh, dy, sdg[pos]) : 0.0;
+                uint32_t pos = 0;
+                const bool hit = in && tree_probe<LV>(leaf, l1, l2, top, z, pos);
+                const double t = hit ? tri_w(dh, dy, hm_.deg(g, pos)) : 0.0;
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
                 if (tb.t && hit)  // edges {lo, z} and {hub, z}
-                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], sed[pos]};
+                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], hm_.ed(g, pos)};
                 c += (uint32_t)__popcll(hm);
                 // Ordered fold: the group's weights in lane order, a miss
                 // adding 0.0 (which leaves the sum's bits unchanged).
@@ -491,7 +512,8 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
 // Small hubs (d_hub <= 64): one wave per unit.  The wave stages its hub in
 // its own LDS slice, one slot per lane in a single load round trip (the
 // neighbours' degrees and offsets come precomputed per slot), so units of
-// a block never wait on each other.
+// a block never wait on each other.  The tree is the list itself plus its 8
+// block pivots, read from the lanes' registers.
 constexpr int HW = 256;
 template <int G>
 __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __restrict__ units,
@@ -504,157 +526,83 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
     __shared__ double s_fold[NW][WAVE];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
-    uint32_t *snb = s_nb[wid], *sed = s_ed[wid], *sdg = s_dg[wid], *sof = s_of[wid],
-             *sown = s_own[wid];
+    const HubMeta<true> meta{s_ed[wid], s_dg[wid], s_of[wid], 0u};
     for (uint32_t un = blockIdx.x * NW + wid; un < nunits; un += gridDim.x * NW) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);  // dh, cnt <= 64 in this class
-        uint32_t z = 0xffffffffu;  // padding: never equal to, never below, a vertex id
+        uint32_t z = PAD_ID;
         if (lane < dh) {
             const uint2 a = g.nbe[oh + lane];
             const uint2 b = g.ndo[oh + lane];
             z = a.x;
-            sed[lane] = a.y;
-            sdg[lane] = b.x;
-            sof[lane] = b.y;
+            s_ed[wid][lane] = a.y;
+            s_dg[wid][lane] = b.x;
+            s_of[wid][lane] = b.y;
         }
-        snb[lane] = z;
-        Piv8 pv;
+        s_nb[wid][lane] = z;
+        Piv8 top;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) pv.p[i] = __builtin_amdgcn_readlane(z, 8 * i + 7);
-        if (lane < cnt) sown[lane] = own_idx[k0 + lane];
+        for (int i = 0; i < 8; ++i) top.p[i] = __builtin_amdgcn_readlane(z, 8 * i + 7);
+        if (lane < cnt) s_own[wid][lane] = own_idx[k0 + lane];
         wave_sync();
-        hub_edges<G, WAVE, true>(g, tb, h, dh, snb, sed, sdg, sof, sown, s_fold[wid], 0, cnt,
-                                 WAVE / G, pv);
+        hub_edges<G, 1, true>(g, tb, dh, s_nb[wid], nullptr, nullptr, top, meta, s_own[wid],
+                              s_fold[wid], 0, cnt, WAVE / G);
         wave_sync();  // the next unit restages the slice
     }
 }
 
-// Medium hubs (d_hub <= 512): one workgroup per unit, the hub staged once in
-// LDS and shared by its four waves.
-template <int G, int L>
-__global__ void __launch_bounds__(HT) k_count_hub(GraphDev g, const uint4* __restrict__ units,
+// Medium and large hubs (d_hub <= 512: LV = 2; <= 4096: LV = 3): one
+// workgroup of NT lanes per unit, the hub's tree staged in LDS once and shared
+// by its waves.  Medium hubs also stage the per-slot data and the owned slots;
+// large ones read them from HBM (the hub's own slots: L2-resident).
+template <int G, int LV, int NT>
+__global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __restrict__ units,
                                                   uint32_t nunits,
                                                   const uint32_t* __restrict__ own_idx,
                                                   TermBuf tb) {
-    __shared__ uint32_t snb[L], sed[L], sdg[L], sof[L], sown[L];
-    __shared__ double sfold[HT];
+    constexpr uint32_t L = LV == 2 ? 512 : 4096;
+    constexpr bool META = LV == 2;  // per-slot data staged in LDS
+    constexpr uint32_t ML = META ? L : 1;
+    __shared__ __attribute__((aligned(16))) uint32_t s_leaf[L];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
+    __shared__ uint32_t s_ed[ML], s_dg[ML], s_of[ML], s_own[ML];
+    __shared__ double sfold[NT];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);
-        for (uint32_t i = threadIdx.x; i < dh; i += HT) {
-            const uint2 a = g.nbe[oh + i];
-            const uint2 b = g.ndo[oh + i];
-            snb[i] = a.x;
-            sed[i] = a.y;
-            sdg[i] = b.x;
-            sof[i] = b.y;
+        // Leaf: the list, padded through the block after its last entry (a
+        // probe above every id can land there); pivots for every position.
+        const uint32_t nleaf = min(L, ((dh + 7) & ~7u) + 8);
+        for (uint32_t i = threadIdx.x; i < nleaf; i += NT) {
+            if (META && i < dh) {
+                const uint2 a = g.nbe[oh + i];
+                const uint2 b = g.ndo[oh + i];
+                s_leaf[i] = a.x;
+                s_ed[i] = a.y;
+                s_dg[i] = b.x;
+                s_of[i] = b.y;
+            } else {
+                s_leaf[i] = hub_id(g, oh, dh, i);
+            }
         }
-        for (uint32_t i = threadIdx.x; i < cnt; i += HT) sown[i] = own_idx[k0 + i];
+        for (uint32_t i = threadIdx.x; i < L / 8; i += NT) s_l1[i] = hub_id(g, oh, dh, 8 * i + 7);
+        if (LV == 3)
+            for (uint32_t i = threadIdx.x; i < L / 64; i += NT)
+                s_l2[i] = hub_id(g, oh, dh, 64 * i + 63);
+        Piv8 top;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
+        if (META)
+            for (uint32_t i = threadIdx.x; i < cnt; i += NT) s_own[i] = own_idx[k0 + i];
         __syncthreads();
-        hub_edges<G, L, false>(g, tb, h, dh, snb, sed, sdg, sof, sown, sfold + wid * WAVE,
-                               wid * per_wave, cnt, per_wave * (HT / WAVE), Piv8{});
+        const HubMeta<META> meta{s_ed, s_dg, s_of, oh};
+        hub_edges<G, LV, META>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta,
+                               META ? s_own : own_idx + k0, sfold + wid * WAVE, wid * per_wave,
+                               cnt, per_wave * (NT / WAVE));
         __syncthreads();  // the next unit restages the hub
-    }
-}
-
-// Large hubs (staging class 4096): the hub list is held in an LDS hash table
-// (id -> position, open addressing, 2x capacity), so a probe is ~1 LDS read
-// instead of a 12-step binary search; d_z and the edge id of {hub, z} are
-// gathered for matches only.  Everything else is as in k_count_hub.
-constexpr uint32_t HH = 8192;  // hash slots (>= 2 * 4096)
-constexpr int HTH = 512;       // 8 waves share one table (occupancy: 3 x 48 KB per CU)
-constexpr uint32_t HEMPTY = 0xffffffffu;
-__device__ __forceinline__ uint32_t hub_hash(uint32_t z) { return (z * 2654435761u) >> 19; }
-
-template <int G>
-__global__ void __launch_bounds__(HTH) k_count_hub_hash(GraphDev g,
-                                                       const uint4* __restrict__ units,
-                                                       uint32_t nunits,
-                                                       const uint32_t* __restrict__ own_idx,
-                                                       TermBuf tb) {
-    constexpr int R = 2;
-    __shared__ uint32_t hk[HH];
-    __shared__ uint16_t hv[HH];
-    const int lane = lane_id();
-    const int wid = threadIdx.x / WAVE;
-    const int gl = lane & (G - 1);
-    const int gbase = lane & ~(G - 1);
-    const uint64_t below = ((1ull << gl) - 1ull);
-    constexpr uint32_t per_wave = WAVE / G;
-    constexpr uint32_t per_blk = per_wave * (HTH / WAVE);
-    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
-        uint32_t h, oh, k0, dh, cnt;
-        unit_fields(units[un], h, oh, k0, dh, cnt);
-        for (uint32_t i = threadIdx.x; i < HH; i += HTH) hk[i] = HEMPTY;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < dh; i += HTH) {
-            const uint32_t z = g.nb[oh + i];
-            uint32_t q = hub_hash(z);
-            while (atomicCAS(&hk[q], HEMPTY, z) != HEMPTY) q = (q + 1) & (HH - 1);
-            hv[q] = (uint16_t)i;
-        }
-        __syncthreads();
-        for (uint32_t base = wid * per_wave; base < cnt; base += per_blk) {
-            const uint32_t idx = base + lane / G;
-            const bool act = idx < cnt;
-            const uint32_t li = act ? own_idx[k0 + idx] : 0;
-            const uint2 ye = act ? g.nbe[oh + li] : make_uint2(0u, 0u);
-            const uint2 yo = act ? g.ndo[oh + li] : make_uint2(0u, 0u);
-            const uint32_t e = ye.y;  // edge {hub, lo}
-            const uint32_t dy = yo.x;
-            uint32_t p = yo.y;
-            const uint32_t le = p + dy;
-            const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
-            double S = 0.0;
-            uint32_t c = 0;
-            while (__any(p < le)) {
-                uint32_t zr[R], er[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint32_t q = p + r * G + gl;
-                    const uint2 a = q < le ? g.nbe[q] : make_uint2(0u, 0u);
-                    zr[r] = a.x;
-                    er[r] = a.y;  // edge {lo, z}
-                }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const bool go = p < le;
-                    const bool in = go && p + gl < le;
-                    const uint32_t z = zr[r];
-                    uint32_t pos = 0;
-                    bool hit = false;
-                    if (in) {
-                        uint32_t q = hub_hash(z);
-                        for (;;) {
-                            const uint32_t kz = hk[q];
-                            if (kz == z) {
-                                pos = hv[q];
-                                hit = true;
-                                break;
-                            }
-                            if (kz == HEMPTY) break;
-                            q = (q + 1) & (HH - 1);
-                        }
-                    }
-                    const uint2 zd = hit ? g.ndo[oh + pos] : make_uint2(0u, 0u);
-                    const double t = hit ? tri_w(dh, dy, zd.x) : 0.0;
-                    const uint64_t hm = G == 64 ? ballot64(hit)
-                                                : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
-                    if (tb.t && hit)  // edges {lo, z} and {hub, z}
-                        tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], g.nbe[oh + pos].y};
-                    group_fold<G>(hit, t, S, c);
-                    if (go) p += G;
-                }
-            }
-            if (act && gl == 0) {
-                g.sup[e] = S;
-                g.tcnt[e] = c;
-            }
-        }
-        __syncthreads();  // the next unit rebuilds the table
     }
 }
 
@@ -880,14 +828,17 @@ void launch_hub_gl(const GraphDev& g, const uint4* units, uint32_t nu, const uin
         const uint32_t blocks = std::min<uint32_t>((nu + HW / WAVE - 1) / (HW / WAVE), 16384);
         k_count_hub_w<G><<<blocks, HW, 0, s>>>(g, units, nu, own_idx, tb);
     } else {
-        k_count_hub<G, L><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_idx, tb);
+        k_count_hub<G, 2, HT><<<std::min<uint32_t>(nu, 4096), HT, 0, s>>>(g, units, nu, own_idx,
+                                                                          tb);
     }
 }
 
+// Large hubs: 8 waves per unit share the 4096-entry tree.
+constexpr int HTL = 512;
 template <int G>
-void launch_hub_hash(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
+void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
                      const TermBuf& tb, hipStream_t s) {
-    k_count_hub_hash<G><<<std::min<uint32_t>(nu, 4096), HTH, 0, s>>>(g, units, nu, own_idx, tb);
+    k_count_hub<G, 3, HTL><<<std::min<uint32_t>(nu, 4096), HTL, 0, s>>>(g, units, nu, own_idx, tb);
 }
 
 template <int L>
@@ -914,11 +865,11 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else {
             switch (gc) {
-                case 0: launch_hub_hash<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_hash<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_hash<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_hash<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                default: launch_hub_hash<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 0: launch_hub_large<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 1: launch_hub_large<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 2: launch_hub_large<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 3: launch_hub_large<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                default: launch_hub_large<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
             }
         }
         at += nu;
