@@ -173,7 +173,7 @@ namespace rtrex {
 // each rounded up to 256 bytes.
 uint64_t resident_bytes(uint64_t n, uint64_t m) {
     const uint64_t nnz = 2 * m;
-    return 29 * nnz + 91 * m + 68 * n + 96 * 256 + 8192 + 64;
+    return 37 * nnz + 107 * m + 68 * n + 96 * 256 + 8192 + 64;
 }
 // The largest phase scratch: the build (edge upload, radix keys, partner
 // ranks), the hub plan, the clean (frontiers and buckets) or the extraction

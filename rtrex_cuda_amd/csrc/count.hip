@@ -196,30 +196,32 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
             group_fold<G>(hit, t, S, c);
             if (go) p += G;
         }
-        if (act && gl == 0) {
-            g.sup[e] = S;
-            g.tcnt[e] = c;
-        }
+        if (act && gl == 0) tb.rec[e] = SupRec{S, c, 0u};
     }
 }
 
 // Triangle incidences sum_e tcnt(e) (three per triangle): per-block partial
 // sums, then one block adds them (no contended atomics).  The same stream
-// copies the supports into sup0, the count-phase value the clean starts from
-// (the counting kernels write sup only: one scattered store per edge fewer).
+// spreads the counting kernels' per-edge records (support and count, one
+// scattered 16-byte store per edge) into sup, sup0 (the count-phase value the
+// clean starts from) and tcnt, in edge order.
 constexpr int RED_BLOCKS = 1024;
-__global__ void __launch_bounds__(T) k_sum_tcnt(const uint32_t* __restrict__ tcnt,
-                                                const double* __restrict__ sup,
-                                                double* __restrict__ sup0, uint32_t m,
+__global__ void __launch_bounds__(T) k_sum_tcnt(const SupRec* __restrict__ rec,
+                                                double* __restrict__ sup,
+                                                double* __restrict__ sup0,
+                                                uint32_t* __restrict__ tcnt, uint32_t m,
                                                 unsigned long long* __restrict__ part) {
     __shared__ unsigned long long sw[T / WAVE];
-    unsigned long long c = 0;
+    unsigned long long acc = 0;
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < m; e += gridDim.x * T) {
-        c += tcnt[e];
-        sup0[e] = sup[e];
+        const SupRec r = rec[e];
+        acc += r.c;
+        tcnt[e] = r.c;
+        sup[e] = r.s;
+        sup0[e] = r.s;
     }
-    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-    if (lane_id() == 0) sw[threadIdx.x / WAVE] = c;
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if (lane_id() == 0) sw[threadIdx.x / WAVE] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
@@ -229,10 +231,10 @@ __global__ void __launch_bounds__(T) k_sum_tcnt(const uint32_t* __restrict__ tcn
 }
 
 __global__ void __launch_bounds__(T) k_sum_final(const unsigned long long* __restrict__ part,
-                                                 unsigned long long* out) {
+                                                 uint32_t nparts, unsigned long long* out) {
     __shared__ unsigned long long sw[T / WAVE];
     unsigned long long c = 0;
-    for (int i = threadIdx.x; i < RED_BLOCKS; i += T) c += part[i];
+    for (uint32_t i = threadIdx.x; i < nparts; i += T) c += part[i];
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
     if (lane_id() == 0) sw[threadIdx.x / WAVE] = c;
     __syncthreads();
@@ -416,6 +418,7 @@ __device__ __forceinline__ uint32_t hub_id(const GraphDev& g, uint32_t oh, uint3
 template <bool META>
 struct HubMeta {
     const uint32_t *sed, *sdg, *sof;
+    const uint64_t* szo;
     uint32_t oh;
     __device__ __forceinline__ uint32_t ed(const GraphDev& g, uint32_t i) const {
         return META ? sed[i] : g.nbe[oh + i].y;
@@ -425,6 +428,10 @@ struct HubMeta {
     }
     __device__ __forceinline__ uint32_t deg(const GraphDev& g, uint32_t i) const {
         return META ? sdg[i] : g.ndo[oh + i].x;
+    }
+    // Term offset of the edge of hub slot i.
+    __device__ __forceinline__ uint64_t zo(const TermBuf& tb, uint32_t i) const {
+        return META ? szo[i] : tb.zsl[oh + i];
     }
 };
 
@@ -455,7 +462,7 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
         const uint32_t dy = yo.x;
         uint32_t p = yo.y;
         const uint32_t le = act ? p + dy : p;
-        const uint64_t wbase = (act && tb.t) ? tb.zoff[e] : 0;
+        const uint64_t wbase = (act && tb.t) ? hm_.zo(tb, li) : 0;
         double S = 0.0;
         uint32_t c = 0;
         while (__any(p < le)) {
@@ -502,10 +509,7 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 if (go) p += G;
             }
         }
-        if (act && gl == 0) {
-            g.sup[e] = S;
-            g.tcnt[e] = c;
-        }
+        if (act && gl == 0) tb.rec[e] = SupRec{S, c, 0u};
     }
 }
 
@@ -523,10 +527,11 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
     constexpr int NW = HW / WAVE;
     __shared__ __attribute__((aligned(16))) uint32_t s_nb[NW][WAVE];
     __shared__ uint32_t s_ed[NW][WAVE], s_dg[NW][WAVE], s_of[NW][WAVE], s_own[NW][WAVE];
+    __shared__ uint64_t s_zo[NW][WAVE];
     __shared__ double s_fold[NW][WAVE];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
-    const HubMeta<true> meta{s_ed[wid], s_dg[wid], s_of[wid], 0u};
+    const HubMeta<true> meta{s_ed[wid], s_dg[wid], s_of[wid], s_zo[wid], 0u};
     for (uint32_t un = blockIdx.x * NW + wid; un < nunits; un += gridDim.x * NW) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);  // dh, cnt <= 64 in this class
@@ -538,6 +543,7 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
             s_ed[wid][lane] = a.y;
             s_dg[wid][lane] = b.x;
             s_of[wid][lane] = b.y;
+            if (tb.t) s_zo[wid][lane] = tb.zsl[oh + lane];
         }
         s_nb[wid][lane] = z;
         Piv8 top;
@@ -548,6 +554,164 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
         hub_edges<G, 1, true>(g, tb, dh, s_nb[wid], nullptr, nullptr, top, meta, s_own[wid],
                               s_fold[wid], 0, cnt, WAVE / G);
         wave_sync();  // the next unit restages the slice
+    }
+}
+
+// Small hubs (d_hub <= 64) with a term buffer: one wave per unit, the
+// unit's owned edges' shorter lists streamed as ONE sequence, 128 elements per
+// step (two per lane, two loads in flight), whatever edge each belongs to
+// (edge k covers elements [L_k, L_k + d_k) of the sequence, L the exclusive
+// prefix sum of the list lengths d_k).  A lane finds its edge from the edge
+// starts that fall in the step: each start sets a per-step tag at its
+// position in LDS, two ballots gather them, and the lane's edge is the
+// previous step's last edge plus the starts at or below it.  A match is
+// stored at its edge's term offset plus the edge's earlier matches, so every
+// edge's terms stay in ascending z.  The support is folded in the same order:
+// each half step's match weights are compacted in lane order into LDS and the
+// last lane of every edge's run adds its run's weights, one after another, to
+// the edge's running sum.  The next unit's record and hub staging loads are
+// issued before the current unit's steps (unit records two units ahead), so
+// a unit's setup does not wait on memory.
+constexpr int FW = 256;
+struct FlatStage {
+    uint32_t z, e, d, o, zl, zh, oi;  // hub slot lane: id, edge id, degree, offset, term offset; own_idx
+};
+__device__ __forceinline__ FlatStage flat_stage(const GraphDev& g, const TermBuf& tb,
+                                                const uint32_t* __restrict__ own_idx,
+                                                const uint4& U, bool valid, int lane) {
+    FlatStage r{PAD_ID, 0u, 0u, 0u, 0u, 0u, 0u};
+    if (!valid) return r;
+    const uint32_t oh = U.y, k0 = U.z, dh = U.w & 0xffffu, cnt = U.w >> 16;
+    if ((uint32_t)lane < dh) {
+        const uint2 a = g.nbe[oh + lane];
+        const uint2 b = g.ndo[oh + lane];
+        const uint64_t zo = tb.zsl[oh + lane];
+        r.z = a.x;
+        r.e = a.y;
+        r.d = b.x;
+        r.o = b.y;
+        r.zl = (uint32_t)zo;
+        r.zh = (uint32_t)(zo >> 32);
+    }
+    if ((uint32_t)lane < cnt) r.oi = own_idx[k0 + lane];
+    return r;
+}
+
+__global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* __restrict__ units,
+                                                       uint32_t nunits,
+                                                       const uint32_t* __restrict__ own_idx,
+                                                       TermBuf tb) {
+    constexpr int NW = FW / WAVE;
+    __shared__ __attribute__((aligned(16))) uint32_t s_nb[NW][WAVE];
+    __shared__ uint32_t s_ed[NW][WAVE], s_dg[NW][WAVE];
+    __shared__ uint4 s_ek[NW][WAVE];     // per owned edge k: (L_k, list offset - L_k, d_k, 0)
+    __shared__ uint64_t s_zb[NW][WAVE];  // term offset of edge k
+    __shared__ uint32_t s_cnt[NW][WAVE];  // matches so far
+    __shared__ double s_S[NW][WAVE];      // support so far
+    __shared__ double s_t[NW][WAVE];      // a half step's match weights, in lane order
+    __shared__ uint32_t s_flag[NW][2 * WAVE];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t tag = 0;
+    s_flag[wid][lane] = 0;  // tags start at 1
+    s_flag[wid][WAVE + lane] = 0;
+    const uint32_t stride = gridDim.x * NW;
+    uint32_t un = blockIdx.x * NW + wid;
+    uint4 Ucur = un < nunits ? units[un] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 Unext = un + stride < nunits ? units[un + stride] : make_uint4(0u, 0u, 0u, 0u);
+    FlatStage cur = flat_stage(g, tb, own_idx, Ucur, un < nunits, lane);
+    for (; un < nunits; un += stride) {
+        const uint32_t oh = Ucur.y, dh = Ucur.w & 0xffffu, cnt = Ucur.w >> 16;
+        (void)oh;
+        if ((uint32_t)lane < dh) {
+            s_ed[wid][lane] = cur.e;
+            s_dg[wid][lane] = cur.d;
+        }
+        s_nb[wid][lane] = cur.z;
+        Piv8 top;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) top.p[i] = __builtin_amdgcn_readlane(cur.z, 8 * i + 7);
+        // Owned edge k = lane: the data of its hub slot li, held by lane li.
+        const bool own = (uint32_t)lane < cnt;
+        const int li = own ? (int)cur.oi : 0;
+        const uint32_t vd = __shfl(cur.d, li, 64), vo = __shfl(cur.o, li, 64);
+        const uint32_t ve = __shfl(cur.e, li, 64);
+        const uint32_t zl = __shfl(cur.zl, li, 64), zh = __shfl(cur.zh, li, 64);
+        const uint32_t dk = own ? vd : 0u;
+        uint32_t incl = dk;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, (unsigned)d, 64);
+            if (lane >= d) incl += y;
+        }
+        const uint32_t Lk = incl - dk;
+        const uint32_t W = __builtin_amdgcn_readlane(incl, WAVE - 1);
+        if (own) {
+            s_ek[wid][lane] = make_uint4(Lk, vo - Lk, dk, 0u);
+            s_zb[wid][lane] = ((uint64_t)zh << 32) | zl;
+            s_cnt[wid][lane] = 0;
+            s_S[wid][lane] = 0.0;
+        }
+        // Next unit's staging loads now; the record after it as well.
+        const uint32_t un1 = un + stride, un2 = un + 2 * stride;
+        cur = flat_stage(g, tb, own_idx, Unext, un1 < nunits, lane);
+        Ucur = Unext;
+        Unext = un2 < nunits ? units[un2] : make_uint4(0u, 0u, 0u, 0u);
+        wave_sync();
+        int kprev = -1;
+        for (uint32_t base = 0; base < W; base += 2 * WAVE) {
+            ++tag;
+            if (own && Lk >= base && Lk - base < 2u * WAVE) s_flag[wid][Lk - base] = tag;
+            wave_sync();
+            const bool st1 = s_flag[wid][lane] == tag;
+            const bool st2 = s_flag[wid][WAVE + lane] == tag;
+            const uint64_t sm1 = ballot64(st1), sm2 = ballot64(st2);
+            const int k1 = kprev + (int)__popcll(sm1 & below) + (st1 ? 1 : 0);
+            const int k2 = kprev + (int)__popcll(sm1) + (int)__popcll(sm2 & below) + (st2 ? 1 : 0);
+            const uint32_t f1 = base + lane, f2 = base + WAVE + lane;
+            const bool a1 = f1 < W, a2 = f2 < W;  // then 0 <= k < cnt
+            const int kk1 = a1 ? k1 : 0, kk2 = a2 ? k2 : 0;
+            const uint4 E1 = s_ek[wid][kk1], E2 = s_ek[wid][kk2];
+            const uint2 x1 = a1 ? g.nbe[f1 + E1.y] : make_uint2(0u, 0u);  // (z, edge {lo, z})
+            const uint2 x2 = a2 ? g.nbe[f2 + E2.y] : make_uint2(0u, 0u);
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const uint32_t hb = base + hf * WAVE, f = hf ? f2 : f1;
+                const bool act = hf ? a2 : a1;
+                const int k = hf ? k2 : k1, kk = hf ? kk2 : kk1;
+                const uint4 E = hf ? E2 : E1;
+                const uint2 x = hf ? x2 : x1;
+                uint32_t pos = 0;
+                const bool hit = act && tree_probe<1>(s_nb[wid], nullptr, nullptr, top, x.x, pos);
+                const uint64_t hm = ballot64(hit);
+                // lanes of edge kk in this half step: [run0, last of the run]
+                const uint32_t run0 = E.x > hb ? E.x - hb : 0u;
+                const uint32_t nb_below = (uint32_t)__popcll(hm & below);
+                const uint32_t rank = (uint32_t)__popcll(hm & below & ~((1ull << run0) - 1ull));
+                const uint32_t c0 = s_cnt[wid][kk];
+                if (hit) {  // edges {lo, z} and {hub, z}
+                    const double t = tri_w(dh, E.z, s_dg[wid][pos]);
+                    tb.t[s_zb[wid][kk] + c0 + rank] = Term{t, x.y, s_ed[wid][pos]};
+                    s_t[wid][nb_below] = t;
+                }
+                const int knext = __shfl_down(k, 1u, 64);
+                const bool last = act && (lane == WAVE - 1 || f + 1 == W || knext != k);
+                wave_sync();
+                if (last) {
+                    // the run's matches are s_t[nb_below - rank, nb_below + hit)
+                    double S = s_S[wid][kk];
+                    const uint32_t b = nb_below + (hit ? 1u : 0u);
+                    for (uint32_t i = nb_below - rank; i < b; ++i) S += s_t[wid][i];
+                    s_S[wid][kk] = S;
+                    s_cnt[wid][kk] = c0 + rank + (hit ? 1u : 0u);
+                }
+                wave_sync();
+            }
+            kprev = __builtin_amdgcn_readlane(k2, WAVE - 1);
+        }
+        if (own) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
+        wave_sync();  // the next unit restages the slices
     }
 }
 
@@ -567,6 +731,7 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
     __shared__ uint32_t s_ed[ML], s_dg[ML], s_of[ML], s_own[ML];
+    __shared__ uint64_t s_zo[ML];
     __shared__ double sfold[NT];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
@@ -584,6 +749,7 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
                 s_ed[i] = a.y;
                 s_dg[i] = b.x;
                 s_of[i] = b.y;
+                if (tb.t) s_zo[i] = tb.zsl[oh + i];
             } else {
                 s_leaf[i] = hub_id(g, oh, dh, i);
             }
@@ -598,7 +764,7 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
         if (META)
             for (uint32_t i = threadIdx.x; i < cnt; i += NT) s_own[i] = own_idx[k0 + i];
         __syncthreads();
-        const HubMeta<META> meta{s_ed, s_dg, s_of, oh};
+        const HubMeta<META> meta{s_ed, s_dg, s_of, s_zo, oh};
         hub_edges<G, LV, META>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta,
                                META ? s_own : own_idx + k0, sfold + wid * WAVE, wid * per_wave,
                                cnt, per_wave * (NT / WAVE));
@@ -754,6 +920,15 @@ __global__ void __launch_bounds__(T) k_term_caps(GraphDev g, uint32_t* __restric
     }
 }
 
+// Term offset of every slot's edge (zsl[j] = zoff[s2e[j]]): the hub kernels
+// read a hub's owned edges' offsets with its list, contiguously.
+__global__ void __launch_bounds__(T) k_zsl(GraphDev g, const uint64_t* __restrict__ zoff,
+                                           uint64_t* __restrict__ zsl) {
+    for (uint64_t j = (uint64_t)blockIdx.x * T + threadIdx.x; j < g.nnz;
+         j += (uint64_t)gridDim.x * T)
+        zsl[j] = zoff[g.s2e[j]];
+}
+
 template <int G>
 __global__ void __launch_bounds__(T) k_count_vertices(GraphDev g,
                                                       const uint32_t* __restrict__ list,
@@ -853,10 +1028,24 @@ void launch_hub_l(int gc, const GraphDev& g, const uint4* units, uint32_t nu,
     }
 }
 
-// Every hub-path unit class, one launch each.
+// Every hub-path unit class, one launch each; with a term buffer, the small
+// hubs' classes (stored first) in one flattened launch.
 void hub_pass(rtrex_graph h, const TermBuf& tb) {
     uint64_t at = 0;
-    for (int c = 0; c < HUB_NL * HUB_NG; ++c) {
+    int c0 = 0;
+    if (tb.t) {
+        uint64_t nu = 0;
+        for (int c = 0; c < HUB_NG; ++c) nu += h->hub_ucnt[c];
+        if (nu) {
+            const uint32_t blocks =
+                (uint32_t)std::min<uint64_t>((nu + FW / WAVE - 1) / (FW / WAVE), 16384);
+            k_count_hub_flat<<<blocks, FW, 0, h->stream>>>(h->g, h->hub_units, (uint32_t)nu,
+                                                           h->own_idx, tb);
+        }
+        at = nu;
+        c0 = HUB_NG;
+    }
+    for (int c = c0; c < HUB_NL * HUB_NG; ++c) {
         const uint32_t nu = h->hub_ucnt[c];
         if (!nu) continue;
         const uint4* u = h->hub_units + at;
@@ -976,6 +1165,8 @@ rtrex_status prepare_count(rtrex_graph h) {
     }
     if (h->zoff) return RTREX_OK;
     RT_TRY(dalloc(h, &h->red, RED_BLOCKS));
+    RT_TRY(dalloc(h, &h->srec, std::max<uint64_t>(g.m, 1)));
+    RT_TRY(dalloc(h, &h->zsl, std::max<uint64_t>(g.nnz, 1)));
     // term buffer: capacity min(d_x, d_y) - 1 per edge
     RT_TRY(dalloc(h, &h->zoff, (uint64_t)g.m + 1));
     {
@@ -985,6 +1176,8 @@ rtrex_status prepare_count(rtrex_graph h) {
         if (g.m) k_term_caps<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, cap);
         RT_TRY(exclusive_scan_u32(cap, h->zoff, g.m, s));
     }
+    if (g.nnz) k_zsl<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g, h->zoff, h->zsl);
+    RT_HIP(hipGetLastError());
     uint64_t total = 0;
     RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
@@ -1093,15 +1286,17 @@ rtrex_status run_count(rtrex_graph h) {
         if (counts[2]) k_count_edges<16><<<grid_for(counts[2], 16), T, 0, s>>>(g, ep(2), counts[2], tb);
         if (counts[3]) k_count_edges<64><<<grid_for(counts[3], 64), T, 0, s>>>(g, ep(3), counts[3], tb);
         hub_pass(h, tb);
-        k_sum_tcnt<<<RED_BLOCKS, T, 0, s>>>(g.tcnt, g.sup, g.sup0, g.m, h->red);
-        k_sum_final<<<1, T, 0, s>>>(h->red, &h->ctr->tri_total);
+        k_sum_tcnt<<<RED_BLOCKS, T, 0, s>>>(h->srec, g.sup, g.sup0, g.tcnt, g.m, h->red);
+        k_sum_final<<<1, T, 0, s>>>(h->red, RED_BLOCKS, &h->ctr->tri_total);
     };
     if (h->terms_exact_pending) {
         // Counting pass, exact offsets from the triangle counts, then the
         // buffer is filled by the timed pass below (once per graph).
-        edge_pass(TermBuf{nullptr, h->zoff});
+        edge_pass(TermBuf{nullptr, h->zoff, h->zsl, h->srec});
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(g.tcnt, h->zoff, g.m, s));
+        if (g.nnz) k_zsl<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g, h->zoff, h->zsl);
+        RT_HIP(hipGetLastError());
         uint64_t total = 0;
         RT_HIP(hipMemcpyAsync(&total, h->zoff + g.m, 8, hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
@@ -1109,7 +1304,7 @@ rtrex_status run_count(rtrex_graph h) {
         try_alloc_terms(h, total);
         RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
     }
-    TermBuf tb{h->zt, h->zoff};
+    const TermBuf tb = h->terms();
 
     RT_HIP(hipEventRecord(h->ev[2], s));
     edge_pass(tb);

@@ -93,9 +93,18 @@ struct __attribute__((aligned(16))) Term {
     double w;
     uint32_t sa, sb;
 };
+// A count kernel's result for one edge: its support and triangle count in
+// one 16-byte record, so that each edge costs one scattered store; a
+// streaming pass then fills sup, sup0 and tcnt in edge order.
+struct __attribute__((aligned(16))) SupRec {
+    double s;
+    uint32_t c, pad;
+};
 struct TermBuf {
     Term* t;  // null when the buffer did not fit (intersection fallbacks)
     const uint64_t* zoff;
+    const uint64_t* zsl = nullptr;  // per slot: zoff of the slot's edge
+    SupRec* rec = nullptr;          // count kernels' per-edge results
 };
 
 struct Timings {
@@ -184,6 +193,8 @@ struct rtrex_graph_s {
     rtrex::CountParams cparams;
     bool plan_valid = false;  // buckets and hub units built for cparams
     uint64_t* zoff = nullptr;
+    uint64_t* zsl = nullptr;         // zoff per slot (hub kernels stage it with the hub list)
+    rtrex::SupRec* srec = nullptr;   // count kernels' per-edge results
     rtrex::Term* zt = nullptr;  // term buffer (TermBuf::t)
     bool terms_exact_pending = false;  // size the buffer after a counting pass
     bool pv_full = false;              // per-vertex supports valid everywhere
@@ -197,7 +208,7 @@ struct rtrex_graph_s {
     char* slab = nullptr;
     uint64_t slab_cap = 0, slab_top = 0;
     bool own_stream = true;
-    rtrex::TermBuf terms() const { return {zt, zoff}; }
+    rtrex::TermBuf terms() const { return {zt, zoff, zsl, srec}; }
     hipEvent_t ev[8] = {};
 };
 
