@@ -182,7 +182,7 @@ uint64_t scratch_bytes(uint64_t n, uint64_t m) {
     const uint64_t nnz = 2 * m;
     const uint64_t build = 28 * nnz + 8 * m;
     const uint64_t hubs = 8 * nnz + 4 * n;
-    const uint64_t clean = 92 * m;  // frontiers, buckets, undecided list, exact replay
+    const uint64_t clean = 128 * m;  // frontiers, buckets, alive lists, undecided list, exact replay
     const uint64_t extract = 120 * n + 4 * nnz;
     return std::max(std::max(build, hubs), std::max(clean, extract)) + nnz / 16 +
            (4ull << 20) + 64 * 256;

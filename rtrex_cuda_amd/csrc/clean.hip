@@ -31,6 +31,15 @@ namespace {
 
 constexpr int T = 256;
 
+// Status change of edge e during a peel.  With stored triangle lists the
+// peel reads the per-edge status only, so only est is written and the slot
+// copies are brought in line once the clean ends (k_sync_slot_status); the
+// intersection form reads the slot copies and keeps all three in step.
+__device__ __forceinline__ void mark(const GraphDev& g, uint32_t e, uint8_t c, bool terms) {
+    if (terms) g.est[e] = c;
+    else set_status(g, e, c);
+}
+
 // Initial frontier: alive edges below threshold (Decomposition.h:378-405).
 // The count-phase supports are bit-identical to the reference's, so these
 // decisions are the reference's exactly.
@@ -39,19 +48,24 @@ __device__ __forceinline__ int work_bucket(uint32_t w) {
 }
 
 // When bk is given (stored triangle lists), the frontier is also written
-// split by work bucket of tcnt (bk: 4 lists of capacity m, then 4 counts).
-// At eps = 0.3 nearly every edge starts in the frontier, so appends are
-// aggregated per workgroup (LDS counters, one global atomic per list per
-// tile of SEED_K * T edges) rather than per wave.
+// split by work bucket of tcnt (bk: 4 lists of capacity m, then 4 counts),
+// and the edges that stay alive with triangles go to the alive buckets ak
+// (same layout), the work list of a pull round.  At eps = 0.3 nearly every
+// edge starts in the frontier, so appends are aggregated per workgroup (LDS
+// counters, one global atomic per list per tile of SEED_K * T edges) rather
+// than per wave.
 constexpr int SEED_K = 8;
 __global__ void __launch_bounds__(T) k_seed_frontier(GraphDev g, double eps,
                                                      uint32_t* __restrict__ F,
                                                      unsigned* nF,
-                                                     uint32_t* __restrict__ bk) {
-    __shared__ uint32_t lcnt[5], lbase[5];
+                                                     uint32_t* __restrict__ bk,
+                                                     uint32_t* __restrict__ ak) {
+    // counters: 0-3 frontier buckets, 4 frontier, 5-8 alive buckets
+    __shared__ uint32_t lcnt[9], lbase[9];
+    const bool terms = bk != nullptr;
     for (uint64_t tile = (uint64_t)blockIdx.x * SEED_K * T; tile < g.m;
          tile += (uint64_t)gridDim.x * SEED_K * T) {
-        if (threadIdx.x < 5) lcnt[threadIdx.x] = 0;
+        if (threadIdx.x < 9) lcnt[threadIdx.x] = 0;
         __syncthreads();
         int qb[SEED_K];
         uint32_t rf[SEED_K], rb[SEED_K];
@@ -59,25 +73,37 @@ __global__ void __launch_bounds__(T) k_seed_frontier(GraphDev g, double eps,
         for (int k = 0; k < SEED_K; ++k) {
             const uint64_t e = tile + (uint64_t)k * T + threadIdx.x;
             qb[k] = -1;
-            if (e < g.m && g.sup[e] < eps * g.ew[e] && g.est[e] == ST_ALIVE) {
-                set_status(g, (uint32_t)e, ST_QUEUED);
-                qb[k] = bk ? work_bucket(g.tcnt[e]) : 0;
-                rf[k] = atomicAdd(&lcnt[4], 1u);
-                if (bk) rb[k] = atomicAdd(&lcnt[qb[k]], 1u);
+            if (e < g.m && g.est[e] == ST_ALIVE) {
+                const uint32_t tc = terms ? g.tcnt[e] : 0u;
+                if (g.sup[e] < eps * g.ew[e]) {
+                    mark(g, (uint32_t)e, ST_QUEUED, terms);
+                    qb[k] = terms ? work_bucket(tc) : 0;
+                    rf[k] = atomicAdd(&lcnt[4], 1u);
+                    if (terms) rb[k] = atomicAdd(&lcnt[qb[k]], 1u);
+                } else if (terms && tc) {
+                    qb[k] = 8 + work_bucket(tc);  // alive bucket
+                    rb[k] = atomicAdd(&lcnt[5 + work_bucket(tc)], 1u);
+                }
             }
         }
         __syncthreads();
-        if (threadIdx.x < 5 && lcnt[threadIdx.x]) {
-            unsigned* c = threadIdx.x == 4 ? nF : bk + 4ull * g.m + threadIdx.x;
-            if (threadIdx.x == 4 || bk) lbase[threadIdx.x] = atomicAdd(c, lcnt[threadIdx.x]);
+        if (threadIdx.x < 9 && lcnt[threadIdx.x]) {
+            const int i = threadIdx.x;
+            unsigned* c = i == 4 ? nF : i < 4 ? bk + 4ull * g.m + i : ak + 4ull * g.m + (i - 5);
+            if (i == 4 || terms) lbase[i] = atomicAdd(c, lcnt[i]);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < SEED_K; ++k) {
             if (qb[k] < 0) continue;
             const uint32_t e = (uint32_t)(tile + (uint64_t)k * T + threadIdx.x);
+            if (qb[k] >= 8) {
+                const int b = qb[k] - 8;
+                ak[(uint64_t)b * g.m + lbase[5 + b] + rb[k]] = e;
+                continue;
+            }
             F[lbase[4] + rf[k]] = e;
-            if (bk) bk[(uint64_t)qb[k] * g.m + lbase[qb[k]] + rb[k]] = e;
+            if (terms) bk[(uint64_t)qb[k] * g.m + lbase[qb[k]] + rb[k]] = e;
         }
         __syncthreads();  // lcnt / lbase reused by the next tile
     }
@@ -89,7 +115,7 @@ __global__ void __launch_bounds__(T) k_seed_pairs(GraphDev g,
                                                   const uint32_t* __restrict__ pr,
                                                   uint32_t k,
                                                   uint32_t* __restrict__ F,
-                                                  unsigned* nF, unsigned* err) {
+                                                  unsigned* nF, unsigned* err, bool terms) {
     for (uint32_t q = blockIdx.x * T + threadIdx.x; q < k; q += gridDim.x * T) {
         uint32_t a = pr[2 * q], b = pr[2 * q + 1];
         if (a >= g.n || b >= g.n) {
@@ -106,15 +132,15 @@ __global__ void __launch_bounds__(T) k_seed_pairs(GraphDev g,
             atomicOr(err, ERR_UNSORTED_OR_DUP);
             continue;
         }
-        set_status(g, e, ST_QUEUED);
+        mark(g, e, ST_QUEUED, terms);
         F[atomicAdd(nF, 1u)] = e;
     }
 }
 
 __device__ __forceinline__ void claim_edge(const GraphDev& g, uint32_t f,
-                                           uint32_t* Fn, unsigned* nFn) {
+                                           uint32_t* Fn, unsigned* nFn, bool terms) {
     if (atomicCAS(&g.claim[f], 0u, 1u) == 0u) {
-        set_status(g, f, ST_PENDING);
+        mark(g, f, ST_PENDING, terms);
         Fn[wave_append(nFn)] = f;
     }
 }
@@ -177,7 +203,7 @@ __device__ __forceinline__ void peel_side(const PeelArgs& A, uint32_t f, uint8_t
     const double thr = A.eps * g.ew[f];
     const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
     if (nv < thr - cb) {
-        claim_edge(g, f, A.Fn, A.nFn);
+        claim_edge(g, f, A.Fn, A.nFn, A.tb.t != nullptr);
     } else if (nv < thr + cb && atomicExch(&A.nearst[f], A.stamp) != A.stamp) {
         A.near[wave_append(A.nNear)] = f;
     }
@@ -338,36 +364,102 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
     }
 }
 
+// Item i of the four alive buckets (lists of capacity m, counts c[0..3]), as
+// one sequence.
+__device__ __forceinline__ uint32_t alive_item(const uint32_t* __restrict__ ak, uint64_t m,
+                                               const uint32_t c[4], uint32_t i) {
+    int b = 0;
+    while (b < 3 && i >= c[b]) i -= c[b++];
+    return ak[(uint64_t)b * m + i];
+}
+
 // After the pull kernels of a round: the edges they claimed (claim set,
-// status still ALIVE) go to the next frontier as PENDING, one global atomic
-// per tile of SEED_K * T edges.
-__global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, uint32_t* __restrict__ Fn,
+// status still ALIVE) go to the next frontier as PENDING.  Only the alive
+// buckets the pull kernels walked can hold them.  One global atomic per tile
+// of SEED_K * T items.
+__global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, const uint32_t* __restrict__ ak,
+                                                      uint32_t* __restrict__ Fn,
                                                       unsigned* nFn) {
     __shared__ uint32_t lcnt, lbase;
-    for (uint64_t tile = (uint64_t)blockIdx.x * SEED_K * T; tile < g.m;
-         tile += (uint64_t)gridDim.x * SEED_K * T) {
+    const unsigned* ac = ak + 4ull * g.m;
+    const uint32_t c[4] = {ac[0], ac[1], ac[2], ac[3]};
+    const uint32_t na = c[0] + c[1] + c[2] + c[3];
+    for (uint32_t tile = blockIdx.x * SEED_K * T; tile < na; tile += gridDim.x * SEED_K * T) {
         if (threadIdx.x == 0) lcnt = 0;
         __syncthreads();
-        uint32_t r[SEED_K];
-        bool c[SEED_K];
+        uint32_t r[SEED_K], ev[SEED_K];
+        bool cl[SEED_K];
 #pragma unroll
         for (int k = 0; k < SEED_K; ++k) {
-            const uint64_t e = tile + (uint64_t)k * T + threadIdx.x;
-            c[k] = e < g.m && g.claim[e] && g.est[e] == ST_ALIVE;
-            if (c[k]) r[k] = atomicAdd(&lcnt, 1u);
+            const uint32_t i = tile + k * T + threadIdx.x;
+            cl[k] = false;
+            if (i < na) {
+                ev[k] = alive_item(ak, g.m, c, i);
+                cl[k] = g.claim[ev[k]] && g.est[ev[k]] == ST_ALIVE;
+            }
+            if (cl[k]) r[k] = atomicAdd(&lcnt, 1u);
         }
         __syncthreads();
         if (threadIdx.x == 0 && lcnt) lbase = atomicAdd(nFn, lcnt);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < SEED_K; ++k) {
-            if (!c[k]) continue;
-            const uint32_t e = (uint32_t)(tile + (uint64_t)k * T + threadIdx.x);
-            set_status(g, e, ST_PENDING);
-            Fn[lbase + r[k]] = e;
+            if (!cl[k]) continue;
+            g.est[ev[k]] = ST_PENDING;  // pull rounds run on stored triangle lists only
+            Fn[lbase + r[k]] = ev[k];
         }
         __syncthreads();  // lcnt is reset for the next tile
     }
+}
+
+// The alive buckets of the next round: the items of ak still ALIVE once the
+// round has ended (the frontier dead, the claimed edges queued), into ak2 in
+// the same order per bucket.  Work in proportion to the alive edges rather
+// than to m.
+__global__ void __launch_bounds__(T) k_next_alive(GraphDev g, const uint32_t* __restrict__ ak,
+                                                  uint32_t* __restrict__ ak2) {
+    __shared__ uint32_t lcnt[4], lbase[4];
+    const unsigned* ac = ak + 4ull * g.m;
+    unsigned* ac2 = ak2 + 4ull * g.m;
+    const uint32_t c[4] = {ac[0], ac[1], ac[2], ac[3]};
+    const uint32_t na = c[0] + c[1] + c[2] + c[3];
+    for (uint32_t tile = blockIdx.x * SEED_K * T; tile < na; tile += gridDim.x * SEED_K * T) {
+        if (threadIdx.x < 4) lcnt[threadIdx.x] = 0;
+        __syncthreads();
+        int qb[SEED_K];
+        uint32_t rb[SEED_K], ev[SEED_K];
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k) {
+            const uint32_t i = tile + k * T + threadIdx.x;
+            qb[k] = -1;
+            if (i < na) {
+                uint32_t j = i;
+                int b = 0;
+                while (b < 3 && j >= c[b]) j -= c[b++];
+                ev[k] = ak[(uint64_t)b * g.m + j];
+                if (g.est[ev[k]] == ST_ALIVE) {
+                    qb[k] = b;
+                    rb[k] = atomicAdd(&lcnt[b], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 4 && lcnt[threadIdx.x])
+            lbase[threadIdx.x] = atomicAdd(&ac2[threadIdx.x], lcnt[threadIdx.x]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SEED_K; ++k)
+            if (qb[k] >= 0) ak2[(uint64_t)qb[k] * g.m + lbase[qb[k]] + rb[k]] = ev[k];
+        __syncthreads();
+    }
+}
+
+// Slot statuses from the edge statuses, after a peel on stored triangle
+// lists (which writes est only).
+__global__ void __launch_bounds__(T) k_sync_slot_status(GraphDev g) {
+    for (uint64_t j = (uint64_t)blockIdx.x * T + threadIdx.x; j < g.nnz;
+         j += (uint64_t)gridDim.x * T)
+        g.st[j] = g.est[g.s2e[j]];
 }
 
 // Triangles with all three edges alive, counted once per alive edge.
@@ -444,7 +536,7 @@ __global__ void __launch_bounds__(T) k_peel(PeelArgs A) {
                         const double thr = edge_thr(A.eps, side ? dhi : dlo, dw);
                         const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
                         if (nv < thr - cb) {
-                            claim_edge(g, f, A.Fn, A.nFn);
+                            claim_edge(g, f, A.Fn, A.nFn, false);  // intersection form
                         } else if (nv < thr + cb &&
                                    atomicExch(&A.nearst[f], A.stamp) != A.stamp) {
                             A.near[wave_append(A.nNear)] = f;
@@ -487,7 +579,7 @@ __global__ void __launch_bounds__(T) k_near_check(PeelArgs A) {
             note_undecided(A.undec, A.undec_cap, A.ctr, f);
             c = g.sup[f] < thr ? CERT_REMOVE : CERT_KEEP;
         }
-        if (c == CERT_REMOVE) claim_edge(g, f, A.Fn, A.nFn);
+        if (c == CERT_REMOVE) claim_edge(g, f, A.Fn, A.nFn, A.tb.t != nullptr);
     }
     for (int d = 32; d >= 1; d >>= 1) amb += __shfl_xor(amb, d, 64);
     if (lane_id() == 0 && amb) atomicAdd(&A.ctr->ambiguous_clean, amb);
@@ -499,16 +591,16 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
                                                  uint32_t nF,
                                                  const uint32_t* __restrict__ Fn,
                                                  const unsigned* __restrict__ nFnp,
-                                                 Counters* ctr) {
+                                                 Counters* ctr, bool terms) {
     const uint32_t nFn = *nFnp;
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < max(nF, nFn);
          i += gridDim.x * T) {
         if (i < nF) {
             const uint32_t f = F[i];
-            set_status(g, f, ST_DEAD);
+            mark(g, f, ST_DEAD, terms);
             g.sup[f] = 0.0;
         }
-        if (i < nFn) set_status(g, Fn[i], ST_QUEUED);
+        if (i < nFn) mark(g, Fn[i], ST_QUEUED, terms);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&ctr->edges_removed, (unsigned long long)nF);
@@ -865,8 +957,13 @@ unsigned grid_items(uint32_t items, int G) {
 
 // Frontier rounds to convergence.  first_bk (optional) holds the first
 // frontier already split by work bucket (4 lists of capacity m + 4 counts).
+// With stored triangle lists, ak[0] and ak[1] (same layout) hold the alive
+// buckets a pull round walks: ak[0] valid on entry when ak_valid (seeding
+// wrote it), carried from one pull round to the next by k_next_alive, and
+// rebuilt by a scan of every edge after a push round.
 rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
-                  uint32_t nF, const uint32_t* first_bk = nullptr,
+                  uint32_t nF, uint32_t* ak0, uint32_t* ak1, bool ak_valid,
+                  const uint32_t* first_bk = nullptr,
                   uint32_t* undec = nullptr, uint32_t undec_cap = 0) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
@@ -900,25 +997,31 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_HIP(hipMemsetAsync(A.nNear, 0, sizeof(unsigned), s));
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
-        const bool pull = A.tb.t != nullptr && (uint64_t)nF > h->n_alive;
+        const bool terms = A.tb.t != nullptr;
+        const bool pull = terms && (uint64_t)nF > h->n_alive;
+        // alive-list launches: sized from the alive count at the round's start
+        const unsigned ba = blocks_for((h->n_alive + SEED_K - 1) / SEED_K, T, 4096);
         if (pull) {
-            first_bk = nullptr;  // pull rounds scan the alive edges instead
+            first_bk = nullptr;  // pull rounds walk the alive edges instead
             A.F = F;
             A.nF = nF;
             A.nFd = nullptr;
-            RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
-            k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, bk, bcur);
+            unsigned* acur = ak0 + 4ull * g.m;
+            if (!ak_valid) {
+                RT_HIP(hipMemsetAsync(acur, 0, 4 * sizeof(unsigned), s));
+                k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(
+                    g, ak0, acur);
+            }
             for (int b = 0; b < 4 && h->n_alive; ++b) {
                 const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
-                const uint32_t* L = bk + (uint64_t)b * g.m;
+                const uint32_t* L = ak0 + (uint64_t)b * g.m;
                 const unsigned grid = grid_items((uint32_t)h->n_alive, G);
-                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, bcur + b);
-                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, bcur + b);
-                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, bcur + b);
-                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, bcur + b);
+                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, acur + b);
+                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, acur + b);
+                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, acur + b);
+                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, acur + b);
             }
-            k_collect_claims<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, Fn,
-                                                                                         A.nFn);
+            k_collect_claims<<<ba, T, 0, s>>>(g, ak0, Fn, A.nFn);
         } else {
             // Work buckets of the frontier (1, 4, 16 or 64 lanes per edge):
             // split while it was seeded (first round), split here when large,
@@ -971,7 +1074,13 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         // The frontier dies, the claimed edges are queued.  Both lists are
         // bounded by the edges alive at the start of the round.
         k_end_round<<<blocks_for(std::max<uint64_t>(nF, h->n_alive), T, 4096), T, 0, s>>>(
-            g, F, nF, Fn, A.nFn, h->ctr);
+            g, F, nF, Fn, A.nFn, h->ctr, terms);
+        if (pull) {  // the next pull round's work list
+            RT_HIP(hipMemsetAsync(ak1 + 4ull * g.m, 0, 4 * sizeof(unsigned), s));
+            k_next_alive<<<ba, T, 0, s>>>(g, ak0, ak1);
+            std::swap(ak0, ak1);
+        }
+        ak_valid = pull;
         RT_HIP(hipGetLastError());
         uint32_t nFn = 0;
         RT_HIP(hipMemcpyAsync(&nFn, A.nFn, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -1018,16 +1127,24 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
-    uint32_t *F, *Fn, *bk = nullptr;
+    uint32_t *F, *Fn, *bk = nullptr, *ak0 = nullptr, *ak1 = nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
     const bool terms = h->zt != nullptr;
-    if (terms) RT_TRY(sc.get(&bk, 4ull * g.m + 4));
+    if (terms) {
+        RT_TRY(sc.get(&bk, 4ull * g.m + 4));
+        RT_TRY(sc.get(&ak0, 4ull * g.m + 4));
+        RT_TRY(sc.get(&ak1, 4ull * g.m + 4));
+    }
     RT_HIP(hipEventRecord(h->ev[5], s));
     unsigned* cnt = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
-    if (terms) RT_HIP(hipMemsetAsync(bk + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
-    k_seed_frontier<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, eps, F, cnt, bk);
+    if (terms) {
+        RT_HIP(hipMemsetAsync(bk + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
+        RT_HIP(hipMemsetAsync(ak0 + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
+    }
+    k_seed_frontier<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(g, eps, F, cnt,
+                                                                                 bk, ak0);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
     RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -1044,7 +1161,8 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
         RT_HIP(hipMemcpyAsync(&h->ctr->n_undecided, &nf, 4, hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
     }
-    RT_TRY(peel(h, eps, F, Fn, nF, bk, terms ? undec : nullptr, g.m));
+    RT_TRY(peel(h, eps, F, Fn, nF, ak0, ak1, terms, bk, terms ? undec : nullptr, g.m));
+    if (terms) k_sync_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
     k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
     RT_HIP(hipGetLastError());
@@ -1074,16 +1192,21 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
         pr[i] = (uint32_t)pairs[i];
     }
     Scratch sc;
-    uint32_t *F, *Fn, *dpr;
+    uint32_t *F, *Fn, *dpr, *ak0 = nullptr, *ak1 = nullptr;
+    const bool terms = h->zt != nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
     RT_TRY(sc.get(&dpr, 2 * k));
+    if (terms) {
+        RT_TRY(sc.get(&ak0, 4ull * g.m + 4));
+        RT_TRY(sc.get(&ak1, 4ull * g.m + 4));
+    }
     RT_HIP(hipMemcpyAsync(dpr, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, s));
     unsigned* cnt = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
     RT_HIP(hipMemsetAsync(&h->ctr->error, 0, sizeof(unsigned), s));
     k_seed_pairs<<<blocks_for(k, T, 4096), T, 0, s>>>(g, dpr, (uint32_t)k, F, cnt,
-                                                      &h->ctr->error);
+                                                      &h->ctr->error, terms);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
     unsigned err = 0;
@@ -1097,7 +1220,10 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                     "graph, or is listed twice (state is undefined now)");
     if (h->xact) RT_HIP(hipMemsetAsync(h->xact, 0, g.m, s));  // supports no longer the reference's
     h->n_alive -= nF;
-    RT_TRY(peel(h, eps, F, Fn, nF));
+    RT_TRY(peel(h, eps, F, Fn, nF, ak0, ak1, false));
+    if (terms) k_sync_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }
 
