@@ -540,6 +540,7 @@ rtrex_status time_count(rtrex_graph g, int repeats, double* ms) {
     double best = 1e300;
     for (int r = 0; r < repeats; ++r) {
         RT_TRY(run_count(g));
+        RT_TRY(resolve_timings(g));
         best = std::min(best, (double)g->tm.count_edge_ms);
     }
     *ms = best;
@@ -680,6 +681,7 @@ rtrex_status rtrex_get_audit(rtrex_graph g, rtrex_audit* out) {
 rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out) {
     RT_TRY(check(g));
     if (!out) return fail(RTREX_EINVAL, "null output");
+    RT_TRY(resolve_timings(g));
     out->build_ms = g->tm.build_ms;
     out->count_ms = g->tm.count_ms;
     out->clean_ms = g->tm.clean_ms;

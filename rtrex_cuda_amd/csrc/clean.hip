@@ -462,6 +462,12 @@ __global__ void __launch_bounds__(T) k_sync_slot_status(GraphDev g) {
         g.st[j] = g.est[g.s2e[j]];
 }
 
+// Triangles removed by the clean: the count phase's total minus the live ones
+// (incidences are three per triangle).
+__global__ void k_tri_removed(const unsigned long long* __restrict__ inc, Counters* ctr) {
+    if (threadIdx.x == 0) ctr->tri_removed = ctr->tri_total / 3 - *inc / 3;
+}
+
 // Triangles with all three edges alive, counted once per alive edge.
 __global__ void __launch_bounds__(T) k_live_incidences(GraphDev g, TermBuf tb,
                                                        unsigned long long* out) {
@@ -882,12 +888,10 @@ int bits_of(uint64_t n) {
 }
 
 // Re-runs the regions of the undecided edges exactly (see above).
-rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint32_t cap) {
+rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint32_t cap,
+                          unsigned nund) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
-    unsigned nund = 0;
-    RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
     if (!nund || !h->zt) return RTREX_OK;
     Scratch sc;
     uint32_t *parent, *redge, *rflag, *rpos, *rstart;
@@ -1093,27 +1097,33 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr, undec,
                                                                undec_cap);
     RT_HIP(hipGetLastError());
+    Scratch sc2;
+    unsigned long long* inc = nullptr;
+    if (A.tb.t) RT_TRY(sc2.get(&inc, 1));
+    // Live-triangle total (the peel kernels do not count), on the device.
+    auto removed_total = [&]() -> rtrex_status {
+        if (!A.tb.t) return RTREX_OK;
+        RT_HIP(hipMemsetAsync(inc, 0, 8, s));
+        k_live_incidences<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, inc);
+        k_tri_removed<<<1, WAVE, 0, s>>>(inc, h->ctr);
+        RT_HIP(hipGetLastError());
+        return RTREX_OK;
+    };
     if (undec) {
         if (!h->xact) RT_TRY(dalloc(h, &h->xact, g.m));
         k_xact_init<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, h->xact);
-        RT_TRY(exact_replay(h, eps, undec, undec_cap));
     }
-    if (A.tb.t) {  // live-triangle total (the peel kernels do not count)
-        Scratch sc2;
-        unsigned long long* inc;
-        RT_TRY(sc2.get(&inc, 1));
-        RT_HIP(hipMemsetAsync(inc, 0, 8, s));
-        k_live_incidences<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, inc);
-        unsigned long long c[2] = {0, 0};
-        RT_HIP(hipMemcpyAsync(&c[0], inc, 8, hipMemcpyDeviceToHost, s));
-        RT_HIP(hipMemcpyAsync(&c[1], &h->ctr->tri_total, 8, hipMemcpyDeviceToHost, s));
+    RT_TRY(removed_total());
+    if (undec) {  // the only wait after the last round: were any decisions open?
+        unsigned nund = 0;
+        RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
-        unsigned long long removed = c[1] / 3 - c[0] / 3;
-        RT_HIP(hipMemcpyAsync(&h->ctr->tri_removed, &removed, 8, hipMemcpyHostToDevice, s));
-        RT_HIP(hipStreamSynchronize(s));
+        if (nund && h->zt) {
+            RT_TRY(exact_replay(h, eps, undec, undec_cap, nund));
+            RT_TRY(removed_total());
+        }
     }
     h->pv_full = false;  // vertex supports are pulled again on demand
-    RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
 }
 
@@ -1167,10 +1177,7 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
     RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[6], s));
-    RT_HIP(hipEventSynchronize(h->ev[6]));
-    float ms = 0;
-    RT_HIP(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
-    h->tm.clean_ms = ms;
+    h->tm_clean_pending = true;  // read later (resolve_timings), no wait here
     h->eps = eps;
     h->phase = PH_CLEANED;
     return RTREX_OK;

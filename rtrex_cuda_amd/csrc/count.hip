@@ -1324,17 +1324,34 @@ rtrex_status run_count(rtrex_graph h) {
     RT_HIP(hipEventRecord(h->ev[4], s));
     // Fresh clean state for the supports just computed (one launch).
     k_reset_clean<<<blocks_for(std::max<uint64_t>(g.nnz / 16, 1), T, 4096), T, 0, s>>>(g, h->ctr);
-    RT_HIP(hipEventSynchronize(h->ev[4]));
-    float a = 0, b = 0;
-    RT_HIP(hipEventElapsedTime(&a, h->ev[2], h->ev[3]));
-    RT_HIP(hipEventElapsedTime(&b, h->ev[3], h->ev[4]));
-    h->tm.count_edge_ms = a;
-    h->tm.count_vertex_ms = b;
-    h->tm.count_ms = a + b;
-    RT_HIP(hipStreamSynchronize(s));
+    RT_HIP(hipGetLastError());
+    // No wait here: the next phase's launches queue behind these; the phase
+    // time is read from the events later (resolve_timings).
+    h->tm_count_pending = true;
     h->audit = AuditHost();
     h->n_alive = g.m;
     h->phase = PH_COUNTED;
+    return RTREX_OK;
+}
+
+rtrex_status resolve_timings(rtrex_graph h) {
+    if (h->tm_count_pending) {
+        RT_HIP(hipEventSynchronize(h->ev[4]));
+        float a = 0, b = 0;
+        RT_HIP(hipEventElapsedTime(&a, h->ev[2], h->ev[3]));
+        RT_HIP(hipEventElapsedTime(&b, h->ev[3], h->ev[4]));
+        h->tm.count_edge_ms = a;
+        h->tm.count_vertex_ms = b;
+        h->tm.count_ms = a + b;
+        h->tm_count_pending = false;
+    }
+    if (h->tm_clean_pending) {
+        RT_HIP(hipEventSynchronize(h->ev[6]));
+        float ms = 0;
+        RT_HIP(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
+        h->tm.clean_ms = ms;
+        h->tm_clean_pending = false;
+    }
     return RTREX_OK;
 }
 

@@ -713,7 +713,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     const uint32_t n = g.n;
-    RT_HIP(hipEventRecord(h->ev[5], s));
+    RT_HIP(hipEventRecord(h->ev[7], s));
     Scratch sc;
     uint64_t *keys, *tmp;
     uint32_t *parent, *csize, *vseg = nullptr;
@@ -761,6 +761,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     RT_HIP(hipStreamSynchronize(s));
     h->audit.components = 0;
     h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
+    uint32_t* x_undec_list = nullptr;  // device list of undecided cascade decisions
     if (nk) {
         if (nk <= SB_MAX) k_sort_small<<<1, SB_T, 0, s>>>(keys, nk);
         else RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
@@ -864,15 +865,9 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         k_extract<<<grid, XT, 0, s>>>(a);
         RT_HIP(hipGetLastError());
         // Undecided cascade decisions (inexact post-clean supports): kept for
-        // an exact re-run of their regions (rtrex_extract).
-        unsigned nu = 0;
-        RT_HIP(hipMemcpyAsync(&nu, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
-        h->x_undec.resize(std::min<uint32_t>(nu, g.m));
-        if (nu)
-            RT_HIP(hipMemcpyAsync(h->x_undec.data(), undec, h->x_undec.size() * 4ull,
-                                  hipMemcpyDeviceToHost, s));
-        h->x_undec_all = nu > g.m;
+        // an exact re-run of their regions (rtrex_extract); their count is
+        // read with the phase's last results below.
+        x_undec_list = undec;
     }
     // cluster ids in seed order; they stay in HBM (copy_cluster_map)
     uint32_t *flag2, *cidr;
@@ -900,19 +895,29 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                                                              (uint32_t)(ns + 1), bounds_d);
         RT_HIP(hipGetLastError());
     }
-    unsigned cerr = 0;
+    unsigned cerr = 0, nu = 0;
     uint32_t total = 0;
+    if (x_undec_list)
+        RT_HIP(hipMemcpyAsync(&nu, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipMemcpyAsync(&total, cidr + n, 4, hipMemcpyDeviceToHost, s));
     h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
     if (ns)
         RT_HIP(hipMemcpyAsync(h->seg_bounds.data(), bounds_d, (ns + 1) * 4,
                               hipMemcpyDeviceToHost, s));
-    RT_HIP(hipEventRecord(h->ev[6], s));
-    RT_HIP(hipEventSynchronize(h->ev[6]));
+    RT_HIP(hipEventRecord(h->ev[8], s));
+    RT_HIP(hipEventSynchronize(h->ev[8]));
     if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
+    h->x_undec.resize(std::min<uint32_t>(nu, g.m));
+    h->x_undec_all = nu > g.m;
+    if (nu) {
+        RT_HIP(hipMemcpyAsync(h->x_undec.data(), x_undec_list, h->x_undec.size() * 4ull,
+                              hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    RT_TRY(resolve_timings(h));
     float ms = 0;
-    RT_HIP(hipEventElapsedTime(&ms, h->ev[5], h->ev[6]));
+    RT_HIP(hipEventElapsedTime(&ms, h->ev[7], h->ev[8]));
     h->tm.extract_ms = ms;
     h->n_clusters = total;
     h->seg_clusters.assign(ns, 0);

@@ -209,7 +209,11 @@ struct rtrex_graph_s {
     uint64_t slab_cap = 0, slab_top = 0;
     bool own_stream = true;
     rtrex::TermBuf terms() const { return {zt, zoff, zsl, srec}; }
-    hipEvent_t ev[8] = {};
+    // Events: 0-1 build, 2-4 count, 5-6 clean, 7-8 extract.  The count and
+    // clean times are read once something waits for the stream anyway
+    // (resolve_timings), so those phases end without a host round trip.
+    hipEvent_t ev[9] = {};
+    bool tm_count_pending = false, tm_clean_pending = false;
 };
 
 namespace rtrex {
@@ -352,6 +356,8 @@ rtrex_status build_from_csr(rtrex_graph h, const int64_t* off,
 rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
                               const uint32_t* v, int64_t n, int64_t m);
 rtrex_status run_count(rtrex_graph h);
+// Phase times whose events are still pending (waits for them).
+rtrex_status resolve_timings(rtrex_graph h);
 rtrex_status run_initial_clean(rtrex_graph h, double eps);
 rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                                   int64_t k, double eps);
