@@ -173,7 +173,7 @@ namespace rtrex {
 // each rounded up to 256 bytes.
 uint64_t resident_bytes(uint64_t n, uint64_t m) {
     const uint64_t nnz = 2 * m;
-    return 37 * nnz + 107 * m + 68 * n + 96 * 256 + 8192 + 64;
+    return 37 * nnz + 107 * m + 72 * n + 96 * 256 + 8192 + 64;
 }
 // The largest phase scratch: the build (edge upload, radix keys, partner
 // ranks), the hub plan, the clean (frontiers and buckets) or the extraction
@@ -183,7 +183,7 @@ uint64_t scratch_bytes(uint64_t n, uint64_t m) {
     const uint64_t build = 28 * nnz + 8 * m;
     const uint64_t hubs = 8 * nnz + 4 * n;
     const uint64_t clean = 128 * m;  // frontiers, buckets, alive lists, undecided list, exact replay
-    const uint64_t extract = 120 * n + 4 * nnz;
+    const uint64_t extract = 160 * n + 4 * nnz;
     return std::max(std::max(build, hubs), std::max(clean, extract)) + nnz / 16 +
            (4ull << 20) + 64 * 256;
 }

@@ -169,6 +169,28 @@ __global__ void __launch_bounds__(T) k_comp_layout(const uint64_t* __restrict__ 
     }
 }
 
+// Per component c < ncomp: its unordered_set bucket count bcnt[size] (0 past
+// ncomp, up to bound), and the largest size.
+__global__ void __launch_bounds__(T) k_comp_hsize(const uint32_t* __restrict__ cstart,
+                                                  const uint32_t* __restrict__ ncomp_d,
+                                                  uint32_t bound,
+                                                  const uint32_t* __restrict__ bcnt,
+                                                  uint32_t* __restrict__ hsz, uint32_t* big) {
+    const uint32_t ncomp = *ncomp_d;
+    uint32_t mx = 0;
+    for (uint32_t c = blockIdx.x * T + threadIdx.x; c < bound; c += gridDim.x * T) {
+        uint32_t v = 0;
+        if (c < ncomp) {
+            const uint32_t len = cstart[c + 1] - cstart[c];
+            v = bcnt[len];
+            mx = max(mx, len);
+        }
+        hsz[c] = v;
+    }
+    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+    if (lane_id() == 0 && mx) atomicMax(big, mx);
+}
+
 __global__ void __launch_bounds__(T) k_comp_starts(const uint32_t* __restrict__ flag,
                                                    const uint32_t* __restrict__ fpos,
                                                    uint32_t nk,
@@ -213,7 +235,7 @@ struct XArgs {
     const uint32_t* cv;      // component-grouped vertices, seed order inside
     const uint32_t* cstart;  // ncomp + 1 boundaries into cv
     const uint64_t* dpre;    // prefix of degrees over cv (scratch placement)
-    uint32_t ncomp;
+    const uint32_t* ncomp;   // number of components (device)
     uint32_t* cseed;         // seed of each vertex's cluster, NONE if not yet
     uint32_t* mrank;         // position at which the vertex joined its cluster
     uint32_t* mk;            // neighbour stamp (seed + 1)
@@ -297,12 +319,13 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     const GraphDev& g = A.g;
     const int lane = lane_id();
     unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
+    const uint32_t ncomp = *A.ncomp;
 
     while (true) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(&A.ctr->work_next, 1u);
         c = __shfl(c, 0, 64);
-        if (c >= A.ncomp) break;
+        if (c >= ncomp) break;
         const uint32_t start = A.cstart[c], len = A.cstart[c + 1] - start;
         const uint64_t ebase = A.dpre[start];
 
@@ -704,6 +727,37 @@ int bits_for(uint64_t n) {
 
 }  // namespace
 
+// unordered_set<int64> bucket counts after k insertions, k = 0..upto,
+// measured from the library (the iteration order the extraction emulates
+// depends on them), with their device copy and the largest ratio
+// bcnt[k] / k over k >= 2 (which bounds the set scratch of components).
+rtrex_status ensure_bucket_table(rtrex_graph h, uint64_t upto) {
+    if (h->bcnt.size() < upto + 1) {
+        std::unordered_set<int64_t> probe;
+        h->bcnt.assign(1, (uint32_t)probe.bucket_count());
+        uint32_t ratio = 1;
+        for (uint64_t k = 1; k <= upto; ++k) {
+            probe.insert((int64_t)k);
+            const uint32_t b = (uint32_t)probe.bucket_count();
+            h->bcnt.push_back(b);
+            if (k >= 2) ratio = std::max<uint32_t>(ratio, (uint32_t)((b + k - 1) / k));
+        }
+        h->bcnt_ratio = ratio;
+    }
+    if (!h->bcnt_dev) {
+        RT_TRY(dalloc(h, &h->bcnt_dev, (uint64_t)h->g.n + 1));
+        h->bcnt_dev_len = 0;
+    }
+    if (h->bcnt_dev_len < upto + 1) {
+        // the host table stays unchanged until a later extraction, which
+        // begins after this one has synchronised
+        RT_HIP(hipMemcpyAsync(h->bcnt_dev, h->bcnt.data(), (upto + 1) * 4,
+                              hipMemcpyHostToDevice, h->stream));
+        h->bcnt_dev_len = upto + 1;
+    }
+    return RTREX_OK;
+}
+
 rtrex_status run_extract(rtrex_graph h, double eps) {
     if (h->phase < PH_CLEANED)
         return fail(RTREX_ESTATE, "extract needs count and initial clean first");
@@ -762,6 +816,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     h->audit.components = 0;
     h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
     uint32_t* x_undec_list = nullptr;  // device list of undecided cascade decisions
+    const uint32_t* ncomp_d = nullptr;  // component count (device)
+    uint32_t* big_d = nullptr;          // largest component (device)
     if (nk) {
         if (nk <= SB_MAX) k_sort_small<<<1, SB_T, 0, s>>>(keys, nk);
         else RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
@@ -776,48 +832,28 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         k_comp_layout<<<bk, T, 0, s>>>(keys, nk, order, g, cv, vdeg, flag);
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32_32(flag, fpos, nk, s));
-        uint32_t ncomp = 0;
-        RT_HIP(hipMemcpyAsync(&ncomp, fpos + nk, 4, hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
-        RT_TRY(sc.get(&cstart, (uint64_t)ncomp + 1));
+        // The component count stays on the device (fpos[nk]); everything
+        // below is sized from nk (components have two vertices or more).
+        ncomp_d = fpos + nk;
+        RT_TRY(sc.get(&cstart, (uint64_t)nk + 1));
         k_comp_starts<<<bk, T, 0, s>>>(flag, fpos, nk, cstart);
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
         RT_TRY(ensure_vertex_supports(h, cv, nk));  // candidates' supports
         // Bucket counts of the reference's unordered_set (measured from the
-        // library) up to the largest component, and each component's share
-        // of the set scratch.
-        uint32_t *bcnt_d, *hoff, *hnext, *hbkt;
-        {
-            std::vector<uint32_t> cs(ncomp + 1);
-            RT_HIP(hipMemcpyAsync(cs.data(), cstart, (ncomp + 1) * 4ull,
-                                  hipMemcpyDeviceToHost, s));
-            RT_HIP(hipStreamSynchronize(s));
-            int64_t big = 0;
-            for (uint32_t c = 0; c < ncomp; ++c)
-                big = std::max<int64_t>(big, cs[c + 1] - cs[c]);
-            h->audit.components = ncomp;
-            h->audit.largest_component = big;
-            if ((int64_t)h->bcnt.size() < big + 1) {
-                std::unordered_set<int64_t> probe;
-                h->bcnt.assign(1, (uint32_t)probe.bucket_count());
-                for (int64_t k = 1; k <= big; ++k) {
-                    probe.insert(k);
-                    h->bcnt.push_back((uint32_t)probe.bucket_count());
-                }
-            }
-            std::vector<uint32_t> ho(ncomp + 1, 0);
-            for (uint32_t c = 0; c < ncomp; ++c) ho[c + 1] = ho[c] + h->bcnt[cs[c + 1] - cs[c]];
-            RT_TRY(sc.get(&bcnt_d, (uint64_t)big + 1));
-            RT_TRY(sc.get(&hoff, (uint64_t)ncomp + 1));
-            RT_TRY(sc.get(&hnext, nk));
-            RT_TRY(sc.get(&hbkt, std::max<uint32_t>(ho[ncomp], 1)));
-            RT_HIP(hipMemcpyAsync(bcnt_d, h->bcnt.data(), ((uint64_t)big + 1) * 4,
-                                  hipMemcpyHostToDevice, s));
-            RT_HIP(hipMemcpyAsync(hoff, ho.data(), ((uint64_t)ncomp + 1) * 4,
-                                  hipMemcpyHostToDevice, s));
-            RT_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
-        }
+        // library) for up to nk elements, and each component's share of the
+        // set scratch: hoff = prefix of bcnt[size] (at most ratio * nk).
+        RT_TRY(ensure_bucket_table(h, nk));
+        uint32_t *hsz, *hoff, *hnext, *hbkt;
+        RT_TRY(sc.get(&hsz, nk));
+        RT_TRY(sc.get(&hoff, (uint64_t)nk + 1));
+        RT_TRY(sc.get(&hnext, nk));
+        RT_TRY(sc.get(&hbkt, (uint64_t)h->bcnt_ratio * nk + 1));
+        RT_TRY(sc.get(&big_d, 1));
+        RT_HIP(hipMemsetAsync(big_d, 0, 4, s));
+        k_comp_hsize<<<bk, T, 0, s>>>(cstart, ncomp_d, nk, h->bcnt_dev, hsz, big_d);
+        RT_HIP(hipGetLastError());
+        RT_TRY(exclusive_scan_u32_32(hsz, hoff, nk, s));
         uint32_t *mk, *cm, *mem, *cand, *undec;
         uint64_t* wl;
         double *tw, *gpv;
@@ -841,7 +877,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.cv = cv;
         a.cstart = cstart;
         a.dpre = dpre;
-        a.ncomp = ncomp;
+        a.ncomp = ncomp_d;
         a.cseed = h->cseed;
         a.mrank = h->mrank;
         a.mk = mk;
@@ -855,13 +891,13 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.tb = h->terms();
         a.ctr = h->ctr;
         a.xact = h->xact;
-        a.bcnt = bcnt_d;
+        a.bcnt = h->bcnt_dev;
         a.hoff = hoff;
         a.hnext = hnext;
         a.hbkt = hbkt;
         a.undec = undec;
         a.undec_cap = g.m;
-        const unsigned grid = std::min<uint32_t>(ncomp, 8192);
+        const unsigned grid = std::min<uint32_t>(nk / 2, 8192);
         k_extract<<<grid, XT, 0, s>>>(a);
         RT_HIP(hipGetLastError());
         // Undecided cascade decisions (inexact post-clean supports): kept for
@@ -896,7 +932,11 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         RT_HIP(hipGetLastError());
     }
     unsigned cerr = 0, nu = 0;
-    uint32_t total = 0;
+    uint32_t total = 0, ncomp = 0, big = 0;
+    if (ncomp_d) {
+        RT_HIP(hipMemcpyAsync(&ncomp, ncomp_d, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipMemcpyAsync(&big, big_d, 4, hipMemcpyDeviceToHost, s));
+    }
     if (x_undec_list)
         RT_HIP(hipMemcpyAsync(&nu, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
@@ -908,6 +948,10 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     RT_HIP(hipEventRecord(h->ev[8], s));
     RT_HIP(hipEventSynchronize(h->ev[8]));
     if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
+    if (ncomp_d) {
+        h->audit.components = ncomp;
+        h->audit.largest_component = big;
+    }
     h->x_undec.resize(std::min<uint32_t>(nu, g.m));
     h->x_undec_all = nu > g.m;
     if (nu) {

@@ -159,6 +159,9 @@ struct rtrex_graph_s {
     uint32_t* mrank = nullptr;  // position at which each vertex joined its cluster
     uint8_t* xact = nullptr;    // per edge: post-clean support is the reference's bits
     std::vector<uint32_t> bcnt;     // unordered_set bucket count after k insertions
+    uint32_t* bcnt_dev = nullptr;   // device copy of bcnt[0, bcnt_dev_len), capacity n + 1
+    uint64_t bcnt_dev_len = 0;
+    uint32_t bcnt_ratio = 0;        // max over k >= 2 of ceil(bcnt[k] / k)
     std::vector<uint32_t> x_undec;  // edges the last extraction could not decide
     bool x_undec_all = false;       // ... more than the list holds
     std::vector<uint32_t> force_replay;  // edges whose regions the clean re-runs
