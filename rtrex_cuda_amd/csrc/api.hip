@@ -587,7 +587,7 @@ rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params
         CountParams c = cur;
         switch (coord) {
             case 0:
-                for (uint32_t v : {8u, 16u, 32u, 64u}) { c.hub_min = v; cand.push_back(c); }
+                for (uint32_t v : {4u, 8u, 16u, 32u, 64u}) { c.hub_min = v; cand.push_back(c); }
                 break;
             case 1:
                 for (auto lv : {std::pair<uint32_t, uint32_t>{64, 512}, {32, 512}, {64, 256},

@@ -286,12 +286,16 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
                                                  uint32_t* __restrict__ cls_cnt,
                                                  uint4* __restrict__ units,
                                                  const uint32_t* __restrict__ cls_base,
-                                                 CountParams Pm) {
-    // One thread per hub; class totals aggregated per block (LDS counters,
-    // one global atomic per class per block).
+                                                 CountParams Pm, uint32_t* __restrict__ cls_max) {
+    // One thread per hub; class totals (and, COUNT, largest hub degrees)
+    // aggregated per block (LDS counters, one global atomic per class per
+    // block).
     constexpr int NC = HUB_NL * HUB_NG;
-    __shared__ uint32_t lc[NC], lb[NC];
-    if (threadIdx.x < NC) lc[threadIdx.x] = 0;
+    __shared__ uint32_t lc[NC], lb[NC], lm[NC];
+    if (threadIdx.x < NC) {
+        lc[threadIdx.x] = 0;
+        lm[threadIdx.x] = 0;
+    }
     __syncthreads();
     const uint32_t h = blockIdx.x * T + threadIdx.x;
     int c = -1;
@@ -313,11 +317,14 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = lc_ * HUB_NG + hub_gclass(work, oc, Pm);
             at = atomicAdd(&lc[c], (uint32_t)P);
+            if (COUNT) atomicMax(&lm[c], dh);
         }
     }
     __syncthreads();
-    if (threadIdx.x < NC && lc[threadIdx.x])
+    if (threadIdx.x < NC && lc[threadIdx.x]) {
         lb[threadIdx.x] = atomicAdd(&cls_cnt[threadIdx.x], lc[threadIdx.x]);
+        if (COUNT) atomicMax(&cls_max[threadIdx.x], lm[threadIdx.x]);
+    }
     __syncthreads();
     if (!COUNT && c >= 0) {
         const uint32_t o0 = own_off[h], oc = own_off[h + 1] - o0;
@@ -415,19 +422,21 @@ __device__ __forceinline__ uint32_t hub_id(const GraphDev& g, uint32_t oh, uint3
 // small and medium hubs); otherwise read from the per-slot arrays in HBM at
 // oh (large hubs, whose 4096-entry staging would leave no room for other
 // workgroups).
-template <bool META>
+template <bool META, bool HIT = META>
 struct HubMeta {
+    // HIT: the edge ids and degrees a match needs are in LDS (sed, sdg) even
+    // when the owned edges' data is read from HBM.
     const uint32_t *sed, *sdg, *sof;
     const uint64_t* szo;
     uint32_t oh;
     __device__ __forceinline__ uint32_t ed(const GraphDev& g, uint32_t i) const {
-        return META ? sed[i] : g.nbe[oh + i].y;
+        return META || HIT ? sed[i] : g.nbe[oh + i].y;
     }
     __device__ __forceinline__ uint2 dego(const GraphDev& g, uint32_t i) const {
         return META ? make_uint2(sdg[i], sof[i]) : g.ndo[oh + i];
     }
     __device__ __forceinline__ uint32_t deg(const GraphDev& g, uint32_t i) const {
-        return META ? sdg[i] : g.ndo[oh + i].x;
+        return META || HIT ? sdg[i] : g.ndo[oh + i].x;
     }
     // Term offset of the edge of hub slot i.
     __device__ __forceinline__ uint64_t zo(const TermBuf& tb, uint32_t i) const {
@@ -441,11 +450,11 @@ struct HubMeta {
 // the hub's tree; own[] holds the owned slots (LDS or HBM).  Matches go to
 // the term buffer in ascending z; the support is folded in lane order.
 // sfold is this wave's 64-entry LDS slice.
-template <int G, int LV, bool META>
+template <int G, int LV, class HM>
 __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t dh,
                                           const uint32_t* leaf, const uint32_t* l1,
                                           const uint32_t* l2, const Piv8& top,
-                                          const HubMeta<META>& hm_, const uint32_t* own,
+                                          const HM& hm_, const uint32_t* own,
                                           double* sfold, uint32_t first, uint32_t cnt,
                                           uint32_t stride) {
     constexpr int R = G <= 8 ? 4 : 2;
@@ -551,7 +560,7 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
         for (int i = 0; i < 8; ++i) top.p[i] = __builtin_amdgcn_readlane(z, 8 * i + 7);
         if (lane < cnt) s_own[wid][lane] = own_idx[k0 + lane];
         wave_sync();
-        hub_edges<G, 1, true>(g, tb, dh, s_nb[wid], nullptr, nullptr, top, meta, s_own[wid],
+        hub_edges<G, 1>(g, tb, dh, s_nb[wid], nullptr, nullptr, top, meta, s_own[wid],
                               s_fold[wid], 0, cnt, WAVE / G);
         wave_sync();  // the next unit restages the slice
     }
@@ -765,9 +774,60 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
             for (uint32_t i = threadIdx.x; i < cnt; i += NT) s_own[i] = own_idx[k0 + i];
         __syncthreads();
         const HubMeta<META> meta{s_ed, s_dg, s_of, s_zo, oh};
-        hub_edges<G, LV, META>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta,
+        hub_edges<G, LV>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta,
                                META ? s_own : own_idx + k0, sfold + wid * WAVE, wid * per_wave,
                                cnt, per_wave * (NT / WAVE));
+        __syncthreads();  // the next unit restages the hub
+    }
+}
+
+// Large hubs (degree in (l1_max, 4096]): one workgroup of HTL lanes per unit,
+// the hub's 8-ary tree staged in LDS once and shared by its waves, with the
+// edge id and degree of every hub slot beside it so that a match needs no
+// memory access; the owned edges' own data is read from HBM (the hub's
+// slots, L2-resident).  The leaf and slot arrays live in dynamic LDS of lcap
+// entries each, lcap >= the launch's largest hub degree + 8: a probe ends in
+// the leaf block holding the hub's first padding entry at the latest.
+constexpr int HTL = 512;
+template <int G>
+__global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
+                                                         const uint4* __restrict__ units,
+                                                         uint32_t nunits,
+                                                         const uint32_t* __restrict__ own_idx,
+                                                         TermBuf tb, uint32_t lcap) {
+    constexpr uint32_t L = 4096;
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    uint32_t* s_leaf = s_dyn;
+    uint32_t* s_ed = s_dyn + lcap;
+    uint32_t* s_dg = s_dyn + 2 * lcap;
+    __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
+    __shared__ double sfold[HTL];
+    const int wid = threadIdx.x / WAVE;
+    constexpr uint32_t per_wave = WAVE / G;
+    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);
+        const uint32_t nleaf = ((dh + 7) & ~7u) + 8;  // <= lcap
+        for (uint32_t i = threadIdx.x; i < nleaf; i += HTL) {
+            uint32_t z = PAD_ID;
+            if (i < dh) {
+                const uint2 a = g.nbe[oh + i];
+                z = a.x;
+                s_ed[i] = a.y;
+                s_dg[i] = g.ndo[oh + i].x;
+            }
+            s_leaf[i] = z;
+        }
+        for (uint32_t i = threadIdx.x; i < L / 8; i += HTL) s_l1[i] = hub_id(g, oh, dh, 8 * i + 7);
+        for (uint32_t i = threadIdx.x; i < L / 64; i += HTL) s_l2[i] = hub_id(g, oh, dh, 64 * i + 63);
+        Piv8 top;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
+        __syncthreads();
+        const HubMeta<false, true> meta{s_ed, s_dg, nullptr, nullptr, oh};
+        hub_edges<G, 3>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta, own_idx + k0,
+                        sfold + wid * WAVE, wid * per_wave, cnt, per_wave * (HTL / WAVE));
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -1008,12 +1068,14 @@ void launch_hub_gl(const GraphDev& g, const uint4* units, uint32_t nu, const uin
     }
 }
 
-// Large hubs: 8 waves per unit share the 4096-entry tree.
-constexpr int HTL = 512;
+// Large hubs: 8 waves per unit share the tree; LDS sized for the class's
+// largest hub.
 template <int G>
 void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
-                     const TermBuf& tb, hipStream_t s) {
-    k_count_hub<G, 3, HTL><<<std::min<uint32_t>(nu, 4096), HTL, 0, s>>>(g, units, nu, own_idx, tb);
+                     const TermBuf& tb, uint32_t maxdh, hipStream_t s) {
+    const uint32_t lcap = (std::min<uint32_t>(maxdh, 4096) + 8 + 15) & ~15u;
+    k_count_hub_large<G><<<std::min<uint32_t>(nu, 4096), HTL, 3ull * lcap * 4, s>>>(
+        g, units, nu, own_idx, tb, lcap);
 }
 
 template <int L>
@@ -1053,12 +1115,13 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else {
+            const uint32_t mx = h->hub_umax[c];
             switch (gc) {
-                case 0: launch_hub_large<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_large<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_large<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_large<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                default: launch_hub_large<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 0: launch_hub_large<4>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;
+                case 1: launch_hub_large<8>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;
+                case 2: launch_hub_large<16>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;
+                case 3: launch_hub_large<32>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;
+                default: launch_hub_large<64>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;
             }
         }
         at += nu;
@@ -1078,7 +1141,7 @@ rtrex_status prepare_hubs(rtrex_graph h) {
     RT_TRY(sc.get(&hwork, g.n));
     RT_HIP(hipMemsetAsync(hwork, 0, (uint64_t)g.n * 4, s));
     RT_TRY(sc.get(&pos, g.nnz + 1));
-    RT_TRY(sc.get(&ccnt, HUB_NL * HUB_NG));
+    RT_TRY(sc.get(&ccnt, 2 * HUB_NL * HUB_NG));  // unit counts, then largest degrees
     RT_TRY(sc.get(&cbase, HUB_NL * HUB_NG));
     RT_HIP(hipMemsetAsync(flag, 0, g.nnz * 4, s));
     const unsigned be = blocks_for(g.m, T, 8192);
@@ -1094,12 +1157,15 @@ rtrex_status prepare_hubs(rtrex_graph h) {
     }
     k_own_scatter<<<blocks_for(std::max<uint64_t>(g.m, g.n + 1ull), T, 8192), T, 0, s>>>(
         g, pos, h->own_off, h->own_idx, P);
-    RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
+    RT_HIP(hipMemsetAsync(ccnt, 0, 2 * HUB_NL * HUB_NG * 4, s));
     const unsigned bv = (unsigned)std::max<uint64_t>(1, ((uint64_t)g.n + T - 1) / T);  // one thread per hub
     if (nown)
-        k_hub_units<true><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, nullptr, nullptr, P);
+        k_hub_units<true><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, nullptr, nullptr, P,
+                                            ccnt + HUB_NL * HUB_NG);
     RT_HIP(hipGetLastError());
     RT_HIP(hipMemcpyAsync(h->hub_ucnt, ccnt, sizeof(h->hub_ucnt), hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(h->hub_umax, ccnt + HUB_NL * HUB_NG, sizeof(h->hub_umax),
+                          hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
     uint32_t base[HUB_NL * HUB_NG];
     uint64_t total = 0;
@@ -1110,7 +1176,8 @@ rtrex_status prepare_hubs(rtrex_graph h) {
     if (total) {
         RT_HIP(hipMemcpyAsync(cbase, base, sizeof(base), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemsetAsync(ccnt, 0, HUB_NL * HUB_NG * 4, s));
-        k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase, P);
+        k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase, P,
+                                             nullptr);
         RT_HIP(hipGetLastError());
     }
     RT_HIP(hipStreamSynchronize(s));
