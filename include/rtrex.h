@@ -194,8 +194,9 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 /* An edge whose longer-list endpoint (its hub) has degree in                 */
 /* (hub_min, hub_max] is counted per hub with the hub's list staged in LDS:   */
 /* a sorted list of up to 64 entries (degree <= l0_max), 512 (<= l1_max) or   */
-/* 4096 (above, hub_max <= 4096) with an 8-ary pivot tree; other edges are     */
-/* counted per edge.                                                           */
+/* 4096 (<= l2_max) with an 8-ary pivot tree; above l2_max (hub_max <= 32760)  */
+/* the tree's upper levels are staged and its leaf is the list in device      */
+/* memory; other edges are counted per edge.                                   */
 /* A hub's edges get 4, 8, 16, 32 or 64 lanes each by their mean shorter-list */
 /* length against g_thr[0..3].  Units of work hold at least unit_min streamed  */
 /* elements (large class: unit_hash_mult * hub degree).  Results never depend   */
@@ -204,6 +205,7 @@ typedef struct {
     int32_t hub_min, hub_max, l0_max, l1_max;
     int32_t g_thr[4];
     int32_t unit_min, unit_hash_mult;
+    int32_t l2_max; /* l1_max <= l2_max <= 4096 */
 } rtrex_count_params;
 
 rtrex_status rtrex_get_count_params(rtrex_graph g, rtrex_count_params* out);

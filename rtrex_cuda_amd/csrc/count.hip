@@ -46,12 +46,13 @@ __device__ __forceinline__ int bucket_of(uint32_t w) {
 // endpoint, its hub.  Edges whose hub degree is in (hub_min, hub_max] are
 // counted per hub (k_count_hub*), the hub's list staged once in LDS; the rest
 // per edge (k_count_edges).
-constexpr int HUB_NL = 3, HUB_NG = 5;  // staging classes 64/512/4096; G = 4..64
+constexpr int HUB_NL = 4, HUB_NG = 5;  // staging classes 64/512/4096/32768; G = 4..64
+constexpr uint32_t LARGE_UNIT = 32;   // owned edges per unit, at most, in the large class
 __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
     return dhub > P.hub_min && dhub <= P.hub_max;
 }
 __host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub, const CountParams& P) {
-    return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : 2;
+    return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : dhub <= P.l2_max ? 2 : 3;
 }
 // Lanes per edge from the mean shorter-list length of the hub's edges.
 __host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges,
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(T) k_sum_final(const unsigned long long* __res
 
 // ---------------------------------------------------------------- hub path
 // Ownership flags: the hub-side slot of every hub-path edge; hwork[h] = the
-// streamed work of hub h (sum of its owned edges' shorter lists, <= 4096^2).
+// streamed work of hub h (sum of its owned edges' shorter lists, <= 32768^2).
 __global__ void __launch_bounds__(T) k_own_flags(GraphDev g, uint32_t* __restrict__ flag,
                                                  uint32_t* __restrict__ hwork, CountParams P) {
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
@@ -307,13 +308,13 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
             const uint32_t dh = g.deg[h];
             const uint64_t work = hwork[h];
             // Units of at least max(unit_min, d_h) elements; the large class
-            // restages a 4096-entry tree per unit: unit_hash_mult * d_h, but
-            // at most 32 owned edges apart so that hubs still spread out.
+            // restages its tree per unit: unit_hash_mult * d_h, but at most
+            // LARGE_UNIT owned edges apart so that hubs still spread out.
             const int lc_ = hub_lclass(dh, Pm);
-            const uint64_t uw = lc_ == 2 ? (uint64_t)Pm.unit_hash_mult * dh
+            const uint64_t uw = lc_ >= 2 ? (uint64_t)Pm.unit_hash_mult * dh
                                          : max<uint64_t>(Pm.unit_min, dh);
             P = (work + uw - 1) / uw;
-            if (lc_ == 2) P = max<uint64_t>(P, (oc + 31) / 32);
+            if (lc_ >= 2) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = lc_ * HUB_NG + hub_gclass(work, oc, Pm);
             at = atomicAdd(&lc[c], (uint32_t)P);
@@ -355,8 +356,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Hub-path work unit: part of one hub's owned edges.  x = hub, y = offset of
-// its list, z = first own_idx entry, w = d_hub | (edge count << 16) (both
-// <= 4096: hub_max is at most 4096).
+// its list, z = first own_idx entry, w = d_hub | (edge count << 16) (d_hub <=
+// 32768, hub_max's limit; at most 512 edges per unit).
 __device__ __forceinline__ void unit_fields(const uint4& U, uint32_t& h, uint32_t& oh,
                                             uint32_t& k0, uint32_t& dh, uint32_t& cnt) {
     h = U.x;
@@ -411,6 +412,16 @@ __device__ __forceinline__ bool tree_probe(const uint32_t* leaf, const uint32_t*
     return ok && j >= 0;
 }
 
+// The probe of a staged tree as a callable for hub_edges.
+template <int LV>
+struct TreeProbe {
+    const uint32_t *leaf, *l1, *l2;
+    Piv8 top;
+    __device__ __forceinline__ bool operator()(uint32_t z, uint32_t& pos) const {
+        return tree_probe<LV>(leaf, l1, l2, top, z, pos);
+    }
+};
+
 // Entry i of the padded hub list (global loads; the pivots of the tree).
 __device__ __forceinline__ uint32_t hub_id(const GraphDev& g, uint32_t oh, uint32_t dh,
                                            uint32_t i) {
@@ -450,14 +461,17 @@ struct HubMeta {
 // the hub's tree; own[] holds the owned slots (LDS or HBM).  Matches go to
 // the term buffer in ascending z; the support is folded in lane order.
 // sfold is this wave's 64-entry LDS slice.
-template <int G, int LV, class HM>
+// PRE: the owned edges' data is prefetched per owned position (pre: edge id,
+// shorter-list length and offset; prez: term offset) instead of read through
+// own[] and the hub's slots.
+template <int G, class PR, class HM, bool PRE = false>
 __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, uint32_t dh,
-                                          const uint32_t* leaf, const uint32_t* l1,
-                                          const uint32_t* l2, const Piv8& top,
+                                          const PR& probe,
                                           const HM& hm_, const uint32_t* own,
                                           double* sfold, uint32_t first, uint32_t cnt,
-                                          uint32_t stride) {
-    constexpr int R = G <= 8 ? 4 : 2;
+                                          uint32_t stride, const uint4* pre = nullptr,
+                                          const uint64_t* prez = nullptr) {
+    constexpr int R = G <= 16 ? 4 : 2;  // 64 ids per group and load round up to G = 16
     const int lane = lane_id();
     const int gl = lane & (G - 1);
     const int gbase = lane & ~(G - 1);
@@ -465,13 +479,24 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
     for (uint32_t base = first; base < cnt; base += stride) {
         const uint32_t idx = base + lane / G;
         const bool act = idx < cnt;
-        const uint32_t li = act ? own[idx] : 0;
-        const uint32_t e = hm_.ed(g, li);  // edge {hub, lo}
-        const uint2 yo = hm_.dego(g, li);
+        uint32_t e;  // edge {hub, lo}
+        uint2 yo;
+        uint64_t wb0;
+        if (PRE) {
+            const uint4 q = pre[act ? idx : 0];
+            e = q.x;
+            yo = make_uint2(q.y, q.z);
+            wb0 = prez[act ? idx : 0];
+        } else {
+            const uint32_t li = act ? own[idx] : 0;
+            e = hm_.ed(g, li);
+            yo = hm_.dego(g, li);
+            wb0 = (act && tb.t) ? hm_.zo(tb, li) : 0;
+        }
         const uint32_t dy = yo.x;
         uint32_t p = yo.y;
         const uint32_t le = act ? p + dy : p;
-        const uint64_t wbase = (act && tb.t) ? hm_.zo(tb, li) : 0;
+        const uint64_t wbase = (act && tb.t) ? wb0 : 0;
         double S = 0.0;
         uint32_t c = 0;
         while (__any(p < le)) {
@@ -489,7 +514,7 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 const bool in = go && p + gl < le;
                 const uint32_t z = zr[r];
                 uint32_t pos = 0;
-                const bool hit = in && tree_probe<LV>(leaf, l1, l2, top, z, pos);
+                const bool hit = in && probe(z, pos);
                 const double t = hit ? tri_w(dh, dy, hm_.deg(g, pos)) : 0.0;
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
@@ -560,7 +585,7 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
         for (int i = 0; i < 8; ++i) top.p[i] = __builtin_amdgcn_readlane(z, 8 * i + 7);
         if (lane < cnt) s_own[wid][lane] = own_idx[k0 + lane];
         wave_sync();
-        hub_edges<G, 1>(g, tb, dh, s_nb[wid], nullptr, nullptr, top, meta, s_own[wid],
+        hub_edges<G>(g, tb, dh, TreeProbe<1>{s_nb[wid], nullptr, nullptr, top}, meta, s_own[wid],
                               s_fold[wid], 0, cnt, WAVE / G);
         wave_sync();  // the next unit restages the slice
     }
@@ -774,18 +799,19 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
             for (uint32_t i = threadIdx.x; i < cnt; i += NT) s_own[i] = own_idx[k0 + i];
         __syncthreads();
         const HubMeta<META> meta{s_ed, s_dg, s_of, s_zo, oh};
-        hub_edges<G, LV>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta,
+        hub_edges<G>(g, tb, dh, TreeProbe<LV>{s_leaf, s_l1, s_l2, top}, meta,
                                META ? s_own : own_idx + k0, sfold + wid * WAVE, wid * per_wave,
                                cnt, per_wave * (NT / WAVE));
         __syncthreads();  // the next unit restages the hub
     }
 }
 
-// Large hubs (degree in (l1_max, 4096]): one workgroup of HTL lanes per unit,
+// Large hubs (degree in (l1_max, l2_max], l2_max <= 4096): one workgroup of HTL lanes per unit,
 // the hub's 8-ary tree staged in LDS once and shared by its waves, with the
 // edge id and degree of every hub slot beside it so that a match needs no
-// memory access; the owned edges' own data is read from HBM (the hub's
-// slots, L2-resident).  The leaf and slot arrays live in dynamic LDS of lcap
+// memory access; the unit's owned edges (at most LARGE_UNIT) have their edge id,
+// list and term offset fetched during the staging, so each edge starts with
+// its list loads.  The leaf and slot arrays live in dynamic LDS of lcap
 // entries each, lcap >= the launch's largest hub degree + 8: a probe ends in
 // the leaf block holding the hub's first padding entry at the latest.
 constexpr int HTL = 512;
@@ -803,11 +829,19 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
     __shared__ double sfold[HTL];
+    __shared__ uint4 s_pre[LARGE_UNIT];
+    __shared__ uint64_t s_prez[LARGE_UNIT];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
-        unit_fields(units[un], h, oh, k0, dh, cnt);
+        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= LARGE_UNIT in this class
+        if (threadIdx.x < cnt) {
+            const uint32_t li = own_idx[k0 + threadIdx.x];
+            const uint2 yo = g.ndo[oh + li];
+            s_pre[threadIdx.x] = make_uint4(g.nbe[oh + li].y, yo.x, yo.y, 0u);
+            s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
+        }
         const uint32_t nleaf = ((dh + 7) & ~7u) + 8;  // <= lcap
         for (uint32_t i = threadIdx.x; i < nleaf; i += HTL) {
             uint32_t z = PAD_ID;
@@ -826,8 +860,101 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
         for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
         __syncthreads();
         const HubMeta<false, true> meta{s_ed, s_dg, nullptr, nullptr, oh};
-        hub_edges<G, 3>(g, tb, dh, s_leaf, s_l1, s_l2, top, meta, own_idx + k0,
-                        sfold + wid * WAVE, wid * per_wave, cnt, per_wave * (HTL / WAVE));
+        hub_edges<G, TreeProbe<3>, HubMeta<false, true>, true>(
+            g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta, nullptr, sfold + wid * WAVE,
+            wid * per_wave, cnt, per_wave * (HTL / WAVE), s_pre, s_prez);
+        __syncthreads();  // the next unit restages the hub
+    }
+}
+
+// Huge hubs (degree in (l2_max, HUGE_MAX]): a four-level 8-ary tree whose
+// upper levels (every 8th, 64th and 512th entry; 4096 + 512 + 64 ids, 18 KB)
+// are staged in LDS and whose leaf is the hub's list itself in HBM: a probe
+// descends in LDS, then reads one 8-entry block of the list (two aligned
+// 16-byte loads; the hub's list is read by all of its units, L2-resident
+// meanwhile) instead of the dozen dependent reads of a binary search.  The
+// tree indexes the list from the 32-byte boundary at or before its start
+// (sh = oh mod 8 entries earlier): those sh entries read as 0, so the sequence
+// stays non-decreasing, and are never a match.
+constexpr uint32_t HUGE_MAX = 32768 - 8;
+__device__ __forceinline__ uint32_t huge_id(const GraphDev& g, uint32_t ob, uint32_t sh,
+                                            uint32_t dh, uint32_t s) {
+    return s < sh ? 0u : s < sh + dh ? g.nb[ob + s] : PAD_ID;
+}
+struct HugeProbe {
+    const uint4* blk;  // the list's 32-byte blocks, from its aligned base
+    uint32_t sh, dh;
+    const uint32_t *l1, *l2, *l3;
+    Piv8 top;
+    __device__ __forceinline__ bool operator()(uint32_t z, uint32_t& pos) const {
+        uint32_t b = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) b += top.p[i] < z ? 1u : 0u;
+        const bool ok = b < 8;
+        b = min(b, 7u);
+        const uint4* q = reinterpret_cast<const uint4*>(l3) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        q = reinterpret_cast<const uint4*>(l2) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        q = reinterpret_cast<const uint4*>(l1) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        // leaf block b: positions 8b .. 8b + 7 of the shifted sequence, read
+        // from memory as they are (the entries before sh belong to the
+        // previous list, those from sh + dh on to the next one); a match
+        // counts only inside the list, k in [kmin, kend)
+        const uint4 a = blk[2 * b], c = blk[2 * b + 1];
+        const uint32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+        const uint32_t base = 8 * b;
+        const uint32_t kmin = base >= sh ? 0u : sh - base;
+        const uint32_t kend = sh + dh > base ? min(8u, sh + dh - base) : 0u;
+        int j = -1;
+#pragma unroll
+        for (int k = 7; k >= 0; --k)
+            if (v[k] == z && (uint32_t)k >= kmin && (uint32_t)k < kend) j = k;
+        pos = base + (uint32_t)max(j, 0) - sh;
+        return ok && j >= 0;
+    }
+};
+
+template <int G>
+__global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
+                                                        const uint4* __restrict__ units,
+                                                        uint32_t nunits,
+                                                        const uint32_t* __restrict__ own_idx,
+                                                        TermBuf tb) {
+    constexpr uint32_t L = 32768;
+    __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l3[L / 512];
+    __shared__ double sfold[HTL];
+    __shared__ uint4 s_pre[LARGE_UNIT];
+    __shared__ uint64_t s_prez[LARGE_UNIT];
+    const int wid = threadIdx.x / WAVE;
+    constexpr uint32_t per_wave = WAVE / G;
+    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= LARGE_UNIT in this class
+        if (threadIdx.x < cnt) {
+            const uint32_t li = own_idx[k0 + threadIdx.x];
+            const uint2 yo = g.ndo[oh + li];
+            s_pre[threadIdx.x] = make_uint4(g.nbe[oh + li].y, yo.x, yo.y, 0u);
+            s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
+        }
+        const uint32_t sh = oh & 7u, ob = oh - sh;  // dh + sh <= L
+        for (uint32_t i = threadIdx.x; i < L / 8; i += HTL) s_l1[i] = huge_id(g, ob, sh, dh, 8 * i + 7);
+        for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
+            s_l2[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
+        for (uint32_t i = threadIdx.x; i < L / 512; i += HTL)
+            s_l3[i] = huge_id(g, ob, sh, dh, 512 * i + 511);
+        HugeProbe pr{reinterpret_cast<const uint4*>(g.nb + ob), sh, dh, s_l1, s_l2, s_l3, {}};
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            pr.top.p[i] = huge_id(g, ob, sh, dh, (L / 8) * i + L / 8 - 1);
+        __syncthreads();
+        const HubMeta<false, false> meta{nullptr, nullptr, nullptr, nullptr, oh};
+        hub_edges<G, HugeProbe, HubMeta<false, false>, true>(
+            g, tb, dh, pr, meta, nullptr, sfold + wid * WAVE, wid * per_wave, cnt,
+            per_wave * (HTL / WAVE), s_pre, s_prez);
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -1078,6 +1205,12 @@ void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const 
         g, units, nu, own_idx, tb, lcap);
 }
 
+template <int G>
+void launch_hub_huge(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
+                     const TermBuf& tb, hipStream_t s) {
+    k_count_hub_huge<G><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu, own_idx, tb);
+}
+
 template <int L>
 void launch_hub_l(int gc, const GraphDev& g, const uint4* units, uint32_t nu,
                   const uint32_t* own_idx, const TermBuf& tb, hipStream_t s) {
@@ -1114,7 +1247,15 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         const int lc = c / HUB_NG, gc = c % HUB_NG;
         if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
-        else {
+        else if (lc == 3) {
+            switch (gc) {
+                case 0: launch_hub_huge<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 1: launch_hub_huge<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 2: launch_hub_huge<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 3: launch_hub_huge<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                default: launch_hub_huge<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+            }
+        } else {
             const uint32_t mx = h->hub_umax[c];
             switch (gc) {
                 case 0: launch_hub_large<4>(h->g, u, nu, h->own_idx, tb, mx, h->stream); break;

@@ -191,7 +191,9 @@ rtrex_status alloc_state(rtrex_graph h, uint32_t n, uint64_t nnz) {
     g.nnz = nnz;
     g.m = (uint32_t)m;
     RT_TRY(dalloc(h, &g.off, (uint64_t)n + 1));
-    RT_TRY(dalloc(h, &g.nb, nnz));
+    // 8 entries of padding: a hub-list block read (32 aligned bytes) may run
+    // past the last list
+    RT_TRY(dalloc(h, &g.nb, nnz + 8));
     RT_TRY(dalloc(h, &g.deg, n));
     RT_TRY(dalloc(h, &g.part, nnz));
     RT_TRY(dalloc(h, &g.s2e, nnz));

@@ -126,8 +126,8 @@ struct AuditHost {
 // mean shorter-list length against g_thr[0..3].  Work units carry at least
 // unit_min streamed elements (hash class: unit_hash_mult * d_h).
 struct CountParams {
-    uint32_t hub_min = 8, hub_max = 4096;
-    uint32_t l0_max = 64, l1_max = 512;
+    uint32_t hub_min = 8, hub_max = 32760;
+    uint32_t l0_max = 64, l1_max = 512, l2_max = 4096;
     uint32_t g_thr[4] = {3, 6, 24, 80};
     uint32_t unit_min = 1024, unit_hash_mult = 8;
 };
@@ -191,8 +191,8 @@ struct rtrex_graph_s {
     uint32_t* own_off = nullptr;
     uint32_t* own_idx = nullptr;
     uint4* hub_units = nullptr;
-    uint32_t hub_ucnt[15] = {};
-    uint32_t hub_umax[15] = {};  // largest hub degree per class
+    uint32_t hub_ucnt[20] = {};
+    uint32_t hub_umax[20] = {};  // largest hub degree per class
     unsigned long long* red = nullptr;  // per-block partial sums (count phase)
     rtrex::CountParams cparams;
     bool plan_valid = false;  // buckets and hub units built for cparams

@@ -25,7 +25,11 @@ SETTINGS = [
     {"hub_min": 4, "l0_max": 16, "l1_max": 128, "g_thr": [1, 2, 4, 8]},
     {"hub_min": 64, "hub_max": 1024, "unit_min": 1, "unit_hash_mult": 1,
      "g_thr": [100, 200, 300, 400]},
-    {"hub_min": 1, "hub_max": 4096, "l0_max": 0, "l1_max": 0},  # all hubs hashed
+    {"hub_min": 1, "hub_max": 4096, "l0_max": 0, "l1_max": 0},  # all hubs: 4096-entry trees
+    # all hubs through the huge-hub kernel (upper tree levels staged, leaf in HBM)
+    {"hub_min": 0, "hub_max": 32760, "l0_max": 0, "l1_max": 0, "l2_max": 0},
+    {"hub_min": 2, "hub_max": 32760, "l0_max": 8, "l1_max": 32, "l2_max": 64,
+     "g_thr": [2, 4, 8, 16]},
 ]
 
 
@@ -48,7 +52,8 @@ def test_supports_do_not_depend_on_parameters(i):
 def test_invalid_parameters_rejected():
     n, u, v = golden_graph("small_test")
     with R.DeviceGraph.from_edges(n, u, v) as g:
-        for bad in ({"hub_max": 8192}, {"l0_max": 65}, {"l1_max": 600},
+        for bad in ({"hub_max": 40000}, {"l0_max": 65}, {"l1_max": 600},
+                    {"l2_max": 5000}, {"l1_max": 256, "l2_max": 128},
                     {"g_thr": [5, 4, 3, 2]}, {"hub_min": 100, "hub_max": 50}):
             with pytest.raises(R.RtrexError) as e:
                 g.set_count_params(bad)
