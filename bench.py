@@ -248,6 +248,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="C5 serving leg: length of the request trace (0 skips it)")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=128.0,
+                    help="serving pool size (HBM footprint admission), GiB of the 288 GB")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the calibration copy kernel over BYTES and print "
                          "its time (scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -351,7 +353,8 @@ def main() -> int:
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))
     if args.serve_seconds > 0:
-        out["serving"] = serving_leg(args.eps, args.serve_seconds, 200.0, args.serve_workers, 64.0)
+        out["serving"] = serving_leg(args.eps, args.serve_seconds, 200.0, args.serve_workers,
+                                     args.serve_budget_gb)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, u, v, args.eps)
     print(json.dumps(out), flush=True)

@@ -34,6 +34,7 @@
 // subtractions, so they may differ from the reference's by a few ulps; each
 // ratio decision is audited against that bound (ambiguous_ratio).
 #include <algorithm>
+#include <mutex>
 #include <unordered_set>
 
 #include "audit.hpp"
@@ -727,22 +728,37 @@ int bits_for(uint64_t n) {
 
 }  // namespace
 
-// unordered_set<int64> bucket counts after k insertions, k = 0..upto,
-// measured from the library (the iteration order the extraction emulates
-// depends on them), with their device copy and the largest ratio
-// bcnt[k] / k over k >= 2 (which bounds the set scratch of components).
+// unordered_set<int64> bucket counts after k insertions, measured once per
+// process from the library (the iteration order the extraction emulates
+// depends on them) and grown on demand by further insertions, with the
+// largest ratio bcnt[k] / k over k >= 2.
+struct BucketTable {
+    std::mutex mu;
+    std::unordered_set<int64_t> probe;
+    std::vector<uint32_t> bcnt;
+    uint32_t ratio = 1;
+};
+BucketTable& bucket_table() {
+    static BucketTable t;
+    return t;
+}
+
+// The handle's copy of bcnt[0..upto] (its own buffer, stable while an
+// upload reads it), its device copy, and the ratio (which bounds the set
+// scratch of components).
 rtrex_status ensure_bucket_table(rtrex_graph h, uint64_t upto) {
     if (h->bcnt.size() < upto + 1) {
-        std::unordered_set<int64_t> probe;
-        h->bcnt.assign(1, (uint32_t)probe.bucket_count());
-        uint32_t ratio = 1;
-        for (uint64_t k = 1; k <= upto; ++k) {
-            probe.insert((int64_t)k);
-            const uint32_t b = (uint32_t)probe.bucket_count();
-            h->bcnt.push_back(b);
-            if (k >= 2) ratio = std::max<uint32_t>(ratio, (uint32_t)((b + k - 1) / k));
+        BucketTable& t = bucket_table();
+        std::lock_guard<std::mutex> lk(t.mu);
+        if (t.bcnt.empty()) t.bcnt.push_back((uint32_t)t.probe.bucket_count());
+        for (uint64_t k = t.bcnt.size(); k <= upto; ++k) {
+            t.probe.insert((int64_t)k);
+            const uint32_t b = (uint32_t)t.probe.bucket_count();
+            t.bcnt.push_back(b);
+            if (k >= 2) t.ratio = std::max<uint32_t>(t.ratio, (uint32_t)((b + k - 1) / k));
         }
-        h->bcnt_ratio = ratio;
+        h->bcnt.assign(t.bcnt.begin(), t.bcnt.begin() + (upto + 1));
+        h->bcnt_ratio = t.ratio;
     }
     if (!h->bcnt_dev) {
         RT_TRY(dalloc(h, &h->bcnt_dev, (uint64_t)h->g.n + 1));

@@ -7,9 +7,11 @@
 // when the head is small, the head and the small jobs queued right behind it
 // (same eps, up to an edge and a graph cap) as one disjoint-union batch.  The
 // launch is admitted when its footprint -- resident arrays, the upper bound of
-// its stored triangle lists and its phase scratch -- fits in one free range of
-// the server's device pool; every allocation of the launch is carved from that
-// range, so a job costs no hipMalloc/hipFree.  Launches of different workers
+// its stored triangle lists (at most half the pool) and its phase
+// scratch -- fits in one free range of the server's device pool; every
+// allocation of the launch is carved from that range, so a job costs no
+// hipMalloc/hipFree (except exactly sized triangle lists that outgrow the
+// range, allocated outside the pool).  Launches of different workers
 // run concurrently on their streams, and jobs that arrive while others run
 // join the next launch that a worker forms.  A launch that does not fit while
 // nothing else runs is run outside the pool.  A batch is the batched path:
@@ -230,7 +232,14 @@ void worker_main(rtrex_server s, int w) {
                 Cp += j->cap;
             }
         }
-        const uint64_t bytes = launch_bytes(N, M, Cp);
+        // The stored triangle lists count at most half the pool: a job whose
+        // upper bound is larger (skewed graphs: the bound can be several
+        // times the exact size) sizes its lists exactly after a counting pass
+        // and takes them from its range when they fit, else outside the pool
+        // -- so it runs beside other jobs instead of waiting for the whole
+        // server to drain.
+        const uint64_t bytes =
+            launch_bytes(N, M, std::min<uint64_t>(Cp, s->pool.cap / 2 / sizeof(Term)));
         uint64_t off = 0;
         const bool pooled = s->pool.take(bytes, &off);
         if (!pooled && s->running > 0) {
