@@ -1327,8 +1327,8 @@ rtrex_status prepare_hubs(rtrex_graph h) {
 
 }  // namespace
 
-// Term buffer of `total` 16-byte records when it fits in a third of the free
-// HBM.
+// Term buffer of `total` 16-byte records when it fits in 60 % of the free HBM
+// (the rest stays for the phase scratch and other work).
 bool try_alloc_terms(rtrex_graph h, uint64_t total) {
     const uint64_t bytes = (total + 1) * sizeof(Term);
     if (h->slab && h->slab_top + bytes + 256 <= h->slab_cap) {
@@ -1340,7 +1340,7 @@ bool try_alloc_terms(rtrex_graph h, uint64_t total) {
         (void)hipGetLastError();
         return false;
     }
-    if (bytes >= freeb / 3) return false;
+    if (bytes >= freeb / 5 * 3) return false;
     void* q = nullptr;
     if (hipMalloc(&q, bytes) == hipSuccess) {
         h->allocs.push_back(q);
