@@ -746,6 +746,99 @@ __global__ void __launch_bounds__(T) k_rg_keys(GraphDev g, const uint32_t* __res
     }
 }
 
+// ---- regions of the undecided edges only: a breadth-first search over stored
+// triangles from them (the visited set is the union of their regions), then
+// the same union-find restricted to the visited edges.
+__device__ __forceinline__ bool first_visit(uint8_t* vis, uint32_t e) {
+    unsigned* w = reinterpret_cast<unsigned*>(vis + (e & ~3u));
+    const unsigned bit = 1u << (8 * (e & 3u));
+    return (atomicOr(w, bit) & bit) == 0u;
+}
+
+// Queue counters c: [0, 1) the round's frontier, [2] the tail.
+__global__ void __launch_bounds__(T) k_bfs_seed(const uint32_t* __restrict__ undec, uint32_t n,
+                                                uint8_t* vis, uint32_t* __restrict__ q,
+                                                unsigned* c) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < n; i += gridDim.x * T) {
+        const uint32_t e = undec[i];
+        if (first_visit(vis, e)) q[wave_append(&c[2])] = e;
+    }
+}
+
+__global__ void k_bfs_begin(unsigned* c) {
+    if (threadIdx.x == 0) {
+        c[0] = c[1];
+        c[1] = c[2];
+    }
+}
+
+// One round: every frontier edge's triangles; an edge seen for the first
+// time joins the queue (the next frontier).
+__global__ void __launch_bounds__(T) k_bfs_expand(GraphDev g, TermBuf tb, uint8_t* vis,
+                                                  uint32_t* __restrict__ q, unsigned* c) {
+    const uint32_t h0 = c[0], h1 = c[1];  // set by k_bfs_begin; only c[2] moves here
+    for (uint32_t i = h0 + blockIdx.x * T + threadIdx.x; i < h1; i += gridDim.x * T) {
+        const uint32_t e = q[i];
+        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+        for (uint64_t k = k0; k < k1; ++k) {
+            const uint32_t a = tb.t[k].sa, b = tb.t[k].sb;
+            if (first_visit(vis, a)) q[wave_append(&c[2])] = a;
+            if (first_visit(vis, b)) q[wave_append(&c[2])] = b;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rgl_init(const uint32_t* __restrict__ q, uint32_t nv,
+                                                uint32_t* parent) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nv; i += gridDim.x * T) parent[q[i]] = q[i];
+}
+
+__global__ void __launch_bounds__(T) k_rgl_hook(GraphDev g, TermBuf tb,
+                                                const uint32_t* __restrict__ q, uint32_t nv,
+                                                uint32_t* parent) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nv; i += gridDim.x * T) {
+        const uint32_t e = q[i];
+        const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+        for (uint64_t k = k0; k < k1; ++k) {
+            const uint32_t a = tb.t[k].sa, b = tb.t[k].sb;
+            if (e < a && e < b) {
+                rg_union(parent, e, a);
+                rg_union(parent, e, b);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) k_rgl_compress(const uint32_t* __restrict__ q, uint32_t nv,
+                                                    uint32_t* parent) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nv; i += gridDim.x * T) {
+        const uint32_t e = q[i];
+        uint32_t r = e, p = parent[e];
+        while (p != r) {
+            r = p;
+            p = parent[r];
+        }
+        parent[e] = r;
+    }
+}
+
+// Keys (root, edge id) of the visited edges, which are reset to their state
+// before the clean.
+__global__ void __launch_bounds__(T) k_rgl_keys(GraphDev g, const uint32_t* __restrict__ parent,
+                                                const uint32_t* __restrict__ q, uint32_t nv,
+                                                int Bi, uint64_t* __restrict__ keys,
+                                                uint8_t* __restrict__ xact) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nv; i += gridDim.x * T) {
+        const uint32_t e = q[i];
+        keys[i] = ((uint64_t)parent[e] << Bi) | e;
+        if (xact) xact[e] = 1;  // replayed: the reference's own values
+        set_status(g, e, ST_ALIVE);
+        g.sup[e] = g.sup0[e];
+        g.touched[e] = 0;
+        g.claim[e] = 0;
+    }
+}
+
 __global__ void __launch_bounds__(T) k_rg_layout(const uint64_t* __restrict__ keys, uint32_t nk,
                                                  int Bi, uint32_t* __restrict__ redge,
                                                  uint32_t* __restrict__ rflag) {
@@ -899,24 +992,58 @@ rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint
     uint8_t* flag;
     unsigned long long* cnt;
     RT_TRY(sc.get(&parent, g.m));
-    RT_TRY(sc.get(&flag, g.m));
+    RT_TRY(sc.get(&flag, (g.m + 3) & ~3ull));
     RT_TRY(sc.get(&cnt, 2));
     const unsigned be = blocks_for(g.m, T, 8192);
-    k_rg_init<<<be, T, 0, s>>>(g.m, parent);
-    k_rg_hook<<<be, T, 0, s>>>(g, h->terms(), parent);
-    k_rg_compress<<<be, T, 0, s>>>(g.m, parent);
-    RT_HIP(hipMemsetAsync(flag, 0, g.m, s));
     const bool all = nund > cap;
-    k_rg_flag<<<blocks_for(all ? g.m : nund, T, 4096), T, 0, s>>>(undec, nund, parent, flag, all,
-                                                                  g.m);
-    RT_HIP(hipMemsetAsync(cnt, 0, 16, s));
     const int Bi = bits_of((uint64_t)g.m + 1);
     RT_TRY(sc.get(&keys, g.m));
-    k_rg_keys<<<be, T, 0, s>>>(g, parent, flag, Bi, keys, (unsigned*)cnt, h->xact);
-    RT_HIP(hipGetLastError());
+    RT_HIP(hipMemsetAsync(cnt, 0, 16, s));  // [0] region edges (all path), [1] alive edges
     uint32_t nk = 0;
-    RT_HIP(hipMemcpyAsync(&nk, cnt, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
+    if (all) {  // every region: a union-find over all stored triangles
+        k_rg_init<<<be, T, 0, s>>>(g.m, parent);
+        k_rg_hook<<<be, T, 0, s>>>(g, h->terms(), parent);
+        k_rg_compress<<<be, T, 0, s>>>(g.m, parent);
+        RT_HIP(hipMemsetAsync(flag, 0, g.m, s));
+        k_rg_flag<<<be, T, 0, s>>>(undec, nund, parent, flag, true, g.m);
+        k_rg_keys<<<be, T, 0, s>>>(g, parent, flag, Bi, keys, (unsigned*)cnt, h->xact);
+        RT_HIP(hipGetLastError());
+        RT_HIP(hipMemcpyAsync(&nk, cnt, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+    } else {
+        // The regions of the undecided edges: breadth-first over stored
+        // triangles, eight rounds per host round trip (rounds past the end
+        // find an empty frontier), then the union-find over the visited
+        // edges only.
+        uint32_t* q;
+        unsigned* c;
+        RT_TRY(sc.get(&q, g.m));
+        RT_TRY(sc.get(&c, 4));
+        RT_HIP(hipMemsetAsync(flag, 0, (g.m + 3) & ~3ull, s));
+        RT_HIP(hipMemsetAsync(c, 0, 16, s));
+        k_bfs_seed<<<blocks_for(nund, T, 4096), T, 0, s>>>(undec, nund, flag, q, c);
+        const TermBuf tb = h->terms();
+        unsigned hc[3] = {0, 0, 0};
+        for (;;) {
+            for (int r = 0; r < 8; ++r) {
+                k_bfs_begin<<<1, WAVE, 0, s>>>(c);
+                k_bfs_expand<<<1024, T, 0, s>>>(g, tb, flag, q, c);
+            }
+            RT_HIP(hipGetLastError());
+            RT_HIP(hipMemcpyAsync(hc, c, 12, hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            if (hc[1] == hc[2]) break;  // the last round added no edge
+        }
+        nk = hc[2];
+        const unsigned bv = blocks_for(nk, T, 8192);
+        if (nk) {
+            k_rgl_init<<<bv, T, 0, s>>>(q, nk, parent);
+            k_rgl_hook<<<bv, T, 0, s>>>(g, tb, q, nk, parent);
+            k_rgl_compress<<<bv, T, 0, s>>>(q, nk, parent);
+            k_rgl_keys<<<bv, T, 0, s>>>(g, parent, q, nk, Bi, keys, h->xact);
+            RT_HIP(hipGetLastError());
+        }
+    }
     if (nk) {
         RT_TRY(sc.get(&tmp, nk));
         RT_TRY(radix_sort_u64(keys, tmp, nk, 2 * Bi, s));

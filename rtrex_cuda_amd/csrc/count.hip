@@ -597,16 +597,17 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
 // (edge k covers elements [L_k, L_k + d_k) of the sequence, L the exclusive
 // prefix sum of the list lengths d_k).  A lane finds its edge from the edge
 // starts that fall in the step: each start sets a per-step tag at its
-// position in LDS, two ballots gather them, and the lane's edge is the
+// position in LDS, one ballot per 64-element part gathers them, and the lane's edge is the
 // previous step's last edge plus the starts at or below it.  A match is
 // stored at its edge's term offset plus the edge's earlier matches, so every
 // edge's terms stay in ascending z.  The support is folded in the same order:
-// each half step's match weights are compacted in lane order into LDS and the
+// each part's match weights are compacted in lane order into LDS and the
 // last lane of every edge's run adds its run's weights, one after another, to
 // the edge's running sum.  The next unit's record and hub staging loads are
 // issued before the current unit's steps (unit records two units ahead), so
 // a unit's setup does not wait on memory.
 constexpr int FW = 256;
+constexpr int FH = 2;  // 64-element parts per step (4 measured slower on C2)
 struct FlatStage {
     uint32_t z, e, d, o, zl, zh, oi;  // hub slot lane: id, edge id, degree, offset, term offset; own_idx
 };
@@ -643,13 +644,13 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
     __shared__ uint32_t s_cnt[NW][WAVE];  // matches so far
     __shared__ double s_S[NW][WAVE];      // support so far
     __shared__ double s_t[NW][WAVE];      // a half step's match weights, in lane order
-    __shared__ uint32_t s_flag[NW][2 * WAVE];
+    __shared__ uint32_t s_flag[NW][FH * WAVE];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t tag = 0;
-    s_flag[wid][lane] = 0;  // tags start at 1
-    s_flag[wid][WAVE + lane] = 0;
+#pragma unroll
+    for (int q = 0; q < FH; ++q) s_flag[wid][q * WAVE + lane] = 0;  // tags start at 1
     const uint32_t stride = gridDim.x * NW;
     uint32_t un = blockIdx.x * NW + wid;
     uint4 Ucur = un < nunits ? units[un] : make_uint4(0u, 0u, 0u, 0u);
@@ -694,28 +695,37 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
         Unext = un2 < nunits ? units[un2] : make_uint4(0u, 0u, 0u, 0u);
         wave_sync();
         int kprev = -1;
-        for (uint32_t base = 0; base < W; base += 2 * WAVE) {
+        for (uint32_t base = 0; base < W; base += FH * WAVE) {
             ++tag;
-            if (own && Lk >= base && Lk - base < 2u * WAVE) s_flag[wid][Lk - base] = tag;
+            if (own && Lk >= base && Lk - base < (uint32_t)(FH * WAVE))
+                s_flag[wid][Lk - base] = tag;
             wave_sync();
-            const bool st1 = s_flag[wid][lane] == tag;
-            const bool st2 = s_flag[wid][WAVE + lane] == tag;
-            const uint64_t sm1 = ballot64(st1), sm2 = ballot64(st2);
-            const int k1 = kprev + (int)__popcll(sm1 & below) + (st1 ? 1 : 0);
-            const int k2 = kprev + (int)__popcll(sm1) + (int)__popcll(sm2 & below) + (st2 ? 1 : 0);
-            const uint32_t f1 = base + lane, f2 = base + WAVE + lane;
-            const bool a1 = f1 < W, a2 = f2 < W;  // then 0 <= k < cnt
-            const int kk1 = a1 ? k1 : 0, kk2 = a2 ? k2 : 0;
-            const uint4 E1 = s_ek[wid][kk1], E2 = s_ek[wid][kk2];
-            const uint2 x1 = a1 ? g.nbe[f1 + E1.y] : make_uint2(0u, 0u);  // (z, edge {lo, z})
-            const uint2 x2 = a2 ? g.nbe[f2 + E2.y] : make_uint2(0u, 0u);
+            int kq[FH], kkq[FH];
+            bool aq[FH];
+            uint4 Eq[FH];
+            uint2 xq[FH];
+            int kb = kprev;  // edge before this quarter
 #pragma unroll
-            for (int hf = 0; hf < 2; ++hf) {
-                const uint32_t hb = base + hf * WAVE, f = hf ? f2 : f1;
-                const bool act = hf ? a2 : a1;
-                const int k = hf ? k2 : k1, kk = hf ? kk2 : kk1;
-                const uint4 E = hf ? E2 : E1;
-                const uint2 x = hf ? x2 : x1;
+            for (int q = 0; q < FH; ++q) {
+                const bool st = s_flag[wid][q * WAVE + lane] == tag;
+                const uint64_t sm = ballot64(st);
+                kq[q] = kb + (int)__popcll(sm & below) + (st ? 1 : 0);
+                kb += (int)__popcll(sm);
+                aq[q] = base + q * WAVE + lane < W;  // then 0 <= k < cnt
+                kkq[q] = aq[q] ? kq[q] : 0;
+                Eq[q] = s_ek[wid][kkq[q]];
+            }
+            // all of the window's list loads in flight at once
+#pragma unroll
+            for (int q = 0; q < FH; ++q)  // (z, edge {lo, z})
+                xq[q] = aq[q] ? g.nbe[base + q * WAVE + lane + Eq[q].y] : make_uint2(0u, 0u);
+#pragma unroll
+            for (int hf = 0; hf < FH; ++hf) {
+                const uint32_t hb = base + hf * WAVE, f = hb + lane;
+                const bool act = aq[hf];
+                const int k = kq[hf], kk = kkq[hf];
+                const uint4 E = Eq[hf];
+                const uint2 x = xq[hf];
                 uint32_t pos = 0;
                 const bool hit = act && tree_probe<1>(s_nb[wid], nullptr, nullptr, top, x.x, pos);
                 const uint64_t hm = ballot64(hit);
@@ -742,7 +752,7 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
                 }
                 wave_sync();
             }
-            kprev = __builtin_amdgcn_readlane(k2, WAVE - 1);
+            kprev = __builtin_amdgcn_readlane(kq[FH - 1], WAVE - 1);
         }
         if (own) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
         wave_sync();  // the next unit restages the slices
@@ -809,9 +819,9 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
 // Large hubs (degree in (l1_max, l2_max], l2_max <= 4096): one workgroup of HTL lanes per unit,
 // the hub's 8-ary tree staged in LDS once and shared by its waves, with the
 // edge id and degree of every hub slot beside it so that a match needs no
-// memory access; the unit's owned edges (at most LARGE_UNIT) have their edge id,
-// list and term offset fetched during the staging, so each edge starts with
-// its list loads.  The leaf and slot arrays live in dynamic LDS of lcap
+// memory access; the owned edges' own data is read from HBM (the hub's slots,
+// L2-resident; fetching it during the staging measured slower, it delays the
+// staging barrier).  The leaf and slot arrays live in dynamic LDS of lcap
 // entries each, lcap >= the launch's largest hub degree + 8: a probe ends in
 // the leaf block holding the hub's first padding entry at the latest.
 constexpr int HTL = 512;
@@ -829,19 +839,11 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
     __shared__ double sfold[HTL];
-    __shared__ uint4 s_pre[LARGE_UNIT];
-    __shared__ uint64_t s_prez[LARGE_UNIT];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
-        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= LARGE_UNIT in this class
-        if (threadIdx.x < cnt) {
-            const uint32_t li = own_idx[k0 + threadIdx.x];
-            const uint2 yo = g.ndo[oh + li];
-            s_pre[threadIdx.x] = make_uint4(g.nbe[oh + li].y, yo.x, yo.y, 0u);
-            s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
-        }
+        unit_fields(units[un], h, oh, k0, dh, cnt);
         const uint32_t nleaf = ((dh + 7) & ~7u) + 8;  // <= lcap
         for (uint32_t i = threadIdx.x; i < nleaf; i += HTL) {
             uint32_t z = PAD_ID;
@@ -860,9 +862,8 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
         for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
         __syncthreads();
         const HubMeta<false, true> meta{s_ed, s_dg, nullptr, nullptr, oh};
-        hub_edges<G, TreeProbe<3>, HubMeta<false, true>, true>(
-            g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta, nullptr, sfold + wid * WAVE,
-            wid * per_wave, cnt, per_wave * (HTL / WAVE), s_pre, s_prez);
+        hub_edges<G>(g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta, own_idx + k0,
+                     sfold + wid * WAVE, wid * per_wave, cnt, per_wave * (HTL / WAVE));
         __syncthreads();  // the next unit restages the hub
     }
 }

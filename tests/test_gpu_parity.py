@@ -107,6 +107,19 @@ def test_no_silent_mismatch(tag):
     assert a["ambiguous_clean"] == 0  # every cleaning decision is the reference's
 
 
+@pytest.mark.parametrize("tag", GOLDEN_TAGS)
+def test_audit_counts_every_edge(tag):
+    # After a full decomposition every edge was removed by the clean or
+    # deleted with its cluster, replayed regions included.
+    r = result(tag)
+    assert r["audit"]["edges_removed"] == len(r["nb"]) // 2
+
+
+def test_some_golden_case_replays():
+    # the exact replay (and its region search) is exercised by the golden set
+    assert any(result(t)["audit"]["replayed_edges"] > 0 for t in GOLDEN_TAGS)
+
+
 _c2 = {}
 
 
