@@ -406,10 +406,14 @@ __device__ __forceinline__ bool tree_probe(const uint32_t* leaf, const uint32_t*
     }
     const uint4* q = reinterpret_cast<const uint4*>(leaf) + 2 * b;
     const uint4 a = q[0], c = q[1];
-    const int j = a.x == z ? 0 : a.y == z ? 1 : a.z == z ? 2 : a.w == z ? 3
-                : c.x == z ? 4 : c.y == z ? 5 : c.z == z ? 6 : c.w == z ? 7 : -1;
-    pos = 8 * b + (uint32_t)max(j, 0);
-    return ok && j >= 0;
+    // every comparison evaluated (a ?: chain lets the compiler read the block
+    // entry by entry behind branches): the match's bit, then its position
+    const uint32_t mk = (uint32_t)(a.x == z) | (uint32_t)(a.y == z) << 1 |
+                        (uint32_t)(a.z == z) << 2 | (uint32_t)(a.w == z) << 3 |
+                        (uint32_t)(c.x == z) << 4 | (uint32_t)(c.y == z) << 5 |
+                        (uint32_t)(c.z == z) << 6 | (uint32_t)(c.w == z) << 7;
+    pos = 8 * b + (mk ? (uint32_t)__ffs(mk) - 1u : 0u);
+    return ok && mk != 0;
 }
 
 // The probe of a staged tree as a callable for hub_edges.
