@@ -1123,9 +1123,12 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     // One host round trip per round (the next frontier's length, read after
     // the round ends); every launch in between takes its item count from
     // device memory and is sized from a host-side upper bound.
+    static_assert(offsetof(Counters, changed) == offsetof(Counters, frontier_next) + 12,
+                  "next-frontier .. near-list counters are adjacent");
     while (nF) {
-        RT_HIP(hipMemsetAsync(A.nFn, 0, sizeof(unsigned), s));
-        RT_HIP(hipMemsetAsync(A.nNear, 0, sizeof(unsigned), s));
+        // next-frontier and near-list counts (with the error flag and work
+        // queue head between them, both re-zeroed before their own uses)
+        RT_HIP(hipMemsetAsync(A.nFn, 0, 4 * sizeof(unsigned), s));
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
         const bool terms = A.tb.t != nullptr;

@@ -170,6 +170,21 @@ __global__ void __launch_bounds__(T) k_comp_layout(const uint64_t* __restrict__ 
     }
 }
 
+// Scalars read back at the end of an extraction, in one copy: component
+// count, largest component, undecided cascade decisions, error flag, clusters
+// (absent sources read as 0).
+__global__ void k_final_scalars(const uint32_t* ncomp, const uint32_t* big,
+                                const unsigned* nund, const unsigned* err,
+                                const uint32_t* total, uint32_t* out) {
+    if (threadIdx.x == 0) {
+        out[0] = ncomp ? *ncomp : 0u;
+        out[1] = big ? *big : 0u;
+        out[2] = nund ? *nund : 0u;
+        out[3] = *err;
+        out[4] = *total;
+    }
+}
+
 // Per component c < ncomp: its unordered_set bucket count bcnt[size] (0 past
 // ncomp, up to bound), and the largest size.
 __global__ void __launch_bounds__(T) k_comp_hsize(const uint32_t* __restrict__ cstart,
@@ -947,22 +962,20 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                                                              (uint32_t)(ns + 1), bounds_d);
         RT_HIP(hipGetLastError());
     }
-    unsigned cerr = 0, nu = 0;
-    uint32_t total = 0, ncomp = 0, big = 0;
-    if (ncomp_d) {
-        RT_HIP(hipMemcpyAsync(&ncomp, ncomp_d, 4, hipMemcpyDeviceToHost, s));
-        RT_HIP(hipMemcpyAsync(&big, big_d, 4, hipMemcpyDeviceToHost, s));
-    }
-    if (x_undec_list)
-        RT_HIP(hipMemcpyAsync(&nu, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipMemcpyAsync(&cerr, &h->ctr->error, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipMemcpyAsync(&total, cidr + n, 4, hipMemcpyDeviceToHost, s));
+    // the phase's scalar results, gathered on the device: one copy back
+    uint32_t* fin;
+    RT_TRY(sc.get(&fin, 8));
+    k_final_scalars<<<1, WAVE, 0, s>>>(ncomp_d, big_d, x_undec_list ? &h->ctr->n_undecided : nullptr,
+                                       &h->ctr->error, cidr + n, fin);
+    uint32_t hf[5] = {0, 0, 0, 0, 0};
+    RT_HIP(hipMemcpyAsync(hf, fin, sizeof(hf), hipMemcpyDeviceToHost, s));
     h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
     if (ns)
         RT_HIP(hipMemcpyAsync(h->seg_bounds.data(), bounds_d, (ns + 1) * 4,
                               hipMemcpyDeviceToHost, s));
     RT_HIP(hipEventRecord(h->ev[8], s));
     RT_HIP(hipEventSynchronize(h->ev[8]));
+    const uint32_t ncomp = hf[0], big = hf[1], nu = hf[2], cerr = hf[3], total = hf[4];
     if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
     if (ncomp_d) {
         h->audit.components = ncomp;

@@ -95,6 +95,34 @@ __global__ void __launch_bounds__(SCAN_T) k_block_scan(const TI* in, uint64_t n,
         out[n] = (TO)bsum[gridDim.x];
 }
 
+// Small inputs (n <= SCAN_SMALL): the whole exclusive scan in one workgroup,
+// tile after tile with a carry -- one launch instead of three.
+constexpr uint64_t SCAN_SMALL = 4 * SCAN_TILE;
+template <class TI, class TO>
+__global__ void __launch_bounds__(SCAN_T) k_scan_small(const TI* in, uint64_t n, TO* out) {
+    uint64_t carry = 0;
+    for (uint64_t t0 = 0; t0 < n; t0 += SCAN_TILE) {
+        const uint64_t base = t0 + (uint64_t)threadIdx.x * SCAN_ITEMS;
+        uint64_t v[SCAN_ITEMS];
+        uint64_t s = 0;
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint64_t k = base + i;
+            v[i] = k < n ? (uint64_t)in[k] : 0;
+            s += v[i];
+        }
+        uint64_t tot;
+        const uint64_t inc = block_inclusive_scan<uint64_t>(s, &tot);
+        uint64_t run = carry + inc - s;
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint64_t k = base + i;
+            if (k < n) out[k] = (TO)run;
+            run += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = (TO)carry;
+}
+
 inline uint64_t scan_blocks(uint64_t n) {
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     return nb ? nb : 1;
@@ -113,6 +141,11 @@ void scan_enqueue(const TI* in, TO* out, uint64_t n, uint64_t* bsum,
 template <class TI, class TO>
 rtrex_status exclusive_scan_impl(const TI* in, TO* out, uint64_t n,
                                  hipStream_t s) {
+    if (n <= SCAN_SMALL) {
+        k_scan_small<TI, TO><<<1, SCAN_T, 0, s>>>(in, n, out);
+        RT_HIP(hipGetLastError());
+        return RTREX_OK;
+    }
     Scratch sc;
     uint64_t* bsum;
     RT_TRY(sc.get(&bsum, scan_blocks(n) + 1));
