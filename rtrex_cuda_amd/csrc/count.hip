@@ -475,7 +475,7 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                                           double* sfold, uint32_t first, uint32_t cnt,
                                           uint32_t stride, const uint4* pre = nullptr,
                                           const uint64_t* prez = nullptr) {
-    constexpr int R = G <= 16 ? 4 : 2;  // 64 ids per group and load round up to G = 16
+    constexpr int R = 4;  // chunks of G ids per group and load round
     const int lane = lane_id();
     const int gl = lane & (G - 1);
     const int gbase = lane & ~(G - 1);
