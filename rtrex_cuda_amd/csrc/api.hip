@@ -516,7 +516,7 @@ rtrex_count_params to_abi(const CountParams& c) {
 }
 
 rtrex_status from_abi(const rtrex_count_params& p, CountParams* c) {
-    if (p.hub_min < 0 || p.hub_min >= p.hub_max || p.hub_max > 32760)
+    if (p.hub_min < 0 || p.hub_min >= p.hub_max || p.hub_max > (int32_t)HUB_MAX_LIMIT)
         return fail(RTREX_EINVAL, "need 0 <= hub_min < hub_max <= 32760");
     if (p.l0_max < 0 || p.l0_max > 64 || p.l1_max < p.l0_max || p.l1_max > 512)
         return fail(RTREX_EINVAL, "need 0 <= l0_max <= 64 and l0_max <= l1_max <= 512");
@@ -615,7 +615,7 @@ rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params
                 for (uint32_t v : {4u, 8u, 16u}) { c.unit_hash_mult = v; cand.push_back(c); }
                 break;
             default:
-                for (uint32_t v : {1024u, 4096u, 32760u}) {
+                for (uint32_t v : {1024u, 4096u, HUB_MAX_LIMIT}) {
                     c.hub_max = v;
                     if (c.hub_min < v) cand.push_back(c);
                 }

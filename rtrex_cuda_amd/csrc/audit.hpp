@@ -66,6 +66,110 @@ __device__ __forceinline__ bool next_perm(double* a, int k) {
     return true;
 }
 
+// Decision from E = s0 - sum(D) held as hi + lo (hi + lo_exact equal to E
+// exactly, lo's own additions rounded; at most nd + 16 of them, each off by
+// at most 2^-53 |lo| <= 2^-53 (nd + 16) ulp(s0) / 2), with nd = |D|: KEEP or
+// REMOVE when every order of the subtractions lands on that side of thr, else
+// CERT_UNDECIDED (the caller may enumerate the orders).
+__device__ __forceinline__ int dd_decide(double hi, double lo, uint32_t nd, double s0,
+                                         double thr) {
+    const double u = ulp_of(s0);
+    const double n = (double)nd;
+    const double half =
+        0.5 * u * (n * (1.0 + 0x1.0p-40) + (n + 16.0) * (n + 16.0) * 0x1.0p-52) + 0x1.0p-1060;
+    double d, de;
+    two_sum(hi, -thr, d, de);
+    d += de + lo;
+    if (d >= half) return CERT_KEEP;
+    if (d < -half) return CERT_REMOVE;
+    return CERT_UNDECIDED;
+}
+
+// Every order of N items as rows of indices (lexicographic), built at
+// compile time for the unrolled enumeration below.
+template <int N>
+struct PermTable {
+    static constexpr int rows() {
+        int f = 1;
+        for (int i = 2; i <= N; ++i) f *= i;
+        return f;
+    }
+    uint8_t p[rows()][N];
+    constexpr PermTable() : p{} {
+        uint8_t a[N] = {};
+        for (int i = 0; i < N; ++i) a[i] = (uint8_t)i;
+        for (int r = 0; r < rows(); ++r) {
+            for (int i = 0; i < N; ++i) p[r][i] = a[i];
+            int i = N - 2;  // next lexicographic permutation of a
+            while (i >= 0 && a[i] > a[i + 1]) --i;
+            if (i < 0) break;
+            int j = N - 1;
+            while (a[j] < a[i]) --j;
+            const uint8_t x = a[i];
+            a[i] = a[j];
+            a[j] = x;
+            for (int l = i + 1, h = N - 1; l < h; ++l, --h) {
+                const uint8_t y = a[l];
+                a[l] = a[h];
+                a[h] = y;
+            }
+        }
+    }
+};
+
+// All N! orders of t[0..N) with static register indices (repeated values
+// give repeated orders; at most 120 of them).
+template <int N>
+__device__ __forceinline__ int enumerate_unrolled(double s0, double thr, const double* src) {
+    constexpr PermTable<N> P{};
+    double t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = src[i];
+    bool seen_lt = false, seen_ge = false;
+#pragma unroll
+    for (int r = 0; r < PermTable<N>::rows(); ++r) {
+        double v = s0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) v -= t[P.p[r][i]];
+        if (v < thr) seen_lt = true;
+        else seen_ge = true;
+    }
+    return seen_lt && seen_ge ? CERT_UNDECIDED : seen_lt ? CERT_REMOVE : CERT_KEEP;
+}
+
+// Outcome over every order of subtracting src[0..nd) (nd <= kMaxPerm) from s0
+// with the reference's plain double subtractions.  Up to five values run
+// unrolled over a table of orders; more are sorted and stepped through their
+// distinct orders (next_perm).
+__device__ inline int enumerate_orders(double s0, double thr, const double* src, int nd) {
+    switch (nd) {
+        case 0: return s0 < thr ? CERT_REMOVE : CERT_KEEP;
+        case 1: return enumerate_unrolled<1>(s0, thr, src);
+        case 2: return enumerate_unrolled<2>(s0, thr, src);
+        case 3: return enumerate_unrolled<3>(s0, thr, src);
+        case 4: return enumerate_unrolled<4>(s0, thr, src);
+        case 5: return enumerate_unrolled<5>(s0, thr, src);
+        default: break;
+    }
+    double t[kMaxPerm];
+    for (int i = 0; i < nd; ++i) t[i] = src[i];
+    for (int i = 1; i < nd; ++i)  // insertion sort ascending
+        for (int j = i; j > 0 && t[j] < t[j - 1]; --j) {
+            const double x = t[j];
+            t[j] = t[j - 1];
+            t[j - 1] = x;
+        }
+    bool seen_lt = false, seen_ge = false;
+    do {
+        double v = s0;
+        for (int i = 0; i < nd; ++i) v -= t[i];
+        if (v < thr) seen_lt = true;
+        else seen_ge = true;
+        if (seen_lt && seen_ge) return CERT_UNDECIDED;
+    } while (next_perm(t, nd));
+    return seen_lt ? CERT_REMOVE : CERT_KEEP;
+}
+
 // Accumulates one dead triangle of weight t into the certification state.
 struct CertAcc {
     double hi, lo;
@@ -82,29 +186,9 @@ struct CertAcc {
     }
     // Outcome over every order of the accumulated subtractions from s0.
     __device__ int finish(double s0, double thr) {
-        const double u = ulp_of(s0);
-        const double half = 0.5 * (double)nd * u * (1.0 + 0x1.0p-40) + 0x1.0p-1060;
-        double d, de;
-        two_sum(hi, -thr, d, de);
-        d += de + lo;
-        if (d >= half) return CERT_KEEP;
-        if (d < -half) return CERT_REMOVE;
-        if (nd > kMaxPerm) return CERT_UNDECIDED;
-        for (int i = 1; i < nd; ++i)  // insertion sort ascending
-            for (int j = i; j > 0 && t8[j] < t8[j - 1]; --j) {
-                double t = t8[j];
-                t8[j] = t8[j - 1];
-                t8[j - 1] = t;
-            }
-        bool seen_lt = false, seen_ge = false;
-        do {  // every distinct order, with the reference's plain subtractions
-            double v = s0;
-            for (int i = 0; i < nd; ++i) v -= t8[i];
-            if (v < thr) seen_lt = true;
-            else seen_ge = true;
-            if (seen_lt && seen_ge) return CERT_UNDECIDED;
-        } while (next_perm(t8, nd));
-        return seen_lt ? CERT_REMOVE : CERT_KEEP;
+        const int c = dd_decide(hi, lo, (uint32_t)nd, s0, thr);
+        if (c != CERT_UNDECIDED || nd > kMaxPerm) return c;
+        return enumerate_orders(s0, thr, t8, nd);
     }
 };
 
@@ -154,6 +238,75 @@ __device__ int certify_terms(const GraphDev& g, const TermBuf& tb, uint32_t f,
             acc.add(q.w);
     }
     return acc.finish(s0, thr);
+}
+
+// certify_terms by a whole wave: every lane calls it with the same arguments;
+// s_t / s_n are kMaxPerm doubles and a counter in this wave's LDS.  The lanes
+// walk the list U records per step (edge ids, then statuses, loaded before
+// use), each sums its dead weights in double-double (exact partial sums) and
+// gathers them into s_t while there are at most kMaxPerm; the partials are
+// added pairwise across the wave (every lane ends with the same sum), and
+// every lane decides from s0 + sum, enumerating the orders of the gathered
+// weights within the error band.  A certain outcome is certify_terms'.
+__device__ inline int certify_terms_wave(const GraphDev& g, const TermBuf& tb, uint32_t f,
+                                         double thr, bool qdead, uint32_t extra_edge,
+                                         double* s_t, uint32_t* s_n) {
+    const int lane = lane_id();
+    const double s0 = g.sup0[f];
+    const uint64_t k0 = tb.zoff[f], k1 = k0 + g.tcnt[f];
+    if (lane == 0) *s_n = 0;
+    wave_sync();
+    double hi = 0.0, lo = 0.0;
+    uint32_t nd = 0;
+    constexpr int U = 2;
+    for (uint64_t k = k0 + lane; k < k1; k += (uint64_t)U * WAVE) {
+        uint32_t fa[U], fb[U];
+        double tw[U];
+        bool dead[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const uint64_t q = k + (uint64_t)r * WAVE;
+            const Term rec = q < k1 ? tb.t[q] : Term{0.0, 0u, 0u};
+            fa[r] = rec.sa;
+            fb[r] = rec.sb;
+            tw[r] = rec.w;
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            const uint64_t q = k + (uint64_t)r * WAVE;
+            dead[r] = q < k1 && (tri_dead(g.est[fa[r]], g.est[fb[r]], qdead) ||
+                                 fa[r] == extra_edge || fb[r] == extra_edge);
+        }
+#pragma unroll
+        for (int r = 0; r < U; ++r) {
+            if (!dead[r]) continue;
+            double sn, e;
+            two_sum(hi, -tw[r], sn, e);
+            hi = sn;
+            lo += e;
+            ++nd;
+            const uint32_t slot = atomicAdd(s_n, 1u);
+            if (slot < (uint32_t)kMaxPerm) s_t[slot] = tw[r];
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const double h2 = __shfl_xor(hi, o, WAVE);
+        const double l2 = __shfl_xor(lo, o, WAVE);
+        nd += __shfl_xor(nd, o, WAVE);
+        double sn, e;
+        two_sum(hi, h2, sn, e);
+        hi = sn;
+        lo = lo + l2 + e;
+    }
+    wave_sync();  // the gathered weights are visible to every lane
+    double sn, e;
+    two_sum(s0, hi, sn, e);
+    int c = dd_decide(sn, lo + e, nd, s0, thr);
+    if (c == CERT_UNDECIDED && nd <= (uint32_t)kMaxPerm)
+        c = enumerate_orders(s0, thr, s_t, (int)nd);
+    wave_sync();  // s_t and s_n are read before the next call resets them
+    return c;
 }
 
 // Either form, by what the graph keeps.  The extra dead triangle is named by

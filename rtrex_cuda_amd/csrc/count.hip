@@ -346,15 +346,6 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
 // are folded in lane order.
 constexpr int HT = 256;
 
-// Orders this wave's LDS writes before its later LDS reads by other lanes
-// (LDS operations of one wave complete in order; the fences keep the
-// compiler from moving accesses across).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Hub-path work unit: part of one hub's owned edges.  x = hub, y = offset of
 // its list, z = first own_idx entry, w = d_hub | (edge count << 16) (d_hub <=
 // 32768, hub_max's limit; at most 512 edges per unit).
@@ -872,7 +863,7 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
     }
 }
 
-// Huge hubs (degree in (l2_max, HUGE_MAX]): a four-level 8-ary tree whose
+// Huge hubs (degree in (l2_max, HUB_MAX_LIMIT]): a four-level 8-ary tree whose
 // upper levels (every 8th, 64th and 512th entry; 4096 + 512 + 64 ids, 18 KB)
 // are staged in LDS and whose leaf is the hub's list itself in HBM: a probe
 // descends in LDS, then reads one 8-entry block of the list (two aligned
@@ -881,7 +872,7 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
 // tree indexes the list from the 32-byte boundary at or before its start
 // (sh = oh mod 8 entries earlier): those sh entries read as 0, so the sequence
 // stays non-decreasing, and are never a match.
-constexpr uint32_t HUGE_MAX = 32768 - 8;
+static_assert(HUB_MAX_LIMIT + 7 < 32768, "a huge hub's list and its shift fit the tree");
 __device__ __forceinline__ uint32_t huge_id(const GraphDev& g, uint32_t ob, uint32_t sh,
                                             uint32_t dh, uint32_t s) {
     return s < sh ? 0u : s < sh + dh ? g.nb[ob + s] : PAD_ID;

@@ -54,6 +54,15 @@ __device__ __forceinline__ uint64_t ballot64(bool p) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (LDS operations of one wave complete in order; the fences keep the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Slot for one item per calling lane in a list whose length is *ctr: one
 // atomic per wave (the lowest active lane reserves for all active lanes).
 // Callable from divergent code; every active lane must call it.

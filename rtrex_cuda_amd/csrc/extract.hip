@@ -275,20 +275,25 @@ struct XArgs {
 };
 
 // Cascade decision for alive edge f whose support just became v (threshold
-// thr); `third` is the third vertex of f's triangle with the edge being
-// deleted.  Certain outside the cheap bound, otherwise certified against every
-// subtraction order (audit.hpp).
+// thr): certain outside the cheap bound, otherwise (need set) certified
+// against every subtraction order (audit.hpp) and settled by settle().
 // An edge whose post-clean support is the reference's own value (xact) has,
 // in the reference's cascade order, the reference's value here too: its
 // comparison is taken as is.
-__device__ bool decide(const XArgs& A, uint32_t f, double v, double thr, uint32_t third,
-                       uint32_t deleting, unsigned long long& amb) {
+__device__ bool decide_cheap(const XArgs& A, uint32_t f, double v, double thr, bool& need) {
     const GraphDev& g = A.g;
     if (A.xact && A.xact[f]) return v < thr;
     const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
     if (v < thr - cb) return true;
     if (v >= thr + cb) return false;
-    int c = certify_edge(g, A.tb, f, thr, false, third, deleting);
+    need = true;
+    return false;
+}
+
+// Outcome c of a certification; an undecided one is recorded for an exact
+// re-run of its region and takes this library's comparison.
+__device__ bool settle(const XArgs& A, uint32_t f, double v, double thr, int c,
+                       unsigned long long& amb) {
     if (c == CERT_UNDECIDED) {
         amb++;
         if (A.undec) {
@@ -332,6 +337,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ double sbd[LDS_CAND];
     __shared__ uint32_t skey[LDS_CAND];
     __shared__ int s_maxind;
+    __shared__ double s_ct[kMaxPerm];  // certification: gathered dead weights
+    __shared__ uint32_t s_cn;
     const GraphDev& g = A.g;
     const int lane = lane_id();
     unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
@@ -559,6 +566,14 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 const uint64_t c1 = terms ? c0 + g.tcnt[e] : g.off[lo + 1];
                 uint32_t hlo = g.off[hi];
                 const uint32_t he = g.off[hi + 1];
+                // The endpoints' supports lose the triangles' weights in w
+                // order on lane 0, in registers until the pop ends (no other
+                // lane and no other wave writes them meanwhile).
+                double px = 0.0, py = 0.0;
+                if (lane == 0) {
+                    px = g.pv[x];
+                    py = g.pv[y];
+                }
                 for (uint64_t p = c0; p < c1; p += XT) {
                     const uint64_t kk = p + lane;
                     const bool in = kk < c1;
@@ -599,8 +614,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                         const int bl = __ffsll((unsigned long long)hb) - 1;
                         const double tv = __shfl(t, bl, 64);
                         if (lane == 0) {
-                            g.pv[x] -= tv;
-                            g.pv[y] -= tv;
+                            px -= tv;
+                            py -= tv;
                         }
                         hb &= hb - 1;
                     }
@@ -610,23 +625,55 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     const bool swp = rlo != (terms ? tlo : lo);
                     const uint32_t jl = swp ? fh : fj, jh = swp ? fj : fh;
                     const uint8_t sl = swp ? sh : sj, shh = swp ? sj : sh;
-                    bool push1 = false, push2 = false;
+                    bool push1 = false, push2 = false, need1 = false, need2 = false;
+                    double v1 = 0.0, v2 = 0.0, thr1 = 0.0, thr2 = 0.0;
                     if (hit) {
                         g.pv[w] -= t;
                         // (lower, w) then (higher, w), as Decomposition.h:238-264.
                         // Queued edges die anyway; their supports are not kept.
                         if (sl == ST_ALIVE) {
-                            const double v1 = g.sup[jl] - t;
+                            v1 = g.sup[jl] - t;
                             g.sup[jl] = v1;
                             g.touched[jl] = 1;
-                            push1 = decide(A, jl, v1, A.eps * g.ew[jl], rhi, e, amb_c);
+                            thr1 = A.eps * g.ew[jl];
+                            push1 = decide_cheap(A, jl, v1, thr1, need1);
                         }
                         if (shh == ST_ALIVE) {
-                            const double v2 = g.sup[jh] - t;
+                            v2 = g.sup[jh] - t;
                             g.sup[jh] = v2;
                             g.touched[jh] = 1;
-                            push2 = decide(A, jh, v2, A.eps * g.ew[jh], rlo, e, amb_c);
+                            thr2 = A.eps * g.ew[jh];
+                            push2 = decide_cheap(A, jh, v2, thr2, need2);
                         }
+                    }
+                    // The decisions the cheap bound leaves open read statuses
+                    // only (no support written above), so they may follow all
+                    // of the chunk's updates: with stored lists one edge at a
+                    // time by the whole wave, else per lane.
+                    if (terms) {
+#pragma unroll
+                        for (int side = 0; side < 2; ++side) {
+                            uint64_t nb = ballot64(side == 0 ? need1 : need2);
+                            while (nb) {  // wave-uniform
+                                const int bl = __ffsll((unsigned long long)nb) - 1;
+                                nb &= nb - 1;
+                                const uint32_t f = __shfl(side == 0 ? jl : jh, bl, 64);
+                                const double th = __shfl(side == 0 ? thr1 : thr2, bl, 64);
+                                const int cc = certify_terms_wave(g, A.tb, f, th, false, e, s_ct,
+                                                                  &s_cn);
+                                if (lane == bl) {
+                                    if (side == 0) push1 = settle(A, f, v1, th, cc, amb_c);
+                                    else push2 = settle(A, f, v2, th, cc, amb_c);
+                                }
+                            }
+                        }
+                    } else {
+                        if (need1)
+                            push1 = settle(A, jl, v1, thr1,
+                                           certify_edge(g, A.tb, jl, thr1, false, rhi, e), amb_c);
+                        if (need2)
+                            push2 = settle(A, jh, v2, thr2,
+                                           certify_edge(g, A.tb, jh, thr2, false, rlo, e), amb_c);
                     }
                     const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
                     // exclusive prefix of pushes over lanes
@@ -652,6 +699,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                     __syncthreads();
                 }
                 if (lane == 0) {
+                    g.pv[x] = px;
+                    g.pv[y] = py;
                     set_status(g, e, ST_DEAD);
                     g.sup[e] = 0.0;
                 }

@@ -118,15 +118,20 @@ struct AuditHost {
     int64_t components = 0, largest_component = 0;
 };
 
+// Largest hub degree the count kernels accept (the huge-hub tree spans
+// 32768 entries from the 8-entry boundary at or before the list's start).
+constexpr uint32_t HUB_MAX_LIMIT = 32760;
+
 // Count-plan parameters (the autotuner's search space).  An edge whose
 // longer-list endpoint (its hub) has degree in (hub_min, hub_max] is counted
-// per hub with the hub list staged in LDS: as a sorted list of 64 entries
-// (degree <= l0_max), of 512 (<= l1_max), or in a hash table (above); the rest
-// per edge.  A hub's edges get G = 4, 8, 16, 32 or 64 lanes each by their
-// mean shorter-list length against g_thr[0..3].  Work units carry at least
-// unit_min streamed elements (hash class: unit_hash_mult * d_h).
+// per hub, the hub list staged as an 8-ary pivot tree: whole in LDS for
+// degrees up to 64 (l0_max), 512 (l1_max) and 4096 (l2_max), its upper levels
+// in LDS and its leaf blocks read from HBM above l2_max; the rest per edge.
+// A hub's edges get G = 4, 8, 16, 32 or 64 lanes each by their mean
+// shorter-list length against g_thr[0..3].  Work units carry at least
+// unit_min streamed elements (classes above l1_max: unit_hash_mult * d_h).
 struct CountParams {
-    uint32_t hub_min = 8, hub_max = 32760;
+    uint32_t hub_min = 8, hub_max = HUB_MAX_LIMIT;
     uint32_t l0_max = 64, l1_max = 512, l2_max = 4096;
     uint32_t g_thr[4] = {3, 6, 24, 80};
     uint32_t unit_min = 1024, unit_hash_mult = 8;
