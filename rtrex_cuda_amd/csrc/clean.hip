@@ -33,7 +33,7 @@ constexpr int T = 256;
 
 // Status change of edge e during a peel.  With stored triangle lists the
 // peel reads the per-edge status only, so only est is written and the slot
-// copies are brought in line once the clean ends (k_sync_slot_status); the
+// copies are brought in line once the clean ends (sync_slot_status); the
 // intersection form reads the slot copies and keeps all three in step.
 __device__ __forceinline__ void mark(const GraphDev& g, uint32_t e, uint8_t c, bool terms) {
     if (terms) g.est[e] = c;
@@ -48,9 +48,9 @@ __device__ __forceinline__ int work_bucket(uint32_t w) {
 }
 
 // When bk is given (stored triangle lists), the frontier is also written
-// split by work bucket of tcnt (bk: 4 lists of capacity m, then 4 counts),
+// split by work bucket of tcnt (bk: 4 lists of capacity m, counts in bkc),
 // and the edges that stay alive with triangles go to the alive buckets ak
-// (same layout), the work list of a pull round.  At eps = 0.3 nearly every
+// (same layout, counts in akc), the work list of a pull round.  At eps = 0.3 nearly every
 // edge starts in the frontier, so appends are aggregated per workgroup (LDS
 // counters, one global atomic per list per tile of SEED_K * T edges) rather
 // than per wave.
@@ -60,10 +60,12 @@ constexpr int SEED_K = 8;
 // occupancy (about 18 % faster on a 2.1M-edge graph).
 constexpr int SEED_T = 1024;
 __global__ void __launch_bounds__(SEED_T) k_seed_frontier(GraphDev g, double eps,
-                                                     uint32_t* __restrict__ F,
-                                                     unsigned* nF,
-                                                     uint32_t* __restrict__ bk,
-                                                     uint32_t* __restrict__ ak) {
+                                                          uint32_t* __restrict__ F,
+                                                          unsigned* nF,
+                                                          uint32_t* __restrict__ bk,
+                                                          unsigned* bkc,
+                                                          uint32_t* __restrict__ ak,
+                                                          unsigned* akc) {
     // counters: 0-3 frontier buckets, 4 frontier, 5-8 alive buckets
     __shared__ uint32_t lcnt[9], lbase[9];
     const bool terms = bk != nullptr;
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(SEED_T) k_seed_frontier(GraphDev g, double eps
         __syncthreads();
         if (threadIdx.x < 9 && lcnt[threadIdx.x]) {
             const int i = threadIdx.x;
-            unsigned* c = i == 4 ? nF : i < 4 ? bk + 4ull * g.m + i : ak + 4ull * g.m + (i - 5);
+            unsigned* c = i == 4 ? nF : i < 4 ? bkc + i : akc + (i - 5);
             if (i == 4 || terms) lbase[i] = atomicAdd(c, lcnt[i]);
         }
         __syncthreads();
@@ -297,17 +299,17 @@ __global__ void __launch_bounds__(T) k_alive_buckets(GraphDev g, uint32_t* __res
 // the dying weights, in any order: the result is S0 minus the dead terms with
 // at most tcnt roundings of size <= ulp(S0)/2 in all, inside the cheap bound,
 // and every decision within that bound is certified from the terms.
+// Blocks blk of nblk of one bucket's launch range.
 template <int G>
-__global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __restrict__ list,
-                                                 const unsigned* __restrict__ cntp) {
-    const uint32_t cnt = *cntp;
+__device__ __forceinline__ void pull_body(const PeelArgs& A, const uint32_t* __restrict__ list,
+                                          uint32_t cnt, uint32_t blk, uint32_t nblk) {
     const GraphDev& g = A.g;
     const TermBuf& tb = A.tb;
     const int lane = lane_id();
     const int gl = lane & (G - 1);
     const uint32_t per_wave = WAVE / G;
-    const uint32_t wave = (blockIdx.x * T + threadIdx.x) / WAVE;
-    const uint32_t nwaves = gridDim.x * T / WAVE;
+    const uint32_t wave = (blk * T + threadIdx.x) / WAVE;
+    const uint32_t nwaves = nblk * T / WAVE;
     for (uint32_t base = wave * per_wave; base < cnt; base += nwaves * per_wave) {
         const uint32_t idx = base + lane / G;
         const bool in = idx < cnt;
@@ -373,19 +375,39 @@ __global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __r
     }
 }
 
-// After the pull kernels of a round, the items of the alive buckets ak that
-// are still ALIVE split two ways: the claimed ones (claim set) go to the next
+// The four alive buckets in one launch: blocks [bb[b], bb[b + 1]) take
+// bucket b of ak (1, 4, 16 or 64 lanes per edge; counts in acnt[b]).
+struct BucketBlocks {
+    uint32_t bb[5];
+};
+__global__ void __launch_bounds__(T) k_peel_pull(PeelArgs A, const uint32_t* __restrict__ ak,
+                                                 const unsigned* __restrict__ acnt,
+                                                 BucketBlocks R) {
+    const uint32_t x = blockIdx.x;
+    const int b = x < R.bb[1] ? 0 : x < R.bb[2] ? 1 : x < R.bb[3] ? 2 : 3;
+    const uint32_t* L = ak + (uint64_t)b * A.g.m;
+    const uint32_t blk = x - R.bb[b], nblk = R.bb[b + 1] - R.bb[b];
+    switch (b) {
+        case 0: pull_body<1>(A, L, acnt[0], blk, nblk); break;
+        case 1: pull_body<4>(A, L, acnt[1], blk, nblk); break;
+        case 2: pull_body<16>(A, L, acnt[2], blk, nblk); break;
+        default: pull_body<64>(A, L, acnt[3], blk, nblk); break;
+    }
+}
+
+// After the pull kernels of a round, the items of the alive buckets ak
+// (counts ac) that are still ALIVE split two ways: the claimed ones (claim set) go to the next
 // frontier as PENDING, the rest to ak2, the next pull round's buckets (same
-// bucket).  An edge k_near_check claims later in the round stays in ak2; the
+// bucket, counts ac2).  An edge k_near_check claims later in the round stays in ak2; the
 // pull kernels skip items that are no longer ALIVE and the next collection
 // drops them.  One global atomic per list per tile of SEED_K * T items.
 __global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, const uint32_t* __restrict__ ak,
+                                                      const unsigned* __restrict__ ac,
                                                       uint32_t* __restrict__ Fn,
                                                       unsigned* nFn,
-                                                      uint32_t* __restrict__ ak2) {
+                                                      uint32_t* __restrict__ ak2,
+                                                      unsigned* ac2) {
     __shared__ uint32_t lcnt[5], lbase[5];  // 0-3 buckets of ak2, 4 next frontier
-    const unsigned* ac = ak + 4ull * g.m;
-    unsigned* ac2 = ak2 + 4ull * g.m;
     const uint32_t c[4] = {ac[0], ac[1], ac[2], ac[3]};
     const uint32_t na = c[0] + c[1] + c[2] + c[3];
     for (uint32_t tile = blockIdx.x * SEED_K * T; tile < na; tile += gridDim.x * SEED_K * T) {
@@ -429,10 +451,12 @@ __global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, const uint32_t
 
 // Slot statuses from the edge statuses, after a peel on stored triangle
 // lists (which writes est only).
-__global__ void __launch_bounds__(T) k_sync_slot_status(GraphDev g) {
-    for (uint64_t j = (uint64_t)blockIdx.x * T + threadIdx.x; j < g.nnz;
-         j += (uint64_t)gridDim.x * T)
-        g.st[j] = g.est[g.s2e[j]];
+__device__ __forceinline__ void sync_slot_status(const GraphDev& g, uint64_t t0,
+                                                 uint64_t stride) {
+    for (uint64_t j = t0; j < g.nnz; j += stride) g.st[j] = g.est[g.s2e[j]];
+}
+__global__ void __launch_bounds__(T) k_slot_status(GraphDev g) {
+    sync_slot_status(g, (uint64_t)blockIdx.x * T + threadIdx.x, (uint64_t)gridDim.x * T);
 }
 
 // Triangles removed by the clean: the count phase's total minus the live ones
@@ -659,13 +683,22 @@ __global__ void __launch_bounds__(T) k_near_check_terms(PeelArgs A) {
 }
 
 // End of a round: the frontier is deleted, the claimed edges are queued.
+// It also clears the counters the next round appends to (zero2: the other
+// parity's next-frontier and near-list counts) and, after a pull round, the
+// counts of the alive lists just consumed (zero4), which become the next
+// round's output lists: no clearing launch per round.
 __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
                                                  const uint32_t* __restrict__ F,
                                                  uint32_t nF,
                                                  const uint32_t* __restrict__ Fn,
                                                  const unsigned* __restrict__ nFnp,
-                                                 Counters* ctr, bool terms) {
+                                                 Counters* ctr, bool terms,
+                                                 unsigned* zero2, unsigned* zero4) {
     const uint32_t nFn = *nFnp;
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        if (threadIdx.x < 2) zero2[threadIdx.x] = 0;
+        if (zero4) zero4[threadIdx.x] = 0;
+    }
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < max(nF, nFn);
          i += gridDim.x * T) {
         if (i < nF) {
@@ -679,15 +712,32 @@ __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
         atomicAdd(&ctr->edges_removed, (unsigned long long)nF);
 }
 
-// After convergence, a safety net: every alive edge that lost triangles and
-// is not clearly above its threshold is certified once more.
-__global__ void __launch_bounds__(T) k_audit_survivors(GraphDev g, TermBuf tb,
-                                                       double eps, Counters* ctr,
-                                                       uint32_t* undec, uint32_t undec_cap) {
-    unsigned long long amb = 0;
+// After convergence, one pass over the edges:
+//  - a safety net: every alive edge that lost triangles and is not clearly
+//    above its threshold is certified once more;
+//  - xact (when given): a support the fast peel never decremented is the
+//    count-phase value, which is the reference's (no triangle of the edge
+//    died there either);
+//  - inc (when given, stored lists): triangles with all three edges alive,
+//    counted once per alive edge.
+__global__ void __launch_bounds__(T) k_post_peel(GraphDev g, TermBuf tb, double eps,
+                                                 Counters* ctr, uint32_t* undec,
+                                                 uint32_t undec_cap, uint8_t* __restrict__ xact,
+                                                 unsigned long long* inc) {
+    unsigned long long amb = 0, live = 0;
     for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m;
          e += gridDim.x * T) {
-        if (!g.touched[e] || g.est[e] != ST_ALIVE) continue;
+        const uint8_t touched = g.touched[e];
+        if (xact) xact[e] = touched ? 0 : 1;
+        if (g.est[e] != ST_ALIVE) continue;
+        if (inc) {
+            const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
+            for (uint64_t k = k0; k < k1; ++k) {
+                const Term q = tb.t[k];
+                live += g.est[q.sa] == ST_ALIVE && g.est[q.sb] == ST_ALIVE;
+            }
+        }
+        if (!touched) continue;
         const double thr = eps * g.ew[e];
         if (g.sup[e] >= thr + cheap_bound(g.tcnt[e], g.sup0[e])) continue;
         if (certify_edge(g, tb, e, thr, false, 0xffffffffu, 0xffffffffu) != CERT_KEEP) {
@@ -695,12 +745,18 @@ __global__ void __launch_bounds__(T) k_audit_survivors(GraphDev g, TermBuf tb,
             note_undecided(undec, undec_cap, ctr, e);
         }
     }
-    for (int d = 32; d >= 1; d >>= 1) amb += __shfl_xor(amb, d, 64);
+    for (int d = 32; d >= 1; d >>= 1) {
+        amb += __shfl_xor(amb, d, 64);
+        live += __shfl_xor(live, d, 64);
+    }
     if (lane_id() == 0 && amb) atomicAdd(&ctr->ambiguous_clean, amb);
+    if (lane_id() == 0 && live) atomicAdd(inc, live);
 }
 
-// Packed survivor bitmap (bit e = edge e alive), kept after the initial clean.
-__global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits) {
+// Packed survivor bitmap (bit e = edge e alive), kept after the initial
+// clean; with slots, the slot statuses are brought in line in the same launch.
+__global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits, bool slots) {
+    if (slots) sync_slot_status(g, (uint64_t)blockIdx.x * T + threadIdx.x, (uint64_t)gridDim.x * T);
     const uint32_t nw = (g.m + 31) / 32;
     for (uint32_t w = blockIdx.x * T + threadIdx.x; w < nw; w += gridDim.x * T) {
         uint32_t b = 0;
@@ -1034,13 +1090,6 @@ __global__ void __launch_bounds__(T) k_count_alive(GraphDev g, unsigned long lon
     if (lane_id() == 0 && c) atomicAdd(out, c);
 }
 
-// A support the fast peel never decremented is the count-phase value, which
-// is the reference's (no triangle of the edge died there either).
-__global__ void __launch_bounds__(T) k_xact_init(GraphDev g, uint8_t* __restrict__ xact) {
-    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T)
-        xact[e] = g.touched[e] ? 0 : 1;
-}
-
 int bits_of(uint64_t n) {
     int b = 1;
     while ((1ull << b) < n) ++b;
@@ -1156,15 +1205,56 @@ unsigned grid_items(uint32_t items, int G) {
     return (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (waves * WAVE + T - 1) / T), 8192);
 }
 
-// Frontier rounds to convergence.  first_bk (optional) holds the first
-// frontier already split by work bucket (4 lists of capacity m + 4 counts).
-// With stored triangle lists, ak[0] and ak[1] (same layout) hold the alive
-// buckets a pull round walks: ak[0] valid on entry when ak_valid (seeding
-// wrote it), carried from one pull round to the next by k_next_alive, and
-// rebuilt by a scan of every edge after a push round.
+// Work lists of a clean, all of capacity 4 x m (four buckets): the first
+// frontier split by bucket (bk), and the two alive-bucket lists pull rounds
+// alternate between (ak[0] walked, ak[1] written).  Their counts, the
+// frontier's and the rounds' counters live in one block (cnt, CNT_WORDS
+// words) cleared by one memset before the clean starts.
+enum : uint32_t {
+    C_FRONT = 0,   // first frontier length (seeding)
+    C_BK = 4,      // 4 bucket counts of bk
+    C_AK0 = 8,     // 4 bucket counts of ak[0] as passed in
+    C_AK1 = 12,    // ... of ak[1]
+    C_ROUND = 16,  // [2p, 2p + 1]: next-frontier and near counts, round parity p
+    C_INC = 20,    // live-incidence total (64 bits)
+    C_FBK = 24,    // 4 bucket counts of a later frontier's split
+    CNT_WORDS = 28,
+};
+struct CleanLists {
+    uint32_t* bk = nullptr;
+    uint32_t* ak[2] = {nullptr, nullptr};
+    uint32_t* cnt = nullptr;
+    unsigned* akc[2] = {nullptr, nullptr};
+};
+
+// The lists are allocated with stored triangle lists only (pull rounds and
+// the first frontier's split need them); bk when seeding writes it.
+rtrex_status clean_lists(rtrex_graph h, Scratch& sc, bool with_bk, CleanLists& L) {
+    const uint64_t m = h->g.m;
+    if (h->zt) {
+        if (with_bk) RT_TRY(sc.get(&L.bk, 4 * m));
+        RT_TRY(sc.get(&L.ak[0], 4 * m));
+        RT_TRY(sc.get(&L.ak[1], 4 * m));
+    }
+    RT_TRY(sc.get(&L.cnt, CNT_WORDS));
+    L.akc[0] = L.cnt + C_AK0;
+    L.akc[1] = L.cnt + C_AK1;
+    RT_HIP(hipMemsetAsync(L.cnt, 0, CNT_WORDS * sizeof(uint32_t), h->stream));
+    return RTREX_OK;
+}
+
+// Frontier rounds to convergence.  first_bk: the first frontier is also
+// split by work bucket in L.bk (seeding).  With stored triangle lists, the
+// alive buckets a pull round walks are L.ak[0], valid on entry when ak_valid
+// (seeding wrote them), carried from one pull round to the next by
+// k_collect_claims, and rebuilt by a scan of every edge after a push round.
+// Per round: one launch per kernel, the counters the kernels append to
+// cleared by the round before (k_end_round), and one host round trip (the
+// next frontier's length, read after the round ends); every launch in between
+// takes its item count from device memory and is sized from a host-side upper
+// bound.  L.cnt must be zero on entry but for the seeding's counts.
 rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
-                  uint32_t nF, uint32_t* ak0, uint32_t* ak1, bool ak_valid,
-                  const uint32_t* first_bk = nullptr,
+                  uint32_t nF, CleanLists L, bool ak_valid, bool first_bk,
                   uint32_t* undec = nullptr, uint32_t undec_cap = 0) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
@@ -1175,58 +1265,47 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_TRY(dalloc(h, &h->nearst, g.m));
         RT_HIP(hipMemsetAsync(h->nearst, 0, (uint64_t)g.m * 4, s));
     }
-    unsigned* cnts = &h->ctr->frontier_next;  // [0] next frontier, [3] near
+    uint32_t* rc = L.cnt + C_ROUND;
+    unsigned long long* inc = reinterpret_cast<unsigned long long*>(L.cnt + C_INC);
     PeelArgs A{};
     A.g = g;
     A.eps = eps;
-    A.nFn = cnts;
     A.near = near;
-    A.nNear = &h->ctr->changed;
     A.nearst = h->nearst;
     A.ctr = h->ctr;
     A.tb = h->terms();
     A.undec = undec;
     A.undec_cap = undec_cap;
-    uint32_t* bk;  // bucketed frontier: 4 x m ids + 4 counters
-    RT_TRY(sc.get(&bk, 4ull * g.m + 4));
-    unsigned* bcur = bk + 4ull * g.m;
-    // One host round trip per round (the next frontier's length, read after
-    // the round ends); every launch in between takes its item count from
-    // device memory and is sized from a host-side upper bound.
-    static_assert(offsetof(Counters, changed) == offsetof(Counters, frontier_next) + 12,
-                  "next-frontier .. near-list counters are adjacent");
-    while (nF) {
-        // next-frontier and near-list counts (with the error flag and work
-        // queue head between them, both re-zeroed before their own uses)
-        RT_HIP(hipMemsetAsync(A.nFn, 0, 4 * sizeof(unsigned), s));
+    uint32_t* fbk;  // a later frontier split by bucket (4 x m ids)
+    RT_TRY(sc.get(&fbk, 4ull * g.m));
+    unsigned* bcur = L.cnt + C_FBK;
+    for (uint32_t round = 0; nF; ++round) {
+        unsigned* rcur = rc + 2 * (round & 1);
+        A.nFn = rcur;
+        A.nNear = rcur + 1;
         A.Fn = Fn;
         A.stamp = ++h->round_stamp;
         const bool terms = A.tb.t != nullptr;
         const bool pull = terms && (uint64_t)nF > h->n_alive;
-        // alive-list launches: sized from the alive count at the round's start
-        const unsigned ba = blocks_for((h->n_alive + SEED_K - 1) / SEED_K, T, 4096);
         if (pull) {
-            first_bk = nullptr;  // pull rounds walk the alive edges instead
+            first_bk = false;  // pull rounds walk the alive edges instead
             A.F = F;
             A.nF = nF;
             A.nFd = nullptr;
-            unsigned* acur = ak0 + 4ull * g.m;
-            if (!ak_valid) {
-                RT_HIP(hipMemsetAsync(acur, 0, 4 * sizeof(unsigned), s));
+            unsigned* acur = L.akc[0];
+            if (!ak_valid)  // its counts were cleared by the round that used it last
                 k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(
-                    g, ak0, acur);
+                    g, L.ak[0], acur);
+            if (h->n_alive) {
+                // one launch over the four buckets, each sized for every
+                // alive edge (the bucket counts stay on the device)
+                BucketBlocks R{};
+                for (int b = 0; b < 4; ++b)
+                    R.bb[b + 1] = R.bb[b] + grid_items((uint32_t)h->n_alive, 1 << (2 * b));
+                k_peel_pull<<<R.bb[4], T, 0, s>>>(A, L.ak[0], acur, R);
+                k_collect_claims<<<blocks_for((h->n_alive + SEED_K - 1) / SEED_K, T, 4096), T, 0,
+                                   s>>>(g, L.ak[0], acur, Fn, A.nFn, L.ak[1], L.akc[1]);
             }
-            RT_HIP(hipMemsetAsync(ak1 + 4ull * g.m, 0, 4 * sizeof(unsigned), s));
-            for (int b = 0; b < 4 && h->n_alive; ++b) {
-                const int G = b == 0 ? 1 : b == 1 ? 4 : b == 2 ? 16 : 64;
-                const uint32_t* L = ak0 + (uint64_t)b * g.m;
-                const unsigned grid = grid_items((uint32_t)h->n_alive, G);
-                if (G == 1) k_peel_pull<1><<<grid, T, 0, s>>>(A, L, acur + b);
-                else if (G == 4) k_peel_pull<4><<<grid, T, 0, s>>>(A, L, acur + b);
-                else if (G == 16) k_peel_pull<16><<<grid, T, 0, s>>>(A, L, acur + b);
-                else k_peel_pull<64><<<grid, T, 0, s>>>(A, L, acur + b);
-            }
-            k_collect_claims<<<ba, T, 0, s>>>(g, ak0, Fn, A.nFn, ak1);
         } else {
             // Work buckets of the frontier (1, 4, 16 or 64 lanes per edge):
             // split while it was seeded (first round), split here when large,
@@ -1235,16 +1314,16 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
             const unsigned* cnts[4] = {nullptr, nullptr, nullptr, nullptr};
             if (first_bk) {
                 for (int b = 0; b < 4; ++b) {
-                    lists[b] = first_bk + (uint64_t)b * g.m;
-                    cnts[b] = first_bk + 4ull * g.m + b;
+                    lists[b] = L.bk + (uint64_t)b * g.m;
+                    cnts[b] = L.cnt + C_BK + b;
                 }
-                first_bk = nullptr;
+                first_bk = false;
             } else if (nF > 4096) {
                 RT_HIP(hipMemsetAsync(bcur, 0, 4 * sizeof(unsigned), s));
                 k_frontier_buckets<<<blocks_for(nF, T, 8192), T, 0, s>>>(g, A.tb.t != nullptr,
-                                                                         F, nF, bk, bcur);
+                                                                         F, nF, fbk, bcur);
                 for (int b = 0; b < 4; ++b) {
-                    lists[b] = bk + (uint64_t)b * nF;
+                    lists[b] = fbk + (uint64_t)b * nF;
                     cnts[b] = bcur + b;
                 }
             } else {
@@ -1278,10 +1357,16 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         if (terms) k_near_check_terms<NEAR_G><<<NEAR_BLOCKS, T, 0, s>>>(A);
         else k_near_check<<<NEAR_BLOCKS, T, 0, s>>>(A);
         // The frontier dies, the claimed edges are queued.  Both lists are
-        // bounded by the edges alive at the start of the round.
+        // bounded by the edges alive at the start of the round.  The next
+        // round's counters are cleared here, and after a pull round the
+        // counts of the lists just walked (the next round's output lists).
         k_end_round<<<blocks_for(std::max<uint64_t>(nF, h->n_alive), T, 4096), T, 0, s>>>(
-            g, F, nF, Fn, A.nFn, h->ctr, terms);
-        if (pull) std::swap(ak0, ak1);  // ak1 (now ak0): the next pull round's work list
+            g, F, nF, Fn, A.nFn, h->ctr, terms, rc + 2 * ((round + 1) & 1),
+            pull ? L.akc[0] : nullptr);
+        if (pull) {  // ak[1] (now ak[0]): the next pull round's work list
+            std::swap(L.ak[0], L.ak[1]);
+            std::swap(L.akc[0], L.akc[1]);
+        }
         ak_valid = pull;
         RT_HIP(hipGetLastError());
         uint32_t nFn = 0;
@@ -1292,33 +1377,25 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         std::swap(F, Fn);
         nF = nFn;
     }
-    k_audit_survivors<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr, undec,
-                                                               undec_cap);
+    // The safety-net certification, the exact-support flags and the live
+    // triangles (the peel kernels do not count them) in one pass.
+    if (undec && !h->xact) RT_TRY(dalloc(h, &h->xact, g.m));
+    k_post_peel<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, eps, h->ctr, undec, undec_cap,
+                                                       undec ? h->xact : nullptr,
+                                                       A.tb.t ? inc : nullptr);
+    if (A.tb.t) k_tri_removed<<<1, WAVE, 0, s>>>(inc, h->ctr);
     RT_HIP(hipGetLastError());
-    Scratch sc2;
-    unsigned long long* inc = nullptr;
-    if (A.tb.t) RT_TRY(sc2.get(&inc, 1));
-    // Live-triangle total (the peel kernels do not count), on the device.
-    auto removed_total = [&]() -> rtrex_status {
-        if (!A.tb.t) return RTREX_OK;
-        RT_HIP(hipMemsetAsync(inc, 0, 8, s));
-        k_live_incidences<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, inc);
-        k_tri_removed<<<1, WAVE, 0, s>>>(inc, h->ctr);
-        RT_HIP(hipGetLastError());
-        return RTREX_OK;
-    };
-    if (undec) {
-        if (!h->xact) RT_TRY(dalloc(h, &h->xact, g.m));
-        k_xact_init<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, h->xact);
-    }
-    RT_TRY(removed_total());
     if (undec) {  // the only wait after the last round: were any decisions open?
         unsigned nund = 0;
         RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
         if (nund && h->zt) {
             RT_TRY(exact_replay(h, eps, undec, undec_cap, nund));
-            RT_TRY(removed_total());
+            // live triangles again, after the replayed regions
+            RT_HIP(hipMemsetAsync(inc, 0, 8, s));
+            k_live_incidences<<<blocks_for(g.m, T, 4096), T, 0, s>>>(g, A.tb, inc);
+            k_tri_removed<<<1, WAVE, 0, s>>>(inc, h->ctr);
+            RT_HIP(hipGetLastError());
         }
     }
     h->pv_full = false;  // vertex supports are pulled again on demand
@@ -1335,24 +1412,16 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
-    uint32_t *F, *Fn, *bk = nullptr, *ak0 = nullptr, *ak1 = nullptr;
+    uint32_t *F, *Fn;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
     const bool terms = h->zt != nullptr;
-    if (terms) {
-        RT_TRY(sc.get(&bk, 4ull * g.m + 4));
-        RT_TRY(sc.get(&ak0, 4ull * g.m + 4));
-        RT_TRY(sc.get(&ak1, 4ull * g.m + 4));
-    }
     RT_HIP(hipEventRecord(h->ev[5], s));
-    unsigned* cnt = &h->ctr->frontier_next;
-    RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
-    if (terms) {
-        RT_HIP(hipMemsetAsync(bk + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
-        RT_HIP(hipMemsetAsync(ak0 + 4ull * g.m, 0, 4 * sizeof(uint32_t), s));
-    }
+    CleanLists L;
+    RT_TRY(clean_lists(h, sc, terms, L));
+    unsigned* cnt = L.cnt + C_FRONT;
     k_seed_frontier<<<blocks_for((g.m + SEED_K - 1) / SEED_K, SEED_T, 4096), SEED_T, 0, s>>>(
-        g, eps, F, cnt, bk, ak0);
+        g, eps, F, cnt, L.bk, L.cnt + C_BK, terms ? L.ak[0] : nullptr, L.akc[0]);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
     RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -1369,10 +1438,10 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
         RT_HIP(hipMemcpyAsync(&h->ctr->n_undecided, &nf, 4, hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
     }
-    RT_TRY(peel(h, eps, F, Fn, nF, ak0, ak1, terms, bk, terms ? undec : nullptr, g.m));
-    if (terms) k_sync_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
+    RT_TRY(peel(h, eps, F, Fn, nF, L, terms, terms, terms ? undec : nullptr, g.m));
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
-    k_survivor_bits<<<blocks_for((g.m + 31) / 32, T, 4096), T, 0, s>>>(g, h->surv);
+    k_survivor_bits<<<blocks_for(terms ? g.nnz : (g.m + 31) / 32, T, 8192), T, 0, s>>>(
+        g, h->surv, terms);
     RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[6], s));
     h->tm_clean_pending = true;  // read later (resolve_timings), no wait here
@@ -1397,15 +1466,13 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
         pr[i] = (uint32_t)pairs[i];
     }
     Scratch sc;
-    uint32_t *F, *Fn, *dpr, *ak0 = nullptr, *ak1 = nullptr;
+    uint32_t *F, *Fn, *dpr;
     const bool terms = h->zt != nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
     RT_TRY(sc.get(&dpr, 2 * k));
-    if (terms) {
-        RT_TRY(sc.get(&ak0, 4ull * g.m + 4));
-        RT_TRY(sc.get(&ak1, 4ull * g.m + 4));
-    }
+    CleanLists L;
+    RT_TRY(clean_lists(h, sc, false, L));
     RT_HIP(hipMemcpyAsync(dpr, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, s));
     unsigned* cnt = &h->ctr->frontier_next;
     RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
@@ -1425,8 +1492,8 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                     "graph, or is listed twice (state is undefined now)");
     if (h->xact) RT_HIP(hipMemsetAsync(h->xact, 0, g.m, s));  // supports no longer the reference's
     h->n_alive -= nF;
-    RT_TRY(peel(h, eps, F, Fn, nF, ak0, ak1, false));
-    if (terms) k_sync_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
+    RT_TRY(peel(h, eps, F, Fn, nF, L, false, false));
+    if (terms) k_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
     RT_HIP(hipGetLastError());
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;
