@@ -9,9 +9,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
+#include "dev.hpp"
 #include "internal.hpp"
 
 namespace rtrex {
@@ -51,6 +53,86 @@ struct ArenaScope {
     ~ArenaScope() { tl_arena = prev; }
 };
 }  // namespace
+
+__global__ void k_publish(uint64_t* box, uint32_t seq, const unsigned* src) {
+    if (threadIdx.x == 0) publish(box, seq, *src);
+}
+
+// Mailboxes come from a process-wide pool of pinned, device-mapped chunks
+// (never returned to the system): serving opens one handle per job, and a
+// pinned allocation per handle would cost more than the waits it saves.
+namespace {
+struct BoxPool {
+    std::mutex mu;
+    std::vector<std::pair<uint64_t*, uint64_t*>> free;  // (host, device)
+};
+BoxPool& box_pool() {
+    static BoxPool p;
+    return p;
+}
+constexpr int BOX_CHUNK = 256;  // mailboxes per pinned chunk
+}  // namespace
+
+rtrex_status box_open(rtrex_graph h) {
+    if (h->box) return RTREX_OK;
+    BoxPool& bp = box_pool();
+    std::lock_guard<std::mutex> lk(bp.mu);
+    if (bp.free.empty()) {
+        void* p = nullptr;
+        RT_HIP(hipHostMalloc(&p, (uint64_t)BOX_CHUNK * BOX_SLOTS * sizeof(uint64_t),
+                             hipHostMallocMapped | hipHostMallocCoherent |
+                                 hipHostMallocPortable));
+        void* d = nullptr;
+        const hipError_t e = hipHostGetDevicePointer(&d, p, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(p);
+            return hip_fail(e, "hipHostGetDevicePointer", __FILE__, __LINE__);
+        }
+        for (int i = BOX_CHUNK - 1; i >= 0; --i)
+            bp.free.emplace_back(static_cast<uint64_t*>(p) + (uint64_t)i * BOX_SLOTS,
+                                 static_cast<uint64_t*>(d) + (uint64_t)i * BOX_SLOTS);
+    }
+    h->box = bp.free.back().first;
+    h->box_dev = bp.free.back().second;
+    bp.free.pop_back();
+    // sequence numbers start at 1 for every handle; the last owner's stream
+    // was drained when it was destroyed
+    for (int i = 0; i < BOX_SLOTS; ++i) __atomic_store_n(h->box + i, 0ull, __ATOMIC_RELEASE);
+    h->box_seq = 0;
+    return RTREX_OK;
+}
+
+void box_close(rtrex_graph h) {
+    if (!h->box) return;
+    BoxPool& bp = box_pool();
+    std::lock_guard<std::mutex> lk(bp.mu);
+    bp.free.emplace_back(h->box, h->box_dev);
+    h->box = h->box_dev = nullptr;
+}
+
+rtrex_status box_wait(rtrex_graph h, int slot, uint32_t seq, uint32_t* out) {
+    const uint64_t* b = h->box + slot;
+    for (uint32_t it = 1;; ++it) {
+        const uint64_t v = __atomic_load_n(b, __ATOMIC_ACQUIRE);
+        if ((uint32_t)(v >> 32) == seq) {
+            *out = (uint32_t)v;
+            return RTREX_OK;
+        }
+        if ((it & 255) == 0) {  // every few microseconds: is the stream still healthy?
+            const hipError_t e = hipStreamQuery(h->stream);
+            if (e == hipSuccess) {
+                const uint64_t w = __atomic_load_n(b, __ATOMIC_ACQUIRE);
+                if ((uint32_t)(w >> 32) == seq) {
+                    *out = (uint32_t)w;
+                    return RTREX_OK;
+                }
+                return fail(RTREX_EDEVICE, "a kernel ended without publishing its result");
+            }
+            if (e != hipErrorNotReady) return hip_fail(e, "stream query", __FILE__, __LINE__);
+        }
+        __builtin_ia32_pause();
+    }
+}
 
 rtrex_status fail(rtrex_status code, const std::string& msg) {
     set_error(msg);
@@ -339,6 +421,7 @@ void rtrex_graph_destroy(rtrex_graph g) {
     for (void* p : g->allocs) (void)hipFree(p);
     if (g->arena.base && g->arena.owned) (void)hipFree(g->arena.base);
     if (g->cm_pin) (void)hipHostFree(g->cm_pin);
+    box_close(g);  // after the stream drained above
     for (auto& ev : g->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);

@@ -686,16 +686,20 @@ __global__ void __launch_bounds__(T) k_near_check_terms(PeelArgs A) {
 // It also clears the counters the next round appends to (zero2: the other
 // parity's next-frontier and near-list counts) and, after a pull round, the
 // counts of the alive lists just consumed (zero4), which become the next
-// round's output lists: no clearing launch per round.
+// round's output lists: no clearing launch per round.  The next frontier's
+// length goes to the host mailbox at once, so the host sizes the next round
+// while this kernel still runs.
 __global__ void __launch_bounds__(T) k_end_round(GraphDev g,
                                                  const uint32_t* __restrict__ F,
                                                  uint32_t nF,
                                                  const uint32_t* __restrict__ Fn,
                                                  const unsigned* __restrict__ nFnp,
                                                  Counters* ctr, bool terms,
-                                                 unsigned* zero2, unsigned* zero4) {
+                                                 unsigned* zero2, unsigned* zero4,
+                                                 uint64_t* box, uint32_t seq) {
     const uint32_t nFn = *nFnp;
     if (blockIdx.x == 0 && threadIdx.x < 4) {
+        if (threadIdx.x == 0) publish(box, seq, nFn);  // final: every appender has ended
         if (threadIdx.x < 2) zero2[threadIdx.x] = 0;
         if (zero4) zero4[threadIdx.x] = 0;
     }
@@ -1265,6 +1269,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         RT_TRY(dalloc(h, &h->nearst, g.m));
         RT_HIP(hipMemsetAsync(h->nearst, 0, (uint64_t)g.m * 4, s));
     }
+    RT_TRY(box_open(h));
     uint32_t* rc = L.cnt + C_ROUND;
     unsigned long long* inc = reinterpret_cast<unsigned long long*>(L.cnt + C_INC);
     PeelArgs A{};
@@ -1362,7 +1367,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         // counts of the lists just walked (the next round's output lists).
         k_end_round<<<blocks_for(std::max<uint64_t>(nF, h->n_alive), T, 4096), T, 0, s>>>(
             g, F, nF, Fn, A.nFn, h->ctr, terms, rc + 2 * ((round + 1) & 1),
-            pull ? L.akc[0] : nullptr);
+            pull ? L.akc[0] : nullptr, h->box_dev, ++h->box_seq);
         if (pull) {  // ak[1] (now ak[0]): the next pull round's work list
             std::swap(L.ak[0], L.ak[1]);
             std::swap(L.akc[0], L.akc[1]);
@@ -1370,8 +1375,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         ak_valid = pull;
         RT_HIP(hipGetLastError());
         uint32_t nFn = 0;
-        RT_HIP(hipMemcpyAsync(&nFn, A.nFn, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
+        RT_TRY(box_wait(h, 0, h->box_seq, &nFn));
         h->audit.clean_rounds++;
         h->n_alive -= nFn;
         std::swap(F, Fn);
@@ -1387,8 +1391,9 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
     RT_HIP(hipGetLastError());
     if (undec) {  // the only wait after the last round: were any decisions open?
         unsigned nund = 0;
-        RT_HIP(hipMemcpyAsync(&nund, &h->ctr->n_undecided, 4, hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
+        k_publish<<<1, WAVE, 0, s>>>(h->box_dev, ++h->box_seq, &h->ctr->n_undecided);
+        RT_HIP(hipGetLastError());
+        RT_TRY(box_wait(h, 0, h->box_seq, &nund));
         if (nund && h->zt) {
             RT_TRY(exact_replay(h, eps, undec, undec_cap, nund));
             // live triangles again, after the replayed regions
@@ -1422,10 +1427,11 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     unsigned* cnt = L.cnt + C_FRONT;
     k_seed_frontier<<<blocks_for((g.m + SEED_K - 1) / SEED_K, SEED_T, 4096), SEED_T, 0, s>>>(
         g, eps, F, cnt, L.bk, L.cnt + C_BK, terms ? L.ak[0] : nullptr, L.akc[0]);
+    RT_TRY(box_open(h));
+    k_publish<<<1, WAVE, 0, s>>>(h->box_dev, ++h->box_seq, cnt);
     RT_HIP(hipGetLastError());
     uint32_t nF = 0;
-    RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
+    RT_TRY(box_wait(h, 0, h->box_seq, &nF));
     h->n_alive = g.m - nF;
     uint32_t* undec = nullptr;
     RT_TRY(sc.get(&undec, g.m));

@@ -1556,6 +1556,13 @@ rtrex_status resolve_timings(rtrex_graph h) {
         h->tm.clean_ms = ms;
         h->tm_clean_pending = false;
     }
+    if (h->tm_extract_pending) {
+        RT_HIP(hipEventSynchronize(h->ev[8]));
+        float ms = 0;
+        RT_HIP(hipEventElapsedTime(&ms, h->ev[7], h->ev[8]));
+        h->tm.extract_ms = ms;
+        h->tm_extract_pending = false;
+    }
     return RTREX_OK;
 }
 

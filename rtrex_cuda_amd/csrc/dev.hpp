@@ -63,6 +63,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A small result for the host: value and sequence number in one 64-bit
+// system-scope store to pinned host memory (ordered after this thread's
+// earlier writes).  Called by one lane.
+__device__ __forceinline__ void publish(uint64_t* box, uint32_t seq, uint32_t v) {
+    __hip_atomic_store(box, ((uint64_t)seq << 32) | v, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Slot for one item per calling lane in a list whose length is *ctr: one
 // atomic per wave (the lowest active lane reserves for all active lanes).
 // Callable from divergent code; every active lane must call it.
@@ -76,5 +84,8 @@ __device__ __forceinline__ uint32_t wave_append(unsigned* ctr) {
     base = __shfl(base, leader, 64);
     return base + (uint32_t)__popcll(act & below);
 }
+
+// Publishes *src (final: earlier kernels of the stream have ended).
+__global__ void k_publish(uint64_t* box, uint32_t seq, const unsigned* src);
 
 }  // namespace rtrex

@@ -170,18 +170,18 @@ __global__ void __launch_bounds__(T) k_comp_layout(const uint64_t* __restrict__ 
     }
 }
 
-// Scalars read back at the end of an extraction, in one copy: component
-// count, largest component, undecided cascade decisions, error flag, clusters
-// (absent sources read as 0).
+// Scalars read back at the end of an extraction, through the host mailbox
+// (slots 1-5): component count, largest component, undecided cascade
+// decisions, error flag, clusters (absent sources read as 0).
 __global__ void k_final_scalars(const uint32_t* ncomp, const uint32_t* big,
                                 const unsigned* nund, const unsigned* err,
-                                const uint32_t* total, uint32_t* out) {
+                                const uint32_t* total, uint64_t* box, uint32_t seq) {
     if (threadIdx.x == 0) {
-        out[0] = ncomp ? *ncomp : 0u;
-        out[1] = big ? *big : 0u;
-        out[2] = nund ? *nund : 0u;
-        out[3] = *err;
-        out[4] = *total;
+        publish(box + 1, seq, ncomp ? *ncomp : 0u);
+        publish(box + 2, seq, big ? *big : 0u);
+        publish(box + 3, seq, nund ? *nund : 0u);
+        publish(box + 4, seq, *err);
+        publish(box + 5, seq, *total);
     }
 }
 
@@ -889,10 +889,11 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     RT_HIP(hipMemsetAsync(nk_d, 0, sizeof(unsigned), s));
     if (!h->mrank) RT_TRY(dalloc(h, &h->mrank, n));
     k_comp_keys<<<bn, T, 0, s>>>(n, parent, csize, rank, h->cseed, h->mrank, keys, nk_d);
+    RT_TRY(box_open(h));
+    k_publish<<<1, WAVE, 0, s>>>(h->box_dev, ++h->box_seq, nk_d);
     RT_HIP(hipGetLastError());
     uint32_t nk = 0;
-    RT_HIP(hipMemcpyAsync(&nk, nk_d, 4, hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
+    RT_TRY(box_wait(h, 0, h->box_seq, &nk));
     h->audit.components = 0;
     h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
     uint32_t* x_undec_list = nullptr;  // device list of undecided cascade decisions
@@ -1011,19 +1012,20 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                                                              (uint32_t)(ns + 1), bounds_d);
         RT_HIP(hipGetLastError());
     }
-    // the phase's scalar results, gathered on the device: one copy back
-    uint32_t* fin;
-    RT_TRY(sc.get(&fin, 8));
+    // the phase's scalar results, published to the host mailbox
+    RT_TRY(box_open(h));
+    const uint32_t fseq = ++h->box_seq;
     k_final_scalars<<<1, WAVE, 0, s>>>(ncomp_d, big_d, x_undec_list ? &h->ctr->n_undecided : nullptr,
-                                       &h->ctr->error, cidr + n, fin);
-    uint32_t hf[5] = {0, 0, 0, 0, 0};
-    RT_HIP(hipMemcpyAsync(hf, fin, sizeof(hf), hipMemcpyDeviceToHost, s));
+                                       &h->ctr->error, cidr + n, h->box_dev, fseq);
+    RT_HIP(hipGetLastError());
     h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
     if (ns)
         RT_HIP(hipMemcpyAsync(h->seg_bounds.data(), bounds_d, (ns + 1) * 4,
                               hipMemcpyDeviceToHost, s));
     RT_HIP(hipEventRecord(h->ev[8], s));
-    RT_HIP(hipEventSynchronize(h->ev[8]));
+    uint32_t hf[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < 5; ++i) RT_TRY(box_wait(h, 1 + i, fseq, &hf[i]));
+    if (ns) RT_HIP(hipEventSynchronize(h->ev[8]));  // the segment bounds
     const uint32_t ncomp = hf[0], big = hf[1], nu = hf[2], cerr = hf[3], total = hf[4];
     if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
     if (ncomp_d) {
@@ -1037,10 +1039,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                               hipMemcpyDeviceToHost, s));
         RT_HIP(hipStreamSynchronize(s));
     }
-    RT_TRY(resolve_timings(h));
-    float ms = 0;
-    RT_HIP(hipEventElapsedTime(&ms, h->ev[7], h->ev[8]));
-    h->tm.extract_ms = ms;
+    h->tm_extract_pending = true;  // read later (resolve_timings), no wait here
     h->n_clusters = total;
     h->seg_clusters.assign(ns, 0);
     for (size_t q = 0; q < ns; ++q)

@@ -222,10 +222,26 @@ struct rtrex_graph_s {
     // clean times are read once something waits for the stream anyway
     // (resolve_timings), so those phases end without a host round trip.
     hipEvent_t ev[9] = {};
-    bool tm_count_pending = false, tm_clean_pending = false;
+    bool tm_count_pending = false, tm_clean_pending = false, tm_extract_pending = false;
+    // Host mailbox: pinned words a kernel publishes small results into
+    // (value | sequence << 32), polled by the host in place of a copy and a
+    // stream synchronisation (box: host address, box_dev: device address).
+    uint64_t* box = nullptr;
+    uint64_t* box_dev = nullptr;
+    uint32_t box_seq = 0;
 };
 
 namespace rtrex {
+
+// Mailbox slots (rtrex_graph_s::box).
+constexpr int BOX_SLOTS = 8;
+// Takes a mailbox for the handle on first use (box_close gives it back; the
+// handle's stream must have drained).
+rtrex_status box_open(rtrex_graph h);
+void box_close(rtrex_graph h);
+// Waits until slot holds sequence number seq and returns its value; fails if
+// the stream reports an error, or drains without the value arriving.
+rtrex_status box_wait(rtrex_graph h, int slot, uint32_t seq, uint32_t* out);
 
 // Bytes taken from the handle's slab, 256-byte aligned; null if it is full.
 inline void* slab_take(rtrex_graph h, uint64_t bytes) {
