@@ -451,9 +451,20 @@ __global__ void __launch_bounds__(T) k_collect_claims(GraphDev g, const uint32_t
 
 // Slot statuses from the edge statuses, after a peel on stored triangle
 // lists (which writes est only).
+// Four slots per step (one 16-byte load of edge ids, one 4-byte store) when
+// the arrays allow it.
 __device__ __forceinline__ void sync_slot_status(const GraphDev& g, uint64_t t0,
                                                  uint64_t stride) {
-    for (uint64_t j = t0; j < g.nnz; j += stride) g.st[j] = g.est[g.s2e[j]];
+    const bool vec = (((uintptr_t)g.s2e & 15) | ((uintptr_t)g.st & 3)) == 0;
+    const uint64_t n4 = vec ? g.nnz / 4 : 0;
+    const uint4* __restrict__ e4 = reinterpret_cast<const uint4*>(g.s2e);
+    uint32_t* __restrict__ st4 = reinterpret_cast<uint32_t*>(g.st);
+    for (uint64_t q = t0; q < n4; q += stride) {
+        const uint4 e = e4[q];
+        st4[q] = (uint32_t)g.est[e.x] | (uint32_t)g.est[e.y] << 8 | (uint32_t)g.est[e.z] << 16 |
+                 (uint32_t)g.est[e.w] << 24;
+    }
+    for (uint64_t j = 4 * n4 + t0; j < g.nnz; j += stride) g.st[j] = g.est[g.s2e[j]];
 }
 __global__ void __launch_bounds__(T) k_slot_status(GraphDev g) {
     sync_slot_status(g, (uint64_t)blockIdx.x * T + threadIdx.x, (uint64_t)gridDim.x * T);
@@ -762,11 +773,24 @@ __global__ void __launch_bounds__(T) k_post_peel(GraphDev g, TermBuf tb, double 
 __global__ void __launch_bounds__(T) k_survivor_bits(GraphDev g, uint32_t* bits, bool slots) {
     if (slots) sync_slot_status(g, (uint64_t)blockIdx.x * T + threadIdx.x, (uint64_t)gridDim.x * T);
     const uint32_t nw = (g.m + 31) / 32;
+    const bool vec = ((uintptr_t)g.est & 15) == 0;
+    static_assert(ST_ALIVE == 0, "alive status bytes are zero");
     for (uint32_t w = blockIdx.x * T + threadIdx.x; w < nw; w += gridDim.x * T) {
         uint32_t b = 0;
-        for (uint32_t i = 0; i < 32; ++i) {
-            const uint32_t e = w * 32 + i;
-            if (e < g.m && g.est[e] == ST_ALIVE) b |= 1u << i;
+        if (vec && (uint64_t)w * 32 + 32 <= g.m) {  // the word's 32 status bytes in two loads
+            const uint4* q = reinterpret_cast<const uint4*>(g.est) + 2ull * w;
+            const uint4 lo = q[0], hi = q[1];
+            const uint32_t x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    b |= (uint32_t)(((x[k] >> (8 * i)) & 0xffu) == 0u) << (4 * k + i);
+        } else {
+            for (uint32_t i = 0; i < 32; ++i) {
+                const uint32_t e = w * 32 + i;
+                if (e < g.m && g.est[e] == ST_ALIVE) b |= 1u << i;
+            }
         }
         bits[w] = b;
     }
@@ -1446,8 +1470,9 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     }
     RT_TRY(peel(h, eps, F, Fn, nF, L, terms, terms, terms ? undec : nullptr, g.m));
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
-    k_survivor_bits<<<blocks_for(terms ? g.nnz : (g.m + 31) / 32, T, 8192), T, 0, s>>>(
-        g, h->surv, terms);
+    k_survivor_bits<<<blocks_for(std::max<uint64_t>(terms ? g.nnz / 4 : 0, (g.m + 31) / 32), T,
+                                 8192),
+                      T, 0, s>>>(g, h->surv, terms);
     RT_HIP(hipGetLastError());
     RT_HIP(hipEventRecord(h->ev[6], s));
     h->tm_clean_pending = true;  // read later (resolve_timings), no wait here
@@ -1499,7 +1524,7 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
     if (h->xact) RT_HIP(hipMemsetAsync(h->xact, 0, g.m, s));  // supports no longer the reference's
     h->n_alive -= nF;
     RT_TRY(peel(h, eps, F, Fn, nF, L, false, false));
-    if (terms) k_slot_status<<<blocks_for(g.nnz, T, 8192), T, 0, s>>>(g);
+    if (terms) k_slot_status<<<blocks_for(g.nnz / 4 + 1, T, 8192), T, 0, s>>>(g);
     RT_HIP(hipGetLastError());
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;

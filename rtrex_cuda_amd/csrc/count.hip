@@ -1482,7 +1482,6 @@ rtrex_status run_count(rtrex_graph h) {
     const uint32_t* counts = h->cnt_counts;
     uint32_t* elist = h->cnt_lists;
     uint32_t* vlist = elist + (uint64_t)NB * g.m;
-    RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
     auto ep = [&](int b) { return elist + (uint64_t)b * g.m; };
     auto edge_pass = [&](const TermBuf& tb) {
         if (counts[0]) k_count_edges<1><<<grid_for(counts[0], 1), T, 0, s>>>(g, ep(0), counts[0], tb);
@@ -1506,7 +1505,6 @@ rtrex_status run_count(rtrex_graph h) {
         RT_HIP(hipStreamSynchronize(s));
         h->terms_exact_pending = false;
         try_alloc_terms(h, total);
-        RT_HIP(hipMemsetAsync(&h->ctr->tri_total, 0, sizeof(unsigned long long), s));
     }
     const TermBuf tb = h->terms();
 

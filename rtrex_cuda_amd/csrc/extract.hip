@@ -87,12 +87,28 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
     return x;
 }
 
+// Also clears what the extraction's later kernels count or stamp into (one
+// launch instead of a memset each): the neighbour and candidate stamps mk, cm
+// (n each), the largest-component slot big, and the counters nk (vertices in
+// components), n_undecided, work_next and error.
 __global__ void __launch_bounds__(T) k_uf_init(uint32_t n, uint32_t* parent,
-                                               uint32_t* csize, uint32_t* cseed) {
+                                               uint32_t* csize, uint32_t* cseed,
+                                               uint32_t* __restrict__ mk,
+                                               uint32_t* __restrict__ cm, uint32_t* big,
+                                               unsigned* nk, Counters* ctr) {
     for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
         parent[v] = v;
         csize[v] = 0;
         cseed[v] = NONE;
+        mk[v] = 0;
+        cm[v] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *big = 0;
+        *nk = 0;
+        ctr->n_undecided = 0;
+        ctr->work_next = 0;
+        ctr->error = 0;
     }
 }
 
@@ -882,11 +898,14 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     uint32_t* order = h->seed_order;
     uint32_t* rank = h->seed_rank;
     // alive components
-    k_uf_init<<<bn, T, 0, s>>>(n, parent, csize, h->cseed);
+    uint32_t *mk, *cm, *big_d;
+    RT_TRY(sc.get(&mk, n));
+    RT_TRY(sc.get(&cm, n));
+    RT_TRY(sc.get(&big_d, 1));
+    unsigned* nk_d = &h->ctr->frontier_next;
+    k_uf_init<<<bn, T, 0, s>>>(n, parent, csize, h->cseed, mk, cm, big_d, nk_d, h->ctr);
     k_uf_hook<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, parent);
     k_uf_compress<<<bn, T, 0, s>>>(n, parent, csize);
-    unsigned* nk_d = &h->ctr->frontier_next;
-    RT_HIP(hipMemsetAsync(nk_d, 0, sizeof(unsigned), s));
     if (!h->mrank) RT_TRY(dalloc(h, &h->mrank, n));
     k_comp_keys<<<bn, T, 0, s>>>(n, parent, csize, rank, h->cseed, h->mrank, keys, nk_d);
     RT_TRY(box_open(h));
@@ -898,7 +917,6 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     h->audit.largest_component = nk ? 0 : (n ? 1 : 0);
     uint32_t* x_undec_list = nullptr;  // device list of undecided cascade decisions
     const uint32_t* ncomp_d = nullptr;  // component count (device)
-    uint32_t* big_d = nullptr;          // largest component (device)
     if (nk) {
         if (nk <= SB_MAX) k_sort_small<<<1, SB_T, 0, s>>>(keys, nk);
         else RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
@@ -930,17 +948,13 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         RT_TRY(sc.get(&hoff, (uint64_t)nk + 1));
         RT_TRY(sc.get(&hnext, nk));
         RT_TRY(sc.get(&hbkt, (uint64_t)h->bcnt_ratio * nk + 1));
-        RT_TRY(sc.get(&big_d, 1));
-        RT_HIP(hipMemsetAsync(big_d, 0, 4, s));
         k_comp_hsize<<<bk, T, 0, s>>>(cstart, ncomp_d, nk, h->bcnt_dev, hsz, big_d);
         RT_HIP(hipGetLastError());
         RT_TRY(exclusive_scan_u32_32(hsz, hoff, nk, s));
-        uint32_t *mk, *cm, *mem, *cand, *undec;
+        uint32_t *mem, *cand, *undec;
         uint64_t* wl;
         double *tw, *gpv;
         Rec* grec;
-        RT_TRY(sc.get(&mk, n));
-        RT_TRY(sc.get(&cm, n));
         RT_TRY(sc.get(&mem, n));
         RT_TRY(sc.get(&cand, 2ull * n));
         RT_TRY(sc.get(&tw, n));
@@ -948,10 +962,6 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         RT_TRY(sc.get(&grec, n));
         RT_TRY(sc.get(&wl, g.nnz));
         RT_TRY(sc.get(&undec, g.m));
-        RT_HIP(hipMemsetAsync(&h->ctr->n_undecided, 0, sizeof(unsigned), s));
-        RT_HIP(hipMemsetAsync(mk, 0, (uint64_t)n * 4, s));
-        RT_HIP(hipMemsetAsync(cm, 0, (uint64_t)n * 4, s));
-        RT_HIP(hipMemsetAsync(&h->ctr->work_next, 0, sizeof(unsigned), s));
         XArgs a;
         a.g = g;
         a.eps = eps;
@@ -993,7 +1003,6 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     if (!h->cid) RT_TRY(dalloc(h, &h->cid, n));
     k_seed_flags<<<bn, T, 0, s>>>(n, order, h->cseed, flag2);
     RT_TRY(exclusive_scan_u32_32(flag2, cidr, n, s));
-    RT_HIP(hipMemsetAsync(&h->ctr->error, 0, sizeof(unsigned), s));
     k_cluster_ids<<<bn, T, 0, s>>>(n, h->cseed, rank, cidr, h->cid, &h->ctr->error);
     RT_HIP(hipGetLastError());
     const size_t ns = h->seg_vtx.empty() ? 0 : h->seg_vtx.size() - 1;
@@ -1015,7 +1024,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     // the phase's scalar results, published to the host mailbox
     RT_TRY(box_open(h));
     const uint32_t fseq = ++h->box_seq;
-    k_final_scalars<<<1, WAVE, 0, s>>>(ncomp_d, big_d, x_undec_list ? &h->ctr->n_undecided : nullptr,
+    k_final_scalars<<<1, WAVE, 0, s>>>(ncomp_d, ncomp_d ? big_d : nullptr,
+                                       x_undec_list ? &h->ctr->n_undecided : nullptr,
                                        &h->ctr->error, cidr + n, h->box_dev, fseq);
     RT_HIP(hipGetLastError());
     h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
