@@ -1025,6 +1025,63 @@ struct ReplayArgs {
 };
 
 constexpr int RT = 64;  // one wave per workgroup
+// One chunk of a popped edge's triangles (lane k of the chunk holds record
+// q of its list, in = within the list; the statuses, supports and thresholds
+// of the record's two edges loaded by the caller): the alive triangles lose
+// their weight from both other edges, which are pushed in (lower, w) then
+// (higher, w) order when they fall below their threshold.  Returns the
+// number of pushes; *last is the entry on top of the stack afterwards when
+// there were any.
+__device__ __forceinline__ uint32_t replay_chunk(const ReplayArgs& A, uint64_t* stk, uint32_t top,
+                                                 bool in, const Term& q, bool swap,
+                                                 uint32_t lower, uint32_t higher, uint8_t sa,
+                                                 uint8_t sb, double sup_a, double sup_b,
+                                                 double ew_a, double ew_b, uint64_t* last) {
+    const GraphDev& g = A.g;
+    const int lane = lane_id();
+    const uint32_t jl = swap ? q.sb : q.sa, jh = swap ? q.sa : q.sb;
+    const uint8_t sl = swap ? sb : sa, sh = swap ? sa : sb;
+    const double sup_l = swap ? sup_b : sup_a, sup_h = swap ? sup_a : sup_b;
+    const double ew_l = swap ? ew_b : ew_a, ew_h = swap ? ew_a : ew_b;
+    const bool alive = in && sl != ST_DEAD && sh != ST_DEAD;
+    bool pl = false, ph = false;
+    if (alive) {
+        const double t = q.w;
+        const double vl = sup_l - t, vh = sup_h - t;
+        g.sup[jl] = vl;
+        g.sup[jh] = vh;
+        g.touched[jl] = 1;
+        g.touched[jh] = 1;
+        pl = sl != ST_QUEUED && vl < A.eps * ew_l;
+        ph = sh != ST_QUEUED && vh < A.eps * ew_h;
+    }
+    const uint32_t np = (uint32_t)pl + (uint32_t)ph;
+    uint32_t pre = np;  // exclusive prefix of pushes over lanes
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(pre, d, 64);
+        if (lane >= d) pre += o;
+    }
+    const uint32_t tot = __shfl(pre, 63, 64);
+    pre -= np;
+    // Edge statuses only: the slot copies are brought in line once the
+    // clean ends (sync_slot_status).
+    const uint64_t el = ((uint64_t)jl << 32) | lower, eh = ((uint64_t)jh << 32) | higher;
+    if (pl) {
+        stk[top + pre] = el;
+        g.est[jl] = ST_QUEUED;
+    }
+    if (ph) {
+        stk[top + pre + (uint32_t)pl] = eh;
+        g.est[jh] = ST_QUEUED;
+    }
+    if (tot) {  // the new top of the stack, from the highest pushing lane
+        const uint64_t pb = ballot64(np != 0);
+        const int hl = 63 - __clzll((long long)pb);
+        *last = __shfl(ph ? eh : el, hl, 64);
+    }
+    return tot;
+}
+
 __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
     const GraphDev& g = A.g;
     const TermBuf& tb = A.tb;
@@ -1045,66 +1102,62 @@ __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
             const uint64_t bal = ballot64(push);
             if (push) {
                 stk[top + __popcll(bal & lanes_lt())] = ((uint64_t)e << 32) | g.esrc[e];
-                set_status(g, e, ST_QUEUED);
+                g.est[e] = ST_QUEUED;
             }
             top += (uint32_t)__popcll(bal);
         }
         __syncthreads();
+        // The entry on top of the stack when the last pop pushed (registers;
+        // otherwise it is read back).
+        bool have = false;
+        uint64_t next = 0;
         while (top > 0) {
             --top;
-            const uint64_t ent = stk[top];
+            const uint64_t ent = have ? next : stk[top];
+            have = false;
             const uint32_t e = (uint32_t)(ent >> 32), a = (uint32_t)ent;
-            const uint32_t x = g.esrc[e], y = g.nb[g.eslot[e]];
+            // Three waves of loads per pop: what depends on e (and a); the
+            // first chunk's records and e's other endpoint; the degrees and
+            // both record edges' statuses, supports and thresholds.  The
+            // first chunk's indices are clamped to valid ones, so that no
+            // load waits on a branch.
+            const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
+            const uint64_t k0 = tb.zoff[e];
+            const uint32_t da = g.deg[a];
+            const bool in0 = (uint32_t)lane < tc;
+            const Term q0 = tb.t[in0 ? k0 + lane : 0];
+            const uint32_t y = g.nb[sle];
             const uint32_t b = a == x ? y : x;
-            const uint32_t da = g.deg[a], db = g.deg[b];
+            const uint32_t db = g.deg[b];
+            const uint8_t sa0 = g.est[q0.sa], sb0 = g.est[q0.sb];
+            const double supa0 = g.sup[q0.sa], supb0 = g.sup[q0.sb];
+            const double ewa0 = g.ew[q0.sa], ewb0 = g.ew[q0.sb];
             const uint32_t lower = da < db ? a : b, higher = da < db ? b : a;
             // stored terms: sa = {lo, z}, sb = {hi, z}, lo = d_x <= d_y ? x : y
-            const uint32_t lo = g.deg[x] <= g.deg[y] ? x : y;
+            const uint32_t lo = (a == x ? da : db) <= (a == x ? db : da) ? x : y;
             const bool swap = lower != lo;
-            const uint64_t k0 = tb.zoff[e], k1 = k0 + g.tcnt[e];
-            for (uint64_t p = k0; p < k1; p += 64) {
+            uint64_t last = 0;
+            uint32_t pushed = replay_chunk(A, stk, top, in0, q0, swap, lower, higher, sa0, sb0,
+                                           supa0, supb0, ewa0, ewb0, &last);
+            top += pushed;
+            bool any = pushed != 0;
+            const uint64_t k1 = k0 + tc;
+            for (uint64_t p = k0 + 64; p < k1; p += 64) {  // longer lists
                 const uint64_t k = p + lane;
                 const bool in = k < k1;
-                const Term q = in ? tb.t[k] : Term{0.0, 0u, 0u};
-                const uint32_t fa = q.sa, fb = q.sb;
-                const uint32_t jl = swap ? fb : fa, jh = swap ? fa : fb;
-                const uint8_t sl = in ? g.est[jl] : (uint8_t)ST_DEAD;
-                const uint8_t sh = in ? g.est[jh] : (uint8_t)ST_DEAD;
-                const bool alive = in && sl != ST_DEAD && sh != ST_DEAD;
-                bool pl = false, ph = false;
-                if (alive) {
-                    const double t = q.w;
-                    const double vl = g.sup[jl] - t, vh = g.sup[jh] - t;
-                    g.sup[jl] = vl;
-                    g.sup[jh] = vh;
-                    g.touched[jl] = 1;
-                    g.touched[jh] = 1;
-                    pl = sl != ST_QUEUED && vl < A.eps * g.ew[jl];
-                    ph = sh != ST_QUEUED && vh < A.eps * g.ew[jh];
-                }
-                const uint32_t np = (uint32_t)pl + (uint32_t)ph;
-                uint32_t pre = np;  // exclusive prefix of pushes over lanes
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t o = __shfl_up(pre, d, 64);
-                    if (lane >= d) pre += o;
-                }
-                const uint32_t tot = __shfl(pre, 63, 64);
-                pre -= np;
-                if (pl) {
-                    stk[top + pre] = ((uint64_t)jl << 32) | lower;
-                    set_status(g, jl, ST_QUEUED);
-                }
-                if (ph) {
-                    stk[top + pre + (uint32_t)pl] = ((uint64_t)jh << 32) | higher;
-                    set_status(g, jh, ST_QUEUED);
-                }
-                top += tot;
-                __syncthreads();
+                const Term q = tb.t[in ? k : 0];
+                const uint32_t pushed2 = replay_chunk(
+                    A, stk, top, in, q, swap, lower, higher, g.est[q.sa], g.est[q.sb],
+                    g.sup[q.sa], g.sup[q.sb], g.ew[q.sa], g.ew[q.sb], &last);
+                top += pushed2;
+                any = any || pushed2 != 0;
             }
             if (lane == 0) {
-                set_status(g, e, ST_DEAD);
+                g.est[e] = ST_DEAD;
                 g.sup[e] = 0.0;
             }
+            have = any;
+            next = last;
             __syncthreads();
         }
     }
