@@ -50,6 +50,7 @@ constexpr int T = 256;
 constexpr int XT = 64;  // one wave per workgroup
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr uint32_t LDS_CAND = 1024;
+constexpr uint32_t XSTAGE = 2048;  // longest N(hi) the inner loop stages in LDS
 
 using isort::Rec;
 
@@ -296,14 +297,18 @@ struct XArgs {
 // An edge whose post-clean support is the reference's own value (xact) has,
 // in the reference's cascade order, the reference's value here too: its
 // comparison is taken as is.
-__device__ bool decide_cheap(const XArgs& A, uint32_t f, double v, double thr, bool& need) {
-    const GraphDev& g = A.g;
-    if (A.xact && A.xact[f]) return v < thr;
-    const double cb = cheap_bound(g.tcnt[f], g.sup0[f]);
+__device__ __forceinline__ bool decide_cheap_v(bool exact, uint32_t tcnt, double sup0, double v,
+                                               double thr, bool& need) {
+    if (exact) return v < thr;
+    const double cb = cheap_bound(tcnt, sup0);
     if (v < thr - cb) return true;
     if (v >= thr + cb) return false;
     need = true;
     return false;
+}
+__device__ bool decide_cheap(const XArgs& A, uint32_t f, double v, double thr, bool& need) {
+    const GraphDev& g = A.g;
+    return decide_cheap_v(A.xact && A.xact[f], g.tcnt[f], g.sup0[f], v, thr, need);
 }
 
 // Outcome c of a certification; an undecided one is recorded for an exact
@@ -347,6 +352,170 @@ __device__ void wave_bitonic(uint32_t* a, uint32_t np) {
     }
 }
 
+// Status change of edge e whose two slots (sl, its mirror sp) are known.
+__device__ __forceinline__ void set_status_at(const GraphDev& g, uint32_t e, uint32_t sl,
+                                              uint32_t sp, uint8_t c) {
+    g.est[e] = c;
+    g.st[sl] = c;
+    g.st[sp] = c;
+}
+
+// The cascade of one cluster (deleteAndClean, Decomposition.h:147-283) over
+// stored triangle lists: the LIFO worklist wl[ebase, ebase + top) popped
+// edge by edge, in the reference's order.  Per pop the loads come in five
+// waves (the popped edge's data; its other endpoint, mirror slot and first
+// chunk of triangle records, indices clamped to valid ones so that no load
+// waits on a branch; the degrees and the records' edges' statuses, supports
+// and decision inputs; the third vertices and mirror slots; the third
+// vertices' supports), and the entry on top of the stack is carried in
+// registers when the pop pushed.
+__device__ void cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
+                              unsigned long long& amb_c, unsigned long long& n_tri,
+                              unsigned long long& n_edges, double* s_ct, uint32_t* s_cn) {
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    bool have = false;
+    uint64_t next = 0;
+    while (top > 0) {
+        --top;
+        const uint64_t ent = have ? next : A.wl[ebase + top];
+        have = false;
+        const uint32_t e = (uint32_t)(ent >> 32), pa = (uint32_t)ent;
+        const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
+        const uint64_t c0 = tb.zoff[e];
+        const uint32_t dpa = g.deg[pa];
+        const uint32_t y = g.nb[sle];
+        const uint32_t spe = g.part[sle];
+        double px = g.pv[x];
+        const uint32_t pb = pa == x ? y : x;
+        const uint32_t dpb = g.deg[pb];
+        double py = g.pv[y];
+        const uint32_t dx = pa == x ? dpa : dpb, dy = pa == x ? dpb : dpa;
+        // the stored lists' lower side: d_x <= d_y ? x : y; the reference's
+        // lower endpoint of the pair (pa, pb): smaller degree, pb on a tie
+        const uint32_t tlo = dx <= dy ? x : y;
+        const uint32_t rlo = dpa < dpb ? pa : pb;
+        const uint32_t rhi = rlo == pa ? pb : pa;
+        const bool swp = rlo != tlo;
+        uint64_t last = 0;
+        bool any = false;
+        const uint64_t c1 = c0 + tc;
+        for (uint64_t p = c0; p < c1; p += XT) {
+            const uint64_t kk = p + lane;
+            const bool in = kk < c1;
+            const Term q = tb.t[in ? kk : 0];
+            const uint32_t fj = q.sa, fh = q.sb;
+            const uint8_t sj = g.est[fj], sh = g.est[fh];
+            const uint32_t aj = g.esrc[fj], slj = g.eslot[fj], slh = g.eslot[fh];
+            const double supj = g.sup[fj], suph = g.sup[fh];
+            const double ewj = g.ew[fj], ewh = g.ew[fh];
+            const bool xj = A.xact && A.xact[fj], xh = A.xact && A.xact[fh];
+            const uint32_t tcj = g.tcnt[fj], tch = g.tcnt[fh];
+            const double s0j = g.sup0[fj], s0h = g.sup0[fh];
+            const uint32_t wj = g.nb[slj];
+            const uint32_t spj = g.part[slj], sph = g.part[slh];
+            const bool hit = in && sj != ST_DEAD && sh != ST_DEAD;
+            const uint32_t w = aj == tlo ? wj : aj;  // third vertex
+            const double pvw = g.pv[w];
+            const double t = q.w;
+            uint64_t hb = ballot64(hit);
+            n_tri += (lane == 0) ? (unsigned long long)__popcll(hb) : 0ull;
+            while (hb) {  // the popped edge's endpoints, in w order
+                const int bl = __ffsll((unsigned long long)hb) - 1;
+                const double tv = __shfl(t, bl, 64);
+                px -= tv;
+                py -= tv;
+                hb &= hb - 1;
+            }
+            // fj is the edge (tlo, w), fh the other side's; the reference's
+            // (lower, w) is fj unless its lower endpoint is the other one
+            const uint32_t jl = swp ? fh : fj, jh = swp ? fj : fh;
+            const uint8_t sl = swp ? sh : sj, shh = swp ? sj : sh;
+            const double supl = swp ? suph : supj, suph2 = swp ? supj : suph;
+            const double ewl = swp ? ewh : ewj, ewh2 = swp ? ewj : ewh;
+            const bool xl = swp ? xh : xj, xh2 = swp ? xj : xh;
+            const uint32_t tcl = swp ? tch : tcj, tch2 = swp ? tcj : tch;
+            const double s0l = swp ? s0h : s0j, s0h2 = swp ? s0j : s0h;
+            bool push1 = false, push2 = false, need1 = false, need2 = false;
+            double v1 = 0.0, v2 = 0.0, thr1 = 0.0, thr2 = 0.0;
+            if (hit) {
+                g.pv[w] = pvw - t;
+                // (lower, w) then (higher, w), as Decomposition.h:238-264.
+                // Queued edges die anyway; their supports are not kept.
+                if (sl == ST_ALIVE) {
+                    v1 = supl - t;
+                    g.sup[jl] = v1;
+                    g.touched[jl] = 1;
+                    thr1 = A.eps * ewl;
+                    push1 = decide_cheap_v(xl, tcl, s0l, v1, thr1, need1);
+                }
+                if (shh == ST_ALIVE) {
+                    v2 = suph2 - t;
+                    g.sup[jh] = v2;
+                    g.touched[jh] = 1;
+                    thr2 = A.eps * ewh2;
+                    push2 = decide_cheap_v(xh2, tch2, s0h2, v2, thr2, need2);
+                }
+            }
+            // The decisions the cheap bound leaves open read statuses only
+            // (no support written above), so they may follow all of the
+            // chunk's updates: one edge at a time by the whole wave.
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                uint64_t nb = ballot64(side == 0 ? need1 : need2);
+                while (nb) {  // wave-uniform
+                    const int bl = __ffsll((unsigned long long)nb) - 1;
+                    nb &= nb - 1;
+                    const uint32_t f = __shfl(side == 0 ? jl : jh, bl, 64);
+                    const double th = __shfl(side == 0 ? thr1 : thr2, bl, 64);
+                    const int cc = certify_terms_wave(g, tb, f, th, false, e, s_ct, s_cn);
+                    if (lane == bl) {
+                        if (side == 0) push1 = settle(A, f, v1, th, cc, amb_c);
+                        else push2 = settle(A, f, v2, th, cc, amb_c);
+                    }
+                }
+            }
+            const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
+            uint32_t pre = np2;  // exclusive prefix of pushes over lanes
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(pre, d, 64);
+                if (lane >= d) pre += o;
+            }
+            const uint32_t tot = __shfl(pre, 63, 64);
+            pre -= np2;
+            const uint64_t e1 = ((uint64_t)jl << 32) | rlo, e2 = ((uint64_t)jh << 32) | rhi;
+            // slots of jl and jh: (slj, spj) belong to fj, (slh, sph) to fh
+            if (push1) {
+                A.wl[ebase + top + pre] = e1;
+                set_status_at(g, jl, swp ? slh : slj, swp ? sph : spj, ST_QUEUED);
+            }
+            if (push2) {
+                A.wl[ebase + top + pre + (uint32_t)push1] = e2;
+                set_status_at(g, jh, swp ? slj : slh, swp ? spj : sph, ST_QUEUED);
+            }
+            if (tot) {  // the new top of the stack, from the highest pushing lane
+                const uint64_t pbm = ballot64(np2 != 0);
+                const int hl = 63 - __clzll((long long)pbm);
+                last = __shfl(push2 ? e2 : e1, hl, 64);
+                any = true;
+            }
+            top += tot;
+            __syncthreads();
+        }
+        if (lane == 0) {
+            g.pv[x] = px;
+            g.pv[y] = py;
+            set_status_at(g, e, sle, spe, ST_DEAD);
+            g.sup[e] = 0.0;
+        }
+        n_edges++;
+        have = any;
+        next = last;
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ Rec srec[LDS_CAND];
     __shared__ double spv[LDS_CAND];
@@ -355,6 +524,8 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ int s_maxind;
     __shared__ double s_ct[kMaxPerm];  // certification: gathered dead weights
     __shared__ uint32_t s_cn;
+    __shared__ uint32_t s_hn[XSTAGE];  // inner loop: the pair's N(hi) ...
+    __shared__ uint8_t s_ha[XSTAGE];   // ... and its alive flags
     const GraphDev& g = A.g;
     const int lane = lane_id();
     unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
@@ -405,30 +576,73 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             double internal = 0.0;
             uint32_t nc = 0;
             const uint64_t cb = 2ull * start;
+            // Up to 64 alive neighbours: ids, degrees and list bounds one per
+            // lane, broadcast per pair (else read per pair).
+            const bool regs = nA <= (uint32_t)XT;
+            uint32_t r_v = 0, r_d = 0, r_o0 = 0, r_o1 = 0;
+            if (regs && (uint32_t)lane < nA) {
+                r_v = A.mem[start + 1 + lane];
+                r_d = g.deg[r_v];
+                r_o0 = g.off[r_v];
+                r_o1 = g.off[r_v + 1];
+            }
             for (uint32_t a = 0; a < nA; ++a) {
-                const uint32_t v = A.mem[start + 1 + a];
-                const uint32_t dv = g.deg[v];
+                const uint32_t v = regs ? __shfl(r_v, (int)a, 64) : A.mem[start + 1 + a];
+                const uint32_t dv = regs ? __shfl(r_d, (int)a, 64) : g.deg[v];
                 if ((double)dv > lim) continue;
+                const uint32_t v0 = regs ? __shfl(r_o0, (int)a, 64) : g.off[v];
+                const uint32_t v1 = regs ? __shfl(r_o1, (int)a, 64) : g.off[v + 1];
                 for (uint32_t b = a; b < nA; ++b) {
-                    const uint32_t w = A.mem[start + 1 + b];
-                    const uint32_t dw = g.deg[w];
+                    const uint32_t w = regs ? __shfl(r_v, (int)b, 64) : A.mem[start + 1 + b];
+                    const uint32_t dw = regs ? __shfl(r_d, (int)b, 64) : g.deg[w];
                     if ((double)dw > lim) continue;
+                    const uint32_t w0 = regs ? __shfl(r_o0, (int)b, 64) : g.off[w];
+                    const uint32_t w1 = regs ? __shfl(r_o1, (int)b, 64) : g.off[w + 1];
                     internal += tri_w(du, dv, dw);
                     const uint32_t lo = dv < dw ? v : w, hi = dv < dw ? w : v;
-                    uint32_t p = g.off[lo];
-                    const uint32_t le = g.off[lo + 1];
-                    uint32_t hlo = g.off[hi];
-                    const uint32_t he = g.off[hi + 1];
-                    for (; p < le; p += XT) {
+                    uint32_t p = dv < dw ? v0 : w0;
+                    const uint32_t le = dv < dw ? v1 : w1;
+                    uint32_t hlo = dv < dw ? w0 : v0;
+                    const uint32_t he = dv < dw ? w1 : v1;
+                    // N(hi) and its alive flags staged in LDS when short
+                    // enough (the first chunk of N(lo) loaded alongside), so
+                    // that each lane's search reads LDS; else searched in HBM.
+                    const uint32_t dh = he - hlo;
+                    const bool staged = dh <= XSTAGE;
+                    uint32_t x0 = 0;
+                    uint8_t st0 = ST_DEAD;
+                    if (staged) {
+                        if (p + lane < le) {
+                            x0 = g.nb[p + lane];
+                            st0 = g.st[p + lane];
+                        }
+                        for (uint32_t i = lane; i < dh; i += XT) {
+                            s_hn[i] = g.nb[hlo + i];
+                            s_ha[i] = g.st[hlo + i] != ST_DEAD;
+                        }
+                        wave_sync();
+                    }
+                    for (bool first = true; p < le; p += XT, first = false) {
                         const uint32_t l = p + lane;
                         const bool in = l < le;
-                        const uint32_t x = in ? g.nb[l] : 0;
-                        const bool ok = in && g.st[l] != ST_DEAD && x != lo &&
-                                        x != hi && x != u;
-                        const uint32_t pos = in ? lower_bound(g.nb, hlo, he, x) : hlo;
-                        const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
-                        const bool hit = ok && pos < he && g.nb[pos] == x &&
-                                         g.st[pos] != ST_DEAD;
+                        const uint32_t x = !in ? 0 : staged && first ? x0 : g.nb[l];
+                        const uint8_t sx = !in ? (uint8_t)ST_DEAD : staged && first ? st0 : g.st[l];
+                        const bool ok = in && sx != ST_DEAD && x != lo && x != hi && x != u;
+                        bool hit;
+                        if (staged) {
+                            uint32_t a0 = 0, a1 = dh;  // lower bound of x in the staged list
+                            while (a0 < a1) {
+                                const uint32_t mid = (a0 + a1) >> 1;
+                                if (s_hn[mid] < x) a0 = mid + 1;
+                                else a1 = mid;
+                            }
+                            hit = ok && a0 < dh && s_hn[a0] == x && s_ha[a0];
+                        } else {
+                            const uint32_t pos = in ? lower_bound(g.nb, hlo, he, x) : hlo;
+                            const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
+                            hit = ok && pos < he && g.nb[pos] == x && g.st[pos] != ST_DEAD;
+                            hlo = nh;
+                        }
                         const double t = hit ? tri_w(g.deg[x], dv, dw) : 0.0;
                         const bool isn = hit && A.mk[x] == stamp;
                         uint64_t ib = ballot64(isn && x > v && x > w);
@@ -448,7 +662,6 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                             A.tw[x] += t;
                         }
                         nc += (uint32_t)__popcll(fb);
-                        hlo = nh;
                     }
                     __syncthreads();  // candidate weights of this pair are final
                 }
@@ -558,6 +771,9 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             }
 
             // ... and cascade (deleteAndClean, Decomposition.h:147-283)
+            if (A.tb.t) {
+                cascade_terms(A, ebase, top, amb_c, n_tri, n_edges, s_ct, &s_cn);
+            } else {
             while (top > 0) {
                 --top;
                 const uint64_t ent = A.wl[ebase + top];
@@ -722,6 +938,7 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 }
                 n_edges++;
                 __syncthreads();
+            }
             }
             __syncthreads();
         }
