@@ -133,7 +133,6 @@ __device__ __forceinline__ void group_fold(bool hit, double t, double& S,
         }
     }
 }
-
 // One step of a chunked intersection: lanes of a group take G consecutive
 // ids L[p .. p+G) of the streamed list and look each up in H[hlo, he).  On
 // return hlo has advanced to the insertion point of the chunk's last id.
@@ -738,10 +737,20 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
                 const bool last = act && (lane == WAVE - 1 || f + 1 == W || knext != k);
                 wave_sync();
                 if (last) {
-                    // the run's matches are s_t[nb_below - rank, nb_below + hit)
+                    // the run's matches are s_t[nb_below - rank, nb_below + hit),
+                    // read four at a time and added in order
                     double S = s_S[wid][kk];
                     const uint32_t b = nb_below + (hit ? 1u : 0u);
-                    for (uint32_t i = nb_below - rank; i < b; ++i) S += s_t[wid][i];
+                    uint32_t i = nb_below - rank;
+                    for (; i + 4 <= b; i += 4) {
+                        const double a0 = s_t[wid][i], a1 = s_t[wid][i + 1];
+                        const double a2 = s_t[wid][i + 2], a3 = s_t[wid][i + 3];
+                        S += a0;
+                        S += a1;
+                        S += a2;
+                        S += a3;
+                    }
+                    for (; i < b; ++i) S += s_t[wid][i];
                     s_S[wid][kk] = S;
                     s_cnt[wid][kk] = c0 + rank + (hit ? 1u : 0u);
                 }
