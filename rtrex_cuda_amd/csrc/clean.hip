@@ -1118,14 +1118,14 @@ __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
             const uint32_t e = (uint32_t)(ent >> 32), a = (uint32_t)ent;
             // Three waves of loads per pop: what depends on e (and a); the
             // first chunk's records and e's other endpoint; the degrees and
-            // both record edges' statuses, supports and thresholds.  The
-            // first chunk's indices are clamped to valid ones, so that no
+            // both record edges' statuses, supports and thresholds.  Lanes
+            // past the list take a blank record (term_or_blank), so that no
             // load waits on a branch.
             const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
             const uint64_t k0 = tb.zoff[e];
             const uint32_t da = g.deg[a];
             const bool in0 = (uint32_t)lane < tc;
-            const Term q0 = tb.t[in0 ? k0 + lane : 0];
+            const Term q0 = term_or_blank(tb.t, k0 + lane, in0);
             const uint32_t y = g.nb[sle];
             const uint32_t b = a == x ? y : x;
             const uint32_t db = g.deg[b];
@@ -1145,7 +1145,7 @@ __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
             for (uint64_t p = k0 + 64; p < k1; p += 64) {  // longer lists
                 const uint64_t k = p + lane;
                 const bool in = k < k1;
-                const Term q = tb.t[in ? k : 0];
+                const Term q = term_or_blank(tb.t, k, in);
                 const uint32_t pushed2 = replay_chunk(
                     A, stk, top, in, q, swap, lower, higher, g.est[q.sa], g.est[q.sb],
                     g.sup[q.sa], g.sup[q.sb], g.ew[q.sa], g.ew[q.sb], &last);

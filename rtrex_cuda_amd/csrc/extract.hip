@@ -364,8 +364,8 @@ __device__ __forceinline__ void set_status_at(const GraphDev& g, uint32_t e, uin
 // stored triangle lists: the LIFO worklist wl[ebase, ebase + top) popped
 // edge by edge, in the reference's order.  Per pop the loads come in five
 // waves (the popped edge's data; its other endpoint, mirror slot and first
-// chunk of triangle records, indices clamped to valid ones so that no load
-// waits on a branch; the degrees and the records' edges' statuses, supports
+// chunk of triangle records, lanes past the list with a blank record so that
+// no load waits on a branch; the degrees and the records' edges' statuses, supports
 // and decision inputs; the third vertices and mirror slots; the third
 // vertices' supports), and the entry on top of the stack is carried in
 // registers when the pop pushed.
@@ -404,7 +404,7 @@ __device__ void cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
         for (uint64_t p = c0; p < c1; p += XT) {
             const uint64_t kk = p + lane;
             const bool in = kk < c1;
-            const Term q = tb.t[in ? kk : 0];
+            const Term q = term_or_blank(tb.t, kk, in);
             const uint32_t fj = q.sa, fh = q.sb;
             const uint8_t sj = g.est[fj], sh = g.est[fh];
             const uint32_t aj = g.esrc[fj], slj = g.eslot[fj], slh = g.eslot[fh];

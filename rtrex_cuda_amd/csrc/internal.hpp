@@ -100,6 +100,17 @@ struct __attribute__((aligned(16))) SupRec {
     double s;
     uint32_t c, pad;
 };
+// Record k of the term buffer for lane-uniform code: every lane loads (k is
+// an in-range index when in is set, and entry 0 otherwise: the buffer always
+// holds at least one entry), but a lane past its list gets the record of edge
+// 0 with weight 0, never the entry it read.  Entries past an edge's matches
+// are capacity, not records, and may hold anything (entry 0 included, when
+// edge 0 has no triangle).
+__device__ __forceinline__ Term term_or_blank(const Term* t, uint64_t k, bool in) {
+    const Term q = t[in ? k : 0];
+    return in ? q : Term{0.0, 0u, 0u};
+}
+
 struct TermBuf {
     Term* t;  // null when the buffer did not fit (intersection fallbacks)
     const uint64_t* zoff;

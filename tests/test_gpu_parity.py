@@ -294,3 +294,28 @@ def test_cluster_map_kept_in_hbm():
         assert np.array_equal(flat, np.concatenate(maps))
     finally:
         b.close()
+
+
+def test_fresh_handles_small_graphs_after_a_large_one():
+    """Regression: a lane past the end of an edge's triangle list must not
+    use the term buffer's entry 0 as a record -- that entry is capacity, never
+    written, when edge 0 has no triangle.  Small graphs on fresh handles
+    (first call without a scratch arena), after a larger graph has used and
+    released device memory, through rtrex_decompose, against the oracle; the
+    last case is built so that edge {0, 1} has room for a triangle and none."""
+    with R.DeviceGraph.from_edges(*G.config2()) as g:
+        g.decompose(0.3)
+    cases = [G.config4_graph(i) for i in range(0, 64, 8)]
+    n, u, v = G.config4_graph(3)
+    u2 = np.concatenate([np.array([0, 0, 1, 2], np.uint32), u + 3]).astype(np.uint32)
+    v2 = np.concatenate([np.array([1, 2, 3, 4], np.uint32), v + 3]).astype(np.uint32)
+    cases.append((n + 3, u2, v2))
+    for i, (n, u, v) in enumerate(cases):
+        print(f"[gpu] fresh handle, small graph {i}: n={n} m={len(u)}", flush=True)
+        off, nb = O.csr_from_undirected(n, u, v)
+        d = O.decompose(off, nb, 0.3, with_clusters=True)
+        with R.DeviceGraph.from_edges(n, u, v) as g:
+            cm, k = g.decompose(0.3)
+            assert g.audit()["ambiguous_clean"] == 0
+        assert k == d.n_clusters
+        assert np.array_equal(cm, d.cluster_map)
