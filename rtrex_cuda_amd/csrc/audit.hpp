@@ -48,24 +48,6 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s,
     e = (a - (s - bb)) + (b - bb);
 }
 
-// Next lexicographic permutation of a[0..k) (ascending start); false at end.
-__device__ __forceinline__ bool next_perm(double* a, int k) {
-    int i = k - 2;
-    while (i >= 0 && !(a[i] < a[i + 1])) --i;
-    if (i < 0) return false;
-    int j = k - 1;
-    while (!(a[i] < a[j])) --j;
-    double t = a[i];
-    a[i] = a[j];
-    a[j] = t;
-    for (int l = i + 1, r = k - 1; l < r; ++l, --r) {
-        t = a[l];
-        a[l] = a[r];
-        a[r] = t;
-    }
-    return true;
-}
-
 // Decision from E = s0 - sum(D) held as hi + lo (hi + lo_exact equal to E
 // exactly, lo's own additions rounded; at most nd + 16 of them, each off by
 // at most 2^-53 |lo| <= 2^-53 (nd + 16) ulp(s0) / 2), with nd = |D|: KEEP or
@@ -137,10 +119,47 @@ __device__ __forceinline__ int enumerate_unrolled(double s0, double thr, const d
     return seen_lt && seen_ge ? CERT_UNDECIDED : seen_lt ? CERT_REMOVE : CERT_KEEP;
 }
 
+// Combines two outcomes over disjoint sets of orders.
+__device__ __forceinline__ int join_outcomes(int a, int b) {
+    if (a < 0) return b;
+    return a == b ? a : CERT_UNDECIDED;
+}
+
+// x[i] for a run-time i < N, by selects (no run-time register index).
+template <int N>
+__device__ __forceinline__ double pick(const double* x, int i) {
+    double r = x[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j) r = i == j ? x[j] : r;
+    return r;
+}
+
+// Next lexicographic permutation of a[0..k) (ascending start); false at end.
+__device__ __forceinline__ bool next_perm(double* a, int k) {
+    int i = k - 2;
+    while (i >= 0 && !(a[i] < a[i + 1])) --i;
+    if (i < 0) return false;
+    int j = k - 1;
+    while (!(a[i] < a[j])) --j;
+    double t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+    for (int l = i + 1, r = k - 1; l < r; ++l, --r) {
+        t = a[l];
+        a[l] = a[r];
+        a[r] = t;
+    }
+    return true;
+}
+
 // Outcome over every order of subtracting src[0..nd) (nd <= kMaxPerm) from s0
-// with the reference's plain double subtractions.  Up to five values run
-// unrolled over a table of orders; more are sorted and stepped through their
-// distinct orders (next_perm).
+// with the reference's plain double subtractions.  FAST: six to eight values
+// go through run-time loops over the first values subtracted (the running
+// value after them is the reference's own), down to five left, whose 120
+// orders run unrolled -- every index static.  Otherwise six or more values
+// are sorted and stepped through their distinct orders, in few registers (for
+// kernels that stream, where these cases are rare).
+template <bool FAST>
 __device__ inline int enumerate_orders(double s0, double thr, const double* src, int nd) {
     switch (nd) {
         case 0: return s0 < thr ? CERT_REMOVE : CERT_KEEP;
@@ -152,7 +171,38 @@ __device__ inline int enumerate_orders(double s0, double thr, const double* src,
         default: break;
     }
     double t[kMaxPerm];
-    for (int i = 0; i < nd; ++i) t[i] = src[i];
+#pragma unroll
+    for (int i = 0; i < kMaxPerm; ++i) t[i] = i < nd ? src[i] : 0.0;
+    if (FAST) {
+        int out = -1;
+        for (int a = 0; a < nd && out != CERT_UNDECIDED; ++a) {
+            double q[7];  // the nd - 1 values after the first
+#pragma unroll
+            for (int j = 0; j < 7; ++j) q[j] = j < a ? t[j] : t[j + 1];
+            const double sa = s0 - pick<8>(t, a);
+            if (nd == 6) {
+                out = join_outcomes(out, enumerate_unrolled<5>(sa, thr, q));
+                continue;
+            }
+            for (int b = 0; b < nd - 1 && out != CERT_UNDECIDED; ++b) {
+                double w[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) w[j] = j < b ? q[j] : q[j + 1];
+                const double sb = sa - pick<7>(q, b);
+                if (nd == 7) {
+                    out = join_outcomes(out, enumerate_unrolled<5>(sb, thr, w));
+                    continue;
+                }
+                for (int c = 0; c < 6 && out != CERT_UNDECIDED; ++c) {  // nd == 8
+                    double x[5];
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) x[j] = j < c ? w[j] : w[j + 1];
+                    out = join_outcomes(out, enumerate_unrolled<5>(sb - pick<6>(w, c), thr, x));
+                }
+            }
+        }
+        return out;
+    }
     for (int i = 1; i < nd; ++i)  // insertion sort ascending
         for (int j = i; j > 0 && t[j] < t[j - 1]; --j) {
             const double x = t[j];
@@ -188,7 +238,7 @@ struct CertAcc {
     __device__ int finish(double s0, double thr) {
         const int c = dd_decide(hi, lo, (uint32_t)nd, s0, thr);
         if (c != CERT_UNDECIDED || nd > kMaxPerm) return c;
-        return enumerate_orders(s0, thr, t8, nd);
+        return enumerate_orders<false>(s0, thr, t8, nd);
     }
 };
 
@@ -304,7 +354,7 @@ __device__ inline int certify_terms_wave(const GraphDev& g, const TermBuf& tb, u
     two_sum(s0, hi, sn, e);
     int c = dd_decide(sn, lo + e, nd, s0, thr);
     if (c == CERT_UNDECIDED && nd <= (uint32_t)kMaxPerm)
-        c = enumerate_orders(s0, thr, s_t, (int)nd);
+        c = enumerate_orders<true>(s0, thr, s_t, (int)nd);
     wave_sync();  // s_t and s_n are read before the next call resets them
     return c;
 }

@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(T) k_near_check_terms(PeelArgs A) {
         two_sum(s0, hi, sn, e);
         int c = dd_decide(sn, lo + e, nd, s0, thr);
         if (c == CERT_UNDECIDED && nd <= (uint32_t)kMaxPerm)
-            c = enumerate_orders(s0, thr, s_t[grp], (int)nd);
+            c = enumerate_orders<true>(s0, thr, s_t[grp], (int)nd);
         if (c == CERT_UNDECIDED) {
             amb++;
             note_undecided(A.undec, A.undec_cap, A.ctr, f);
