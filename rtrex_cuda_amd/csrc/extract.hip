@@ -752,18 +752,33 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             __syncthreads();
             if (lane == 0) uset_order(ins, nmem, A.bcnt, A.hnext + start, A.hbkt + A.hoff[c], ord);
             __syncthreads();
+            // Up to 64 members: ids and list bounds one per lane, read once.
+            // A member's statuses are read after the earlier members' writes
+            // (an edge queued from one end is not queued again from the
+            // other); the queued edge's slots are j and its mirror part[j].
+            const bool mregs = nmem <= (uint32_t)XT;
+            uint32_t m_z = 0, m_b = 0, m_e = 0;
+            if (mregs && (uint32_t)lane < nmem) {
+                m_z = ord[lane];
+                m_b = g.off[m_z];
+                m_e = g.off[m_z + 1];
+            }
             for (uint32_t q = 0; q < nmem; ++q) {
-                const uint32_t z = ord[q];
-                const uint32_t zb = g.off[z], ze = g.off[z + 1];
+                const uint32_t z = mregs ? __shfl(m_z, (int)q, 64) : ord[q];
+                const uint32_t zb = mregs ? __shfl(m_b, (int)q, 64) : g.off[z];
+                const uint32_t ze = mregs ? __shfl(m_e, (int)q, 64) : g.off[z + 1];
                 for (uint32_t p = zb; p < ze; p += XT) {
                     const uint32_t j = p + lane;
-                    const bool al = j < ze && g.st[j] == ST_ALIVE;
+                    const bool in = j < ze;
+                    const uint8_t sj = in ? g.st[j] : (uint8_t)ST_DEAD;
+                    const uint32_t f = in ? g.s2e[j] : 0u;
+                    const uint32_t pj = in ? g.part[j] : 0u;
+                    const bool al = sj == ST_ALIVE;
                     const uint64_t bal = ballot64(al);
                     if (al) {
-                        const uint32_t f = g.s2e[j];
                         A.wl[ebase + top + __popcll(bal & lanes_below())] =
                             ((uint64_t)f << 32) | z;  // the pair (z, nbor)
-                        set_status(g, f, ST_QUEUED);
+                        set_status_at(g, f, j, pj, ST_QUEUED);
                     }
                     top += (uint32_t)__popcll(bal);
                 }

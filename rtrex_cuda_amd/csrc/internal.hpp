@@ -343,6 +343,10 @@ struct Scratch {
     Arena* a = tl_arena;
     uint64_t mark = a ? a->top : 0, vmark = a ? a->vtop : 0;
     ~Scratch() {
+        // Kernels queued on this buffer may still run (callers do not wait
+        // for their last launches): buffers of their own are freed only once
+        // the device has drained.  Arena scratch is reused in stream order.
+        if (!ptrs.empty()) (void)hipDeviceSynchronize();
         for (void* p : ptrs) (void)hipFree(p);
         if (a) {
             a->top = mark;
