@@ -257,8 +257,10 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, max(1, args.steps))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of "
-              "it (each graph / job fits one GPU; see DESIGN.md for how it would shard)",
+        print("bench.py: measures one MI355X only; runs on several GPUs are not supported "
+              "(each graph or job fits one GPU; a larger graph would have its edges partitioned "
+              "across GPUs, with the clean's support decrements exchanged once per peel round, "
+              "which is not implemented)",
               file=sys.stderr)
         return 2
 
