@@ -318,6 +318,10 @@ __device__ __forceinline__ void pull_body(const PeelArgs& A, const uint32_t* __r
         const uint8_t se = in ? g.est[e] : (uint8_t)ST_DEAD;
         const uint64_t z = in ? tb.zoff[e] : 0;
         const uint32_t tc = in ? g.tcnt[e] : 0u;
+        // the decision's inputs, loaded with the list bounds (they depend on
+        // e only) rather than after the group's sum
+        const double sup_e = in ? g.sup[e] : 0.0, ew_e = in ? g.ew[e] : 0.0;
+        const double sup0_e = in ? g.sup0[e] : 0.0;
         const bool act = se == ST_ALIVE;
         const uint64_t k0 = z;
         const uint64_t k1 = act ? z + tc : z;
@@ -358,11 +362,11 @@ __device__ __forceinline__ void pull_body(const PeelArgs& A, const uint32_t* __r
         const uint64_t ab = ballot64(any);
         const bool gany = G == 64 ? ab != 0 : ((ab >> (lane & ~(G - 1))) & ((1ull << G) - 1)) != 0;
         if (!act || gl != 0 || !gany) continue;
-        const double v = g.sup[e] - d;
+        const double v = sup_e - d;
         g.sup[e] = v;
         g.touched[e] = 1;
-        const double thr = A.eps * g.ew[e];
-        const double cb = cheap_bound(g.tcnt[e], g.sup0[e]);
+        const double thr = A.eps * ew_e;
+        const double cb = cheap_bound(tc, sup0_e);
         if (v < thr - cb) {
             // This group is the only writer of claim[e] in a pull round;
             // k_collect_claims appends the claimed edges to the next
