@@ -23,15 +23,18 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 thread_local Arena* tl_arena = nullptr;
+thread_local hipStream_t tl_stream = nullptr;
 
 namespace {
-// Makes the handle's scratch arena current for one API call, growing it
-// first (no scratch is outstanding between calls) when the last call asked
-// for more than it holds.  If the larger buffer cannot be had, calls keep
-// allocating per request.
+// Makes the handle's scratch arena (and its stream) current for one API
+// call, growing the arena first (no scratch is outstanding between calls)
+// when the last call asked for more than it holds.  If the larger buffer
+// cannot be had, calls keep allocating per request.
 struct ArenaScope {
     Arena* prev = tl_arena;
+    hipStream_t prev_stream = tl_stream;
     explicit ArenaScope(rtrex_graph h) {
+        tl_stream = h->stream;
         Arena& a = h->arena;
         if (a.top == 0 && a.demand > a.cap) {
             (void)hipStreamSynchronize(h->stream);
@@ -50,7 +53,10 @@ struct ArenaScope {
         }
         tl_arena = &a;
     }
-    ~ArenaScope() { tl_arena = prev; }
+    ~ArenaScope() {
+        tl_arena = prev;
+        tl_stream = prev_stream;
+    }
 };
 }  // namespace
 

@@ -336,17 +336,24 @@ rtrex_status dalloc(rtrex_graph h, T** p, uint64_t count) {
 // `demand`; the arena grows to the demand at the start of the next call.
 // Arena of the handle whose API call is running on this thread (or null).
 extern thread_local Arena* tl_arena;
+// The stream of the API call in progress on this thread (set with tl_arena).
+extern thread_local hipStream_t tl_stream;
 
 // Scratch that lives for one call.
 struct Scratch {
     std::vector<void*> ptrs;
     Arena* a = tl_arena;
+    hipStream_t stream = tl_stream;
     uint64_t mark = a ? a->top : 0, vmark = a ? a->vtop : 0;
     ~Scratch() {
         // Kernels queued on this buffer may still run (callers do not wait
         // for their last launches): buffers of their own are freed only once
-        // the device has drained.  Arena scratch is reused in stream order.
-        if (!ptrs.empty()) (void)hipDeviceSynchronize();
+        // the call's stream has drained (the device, outside an API call).
+        // Arena scratch is reused in stream order.
+        if (!ptrs.empty()) {
+            if (stream) (void)hipStreamSynchronize(stream);
+            else (void)hipDeviceSynchronize();
+        }
         for (void* p : ptrs) (void)hipFree(p);
         if (a) {
             a->top = mark;
