@@ -381,11 +381,14 @@ template <int LV>
 __device__ __forceinline__ bool tree_probe(const uint32_t* leaf, const uint32_t* l1,
                                            const uint32_t* l2, const Piv8& top, uint32_t z,
                                            uint32_t& pos) {
-    uint32_t b = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) b += top.p[i] < z ? 1u : 0u;
-    const bool ok = b < 8;  // b == 8: z is above every id
-    b = min(b, 7u);
+    // Pivots below z, by a three-step binary search over the sorted top
+    // pivots (the count of p[0..6] below z, at most 7); z above every id
+    // when p[7] < z.
+    const bool ok = !(top.p[7] < z);
+    uint32_t b = top.p[3] < z ? 4u : 0u;
+    b += (b ? top.p[5] : top.p[1]) < z ? 2u : 0u;
+    const uint32_t p2 = b == 0 ? top.p[0] : b == 2 ? top.p[2] : b == 4 ? top.p[4] : top.p[6];
+    b += p2 < z ? 1u : 0u;
     if (LV >= 3) {
         const uint4* q = reinterpret_cast<const uint4*>(l2) + 2 * b;
         b = 8 * b + min(below8(q[0], q[1], z), 7u);
