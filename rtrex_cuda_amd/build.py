@@ -76,8 +76,8 @@ def build_gen(out: str = GEN_LIB) -> str:
 
 def build_hostcheck(out: str = CHECK_LIB) -> str:
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", "-o", tmp,
-          os.path.join(CSRC, "hostcheck.cpp")])
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", "-pthread",
+          "-o", tmp, os.path.join(CSRC, "hostcheck.cpp")])
     os.replace(tmp, out)
     return out
 

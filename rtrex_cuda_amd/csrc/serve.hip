@@ -26,6 +26,7 @@
 #include <unordered_map>
 
 #include "internal.hpp"
+#include "jobtable.hpp"
 
 using namespace rtrex;
 
@@ -43,6 +44,7 @@ struct Job {
     std::string err;
     rtrex_job_info info{};
     bool done = false;
+    bool claimed = false;  // a waiter holds it (see JobTable)
 };
 
 // First-fit ranges of the device pool (offset -> length), coalesced on free.
@@ -99,8 +101,7 @@ struct rtrex_server_s {
     std::deque<Job*> intake;  // submitted, footprint not yet computed
     std::condition_variable cv_in;
     std::thread intake_thread;
-    std::unordered_map<int64_t, Job*> jobs;
-    int64_t next_id = 1;
+    JobTable<Job> jobs;
     int running = 0;
     bool stop = false;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -345,9 +346,8 @@ rtrex_status rtrex_server_submit(rtrex_server s, const uint32_t* u, const uint32
         delete j;
         return fail(RTREX_ESTATE, "server is stopping");
     }
-    j->id = s->next_id++;
     j->info.submit_s = s->now();
-    s->jobs[j->id] = j;
+    s->jobs.add(j);
     s->intake.push_back(j);
     *job = j->id;
     s->cv_in.notify_one();
@@ -360,11 +360,14 @@ rtrex_status rtrex_server_wait(rtrex_server s, int64_t job, int64_t* cluster_map
     Job* j = nullptr;
     {
         std::unique_lock<std::mutex> lk(s->mu);
-        auto it = s->jobs.find(job);
-        if (it == s->jobs.end()) return fail(RTREX_EINVAL, "unknown job id");
-        j = it->second;
+        switch (s->jobs.claim(job, &j)) {
+            case JobTable<Job>::UNKNOWN: return fail(RTREX_EINVAL, "unknown job id");
+            case JobTable<Job>::TAKEN:
+                return fail(RTREX_ESTATE, "another thread is already waiting on this job");
+            case JobTable<Job>::CLAIMED: break;
+        }
         s->cv.wait(lk, [&] { return j->done; });
-        s->jobs.erase(it);
+        s->jobs.release(job);
     }
     const rtrex_status st = j->st;
     if (st != RTREX_OK) {
@@ -397,7 +400,7 @@ rtrex_status rtrex_server_stop(rtrex_server s) {
     (void)hipSetDevice(s->cfg.device);
     for (auto& st : s->streams) (void)hipStreamDestroy(st);
     (void)hipFree(s->pool.base);
-    for (auto& kv : s->jobs) delete kv.second;
+    for (auto& kv : s->jobs.map) delete kv.second;
     delete s;
     return RTREX_OK;
 }

@@ -8,6 +8,10 @@ The driver (RTRex.cpp:141-206, mode s) writes, in its working directory:
   RTRex-<name>-{decomposition,stats,cut,freq}.txt            printDecomposition
   connected-components-<name>-{decomposition,stats,cut,freq}.txt
   amazon_cleaned_edges.txt                                   survivor dump
+Beside the file hashes it records the canonical partition hash
+(rtrex_cuda_amd.validate.canonical_sets_sha) of both decomposition files, so
+that a partition computed elsewhere (cluster ids and member order aside) is
+compared with the reference's: "RTRex-sets" and "connected-components-sets".
 The inputs are the graphs of tests/golden/graphs (the reference's bundled
 graphs, cliques K_5..K_9 and the C1 workload), written as Escape files.
 """
@@ -28,6 +32,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 import oracle as O  # noqa: E402
+from rtrex_cuda_amd import validate as V  # noqa: E402
 
 RTREX = os.path.join(ROOT, "oracle", "_ref", "RTRex")
 EPS = (0.3, 0.1)
@@ -69,6 +74,8 @@ def main():
                             if len(keep) != len(lines):
                                 ent.append(hashlib.sha256(b"\n".join(keep)).hexdigest())
                         files[f"{pre.split('-g')[0]}-{suf}"] = ent
+                    sets = V.load_clusters(os.path.join(d, f"{pre}-decomposition.txt"))
+                    files[f"{pre.split('-g')[0]}-sets"] = V.canonical_sets_sha(sets)
                 p = os.path.join(d, "amazon_cleaned_edges.txt")
                 files["survivors"] = [sha_file(p), os.path.getsize(p)]
             out[f"{name}_eps{eps}"] = files

@@ -78,3 +78,14 @@ def test_weights_sidecar():
     assert w.dtype == np.float32 and np.all((w > 0) & (w <= 1))
     w = G.weights(1000, "lognormal", 1)
     assert np.all((w > 0) & (w <= 10))
+
+
+def test_server_job_table_under_concurrent_submit_and_wait():
+    """The serving loop's job table (jobtable.hpp): submits that rehash the
+    map while waiters hold claims, two waiters per id (one refused), every
+    job waited on exactly once, nothing left behind."""
+    L = C.CDLL(B.CHECK_LIB)
+    L.rtrex_hostcheck_jobtable.restype = C.c_int64
+    L.rtrex_hostcheck_jobtable.argtypes = [C.c_int, C.c_int, C.c_int]
+    for _ in range(3):
+        assert L.rtrex_hostcheck_jobtable(4, 400, 6) == 0

@@ -9,7 +9,11 @@ here, on the CPU, by:
   * the order-independent fixpoint that defines the clean phase, checked in
     exact rational arithmetic;
   * planted mistakes in a Python copy of the fold order, which the golden
-    hashes must reject (so the pins can tell orders apart).
+    hashes must reject (so the pins can tell orders apart);
+  * connected_components: the partition hash of the connected-components file
+    the reference's own driver wrote, scipy's connected_components, and a
+    closed form (cliques, a path, isolated vertices, whose ids would be -1
+    if a start vertex were left unlabelled).
 """
 from __future__ import annotations
 
@@ -188,3 +192,68 @@ def test_delete_and_clean_removes_given_edge():
     s = off[0] + np.searchsorted(nb[off[0]:off[1]], 1)
     assert st.status[s] == ord("N")
     assert st.total == cl.total - 4  # K6: edge {0,1} lies in 4 triangles
+
+
+# ---------------------------------------------------------------- connectedComp
+def _golden_files():
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "files.json")) as f:
+        return json.load(f)
+
+
+GRAPH_NAMES = sorted({t.rsplit("_eps", 1)[0] for t in GOLDEN_TAGS})
+
+
+@pytest.mark.parametrize("name", GRAPH_NAMES)
+def test_components_match_reference_driver(name):
+    """The partition equals the one the reference's own driver wrote
+    (connectedComp, Decomposition.h:1023-1068, via RTRex.cpp:203-204), and
+    component ids follow the BFS start order: ascending smallest vertex."""
+    from rtrex_cuda_amd import validate as V
+    off, nb = _csr(name)
+    comp, k = O.connected_components(off, nb)
+    assert comp.min() >= 0 and comp.max() == k - 1
+    assert V.canonical_cluster_sha(comp) == _golden_files()[f"{name}_eps0.3"][
+        "connected-components-sets"]
+    first = np.full(k, len(comp))
+    np.minimum.at(first, comp, np.arange(len(comp)))
+    assert np.all(np.diff(first) > 0)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_components_equal_textbook_routine(seed):
+    """Same partition as scipy's connected_components on sparse random graphs
+    with isolated vertices and several components."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components as sp_cc
+    n, u, v = _random_graph(300, 0.004, seed)
+    off, nb = O.csr_from_undirected(n, u, v)
+    comp, k = O.connected_components(off, nb)
+    a = coo_matrix((np.ones(len(u)), (u, v)), shape=(n, n))
+    k2, lab = sp_cc(a, directed=False)
+    assert k == k2
+    # same partition: the label pairs form a bijection
+    pairs = set(zip(comp.tolist(), lab.tolist()))
+    assert len(pairs) == k
+
+
+def test_components_closed_form():
+    """Disjoint cliques, a path and isolated vertices: components known."""
+    edges, base, want = [], 0, []
+    for k in (3, 5):
+        edges += [(base + a, base + b) for a in range(k) for b in range(a + 1, k)]
+        want.append(list(range(base, base + k)))
+        base += k
+    want += [[base], [base + 1]]  # two isolated vertices
+    base += 2
+    edges += [(base + i, base + i + 1) for i in range(6)]  # a path of 7
+    want.append(list(range(base, base + 7)))
+    base += 7
+    e = np.array(edges, np.uint32)
+    off, nb = O.csr_from_undirected(base, e[:, 0], e[:, 1])
+    comp, k = O.connected_components(off, nb)
+    assert k == len(want)
+    for i, w in enumerate(want):
+        assert np.all(comp[w] == i)
