@@ -1,0 +1,170 @@
+"""The benchmark workloads at their own sizes, on the GPU, against the
+reference:
+
+  skewed large  R-MAT scale 22 (64.2 M edges): CSR, triangle total, FP64
+                supports, survivors and clusters against hashes of the
+                reference's own run (tests/golden/config3.json, written by
+                make_golden_c3.py from the reference sources).
+  batched       the 4,096 planted-community graphs in one batch: every graph's
+                cluster map equals the graph decomposed alone through the plain
+                path AND the CPU oracle's.
+  serving       a Poisson trace over the serving pool (Chung-Lu, planted
+                communities of 50 k - 2 M edges, R-MAT scale 18) through the
+                server: every request's map equals its graph alone; one graph
+                of each kind against hashes of the reference's run
+                (tests/golden/serve_mix.json, make_golden_serve.py).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rtrex_cuda_amd as R
+from conftest import GOLDEN, survivors_of
+from rtrex_cuda_amd import graphs as G
+from rtrex_cuda_amd import validate as V
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _json(name):
+    p = os.path.join(GOLDEN, name)
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        return json.load(f)
+
+
+def _full_hashes(n, u, v, eps, label):
+    """Every hash the reference fixtures hold, from the HIP path."""
+    t0 = time.time()
+    with R.DeviceGraph.from_edges(n, u, v) as g:
+        off, nb = g.csr()
+        out = dict(csr_sha=sha(np.concatenate([off, nb])))
+        out["count_total"] = g.gpu_common_nbr()
+        pe, pv, _ = g.sync_weights()
+        out["count_perEdge_sha"], out["count_perVertex_sha"] = sha(pe), sha(pv)
+        del pe, pv
+        print(f"[gpu] {label}: counted ({time.time() - t0:.1f} s)", flush=True)
+        g.gpu_initial_clean(eps)
+        st = g.sync_edge_status()
+        surv = survivors_of(off, nb, st)
+        out["survivors"], out["survivors_sha"] = int(len(surv)), sha(surv)
+        del st, surv
+        cm, k = g.disjoint_extract(eps)
+        out["n_clusters"], out["cluster_map_sha"] = k, sha(cm)
+        out["cluster_sets_sha"] = V.canonical_cluster_sha(cm)
+        out["audit"] = g.audit()
+        print(f"[gpu] {label}: decomposed ({time.time() - t0:.1f} s)", flush=True)
+    return out
+
+
+KEYS = ("csr_sha", "count_total", "count_perEdge_sha", "count_perVertex_sha", "survivors",
+        "survivors_sha", "n_clusters", "cluster_map_sha", "cluster_sets_sha")
+
+C3 = _json("config3.json")
+
+
+@pytest.fixture(scope="module")
+def config3_graph():
+    t0 = time.time()
+    g = G.config3()
+    print(f"[gpu] R-MAT scale 22 generated: {len(g[1])} edges ({time.time() - t0:.1f} s)",
+          flush=True)
+    return g
+
+
+@pytest.mark.parametrize("tag", sorted(C3) or [pytest.param("none", marks=pytest.mark.skip(
+    reason="tests/golden/config3.json not written yet (make_golden_c3.py)"))])
+def test_config3_matches_reference(tag, config3_graph):
+    ref = C3[tag]
+    n, u, v = config3_graph
+    assert (n, len(u)) == (ref["n"], ref["m"])
+    got = _full_hashes(n, u, v, ref["eps"], tag)
+    for k in KEYS:
+        assert got[k] == ref[k], k
+    assert got["audit"]["ambiguous_clean"] == 0
+    assert got["audit"]["ambiguous_ratio"] == 0
+
+
+def _oracle_map(g, eps):
+    n, u, v = g
+    off, nb = O.csr_from_undirected(n, u, v)
+    return O.decompose(off, nb, eps, with_clusters=False).cluster_map
+
+
+def test_config4_batch_equals_alone_and_oracle():
+    eps = 0.3
+    t0 = time.time()
+    graphs = G.config4()
+    print(f"[gpu] 4096 graphs generated ({time.time() - t0:.1f} s)", flush=True)
+    maps, counts, _, audit = R.batch_decompose(graphs, eps)
+    print(f"[gpu] batch decomposed ({time.time() - t0:.1f} s); audit {audit}", flush=True)
+    assert audit["ambiguous_clean"] == 0 and audit["ambiguous_ratio"] == 0
+    bad_alone = []
+    for i, (n, u, v) in enumerate(graphs):
+        with R.DeviceGraph.from_edges(n, u, v) as g:
+            cm, k = g.decompose(eps)
+        if k != counts[i] or not np.array_equal(cm, maps[i]):
+            bad_alone.append(i)
+        if i % 512 == 511:
+            print(f"[gpu] alone {i + 1}/4096 ({time.time() - t0:.1f} s)", flush=True)
+    assert not bad_alone, bad_alone[:10]
+    with ThreadPoolExecutor(16) as ex:
+        ref = list(ex.map(lambda g: _oracle_map(g, eps), graphs))
+    print(f"[gpu] oracle on 4096 graphs ({time.time() - t0:.1f} s)", flush=True)
+    bad_ref = [i for i in range(len(graphs)) if not np.array_equal(ref[i], maps[i])]
+    assert not bad_ref, bad_ref[:10]
+
+
+SERVE = _json("serve_mix.json")
+
+
+def test_config5_trace_served_equals_alone_and_reference():
+    eps = 0.3
+    t0 = time.time()
+    pool = G.serve_pool(n_rmat=1)  # R-MAT scale 18
+    trace = G.serve_trace(1.5, pool_sizes=(64, 32, 1))
+    # every kind present at least once, the R-MAT graph among the first requests
+    trace = [(0.0, "rmat", 0), (0.0, "planted", 3)] + trace
+    print(f"[gpu] serving {len(trace)} requests ({time.time() - t0:.1f} s)", flush=True)
+    with R.Server(workers=4, hbm_budget_bytes=48 << 30) as s:
+        start = time.perf_counter()
+        ids = []
+        for at, kind, idx in trace:
+            while time.perf_counter() - start < at:
+                time.sleep(0.0005)
+            n, u, v = pool[kind][idx]
+            ids.append(s.submit(n, u, v, eps))
+        res = [s.wait(j) for j in ids]
+    print(f"[gpu] served ({time.time() - t0:.1f} s)", flush=True)
+    infos = [r[2] for r in res]
+    assert any(i["batch_graphs"] > 1 for i in infos), "no request was batched"
+    alone = {}
+    for (at, kind, idx), (cm, k, _) in zip(trace, res):
+        if (kind, idx) not in alone:
+            n, u, v = pool[kind][idx]
+            with R.DeviceGraph.from_edges(n, u, v) as g:
+                alone[(kind, idx)] = g.decompose(eps)
+        cm0, k0 = alone[(kind, idx)]
+        assert k == k0 and np.array_equal(cm, cm0), (kind, idx)
+    print(f"[gpu] {len(alone)} distinct graphs alone ({time.time() - t0:.1f} s)", flush=True)
+    for tag, ref in sorted(SERVE.items()):
+        n, u, v = pool[ref["kind"]][ref["index"]]
+        got = _full_hashes(n, u, v, ref["eps"], tag)
+        for k in KEYS:
+            assert got[k] == ref[k], (tag, k)
+        cm0, _ = alone.get((ref["kind"], ref["index"]), (None, None))
+        if cm0 is not None:
+            assert sha(cm0) == ref["cluster_map_sha"]
