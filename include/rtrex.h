@@ -159,6 +159,16 @@ rtrex_status rtrex_get_member_order(rtrex_graph g, int64_t* rank);
 rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
                                  int64_t* n_edges, int64_t* cut);
 
+/* rtrex_cluster_stats without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  The same results, with the per-cluster
+ * accumulators in a device array of the library (24 MiB per GPU, held until
+ * the process ends) instead of the handle's scratch.  At most 1,048,576
+ * clusters (RTREX_EINVAL beyond).  Calls on one GPU share that array: calls
+ * that overlap on one GPU race. */
+rtrex_status rtrex_cluster_stats_noalloc(rtrex_graph g, int64_t* n_vertices,
+                                         int64_t* n_edges, int64_t* cut);
+
 /* Decision audit.  Every threshold or ratio decision whose margin is within
  * the FP64 error bound of the reference's own summation is counted as
  * ambiguous; with zero ambiguous decisions the survivor and cluster sets are

@@ -41,6 +41,7 @@ EXPORTS = [
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
     "rtrex_get_survivors",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
+    "rtrex_cluster_stats_noalloc",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
@@ -142,6 +143,7 @@ def lib() -> C.CDLL:
             "rtrex_extract": [G, C.c_double, P64, P64],
             "rtrex_decompose": [G, C.c_double, P64, P64],
             "rtrex_cluster_stats": [G, P64, P64, P64],
+            "rtrex_cluster_stats_noalloc": [G, P64, P64, P64],
             "rtrex_get_audit": [G, C.POINTER(Audit)],
             "rtrex_get_timings": [G, C.POINTER(Timings)],
             "rtrex_batch_decompose": [C.c_int64, P64, P64, PU32, PU32, C.c_double, C.c_int,
@@ -344,6 +346,16 @@ class DeviceGraph:
         ne = np.empty(n_clusters, np.int64)
         cut = np.empty(n_clusters, np.int64)
         _check(lib().rtrex_cluster_stats(self._h, _p(nv, P64), _p(ne, P64), _p(cut, P64)))
+        return nv, ne, cut
+
+    def populate_stats_noalloc(self, n_clusters: int):
+        """populate_stats through the allocation-free entry point (at most
+        2^20 clusters; calls overlapping on one GPU race)."""
+        nv = np.empty(n_clusters, np.int64)
+        ne = np.empty(n_clusters, np.int64)
+        cut = np.empty(n_clusters, np.int64)
+        _check(lib().rtrex_cluster_stats_noalloc(self._h, _p(nv, P64), _p(ne, P64),
+                                                 _p(cut, P64)))
         return nv, ne, cut
 
     def audit(self) -> dict:

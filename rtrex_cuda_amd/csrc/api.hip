@@ -753,6 +753,12 @@ rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
     return run_cluster_stats(g, n_vertices, n_edges, cut);
 }
 
+rtrex_status rtrex_cluster_stats_noalloc(rtrex_graph g, int64_t* n_vertices,
+                                         int64_t* n_edges, int64_t* cut) {
+    RT_TRY(check(g));
+    return run_cluster_stats_noalloc(g, n_vertices, n_edges, cut);
+}
+
 rtrex_status rtrex_get_audit(rtrex_graph g, rtrex_audit* out) {
     RT_TRY(check(g));
     if (!out) return fail(RTREX_EINVAL, "null output");

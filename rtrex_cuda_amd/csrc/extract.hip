@@ -1339,3 +1339,84 @@ rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
 }
 
 }  // namespace rtrex
+
+// ------------------------------------------------- statistics, no allocation
+// rtrex_cluster_stats_noalloc keeps the three per-cluster accumulators of
+// k_cluster_stats in this array instead of the handle's scratch arena, so the
+// call allocates nothing.  Each GPU holds its own copy for the life of the
+// process (24 MiB); every call on one GPU shares it, and a call leaves its
+// results in it, so each call clears what it uses first, and calls that
+// overlap on one GPU race.
+static constexpr uint64_t STATS_MAX_CLUSTERS = 1ull << 20;
+static __device__ __align__(256) unsigned long long scratch[3 * STATS_MAX_CLUSTERS];
+
+namespace rtrex {
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_stats_scratch_mu;
+unsigned long long* g_stats_scratch[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device; host code never uses the array's name as an address).
+rtrex_status stats_scratch(int device, unsigned long long** out) {
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(RTREX_EINVAL, "device index beyond the allocation-free path's table");
+    std::lock_guard<std::mutex> lk(g_stats_scratch_mu);
+    if (!g_stats_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__,
+                                                   __LINE__);
+        g_stats_scratch[device] = static_cast<unsigned long long*>(p);
+    }
+    *out = g_stats_scratch[device];
+    return RTREX_OK;
+}
+
+__global__ void __launch_bounds__(T) k_zero_u64(unsigned long long* p, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)T + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * T)
+        p[i] = 0ull;
+}
+
+// Internal edges were counted from both endpoints (populateStats halves them,
+// Decomposition.h:489).
+__global__ void __launch_bounds__(T) k_halve_u64(unsigned long long* p, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)T + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * T)
+        p[i] >>= 1;
+}
+
+}  // namespace
+
+rtrex_status run_cluster_stats_noalloc(rtrex_graph h, int64_t* nv, int64_t* ne, int64_t* cut) {
+    if (h->phase < PH_EXTRACTED || !h->cid)
+        return fail(RTREX_ESTATE, "extract has not run");
+    const uint64_t k = (uint64_t)h->n_clusters;
+    if (k > STATS_MAX_CLUSTERS)
+        return fail(RTREX_EINVAL, "the graph has " + std::to_string(k) +
+                                      " clusters; rtrex_cluster_stats_noalloc holds at most " +
+                                      std::to_string(STATS_MAX_CLUSTERS) +
+                                      " (rtrex_cluster_stats has no limit)");
+    if (k == 0) return RTREX_OK;
+    unsigned long long* base = nullptr;
+    RT_TRY(stats_scratch(h->device, &base));
+    unsigned long long *a = base, *b = base + STATS_MAX_CLUSTERS, *c = base + 2 * STATS_MAX_CLUSTERS;
+    hipStream_t s = h->stream;
+    // the last call's results are still there: clear the three ranges used
+    k_zero_u64<<<blocks_for(k, T, 2048), T, 0, s>>>(a, k);
+    k_zero_u64<<<blocks_for(k, T, 2048), T, 0, s>>>(b, k);
+    k_zero_u64<<<blocks_for(k, T, 2048), T, 0, s>>>(c, k);
+    k_cluster_stats<<<blocks_for(h->g.n, T, 8192), T, 0, s>>>(h->g, h->cid, a, b, c);
+    k_halve_u64<<<blocks_for(k, T, 2048), T, 0, s>>>(b, k);
+    RT_HIP(hipGetLastError());
+    // counts are below 2^63: the u64 words are the int64 results
+    if (nv) RT_HIP(hipMemcpyAsync(nv, a, k * 8, hipMemcpyDeviceToHost, s));
+    if (ne) RT_HIP(hipMemcpyAsync(ne, b, k * 8, hipMemcpyDeviceToHost, s));
+    if (cut) RT_HIP(hipMemcpyAsync(cut, c, k * 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RTREX_OK;
+}
+
+}  // namespace rtrex

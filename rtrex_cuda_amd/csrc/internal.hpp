@@ -414,6 +414,8 @@ rtrex_status run_extract(rtrex_graph h, double eps);
 rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
+rtrex_status run_cluster_stats_noalloc(rtrex_graph h, int64_t* nv, int64_t* ne,
+                                       int64_t* cut);
 // Per-vertex supports of the listed vertices (all when list is null) on
 // demand, when stored triangle lists are kept (count.hip).
 rtrex_status ensure_vertex_supports(rtrex_graph h, const uint32_t* list, uint32_t cnt);
