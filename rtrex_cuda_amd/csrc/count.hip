@@ -384,11 +384,15 @@ __device__ __forceinline__ bool tree_probe(const uint32_t* leaf, const uint32_t*
     // Pivots below z, by a three-step binary search over the sorted top
     // pivots (the count of p[0..6] below z, at most 7); z above every id
     // when p[7] < z.
+    // (Every pivot is chosen by selects on the comparison bits, so the
+    // search stays branch-free.)
     const bool ok = !(top.p[7] < z);
-    uint32_t b = top.p[3] < z ? 4u : 0u;
-    b += (b ? top.p[5] : top.p[1]) < z ? 2u : 0u;
-    const uint32_t p2 = b == 0 ? top.p[0] : b == 2 ? top.p[2] : b == 4 ? top.p[4] : top.p[6];
-    b += p2 < z ? 1u : 0u;
+    const bool c2 = top.p[3] < z;
+    const bool c1 = (c2 ? top.p[5] : top.p[1]) < z;
+    const uint32_t q1 = c2 ? top.p[6] : top.p[2];
+    const uint32_t q0 = c2 ? top.p[4] : top.p[0];
+    const bool c0 = (c1 ? q1 : q0) < z;
+    uint32_t b = (c2 ? 4u : 0u) | (c1 ? 2u : 0u) | (c0 ? 1u : 0u);
     if (LV >= 3) {
         const uint4* q = reinterpret_cast<const uint4*>(l2) + 2 * b;
         b = 8 * b + min(below8(q[0], q[1], z), 7u);
@@ -518,8 +522,8 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 if (tb.t && hit)  // edges {lo, z} and {hub, z}
                     tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], hm_.ed(g, pos)};
                 c += (uint32_t)__popcll(hm);
-                // Ordered fold: the group's weights in lane order, a miss
-                // adding 0.0 (which leaves the sum's bits unchanged).
+                // Ordered fold: the group's match weights in lane order
+                // (ascending z).
                 if (G == 64) {
                     uint64_t mask = hm;
                     while (mask) {
@@ -528,12 +532,22 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                         mask &= mask - 1;
                     }
                 } else if (__any(hm != 0)) {
-                    sfold[lane] = t;
+                    // the group's match weights compacted in lane order; its
+                    // first lane adds them one after another
+                    if (hit) sfold[gbase + __popcll(hm & below)] = t;
                     wave_sync();
                     if (gl == 0 && hm) {
                         const double* f = sfold + gbase;
-#pragma unroll
-                        for (int i = 0; i < G; ++i) S += f[i];
+                        const int nh = __popcll(hm);
+                        int i = 0;
+                        for (; i + 4 <= nh; i += 4) {
+                            const double a0 = f[i], a1 = f[i + 1], a2 = f[i + 2], a3 = f[i + 3];
+                            S += a0;
+                            S += a1;
+                            S += a2;
+                            S += a3;
+                        }
+                        for (; i < nh; ++i) S += f[i];
                     }
                     wave_sync();
                 }
@@ -594,17 +608,28 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
 // (edge k covers elements [L_k, L_k + d_k) of the sequence, L the exclusive
 // prefix sum of the list lengths d_k).  A lane finds its edge from the edge
 // starts that fall in the step: each start sets a per-step tag at its
-// position in LDS, one ballot per 64-element part gathers them, and the lane's edge is the
-// previous step's last edge plus the starts at or below it.  A match is
-// stored at its edge's term offset plus the edge's earlier matches, so every
-// edge's terms stay in ascending z.  The support is folded in the same order:
-// each part's match weights are compacted in lane order into LDS and the
-// last lane of every edge's run adds its run's weights, one after another, to
-// the edge's running sum.  The next unit's record and hub staging loads are
-// issued before the current unit's steps (unit records two units ahead), so
-// a unit's setup does not wait on memory.
+// position in LDS, one ballot per 64-element part gathers them, and the lane's
+// edge is the previous step's last edge plus the starts at or below it.
+//
+// The scan is split from the work a match needs.  Streaming reads only the
+// 4-byte ids of the lists; a match (the id found in the hub's list) is
+// appended to the wave's ring of pending matches in LDS -- its slot in the
+// streamed list, its position in the hub's list and its edge -- in stream
+// order, so every edge's matches are in ascending z and contiguous.  Each time
+// 64 matches are pending they are handled at once, one per lane: the edge id
+// of {lo, z} (gathered by slot), the FP64 weight, the 16-byte term at the
+// edge's term offset plus the edge's earlier matches, and the support fold:
+// the weights go to LDS in ring order and the last lane of every edge's run
+// adds the run's weights, one after another, to the edge's running sum
+// (the reference's ascending-z order).  So the divisions, the term stores and
+// the folds run with every lane busy instead of once per probed element
+// step, and no edge id is read for the ids that do not match.  The next
+// unit's record and hub staging loads are issued before the current unit's
+// steps (unit records two units ahead), so a unit's setup does not wait on
+// memory.
 constexpr int FW = 256;
-constexpr int FH = 2;  // 64-element parts per step (4 measured slower on C2)
+constexpr int FH = 2;        // 64-element parts per step
+constexpr uint32_t FQ = 128;  // pending-match ring per wave: < 64 left + 64 appended
 struct FlatStage {
     uint32_t z, e, d, o, zl, zh, oi;  // hub slot lane: id, edge id, degree, offset, term offset; own_idx
 };
@@ -640,11 +665,13 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
     __shared__ uint64_t s_zb[NW][WAVE];  // term offset of edge k
     __shared__ uint32_t s_cnt[NW][WAVE];  // matches so far
     __shared__ double s_S[NW][WAVE];      // support so far
-    __shared__ double s_t[NW][WAVE];      // a half step's match weights, in lane order
+    __shared__ double s_t[NW][WAVE];      // a batch's match weights, in ring order
     __shared__ uint32_t s_flag[NW][FH * WAVE];
+    __shared__ uint2 s_q[NW][FQ];         // pending matches: (slot, hub position | edge << 8)
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t upto = below | (1ull << lane);
     uint32_t tag = 0;
 #pragma unroll
     for (int q = 0; q < FH; ++q) s_flag[wid][q * WAVE + lane] = 0;  // tags start at 1
@@ -654,8 +681,7 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
     uint4 Unext = un + stride < nunits ? units[un + stride] : make_uint4(0u, 0u, 0u, 0u);
     FlatStage cur = flat_stage(g, tb, own_idx, Ucur, un < nunits, lane);
     for (; un < nunits; un += stride) {
-        const uint32_t oh = Ucur.y, dh = Ucur.w & 0xffffu, cnt = Ucur.w >> 16;
-        (void)oh;
+        const uint32_t dh = Ucur.w & 0xffffu, cnt = Ucur.w >> 16;
         if ((uint32_t)lane < dh) {
             s_ed[wid][lane] = cur.e;
             s_dg[wid][lane] = cur.d;
@@ -691,17 +717,54 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
         Ucur = Unext;
         Unext = un2 < nunits ? units[un2] : make_uint4(0u, 0u, 0u, 0u);
         wave_sync();
+        // The pending matches [qh, qt) of the ring (nh <= 64 of them from qh
+        // on) handled one per lane, the support folded in ring order.
+        auto handle = [&](uint32_t qh, uint32_t nh) {
+            const bool act = (uint32_t)lane < nh;
+            const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (FQ - 1)];
+            const uint32_t pos = q.y & 0xffu, k = act ? q.y >> 8 : 0u;
+            const uint32_t elo = act ? g.s2e[q.x] : 0u;  // edge {lo, z}
+            const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
+            const bool start = act && (lane == 0 || kup != k);
+            const uint64_t sm = ballot64(start) | 1ull;
+            const int r0 = 63 - __clzll((long long)(sm & upto));  // first lane of the run
+            const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
+            uint32_t c0 = 0;
+            if (act) {
+                c0 = s_cnt[wid][k];
+                const double t = tri_w(dh, s_ek[wid][k].z, s_dg[wid][pos]);
+                tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, elo, s_ed[wid][pos]};
+                s_t[wid][lane] = t;
+            }
+            wave_sync();
+            if (last) {
+                double S = s_S[wid][k];
+                int i = r0;
+                for (; i + 3 <= lane; i += 4) {
+                    const double a0 = s_t[wid][i], a1 = s_t[wid][i + 1];
+                    const double a2 = s_t[wid][i + 2], a3 = s_t[wid][i + 3];
+                    S += a0;
+                    S += a1;
+                    S += a2;
+                    S += a3;
+                }
+                for (; i <= lane; ++i) S += s_t[wid][i];
+                s_S[wid][k] = S;
+                s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
+            }
+            wave_sync();
+        };
+        uint32_t qh = 0, qt = 0;
         int kprev = -1;
         for (uint32_t base = 0; base < W; base += FH * WAVE) {
             ++tag;
             if (own && Lk >= base && Lk - base < (uint32_t)(FH * WAVE))
                 s_flag[wid][Lk - base] = tag;
             wave_sync();
-            int kq[FH], kkq[FH];
+            int kq[FH];
             bool aq[FH];
-            uint4 Eq[FH];
-            uint2 xq[FH];
-            int kb = kprev;  // edge before this quarter
+            uint32_t sq[FH], zq[FH];
+            int kb = kprev;  // edge before this part
 #pragma unroll
             for (int q = 0; q < FH; ++q) {
                 const bool st = s_flag[wid][q * WAVE + lane] == tag;
@@ -709,57 +772,32 @@ __global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* 
                 kq[q] = kb + (int)__popcll(sm & below) + (st ? 1 : 0);
                 kb += (int)__popcll(sm);
                 aq[q] = base + q * WAVE + lane < W;  // then 0 <= k < cnt
-                kkq[q] = aq[q] ? kq[q] : 0;
-                Eq[q] = s_ek[wid][kkq[q]];
+                sq[q] = base + q * WAVE + lane + s_ek[wid][aq[q] ? kq[q] : 0].y;
             }
-            // all of the window's list loads in flight at once
+            // all of the step's list loads in flight at once
 #pragma unroll
-            for (int q = 0; q < FH; ++q)  // (z, edge {lo, z})
-                xq[q] = aq[q] ? g.nbe[base + q * WAVE + lane + Eq[q].y] : make_uint2(0u, 0u);
+            for (int q = 0; q < FH; ++q) zq[q] = aq[q] ? g.nb[sq[q]] : PAD_ID;
 #pragma unroll
             for (int hf = 0; hf < FH; ++hf) {
-                const uint32_t hb = base + hf * WAVE, f = hb + lane;
-                const bool act = aq[hf];
-                const int k = kq[hf], kk = kkq[hf];
-                const uint4 E = Eq[hf];
-                const uint2 x = xq[hf];
                 uint32_t pos = 0;
-                const bool hit = act && tree_probe<1>(s_nb[wid], nullptr, nullptr, top, x.x, pos);
+                const bool hit =
+                    aq[hf] && tree_probe<1>(s_nb[wid], nullptr, nullptr, top, zq[hf], pos);
                 const uint64_t hm = ballot64(hit);
-                // lanes of edge kk in this half step: [run0, last of the run]
-                const uint32_t run0 = E.x > hb ? E.x - hb : 0u;
-                const uint32_t nb_below = (uint32_t)__popcll(hm & below);
-                const uint32_t rank = (uint32_t)__popcll(hm & below & ~((1ull << run0) - 1ull));
-                const uint32_t c0 = s_cnt[wid][kk];
-                if (hit) {  // edges {lo, z} and {hub, z}
-                    const double t = tri_w(dh, E.z, s_dg[wid][pos]);
-                    tb.t[s_zb[wid][kk] + c0 + rank] = Term{t, x.y, s_ed[wid][pos]};
-                    s_t[wid][nb_below] = t;
+                if (hit)
+                    s_q[wid][(qt + (uint32_t)__popcll(hm & below)) & (FQ - 1)] =
+                        make_uint2(sq[hf], pos | ((uint32_t)kq[hf] << 8));
+                qt += (uint32_t)__popcll(hm);
+                if (qt - qh >= (uint32_t)WAVE) {
+                    wave_sync();
+                    handle(qh, WAVE);
+                    qh += WAVE;
                 }
-                const int knext = __shfl_down(k, 1u, 64);
-                const bool last = act && (lane == WAVE - 1 || f + 1 == W || knext != k);
-                wave_sync();
-                if (last) {
-                    // the run's matches are s_t[nb_below - rank, nb_below + hit),
-                    // read four at a time and added in order
-                    double S = s_S[wid][kk];
-                    const uint32_t b = nb_below + (hit ? 1u : 0u);
-                    uint32_t i = nb_below - rank;
-                    for (; i + 4 <= b; i += 4) {
-                        const double a0 = s_t[wid][i], a1 = s_t[wid][i + 1];
-                        const double a2 = s_t[wid][i + 2], a3 = s_t[wid][i + 3];
-                        S += a0;
-                        S += a1;
-                        S += a2;
-                        S += a3;
-                    }
-                    for (; i < b; ++i) S += s_t[wid][i];
-                    s_S[wid][kk] = S;
-                    s_cnt[wid][kk] = c0 + rank + (hit ? 1u : 0u);
-                }
-                wave_sync();
             }
             kprev = __builtin_amdgcn_readlane(kq[FH - 1], WAVE - 1);
+        }
+        if (qt != qh) {
+            wave_sync();
+            handle(qh, qt - qh);
         }
         if (own) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
         wave_sync();  // the next unit restages the slices
