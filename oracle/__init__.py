@@ -47,9 +47,11 @@ def lib() -> C.CDLL:
         L.oracle_decompose.argtypes = [C.c_int64, p64, p64, C.c_double, p64, p64, pu8,
                                        p64, C.c_int64, p64, pd, p64]
         L.oracle_connected_components.argtypes = [C.c_int64, p64, p64, p64, p64]
+        L.oracle_extract_state.argtypes = [C.c_int64, p64, p64, C.c_double, pd, pd, p64, pu8,
+                                           p64, p64]
         for f in (L.oracle_make_csr, L.oracle_count, L.oracle_initial_clean,
                   L.oracle_delete_and_clean, L.oracle_decompose,
-                  L.oracle_connected_components):
+                  L.oracle_connected_components, L.oracle_extract_state):
             f.restype = C.c_int
         _lib = L
     return _lib
@@ -181,6 +183,26 @@ def decompose(off, nb, eps: float, with_clusters: bool = True) -> Decomposition:
             clusters.append((nv, ne, cut, fl[p + 3 : p + 3 + nv].copy()))
             p += 3 + nv
     return Decomposition(cm, int(nc[0]), st, clusters, tuple(float(x) for x in t), mr)
+
+
+def extract_from_state(off, nb, eps: float, state: Clean):
+    """disjointExtract's seed loop from a state after initial_clean and any
+    delete_and_clean calls.  Returns (cluster_map, n_clusters)."""
+    n = len(off) - 1
+    off = np.ascontiguousarray(off, np.int64)
+    nb = np.ascontiguousarray(nb, np.int64)
+    pe = state.perEdge.copy()
+    pv = state.perVertex.copy()
+    tot = np.array([state.total], np.int64)
+    st = np.ascontiguousarray(state.status, np.uint8)
+    cm = np.zeros(n, np.int64)
+    nc = np.zeros(1, np.int64)
+    rc = lib().oracle_extract_state(n, _p(off, C.c_int64), _p(nb, C.c_int64), eps,
+                                    _p(pe, C.c_double), _p(pv, C.c_double), _p(tot, C.c_int64),
+                                    _p(st, C.c_uint8), _p(cm, C.c_int64), _p(nc, C.c_int64))
+    if rc:
+        raise ValueError(f"oracle_extract_state failed rc={rc}")
+    return cm, int(nc[0])
 
 
 def connected_components(off, nb):

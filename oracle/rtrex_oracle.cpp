@@ -441,6 +441,30 @@ int oracle_delete_and_clean(int64_t n, const int64_t* offsets,
     return 0;
 }
 
+// disjointExtract's seed loop (Decomposition.h:538-940) entered from a given
+// state (supports and statuses after initialClean and any deleteAndClean
+// calls), as disjointExtract continues after its own initialClean (:548-556).
+int oracle_extract_state(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         double eps, double* perEdge, double* perVertex,
+                         int64_t* total, const uint8_t* status, int64_t* cluster_map,
+                         int64_t* n_clusters) {
+    if (!valid(n, offsets, nbors)) return 1;
+    Graph g{n, offsets, nbors};
+    State s(g);
+    s.part = partner_slots(g);
+    s.pe = perEdge;
+    s.pv = perVertex;
+    s.total = *total;
+    s.eps = eps;
+    const i64 nnz = offsets[n];
+    s.st.assign(status, status + nnz);
+    s.st.push_back('Y');
+    std::vector<Cluster> dec = extract(s, cluster_map, nullptr);
+    *n_clusters = (i64)dec.size();
+    *total = s.total;
+    return 0;
+}
+
 int oracle_decompose(int64_t n, const int64_t* offsets, const int64_t* nbors,
                      double eps, int64_t* cluster_map, int64_t* n_clusters,
                      uint8_t* status_after_clean, int64_t* flat,

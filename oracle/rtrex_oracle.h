@@ -47,6 +47,14 @@ int oracle_delete_and_clean(int64_t n, const int64_t* offsets,
                             double* perVertex, int64_t* total,
                             uint8_t* status);
 
+/* disjointExtract's seed loop alone (Decomposition.h:538-940), from a state
+ * after oracle_initial_clean and any oracle_delete_and_clean calls (the
+ * supports are updated in place by the extraction's cascades). */
+int oracle_extract_state(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         double eps, double* perEdge, double* perVertex,
+                         int64_t* total, const uint8_t* status, int64_t* cluster_map,
+                         int64_t* n_clusters);
+
 /* Full disjointExtract (Decomposition.h:538-940): count, clean, then greedy
  * extraction.  Outputs: cluster_map[n] (cluster id per vertex, ids in
  * extraction order), *n_clusters, survivors status[nnz] after the initial

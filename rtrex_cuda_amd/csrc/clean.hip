@@ -115,31 +115,32 @@ __global__ void __launch_bounds__(SEED_T) k_seed_frontier(GraphDev g, double eps
     }
 }
 
-// Explicit deletions (rtrex_delete_and_clean): the listed edges, which must
-// be alive, become the frontier.
-__global__ void __launch_bounds__(T) k_seed_pairs(GraphDev g,
-                                                  const uint32_t* __restrict__ pr,
-                                                  uint32_t k,
-                                                  uint32_t* __restrict__ F,
-                                                  unsigned* nF, unsigned* err, bool terms) {
+// Explicit deletions (rtrex_delete_and_clean).  First the edge of every
+// pair and whether it is alive, with no write to the graph (the host checks
+// them); then the listed edges, all alive and distinct, become the frontier.
+constexpr uint32_t PAIR_NOT_EDGE = 0xffffffffu, PAIR_NOT_ALIVE = 0x80000000u;
+__global__ void __launch_bounds__(T) k_pair_edges(GraphDev g, const uint32_t* __restrict__ pr,
+                                                  uint32_t k, uint32_t* __restrict__ eid) {
     for (uint32_t q = blockIdx.x * T + threadIdx.x; q < k; q += gridDim.x * T) {
-        uint32_t a = pr[2 * q], b = pr[2 * q + 1];
-        if (a >= g.n || b >= g.n) {
-            atomicOr(err, ERR_RANGE);
-            continue;
-        }
-        int64_t sl = find_slot(g.nb, g.off[a], g.off[a + 1], b);
+        const uint32_t a = pr[2 * q], b = pr[2 * q + 1];
+        const int64_t sl = a < g.n && b < g.n ? find_slot(g.nb, g.off[a], g.off[a + 1], b) : -1;
         if (sl < 0) {
-            atomicOr(err, ERR_RANGE);
+            eid[q] = PAIR_NOT_EDGE;
             continue;
         }
         const uint32_t e = g.s2e[sl];
-        if (g.est[e] != ST_ALIVE || atomicCAS(&g.claim[e], 0u, 2u) != 0u) {
-            atomicOr(err, ERR_UNSORTED_OR_DUP);
-            continue;
-        }
+        eid[q] = e | (g.est[e] != ST_ALIVE ? PAIR_NOT_ALIVE : 0u);
+    }
+}
+
+__global__ void __launch_bounds__(T) k_seed_edges(GraphDev g, const uint32_t* __restrict__ eid,
+                                                  uint32_t k, uint32_t* __restrict__ F,
+                                                  bool terms) {
+    for (uint32_t q = blockIdx.x * T + threadIdx.x; q < k; q += gridDim.x * T) {
+        const uint32_t e = eid[q];
+        g.claim[e] = 2u;
         mark(g, e, ST_QUEUED, terms);
-        F[atomicAdd(nF, 1u)] = e;
+        F[q] = e;
     }
 }
 
@@ -1026,6 +1027,12 @@ struct ReplayArgs {
     uint32_t nreg;
     uint64_t* stk;           // per region: (edge << 32) | first endpoint of the pair
     unsigned* next;          // work queue head
+    // Explicit deletions since the initial clean (rtrex_delete_and_clean), in
+    // call order: entry j deletes edge dl_e[j] given as the pair (dl_a[j], .),
+    // in call dl_c[j].  Region r's entries, ascending j, are
+    // dl_idx[dl_start[r] .. dl_end[r]) (no log: null).
+    const uint32_t *dl_e = nullptr, *dl_a = nullptr, *dl_c = nullptr;
+    const uint32_t *dl_idx = nullptr, *dl_start = nullptr, *dl_end = nullptr;
 };
 
 constexpr int RT = 64;  // one wave per workgroup
@@ -1086,9 +1093,68 @@ __device__ __forceinline__ uint32_t replay_chunk(const ReplayArgs& A, uint64_t* 
     return tot;
 }
 
-__global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
+// LIFO pops until the region's stack (top entries) is empty.
+__device__ __forceinline__ void replay_pops(const ReplayArgs& A, uint64_t* stk, uint32_t top) {
     const GraphDev& g = A.g;
     const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    // The entry on top of the stack when the last pop pushed (registers;
+    // otherwise it is read back).
+    bool have = false;
+    uint64_t next = 0;
+    while (top > 0) {
+        --top;
+        const uint64_t ent = have ? next : stk[top];
+        have = false;
+        const uint32_t e = (uint32_t)(ent >> 32), a = (uint32_t)ent;
+        // Three waves of loads per pop: what depends on e (and a); the
+        // first chunk's records and e's other endpoint; the degrees and
+        // both record edges' statuses, supports and thresholds.  Lanes
+        // past the list take a blank record (term_or_blank), so that no
+        // load waits on a branch.
+        const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
+        const uint64_t k0 = tb.zoff[e];
+        const uint32_t da = g.deg[a];
+        const bool in0 = (uint32_t)lane < tc;
+        const Term q0 = term_or_blank(tb.t, k0 + lane, in0);
+        const uint32_t y = g.nb[sle];
+        const uint32_t b = a == x ? y : x;
+        const uint32_t db = g.deg[b];
+        const uint8_t sa0 = g.est[q0.sa], sb0 = g.est[q0.sb];
+        const double supa0 = g.sup[q0.sa], supb0 = g.sup[q0.sb];
+        const double ewa0 = g.ew[q0.sa], ewb0 = g.ew[q0.sb];
+        const uint32_t lower = da < db ? a : b, higher = da < db ? b : a;
+        // stored terms: sa = {lo, z}, sb = {hi, z}, lo = d_x <= d_y ? x : y
+        const uint32_t lo = (a == x ? da : db) <= (a == x ? db : da) ? x : y;
+        const bool swap = lower != lo;
+        uint64_t last = 0;
+        uint32_t pushed = replay_chunk(A, stk, top, in0, q0, swap, lower, higher, sa0, sb0,
+                                       supa0, supb0, ewa0, ewb0, &last);
+        top += pushed;
+        bool any = pushed != 0;
+        const uint64_t k1 = k0 + tc;
+        for (uint64_t p = k0 + 64; p < k1; p += 64) {  // longer lists
+            const uint64_t k = p + lane;
+            const bool in = k < k1;
+            const Term q = term_or_blank(tb.t, k, in);
+            const uint32_t pushed2 = replay_chunk(
+                A, stk, top, in, q, swap, lower, higher, g.est[q.sa], g.est[q.sb],
+                g.sup[q.sa], g.sup[q.sb], g.ew[q.sa], g.ew[q.sb], &last);
+            top += pushed2;
+            any = any || pushed2 != 0;
+        }
+        if (lane == 0) {
+            g.est[e] = ST_DEAD;
+            g.sup[e] = 0.0;
+        }
+        have = any;
+        next = last;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
+    const GraphDev& g = A.g;
     const int lane = lane_id();
     for (;;) {
         uint32_t r = 0;
@@ -1111,58 +1177,34 @@ __global__ void __launch_bounds__(RT) k_replay(ReplayArgs A) {
             top += (uint32_t)__popcll(bal);
         }
         __syncthreads();
-        // The entry on top of the stack when the last pop pushed (registers;
-        // otherwise it is read back).
-        bool have = false;
-        uint64_t next = 0;
-        while (top > 0) {
-            --top;
-            const uint64_t ent = have ? next : stk[top];
-            have = false;
-            const uint32_t e = (uint32_t)(ent >> 32), a = (uint32_t)ent;
-            // Three waves of loads per pop: what depends on e (and a); the
-            // first chunk's records and e's other endpoint; the degrees and
-            // both record edges' statuses, supports and thresholds.  Lanes
-            // past the list take a blank record (term_or_blank), so that no
-            // load waits on a branch.
-            const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
-            const uint64_t k0 = tb.zoff[e];
-            const uint32_t da = g.deg[a];
-            const bool in0 = (uint32_t)lane < tc;
-            const Term q0 = term_or_blank(tb.t, k0 + lane, in0);
-            const uint32_t y = g.nb[sle];
-            const uint32_t b = a == x ? y : x;
-            const uint32_t db = g.deg[b];
-            const uint8_t sa0 = g.est[q0.sa], sb0 = g.est[q0.sb];
-            const double supa0 = g.sup[q0.sa], supb0 = g.sup[q0.sb];
-            const double ewa0 = g.ew[q0.sa], ewb0 = g.ew[q0.sb];
-            const uint32_t lower = da < db ? a : b, higher = da < db ? b : a;
-            // stored terms: sa = {lo, z}, sb = {hi, z}, lo = d_x <= d_y ? x : y
-            const uint32_t lo = (a == x ? da : db) <= (a == x ? db : da) ? x : y;
-            const bool swap = lower != lo;
-            uint64_t last = 0;
-            uint32_t pushed = replay_chunk(A, stk, top, in0, q0, swap, lower, higher, sa0, sb0,
-                                           supa0, supb0, ewa0, ewb0, &last);
-            top += pushed;
-            bool any = pushed != 0;
-            const uint64_t k1 = k0 + tc;
-            for (uint64_t p = k0 + 64; p < k1; p += 64) {  // longer lists
-                const uint64_t k = p + lane;
-                const bool in = k < k1;
-                const Term q = term_or_blank(tb.t, k, in);
-                const uint32_t pushed2 = replay_chunk(
-                    A, stk, top, in, q, swap, lower, higher, g.est[q.sa], g.est[q.sb],
-                    g.sup[q.sa], g.sup[q.sb], g.ew[q.sa], g.ew[q.sb], &last);
-                top += pushed2;
-                any = any || pushed2 != 0;
+        replay_pops(A, stk, top);
+        if (!A.dl_idx) continue;
+        // Then every later deletion call's pairs of this region, pushed in
+        // the caller's order (deleteAndClean pops the last one first), each
+        // call cascaded to convergence before the next.
+        const uint32_t l0 = A.dl_start[r], l1 = A.dl_end[r];
+        uint32_t p = l0;
+        while (p < l1) {
+            const uint32_t call = A.dl_c[A.dl_idx[p]];
+            top = 0;
+            for (;;) {
+                const uint32_t i = p + lane;
+                const uint32_t j = i < l1 ? A.dl_idx[i] : 0;
+                const bool mine = i < l1 && A.dl_c[j] == call;
+                const uint32_t e = mine ? A.dl_e[j] : 0;
+                const bool push = mine && g.est[e] == ST_ALIVE;
+                const uint64_t bal = ballot64(push);
+                if (push) {
+                    stk[top + __popcll(bal & lanes_lt())] = ((uint64_t)e << 32) | A.dl_a[j];
+                    g.est[e] = ST_QUEUED;
+                }
+                top += (uint32_t)__popcll(bal);
+                const uint64_t mm = ballot64(mine);
+                p += (uint32_t)__popcll(mm);
+                if (mm != ~0ull) break;  // this call's entries end in this chunk
             }
-            if (lane == 0) {
-                g.est[e] = ST_DEAD;
-                g.sup[e] = 0.0;
-            }
-            have = any;
-            next = last;
             __syncthreads();
+            replay_pops(A, stk, top);
         }
     }
 }
@@ -1179,6 +1221,94 @@ int bits_of(uint64_t n) {
     int b = 1;
     while ((1ull << b) < n) ++b;
     return b;
+}
+
+// ---- deletions since the initial clean, per replayed region
+constexpr uint32_t NO_REGION = 0xffffffffu;
+
+__global__ void __launch_bounds__(T) k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)T + threadIdx.x; i < n; i += (uint64_t)gridDim.x * T)
+        p[i] = v;
+}
+
+// rid[e] = region of every region edge (position i of the sorted region
+// edges belongs to region rpos[i] + rflag[i] - 1).
+__global__ void __launch_bounds__(T) k_rg_rid(const uint32_t* __restrict__ redge,
+                                              const uint32_t* __restrict__ rflag,
+                                              const uint32_t* __restrict__ rpos, uint32_t nk,
+                                              uint32_t* __restrict__ rid) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nk; i += gridDim.x * T)
+        rid[redge[i]] = rpos[i] + rflag[i] - 1u;
+}
+
+// Keys (region, entry) of the logged deletions inside replayed regions.
+__global__ void __launch_bounds__(T) k_log_keys(const uint32_t* __restrict__ dl_e, uint32_t nl,
+                                                const uint32_t* __restrict__ rid, int Bj,
+                                                uint64_t* __restrict__ keys, unsigned* cnt) {
+    for (uint32_t j = blockIdx.x * T + threadIdx.x; j < nl; j += gridDim.x * T) {
+        const uint32_t r = rid[dl_e[j]];
+        if (r != NO_REGION) keys[wave_append(cnt)] = ((uint64_t)r << Bj) | j;
+    }
+}
+
+__global__ void __launch_bounds__(T) k_log_layout(const uint64_t* __restrict__ keys, uint32_t n,
+                                                  int Bj, uint32_t* __restrict__ idx,
+                                                  uint32_t* __restrict__ start,
+                                                  uint32_t* __restrict__ end) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < n; i += gridDim.x * T) {
+        const uint64_t k = keys[i];
+        const uint32_t r = (uint32_t)(k >> Bj);
+        idx[i] = (uint32_t)(k & ((1ull << Bj) - 1));
+        if (i == 0 || (uint32_t)(keys[i - 1] >> Bj) != r) start[r] = i;
+        if (i + 1 == n || (uint32_t)(keys[i + 1] >> Bj) != r) end[r] = i + 1;
+    }
+}
+
+// Gives the replay the logged deletions of its regions, in call order.
+rtrex_status replay_log(rtrex_graph h, Scratch& sc, const uint32_t* redge, const uint32_t* rflag,
+                        const uint32_t* rpos, uint32_t nk, uint32_t nreg, ReplayArgs& A) {
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    const uint32_t nl = (uint32_t)h->dlog_e.size();
+    uint32_t *rid, *de, *da, *dc, *idx, *st, *en;
+    uint64_t *keys, *tmp;
+    unsigned* cnt;
+    RT_TRY(sc.get(&rid, g.m));
+    RT_TRY(sc.get(&de, nl));
+    RT_TRY(sc.get(&da, nl));
+    RT_TRY(sc.get(&dc, nl));
+    RT_TRY(sc.get(&keys, nl));
+    RT_TRY(sc.get(&tmp, nl));
+    RT_TRY(sc.get(&idx, nl));
+    RT_TRY(sc.get(&st, nreg));
+    RT_TRY(sc.get(&en, nreg));
+    RT_TRY(sc.get(&cnt, 1));
+    RT_HIP(hipMemcpyAsync(de, h->dlog_e.data(), nl * 4ull, hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(da, h->dlog_a.data(), nl * 4ull, hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(dc, h->dlog_c.data(), nl * 4ull, hipMemcpyHostToDevice, s));
+    k_fill_u32<<<blocks_for(g.m, T, 4096), T, 0, s>>>(rid, g.m, NO_REGION);
+    k_fill_u32<<<blocks_for(nreg, T, 4096), T, 0, s>>>(st, nreg, 0u);
+    k_fill_u32<<<blocks_for(nreg, T, 4096), T, 0, s>>>(en, nreg, 0u);
+    k_fill_u32<<<1, T, 0, s>>>(cnt, 1, 0u);
+    k_rg_rid<<<blocks_for(nk, T, 8192), T, 0, s>>>(redge, rflag, rpos, nk, rid);
+    const int Bj = bits_of((uint64_t)nl + 1), Br = bits_of((uint64_t)nreg + 1);
+    k_log_keys<<<blocks_for(nl, T, 4096), T, 0, s>>>(de, nl, rid, Bj, keys, cnt);
+    RT_HIP(hipGetLastError());
+    uint32_t n = 0;
+    RT_HIP(hipMemcpyAsync(&n, cnt, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (n) {
+        RT_TRY(radix_sort_u64(keys, tmp, n, Bj + Br, s));
+        k_log_layout<<<blocks_for(n, T, 4096), T, 0, s>>>(keys, n, Bj, idx, st, en);
+        RT_HIP(hipGetLastError());
+    }
+    A.dl_e = de;
+    A.dl_a = da;
+    A.dl_c = dc;
+    A.dl_idx = idx;
+    A.dl_start = st;
+    A.dl_end = en;
+    return RTREX_OK;
 }
 
 // Re-runs the regions of the undecided edges exactly (see above).
@@ -1262,6 +1392,7 @@ rtrex_status exact_replay(rtrex_graph h, double eps, const uint32_t* undec, uint
         RT_TRY(sc.get(&stk, nk));
         RT_HIP(hipMemsetAsync(&h->ctr->work_next, 0, 4, s));
         ReplayArgs A{g, h->terms(), eps, redge, rstart, nreg, stk, &h->ctr->work_next};
+        if (!h->dlog_e.empty()) RT_TRY(replay_log(h, sc, redge, rflag, rpos, nk, nreg, A));
         k_replay<<<std::min<uint32_t>(nreg, 8192), RT, 0, s>>>(A);
         RT_HIP(hipGetLastError());
     }
@@ -1517,14 +1648,20 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
     uint32_t* undec = nullptr;
     RT_TRY(sc.get(&undec, g.m));
     // Regions an earlier extraction could not decide are re-run as well.
-    if (terms && (h->force_all || !h->force_replay.empty())) {
-        const uint32_t nf = h->force_all ? g.m + 1 : (uint32_t)h->force_replay.size();
-        if (!h->force_all)
+    if (terms && (h->force_all || h->replay_all || !h->force_replay.empty())) {
+        const bool all = h->force_all || h->replay_all;
+        const uint32_t nf = all ? g.m + 1 : (uint32_t)h->force_replay.size();
+        if (!all)
             RT_HIP(hipMemcpyAsync(undec, h->force_replay.data(), nf * 4ull, hipMemcpyHostToDevice,
                                   s));
         RT_HIP(hipMemcpyAsync(&h->ctr->n_undecided, &nf, 4, hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
     }
+    h->dlog_e.clear();  // a new clean: no deletion since
+    h->dlog_a.clear();
+    h->dlog_b.clear();
+    h->dlog_c.clear();
+    h->dlog_calls = 0;
     RT_TRY(peel(h, eps, F, Fn, nF, L, terms, terms, terms ? undec : nullptr, g.m));
     if (!h->surv) RT_TRY(dalloc(h, &h->surv, (g.m + 31) / 32));
     k_survivor_bits<<<blocks_for(std::max<uint64_t>(terms ? g.nnz / 4 : 0, (g.m + 31) / 32), T,
@@ -1540,13 +1677,15 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps) {
 
 rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                                   int64_t k, double eps) {
-    if (h->phase < PH_COUNTED) return fail(RTREX_ESTATE, "count has not run");
+    if (h->phase < PH_CLEANED) return fail(RTREX_ESTATE, "initial clean has not run");
     if (h->phase >= PH_EXTRACTED)
         return fail(RTREX_ESTATE, "graph already extracted; run count again");
+    if (eps != h->eps) return fail(RTREX_EINVAL, "delete_and_clean needs the eps of the clean");
     if (k < 0 || (k && !pairs)) return fail(RTREX_EINVAL, "bad pair list");
     if (k == 0) return RTREX_OK;
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
+    if ((uint64_t)k > g.m) return fail(RTREX_EINVAL, "more pairs than edges");
     std::vector<uint32_t> pr(2 * k);
     for (int64_t i = 0; i < 2 * k; ++i) {
         if (pairs[i] < 0 || pairs[i] >= (int64_t)g.n)
@@ -1554,33 +1693,55 @@ rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
         pr[i] = (uint32_t)pairs[i];
     }
     Scratch sc;
-    uint32_t *F, *Fn, *dpr;
+    uint32_t *F, *Fn, *dpr, *eid;
     const bool terms = h->zt != nullptr;
     RT_TRY(sc.get(&F, g.m));
     RT_TRY(sc.get(&Fn, g.m));
     RT_TRY(sc.get(&dpr, 2 * k));
+    RT_TRY(sc.get(&eid, k));
+    // Every pair checked before anything changes: an edge of the graph,
+    // alive, listed once (the state is untouched when one is not).
+    RT_HIP(hipMemcpyAsync(dpr, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, s));
+    k_pair_edges<<<blocks_for(k, T, 4096), T, 0, s>>>(g, dpr, (uint32_t)k, eid);
+    RT_HIP(hipGetLastError());
+    std::vector<uint32_t> he(k);
+    RT_HIP(hipMemcpyAsync(he.data(), eid, k * 4ull, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    for (int64_t q = 0; q < k; ++q) {
+        if (he[q] == PAIR_NOT_EDGE)
+            return fail(RTREX_EINVAL, "pair " + std::to_string(q) + " is not an edge of the graph");
+        if (he[q] & PAIR_NOT_ALIVE)
+            return fail(RTREX_EINVAL, "pair " + std::to_string(q) + " is not an alive edge");
+    }
+    std::vector<uint32_t> sorted(he);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(RTREX_EINVAL, "an edge is listed twice");
     CleanLists L;
     RT_TRY(clean_lists(h, sc, false, L));
-    RT_HIP(hipMemcpyAsync(dpr, pr.data(), pr.size() * 4, hipMemcpyHostToDevice, s));
-    unsigned* cnt = &h->ctr->frontier_next;
-    RT_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), s));
-    RT_HIP(hipMemsetAsync(&h->ctr->error, 0, sizeof(unsigned), s));
-    k_seed_pairs<<<blocks_for(k, T, 4096), T, 0, s>>>(g, dpr, (uint32_t)k, F, cnt,
-                                                      &h->ctr->error, terms);
+    k_seed_edges<<<blocks_for(k, T, 4096), T, 0, s>>>(g, eid, (uint32_t)k, F, terms);
     RT_HIP(hipGetLastError());
-    uint32_t nF = 0;
-    unsigned err = 0;
-    RT_HIP(hipMemcpyAsync(&nF, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    RT_HIP(hipMemcpyAsync(&err, &h->ctr->error, sizeof(unsigned),
-                          hipMemcpyDeviceToHost, s));
-    RT_HIP(hipStreamSynchronize(s));
-    if (err)
-        return fail(RTREX_EINVAL,
-                    "delete_and_clean: a pair is not an alive edge of the "
-                    "graph, or is listed twice (state is undefined now)");
+    for (int64_t q = 0; q < k; ++q) {
+        h->dlog_e.push_back(he[q]);
+        h->dlog_a.push_back(pr[2 * q]);
+        h->dlog_b.push_back(pr[2 * q + 1]);
+        h->dlog_c.push_back(h->dlog_calls);
+    }
+    h->dlog_calls++;
     if (h->xact) RT_HIP(hipMemsetAsync(h->xact, 0, g.m, s));  // supports no longer the reference's
-    h->n_alive -= nF;
-    RT_TRY(peel(h, eps, F, Fn, nF, L, false, false));
+    h->n_alive -= (uint64_t)k;
+    uint32_t* undec = nullptr;
+    if (terms) {
+        RT_TRY(sc.get(&undec, g.m));
+        RT_HIP(hipMemsetAsync(&h->ctr->n_undecided, 0, 4, s));
+        if (h->replay_all) {  // every deleted edge's region re-run
+            RT_HIP(hipMemcpyAsync(undec, eid, k * 4ull, hipMemcpyDeviceToDevice, s));
+            const uint32_t nf = (uint32_t)k;
+            RT_HIP(hipMemcpyAsync(&h->ctr->n_undecided, &nf, 4, hipMemcpyHostToDevice, s));
+            RT_HIP(hipStreamSynchronize(s));
+        }
+    }
+    RT_TRY(peel(h, eps, F, Fn, (uint32_t)k, L, false, false, undec, g.m));
     if (terms) k_slot_status<<<blocks_for(g.nnz / 4 + 1, T, 8192), T, 0, s>>>(g);
     RT_HIP(hipGetLastError());
     RT_HIP(hipStreamSynchronize(s));

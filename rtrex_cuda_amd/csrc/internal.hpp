@@ -142,7 +142,7 @@ constexpr uint32_t HUB_MAX_LIMIT = 32760;
 // shorter-list length against g_thr[0..3].  Work units carry at least
 // unit_min streamed elements (classes above l1_max: unit_hash_mult * d_h).
 struct CountParams {
-    uint32_t hub_min = 8, hub_max = HUB_MAX_LIMIT;
+    uint32_t hub_min = 0, hub_max = HUB_MAX_LIMIT;
     uint32_t l0_max = 64, l1_max = 512, l2_max = 4096;
     uint32_t g_thr[4] = {3, 6, 24, 80};
     uint32_t unit_min = 1024, unit_hash_mult = 8;
@@ -180,6 +180,12 @@ struct rtrex_graph_s {
     uint32_t bcnt_ratio = 0;        // max over k >= 2 of ceil(bcnt[k] / k)
     std::vector<uint32_t> x_undec;  // edges the last extraction could not decide
     bool x_undec_all = false;       // ... more than the list holds
+    // Deletions since the initial clean (rtrex_delete_and_clean), in call
+    // order: per entry the edge, the first endpoint of the pair as given
+    // (deleteAndClean's "lower" on a degree tie is the second), the call.
+    std::vector<uint32_t> dlog_e, dlog_a, dlog_b, dlog_c;
+    uint32_t dlog_calls = 0;
+    bool replay_all = false;  // every clean decision re-run in the reference's order (checking)
     std::vector<uint32_t> force_replay;  // edges whose regions the clean re-runs
     bool force_all = false;
     double force_eps = -1;  // the eps force_replay was found at
@@ -412,6 +418,9 @@ rtrex_status run_extract(rtrex_graph h, double eps);
 // The cluster map of the last extraction, as int64 per vertex (per-graph ids
 // on a batch handle).
 rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out);
+// True when p is page-locked host memory the device can address; *dptr is
+// its device address.
+bool mapped_host(const void* p, void** dptr);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
 rtrex_status run_cluster_stats_noalloc(rtrex_graph h, int64_t* nv, int64_t* ne,

@@ -257,3 +257,15 @@ def test_components_closed_form():
     assert k == len(want)
     for i, w in enumerate(want):
         assert np.all(comp[w] == i)
+
+
+@pytest.mark.parametrize("tag", ["clean_astro_eps0.1", "clean_email_eps0.3", "config1_eps0.1"])
+def test_extract_from_state_equals_decompose(tag):
+    """The extraction entered from the initial clean's state is the whole
+    decomposition's (the same seed loop; pinned by the golden clusters)."""
+    meta = golden_meta()[tag]
+    off, nb = _csr(tag.rsplit("_eps", 1)[0])
+    cl = O.initial_clean(off, nb, meta["eps"])
+    cm, k = O.extract_from_state(off, nb, meta["eps"], cl)
+    assert k == meta["n_clusters"]
+    assert np.array_equal(cm, golden_result(tag)["cluster_map"])
