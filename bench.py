@@ -1,20 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark: RTRExtractor decomposition (count -> clean -> extract) on MI355X.
 
-One step = one full decomposition of the workload graph that is already
-resident in HBM (device CSR built once before timing): weighted triangle
-supports, cascading clean to convergence, greedy extraction, cluster ids and
-survivor status back in host memory.  Prints ONE JSON line.
+One step = one full decomposition of the workload graph, whose device CSR was
+built before timing: weighted triangle supports, cascading clean to
+convergence, greedy extraction, and the results -- cluster id of every vertex
+and the packed survivor bitmap -- in host memory (page-locked buffers the
+device writes directly).  Prints ONE JSON line; `value` is that rate for the
+medium planted-community workload (C2).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--eps 0.3]
-                    [--workload c2|c1] [--no-cpu]
+                    [--workload c2|c1|c3|rmat20] [--no-cpu] [--c3-steps S]
+                    [--batch-graphs B] [--serve-seconds T]
 
-Roofline: the count phase (per-edge + per-vertex fold kernels) is priced with
-the algorithmic bytes B = 8*S_min + 28*m + 16*n, S_min = sum over edges of
-min(d_x, d_y) (4-byte id streamed + 4-byte probe per element of the shorter
-list, CSR read once, two FP64 supports per edge, one per vertex), divided by
-the count phase's device time measured with HIP events on the library's
-stream.  Peak: MI355X HBM3E 8 TB/s.
+Beside the headline: the same steps with the results left in HBM, with the
+upload and device CSR build, the skewed large R-MAT scale-22 graph (C3), the
+4,096-graph batch against the plain path (C4, with its first call cold), and a
+60 s Poisson serving trace with p50/p99 latency overall and per request kind
+(C5).
+
+Roofline: the count phase (its kernels' HIP-event time on the library's
+stream) priced with the algorithmic bytes B = 8*S_min + 28*m + 16*n, S_min =
+sum over edges of min(d_x, d_y) (4-byte id streamed + 4-byte probe per element
+of the shorter list, CSR read once, two FP64 supports per edge, one per
+vertex).  Peak: MI355X HBM3E 8 TB/s.  `traffic` is the HBM bytes the count
+kernels moved per phase (FETCH_SIZE x 2 + WRITE_SIZE, from rocprofv3; every
+read of these kernels is a 128-byte request, which FETCH_SIZE counts as 64 B on
+gfx950), read from profiles/ when present.
 
 CPU baseline: the reference's own sequential code (oracle/_ref/ref_dump, built
 from the reference sources) on the same graph, 1 core, when that binary is
@@ -37,6 +48,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench.py [{time.perf_counter() - T_START:6.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def workload(name: str):
@@ -71,8 +88,15 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
     m = int(sum(len(g[1]) for g in graphs))
     n = int(sum(g[0] for g in graphs))
     B_count = sum(algorithmic_count_bytes(*g)[0] for g in graphs)
+    t0 = time.perf_counter()
     b = R.Batch(graphs)
-    b.decompose(eps)  # warmup (plans, triangle lists)
+    build_s = time.perf_counter() - t0
+    # The first call on the fresh batch: count plans, triangle-list sizing,
+    # and the discovery and exact re-run of undecided regions (later calls at
+    # the same eps re-run the remembered regions directly).
+    t0 = time.perf_counter()
+    b.decompose(eps)
+    cold_ms = (time.perf_counter() - t0) * 1e3
     params, tbest, tstart = b.autotune(repeats=2)
     b.decompose(eps)
     tms = []
@@ -104,6 +128,8 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
                         "communities 5-40 p=0.4, background mean degree 6, seed 23",
             "graphs": count, "n": n, "m": m, "generation_s": round(gen_s, 1),
             "batched_value": round(m / bt / 1e6, 3), "batched_ms": round(bt * 1e3, 3),
+            "upload_and_build_s": round(build_s, 3),
+            "first_call_cold_ms": round(cold_ms, 3),
             "plain_value": round(pm / pt / 1e6, 3),
             "plain_sample": f"{len(sample)} of {count} graphs (every {stride}th), "
                             f"{pm} edges in {pt:.3f} s",
@@ -168,13 +194,25 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
         th = threading.Thread(target=waiter)
         th.start()
         start = srv.clock() + 0.05
+        next_note = 10.0
         for i, (t, kind, idx) in enumerate(trace):
+            if t >= next_note:
+                progress(f"c5: {t:.0f} s of the trace submitted")
+                next_note += 10.0
             while srv.clock() < start + t:
                 time.sleep(min(0.0005, max(0.0, start + t - srv.clock())))
             q.put((i, srv.submit(*graph(kind, idx), eps)))
         q.put(None)
         th.join()
     lat = np.array([done[i][1]["end_s"] - (start + t) for i, (t, _, _) in enumerate(trace)])
+    kinds = np.array([k for _, k, _ in trace])
+    by_kind = {}
+    for kind in ("cl", "planted", "rmat"):
+        lk = lat[kinds == kind]
+        if len(lk):
+            by_kind[kind] = {"requests": int(len(lk)),
+                             "p50": round(float(np.percentile(lk, 50)) * 1e3, 2),
+                             "p99": round(float(np.percentile(lk, 99)) * 1e3, 2)}
     edges = np.array([len(graph(k, i)[1]) for _, k, i in trace], np.float64)
     makespan = max(d[1]["end_s"] for d in done) - start
     batched = np.array([d[1]["batch_graphs"] for d in done])
@@ -200,13 +238,105 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
             "makespan_s": round(makespan, 3),
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                           "max": round(float(lat.max()) * 1e3, 2)},
+                           "max": round(float(lat.max()) * 1e3, 2), "by_kind": by_kind},
             "mean_graphs_per_launch": round(float(batched.mean()), 2),
             "workers": workers, "hbm_budget_GB": budget_gb,
             "serial_plain_Medges_per_s": round(sm / st / 1e6, 2),
             "serial_plain_sample": f"{len(sample)} requests (every 10th), {st:.2f} s",
             "served_vs_serial": round((edges.sum() / makespan) / (sm / st), 2),
             "same_cluster_map_as_alone": f"{same}/{len(sample)} sampled requests"}
+
+
+def plain_leg(R, n, u, v, eps, steps, warmup, tune, upload) -> dict:
+    """One graph through the plain path: `steps` timed decompositions with the
+    results in host memory, the same with the results left in HBM, and
+    (upload) with the edge list uploaded and the device CSR built each step."""
+    m = len(u)
+    g = R.DeviceGraph.from_edges(n, u, v)  # resident in HBM before timing
+    cm = R.host_empty(n, np.int64)  # page-locked: the device writes the ids
+    words = R.host_empty((m + 31) // 32, np.uint32)
+    for _ in range(1 + warmup):  # the first call builds the count plan and lists
+        g.decompose(eps, out=cm)
+        g.survivor_words(out=words)
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, k = g.decompose(eps, out=cm)  # cluster ids in host memory
+        g.survivor_words(out=words)     # survivor bitmap in host memory
+        phases.append(g.timings())
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    audit = g.audit()
+    survivors = int(np.unpackbits(words.view(np.uint8))[:m].sum())
+    # The same steps with both results left in device memory.
+    nr = max(1, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(nr):
+        g.decompose(eps, host_map=False)
+    ms_res = (time.perf_counter() - t0) * 1e3 / nr
+    tuned = {}
+    if tune:
+        params, best, start = g.autotune(repeats=2)
+        tuned = {"params": params, "count_ms_default": round(start, 3),
+                 "count_ms_tuned": round(best, 3)}
+    g.close()
+    out = {"ms_per_step": ms, "clusters": k, "survivors": survivors, "audit": audit,
+           "autotune": tuned,
+           "phases_ms": {key: round(float(np.mean([p[key] for p in phases])), 3)
+                         for key in ("count_ms", "clean_ms", "extract_ms",
+                                     "count_edge_kernel_ms", "count_vertex_kernel_ms")},
+           "results_resident": {"value": round(m / (ms_res / 1e3) / 1e6, 3),
+                                "ms_per_step": round(ms_res, 3),
+                                "note": "the same steps, cluster ids and survivor bits left "
+                                        "in HBM"}}
+    if upload:
+        nb_steps = max(1, steps // 3)
+        t0 = time.perf_counter()
+        for _ in range(nb_steps):
+            with R.DeviceGraph.from_edges(n, u, v) as g2:
+                g2.decompose(eps, out=cm)
+                g2.survivor_words(out=words)
+        ms_build = (time.perf_counter() - t0) * 1e3 / nb_steps
+        out["with_upload_and_build"] = {
+            "value": round(m / (ms_build / 1e3) / 1e6, 3), "ms_per_step": round(ms_build, 3),
+            "note": "host edge list -> device CSR build -> decomposition -> results on the host"}
+    return out
+
+
+def c3_leg(R, eps: float, steps: int) -> dict:
+    """C3: the skewed large R-MAT scale-22 graph through the plain path, and
+    its count phase against the roofline (S_min computed exactly)."""
+    desc, n, u, v = workload("c3")
+    progress(f"c3: generated, m={len(u)}")
+    B, s_min = algorithmic_count_bytes(n, u, v)
+    t0 = time.perf_counter()
+    r = plain_leg(R, n, u, v, eps, steps, 0, False, upload=False)
+    total_s = time.perf_counter() - t0
+    m = len(u)
+    cnt_ms = r["phases_ms"]["count_ms"]
+    ach = B / (cnt_ms / 1e3) / 1e9
+    return {"workload": desc, "n": n, "m": m, "eps": eps, "steps": steps,
+            "value": round(m / (r["ms_per_step"] / 1e3) / 1e6, 3), "unit": "M edges/s",
+            "ms_per_step": round(r["ms_per_step"], 3), "phases_ms": r["phases_ms"],
+            "clusters": r["clusters"], "survivors": r["survivors"], "audit": r["audit"],
+            "count_roofline": {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": B,
+                               "s_min": s_min},
+            "leg_wall_s": round(total_s, 1),
+            "note": "graph resident (its build and first decomposition untimed), results in "
+                    "host memory"}
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(n, u, v, eps) -> dict:
@@ -221,7 +351,7 @@ def cpu_baseline(n, u, v, eps) -> dict:
             r = O.ref_dump(p, eps, extract=True)
         t = r["times"]["full_pipeline"]  # disjointExtract: count + clean + extract
         return {"value": round(len(u) / t / 1e6, 4), "unit": "M edges/s", "cores": 1,
-                "kind": "reference",
+                "kind": "reference", "host": host_cpu(),
                 "sample": f"full workload once through the reference's disjointExtract "
                           f"(RTRex_Sequential, mode s), {t:.2f} s; file load excluded"}
     off, nb = O.csr_from_undirected(n, u, v)
@@ -229,7 +359,7 @@ def cpu_baseline(n, u, v, eps) -> dict:
     O.decompose(off, nb, eps, with_clusters=False)
     t = time.perf_counter() - t0
     return {"value": round(len(u) / t / 1e6, 4), "unit": "M edges/s", "cores": 1,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"full workload once through the in-repo C++ restatement, {t:.2f} s"}
 
 
@@ -246,7 +376,9 @@ def main() -> int:
                     help="skip the count-plan autotuner (default parameters)")
     ap.add_argument("--batch-graphs", type=int, default=4096,
                     help="C4 batched-vs-plain leg (0 skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--c3-steps", type=int, default=2,
+                    help="C3 leg (R-MAT scale 22): timed decompositions (0 skips it)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="C5 serving leg: length of the request trace (0 skips it)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=128.0,
@@ -269,48 +401,13 @@ def main() -> int:
     desc, n, u, v = workload(args.workload)
     m = len(u)
     B_count, s_min = algorithmic_count_bytes(n, u, v)
-
-    g = R.DeviceGraph.from_edges(n, u, v)  # resident in HBM before timing
-    g.decompose(args.eps)  # count plan and term buffer
-    for _ in range(args.warmup):
-        g.decompose(args.eps)
-    phases = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        # count -> clean -> extract; cluster ids and survivor bits stay in HBM
-        _, k = g.decompose(args.eps, host_map=False)
-        phases.append(g.timings())
-    wall = time.perf_counter() - t0
-    audit = g.audit()
-    # The same steps with both results copied to host memory each time.
-    nh_steps = max(1, args.steps // 2)
-    t0 = time.perf_counter()
-    for _ in range(nh_steps):
-        cm, k = g.decompose(args.eps)  # cluster ids in host memory
-        alive = g.survivors()          # survivor bitmap in host memory
-    ms_host = (time.perf_counter() - t0) * 1e3 / nh_steps
-    # The timed steps use the library's default count plan (so that the
-    # rocprof summaries of this command match them); the autotuner's result on
-    # this graph is reported beside them.
-    tuned = {}
-    if not args.no_tune:
-        params, best, start = g.autotune(repeats=2)
-        tuned = {"params": params, "count_ms_default": round(start, 3),
-                 "count_ms_tuned": round(best, 3)}
-    g.close()
-    ms = wall * 1e3 / args.steps
-    # Same work including the host->device copy of the edge list and the
-    # device CSR build (not the headline value: inputs are not resident).
-    nb_steps = max(1, args.steps // 3)
-    t0 = time.perf_counter()
-    for _ in range(nb_steps):
-        with R.DeviceGraph.from_edges(n, u, v) as g2:
-            g2.decompose(args.eps)
-            g2.survivors()
-    ms_build = (time.perf_counter() - t0) * 1e3 / nb_steps
+    progress(f"{args.workload}: n={n} m={m}")
+    plain = plain_leg(R, n, u, v, args.eps, args.steps, args.warmup, not args.no_tune,
+                      upload=True)
+    progress(f"{args.workload}: {plain['ms_per_step']:.3f} ms/step")
+    ms = plain["ms_per_step"]
     value = m / (ms / 1e3) / 1e6
-
-    cnt_ms = float(np.mean([p["count_ms"] for p in phases]))
+    cnt_ms = plain["phases_ms"]["count_ms"]
     achieved = B_count / (cnt_ms / 1e3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"count_{args.workload}_traffic.json")
@@ -319,7 +416,8 @@ def main() -> int:
             traffic = json.load(f).get("bytes_per_count_phase")
 
     out = {
-        "metric": "M edges/s count+clean+extract (plain, graph and results resident in HBM)",
+        "metric": "M edges/s count+clean+extract (plain; graph resident in HBM, cluster ids "
+                  "and survivor bitmap in host memory)",
         "value": round(value, 3),
         "unit": "M edges/s",
         "n_gpus": 1,
@@ -332,33 +430,30 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (seeded generator; float32 edge weights generated but unused by "
                 "the path, as in the reference)",
-        "config": {"workload": desc, "n": n, "m": m, "eps": args.eps, "clusters": k,
-                   "survivors": int(alive.sum())},
-        "phases_ms": {key: round(float(np.mean([p[key] for p in phases])), 3)
-                      for key in ("count_ms", "clean_ms", "extract_ms",
-                                  "count_edge_kernel_ms", "count_vertex_kernel_ms")},
+        "config": {"workload": desc, "n": n, "m": m, "eps": args.eps,
+                   "clusters": plain["clusters"], "survivors": plain["survivors"]},
+        "phases_ms": plain["phases_ms"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "count phase (k_count_hub_flat, "
                      "k_count_hub, k_count_hub_large, k_count_hub_huge, k_count_edges, "
                      "k_sum_tcnt)",
                      "algorithmic_bytes": B_count, "s_min": s_min},
-        "audit": audit,
-        "autotune": tuned,
-        "with_results_on_host": {"value": round(m / (ms_host / 1e3) / 1e6, 3),
-                                 "ms_per_step": round(ms_host, 3),
-                                 "note": "the same steps, cluster map (int64 per vertex) and "
-                                         "survivor bits copied to host memory each step"},
-        "with_upload_and_build": {"value": round(m / (ms_build / 1e3) / 1e6, 3),
-                                  "ms_per_step": round(ms_build, 3),
-                                  "note": "host edge list -> device CSR build -> "
-                                          "decomposition -> results on the host"},
+        "audit": plain["audit"],
+        "autotune": plain["autotune"],
+        "results_resident": plain["results_resident"],
+        "with_upload_and_build": plain["with_upload_and_build"],
     }
+    if args.c3_steps > 0 and args.workload != "c3":
+        out["skewed_large"] = c3_leg(R, args.eps, args.c3_steps)
+        progress(f"c3: {out['skewed_large']['ms_per_step']:.1f} ms/step")
     if args.batch_graphs > 0:
         out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))
+        progress(f"c4: {out['batched']['batched_ms']:.1f} ms/step batched")
     if args.serve_seconds > 0:
         out["serving"] = serving_leg(args.eps, args.serve_seconds, 200.0, args.serve_workers,
                                      args.serve_budget_gb)
+        progress("c5: done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, u, v, args.eps)
     print(json.dumps(out), flush=True)

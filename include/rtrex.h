@@ -116,6 +116,14 @@ rtrex_status rtrex_initial_clean(rtrex_graph g, double eps);
  * syncEdgeStatusDeviceToHost (gpu_graph.cuh:76). */
 rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status);
 
+/* Page-locked host memory that the device can address, for result arrays
+ * (cluster maps, survivor words): a single graph's cluster map given in it is
+ * written by the device directly, with no staging copy.  No reference
+ * counterpart (the reference's results are ordinary host arrays,
+ * gpu_graph.cuh:67-81).  rtrex_host_free(NULL) is a no-op. */
+rtrex_status rtrex_host_alloc(size_t bytes, void** out);
+void rtrex_host_free(void* p);
+
 /* Survivors of the last rtrex_initial_clean, packed: bit (e % 32) of
  * bits[e / 32] is set when undirected edge e is alive, edges numbered in
  * (u < v) lexicographic order -- the order of the reference's sorted survivor
@@ -273,6 +281,20 @@ rtrex_status rtrex_write_decomposition(const char* name, int64_t n,
                                        const int64_t* cluster_of,
                                        const int64_t* member_rank,
                                        int64_t n_clusters);
+
+/* expandClusters (Decomposition.h:943-1008) on the clusters given as for
+ * rtrex_write_decomposition, then the same four files for the expanded
+ * clusters plus <name>-scores.txt (printScores, RTRex.cpp:120-135): each
+ * singleton cluster's vertex joins the neighbouring cluster holding the most
+ * of its neighbours, when that count exceeds threshold (the reference
+ * driver's mode e, RTRex.cpp:166-174,196-201, writes them with the name
+ * "RTRex-expanded-<name>").  expanded_of[n] (nullable) receives the cluster
+ * of every vertex after expansion; score[n] (nullable) the moved vertices'
+ * score (neighbours in the cluster / degree), -1 for the others. */
+rtrex_status rtrex_write_expanded(const char* name, int64_t n, const int64_t* offsets,
+                                  const int64_t* nbors, const int64_t* cluster_of,
+                                  const int64_t* member_rank, int64_t n_clusters,
+                                  int32_t threshold, int64_t* expanded_of, double* score);
 
 /* The same four files for the connected components, connectedComp
  * (Decomposition.h:1023-1068), as RTRex.cpp:203-204 writes them. */

@@ -39,14 +39,14 @@ EXPORTS = [
     "rtrex_graph_create_csr", "rtrex_graph_create_edges", "rtrex_graph_destroy",
     "rtrex_graph_info", "rtrex_graph_get_csr", "rtrex_graph_footprint_bytes",
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
-    "rtrex_get_survivors",
+    "rtrex_get_survivors", "rtrex_host_alloc", "rtrex_host_free",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
     "rtrex_cluster_stats_noalloc",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
     "rtrex_server_stop", "rtrex_get_member_order", "rtrex_get_cluster_map", "rtrex_write_decomposition",
-    "rtrex_write_components", "rtrex_write_survivors", "rtrex_get_count_params",
+    "rtrex_write_components", "rtrex_write_survivors", "rtrex_write_expanded", "rtrex_get_count_params",
     "rtrex_set_count_params", "rtrex_autotune_count",
 ]
 
@@ -158,10 +158,13 @@ def lib() -> C.CDLL:
             "rtrex_get_cluster_map": [G, P64],
             "rtrex_write_decomposition": [C.c_char_p, C.c_int64, P64, P64, P64, P64, C.c_int64],
             "rtrex_write_components": [C.c_char_p, C.c_int64, P64, P64],
+            "rtrex_write_expanded": [C.c_char_p, C.c_int64, P64, P64, P64, P64, C.c_int64,
+                                     C.c_int32, P64, PD],
             "rtrex_write_survivors": [C.c_char_p, C.c_int64, P64, P64, PU8],
             "rtrex_get_count_params": [G, C.POINTER(CountParams)],
             "rtrex_set_count_params": [G, C.POINTER(CountParams)],
             "rtrex_autotune_count": [G, C.c_int, C.POINTER(CountParams), PD, PD],
+            "rtrex_host_alloc": [C.c_size_t, C.POINTER(C.c_void_p)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -173,6 +176,8 @@ def lib() -> C.CDLL:
         L.rtrex_graph_footprint_bytes.restype = C.c_int64
         L.rtrex_server_clock.argtypes = [G]
         L.rtrex_server_clock.restype = C.c_double
+        L.rtrex_host_free.argtypes = [C.c_void_p]
+        L.rtrex_host_free.restype = None
         _lib = L
     return _lib
 
@@ -180,6 +185,21 @@ def lib() -> C.CDLL:
 def _check(rc: int) -> None:
     if rc != 0:
         raise RtrexError(rc, lib().rtrex_last_error().decode())
+
+
+def host_empty(shape, dtype=np.int64) -> np.ndarray:
+    """An uninitialised numpy array in page-locked host memory the device can
+    address (rtrex_host_alloc); result arrays there are written by the device
+    directly.  The memory is released when the array is garbage-collected."""
+    import weakref
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(1, count * dt.itemsize)
+    ptr = C.c_void_p()
+    _check(lib().rtrex_host_alloc(nbytes, C.byref(ptr)))
+    buf = (C.c_byte * nbytes).from_address(ptr.value)
+    weakref.finalize(buf, lib().rtrex_host_free, ptr.value)
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
 
 
 def _p(a: np.ndarray, t):
@@ -287,6 +307,14 @@ class DeviceGraph:
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return bits[:m].astype(bool)
 
+    def survivor_words(self, out: np.ndarray | None = None) -> np.ndarray:
+        """The packed survivor bitmap as rtrex_get_survivors returns it: bit
+        e % 32 of word e // 32 set when undirected edge e is alive."""
+        m = self.nnz // 2
+        words = out if out is not None else np.empty((m + 31) // 32, np.uint32)
+        _check(lib().rtrex_get_survivors(self._h, _p(words, PU32), len(words)))
+        return words
+
     def gpu_delete_and_clean(self, pairs, eps: float) -> None:
         pr = np.ascontiguousarray(pairs, np.int64).reshape(-1)
         _check(lib().rtrex_delete_and_clean(self._h, _p(pr, P64), len(pr) // 2, float(eps)))
@@ -297,11 +325,12 @@ class DeviceGraph:
         _check(lib().rtrex_extract(self._h, float(eps), _p(cm, P64), C.byref(k)))
         return cm, int(k.value)
 
-    def decompose(self, eps: float, host_map: bool = True):
+    def decompose(self, eps: float, host_map: bool = True, out: np.ndarray | None = None):
         """count -> clean -> extract.  Returns (cluster map, cluster count);
         with host_map=False the map stays in HBM (returned as None; fetch it
-        later with cluster_map())."""
-        cm = np.empty(self.n, np.int64) if host_map else None
+        later with cluster_map()).  out: an int64 array of n entries to write
+        the map into (host_empty memory is written by the device directly)."""
+        cm = (out if out is not None else np.empty(self.n, np.int64)) if host_map else None
         k = C.c_int64()
         _check(lib().rtrex_decompose(self._h, float(eps), _p(cm, P64) if host_map else None,
                                      C.byref(k)))
@@ -481,6 +510,24 @@ def write_decomposition(name: str, offsets, nbors, cluster_of, member_rank, n_cl
     mr = np.ascontiguousarray(member_rank, np.int64)
     _check(lib().rtrex_write_decomposition(name.encode(), len(off) - 1, _p(off, P64), _p(nb, P64),
                                            _p(co, P64), _p(mr, P64), int(n_clusters)))
+
+
+def write_expanded(name: str, offsets, nbors, cluster_of, member_rank, n_clusters: int,
+                   threshold: int):
+    """expandClusters + the four files and <name>-scores.txt, as the
+    reference driver's mode e writes them (Decomposition.h:943-1008,
+    RTRex.cpp:120-135,166-174,196-201).  Returns (expanded cluster per vertex,
+    score per vertex, -1 where the vertex did not move)."""
+    off, nb = _csr64(offsets, nbors)
+    co = np.ascontiguousarray(cluster_of, np.int64)
+    mr = np.ascontiguousarray(member_rank, np.int64)
+    n = len(off) - 1
+    eo = np.empty(n, np.int64)
+    sc = np.empty(n, np.float64)
+    _check(lib().rtrex_write_expanded(name.encode(), n, _p(off, P64), _p(nb, P64), _p(co, P64),
+                                      _p(mr, P64), int(n_clusters), int(threshold), _p(eo, P64),
+                                      _p(sc, PD)))
+    return eo, sc
 
 
 def write_components(name: str, offsets, nbors):

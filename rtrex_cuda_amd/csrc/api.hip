@@ -178,6 +178,9 @@ rtrex_status new_handle(int device, rtrex_graph* out, hipStream_t stream = nullp
     RT_TRY(open_device(device));
     rtrex_graph h = new rtrex_graph_s();
     h->device = device;
+    // Checking mode: every cleaning decision re-run in the reference's own
+    // order (regions of all removed edges), whatever the certification says.
+    if (const char* ra = std::getenv("RTREX_REPLAY_ALL")) h->replay_all = std::atoi(ra) != 0;
     hipError_t e = hipSuccess;
     if (stream) {
         h->stream = stream;
@@ -514,6 +517,18 @@ rtrex_status rtrex_get_edge_status(rtrex_graph g, uint8_t* status) {
     return RTREX_OK;
 }
 
+rtrex_status rtrex_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(RTREX_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!bytes) return RTREX_OK;
+    RT_HIP(hipHostMalloc(out, bytes, hipHostMallocMapped | hipHostMallocPortable));
+    return RTREX_OK;
+}
+
+void rtrex_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords) {
     RT_TRY(check(g));
     if (!g->surv) return fail(RTREX_ESTATE, "initial clean has not run");
@@ -553,8 +568,21 @@ rtrex_status extract_exact(rtrex_graph g, double eps) {
         g->force_replay.erase(std::unique(g->force_replay.begin(), g->force_replay.end()),
                               g->force_replay.end());
         if (g->force_replay.size() > g->g.m) g->force_all = true;
+        // The deletions made since the initial clean are made again, call by
+        // call, after the clean re-runs (it clears them).
+        const std::vector<uint32_t> la = g->dlog_a, lb = g->dlog_b, lc = g->dlog_c;
         RT_TRY(run_count(g));
         RT_TRY(run_initial_clean(g, eps));
+        for (size_t i = 0; i < lc.size();) {
+            std::vector<int64_t> pr;
+            size_t j = i;
+            for (; j < lc.size() && lc[j] == lc[i]; ++j) {
+                pr.push_back(la[j]);
+                pr.push_back(lb[j]);
+            }
+            RT_TRY(run_delete_and_clean(g, pr.data(), (int64_t)(j - i), eps));
+            i = j;
+        }
         RT_TRY(run_extract(g, eps));
     }
     return RTREX_OK;
@@ -680,7 +708,7 @@ rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params
         CountParams c = cur;
         switch (coord) {
             case 0:
-                for (uint32_t v : {4u, 8u, 16u, 32u, 64u}) { c.hub_min = v; cand.push_back(c); }
+                for (uint32_t v : {0u, 4u, 8u, 16u, 32u, 64u}) { c.hub_min = v; cand.push_back(c); }
                 break;
             case 1:
                 for (auto lv : {std::pair<uint32_t, uint32_t>{64, 512}, {32, 512}, {64, 256},

@@ -1290,10 +1290,38 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     return RTREX_OK;
 }
 
+namespace {
+// Cluster ids widened to int64 straight into page-locked host memory that
+// the device can address (rtrex_host_alloc): no staging copy, no host loop.
+__global__ void __launch_bounds__(T) k_cid_out(const uint32_t* __restrict__ cid, uint32_t n,
+                                              int64_t* __restrict__ out) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) out[v] = cid[v];
+}
+}  // namespace
+
+bool mapped_host(const void* p, void** dptr) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // ordinary pageable memory is not known to HIP
+        return false;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer) return false;
+    *dptr = a.devicePointer;
+    return true;
+}
+
 rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out) {
     if (h->phase < PH_EXTRACTED || !h->cid) return fail(RTREX_ESTATE, "extract has not run");
     const uint32_t n = h->g.n;
     if (!n) return RTREX_OK;
+    void* dout = nullptr;
+    if (h->seg_bounds.empty() && mapped_host(out, &dout)) {
+        k_cid_out<<<blocks_for(n, T, 4096), T, 0, h->stream>>>(h->cid, n,
+                                                               static_cast<int64_t*>(dout));
+        RT_HIP(hipGetLastError());
+        RT_HIP(hipStreamSynchronize(h->stream));
+        return RTREX_OK;
+    }
     if (!h->cm_pin) RT_HIP(hipHostMalloc((void**)&h->cm_pin, (uint64_t)n * 4, hipHostMallocDefault));
     RT_HIP(hipMemcpyAsync(h->cm_pin, h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, h->stream));
     RT_HIP(hipStreamSynchronize(h->stream));

@@ -319,3 +319,18 @@ def test_fresh_handles_small_graphs_after_a_large_one():
             assert g.audit()["ambiguous_clean"] == 0
         assert k == d.n_clusters
         assert np.array_equal(cm, d.cluster_map)
+
+
+def test_results_into_page_locked_host_memory():
+    """Cluster map and survivor words written into rtrex_host_alloc memory
+    (the device writes the map directly) equal the ordinary copies."""
+    n, u, v = golden_graph("clean_astro")
+    with R.DeviceGraph.from_edges(n, u, v) as g:
+        cm0, k0 = g.decompose(0.1)
+        w0 = g.survivor_words()
+        cm = R.host_empty(n, np.int64)
+        cm[:] = -7
+        words = R.host_empty(len(w0), np.uint32)
+        _, k = g.decompose(0.1, out=cm)
+        g.survivor_words(out=words)
+    assert k == k0 and np.array_equal(cm, cm0) and np.array_equal(words, w0)
