@@ -5,9 +5,10 @@
 //   rtrex <graph_file> <output_name> <eps> <gpu_device>  main.cu:343-391
 // A numeric fourth argument selects a GPU the way main.cu does and writes the
 // "RTRex-CUDA-<name>" files only; otherwise the mode letter is RTRex.cpp's
-// (s simple, p parallel: the same device path here; e expandClusters is not
-// part of this build) and the files are those RTRex.cpp writes:
+// (s simple, p parallel: the same device path here; e expand, followed by
+// the threshold, RTRex.cpp:166-174) and the files are those RTRex.cpp writes:
 //   RTRex-<name>-{decomposition,stats,cut,freq}.txt
+//   RTRex-expanded-<name>-{decomposition,stats,cut,freq,scores}.txt (mode e)
 //   connected-components-<name>-{decomposition,stats,cut,freq}.txt
 //   amazon_cleaned_edges.txt (the survivor dump of initialClean)
 // All of them byte-identical to the reference's, written in the working
@@ -51,8 +52,8 @@ int main(int argc, char** argv) {
         else pos.push_back(argv[i]);
     }
     if (pos.size() < 3) {
-        std::printf("Usage: %s <graph_file> <output_name> <eps> [mode s|p | gpu_device] "
-                    "[--device N]\n",
+        std::printf("Usage: %s <graph_file> <output_name> <eps> [mode s|p|e [threshold] | "
+                    "gpu_device] [--device N]\n",
                     argv[0]);
         return 1;
     }
@@ -70,11 +71,16 @@ int main(int argc, char** argv) {
     }
     std::printf("eps = %f\n", eps);
     if (!cuda_names) std::printf("mode = %c\n", mode);
+    int threshold = 0;
     if (mode == 'e') {
-        std::fprintf(stderr, "rtrex: mode e (expandClusters) is not part of this build\n");
-        return 1;
+        if (pos.size() <= 4) {
+            std::printf("Error: Missing threshold value\n");
+            return 1;
+        }
+        threshold = std::atoi(pos[4]);
+        std::printf("threshold = %i\n", threshold);
     }
-    if (mode != 's' && mode != 'p') {
+    if (mode != 's' && mode != 'p' && mode != 'e') {
         std::fprintf(stderr, "rtrex: unknown mode %c\n", mode);
         return 1;
     }
@@ -126,6 +132,11 @@ int main(int argc, char** argv) {
     if (rtrex_write_decomposition(pre.c_str(), nn, off.data(), nb.data(), cluster_of.data(),
                                   rank.data(), k) != RTREX_OK)
         return die("write decomposition");
+    if (mode == 'e' &&
+        rtrex_write_expanded(("RTRex-expanded-" + name).c_str(), nn, off.data(), nb.data(),
+                             cluster_of.data(), rank.data(), k, threshold, nullptr,
+                             nullptr) != RTREX_OK)
+        return die("write expanded");
     if (!cuda_names &&
         rtrex_write_components(("connected-components-" + name).c_str(), nn, off.data(),
                                nb.data()) != RTREX_OK)

@@ -1,7 +1,8 @@
 // Output files of the reference's driver, byte for byte (host code only).
 //
 // Reference: printDecomposition (RTRex_Sequential/clustering/RTRex.cpp:19-118),
-// populateStats (Escape/Decomposition.h:473-491), connectedComp
+// expandClusters (Decomposition.h:943-1008) and printScores
+// (RTRex.cpp:120-135), populateStats (Escape/Decomposition.h:473-491), connectedComp
 // (Decomposition.h:1023-1068) and the survivor dump of initialClean
 // (Decomposition.h:419-452).
 //
@@ -17,8 +18,10 @@
 #include <algorithm>
 #include <cinttypes>
 #include <cstdio>
+#include <map>
 #include <queue>
 #include <string>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
@@ -96,18 +99,13 @@ rtrex_status check_csr(int64_t n, const int64_t* off, const int64_t* nb) {
     return RTREX_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-rtrex_status rtrex_write_decomposition(const char* name, int64_t n, const int64_t* offsets,
-                                       const int64_t* nbors, const int64_t* cluster_of,
-                                       const int64_t* member_rank, int64_t n_clusters) {
-    if (!name) return fail(RTREX_EINVAL, "null name");
-    RT_TRY(check_csr(n, offsets, nbors));
+// Members of every cluster, each set filled in the order its vertices
+// joined the cluster (member_rank), stats over the original graph.
+rtrex_status build_clusters(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                            const int64_t* cluster_of, const int64_t* member_rank,
+                            int64_t n_clusters, std::vector<Clus>& D) {
     if (n && (!cluster_of || !member_rank)) return fail(RTREX_EINVAL, "null cluster arrays");
     if (n_clusters < 0 || n_clusters > n) return fail(RTREX_EINVAL, "bad cluster count");
-    // Members of every cluster in the order they joined it.
     std::vector<int64_t> cnt((size_t)n_clusters + 1, 0);
     for (int64_t v = 0; v < n; ++v) {
         const int64_t c = cluster_of[v];
@@ -122,12 +120,105 @@ rtrex_status rtrex_write_decomposition(const char* name, int64_t n, const int64_
             return fail(RTREX_EINVAL, "member ranks are not a permutation within each cluster");
         mem[cnt[c] + r] = v;
     }
-    std::vector<Clus> D((size_t)n_clusters);
+    D.assign((size_t)n_clusters, Clus());
     for (int64_t c = 0; c < n_clusters; ++c) {
         for (int64_t i = cnt[c]; i < cnt[c + 1]; ++i) D[c].vertices.insert(mem[i]);
         stats(offsets, nbors, D[c]);
     }
+    return RTREX_OK;
+}
+
+// The cluster a singleton v joins (Decomposition.h:970-990): neighbours'
+// clusters counted in slot order; a cluster becomes the best when its count
+// exceeds the best count so far, which starts at the threshold.  cnt is a
+// zeroed per-thread counter array, left zeroed.
+void best_cluster(int64_t v, const int64_t* off, const int64_t* nb, const int64_t* cluster_of,
+                  int32_t threshold, std::vector<int64_t>& cnt, int64_t* best, int64_t* top) {
+    int64_t b = -1, mx = threshold;
+    for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+        const int64_t c = cluster_of[nb[j]];
+        if (c < 0) continue;
+        if (++cnt[c] > mx) {
+            b = c;
+            mx = cnt[c];
+        }
+    }
+    for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+        const int64_t c = cluster_of[nb[j]];
+        if (c >= 0) cnt[c] = 0;
+    }
+    *best = b;
+    *top = mx;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtrex_status rtrex_write_decomposition(const char* name, int64_t n, const int64_t* offsets,
+                                       const int64_t* nbors, const int64_t* cluster_of,
+                                       const int64_t* member_rank, int64_t n_clusters) {
+    if (!name) return fail(RTREX_EINVAL, "null name");
+    RT_TRY(check_csr(n, offsets, nbors));
+    std::vector<Clus> D;
+    RT_TRY(build_clusters(n, offsets, nbors, cluster_of, member_rank, n_clusters, D));
     return write_all(name, n, offsets, nbors, D);
+}
+
+rtrex_status rtrex_write_expanded(const char* name, int64_t n, const int64_t* offsets,
+                                  const int64_t* nbors, const int64_t* cluster_of,
+                                  const int64_t* member_rank, int64_t n_clusters,
+                                  int32_t threshold, int64_t* expanded_of, double* score) {
+    if (!name) return fail(RTREX_EINVAL, "null name");
+    if (threshold < 0) return fail(RTREX_EINVAL, "threshold must be >= 0");
+    RT_TRY(check_csr(n, offsets, nbors));
+    std::vector<Clus> D;
+    RT_TRY(build_clusters(n, offsets, nbors, cluster_of, member_rank, n_clusters, D));
+    // Every singleton's neighbouring cluster, from the extraction's map (which
+    // the moves below never change): in parallel over vertices.
+    std::vector<int64_t> best((size_t)n, -1), top((size_t)n, 0);
+    std::vector<char> single((size_t)n, 0);
+    for (const Clus& c : D)
+        if (c.nVertices == 1) single[(size_t)*c.vertices.cbegin()] = 1;
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            std::vector<int64_t> cnt((size_t)n_clusters, 0);
+            for (int64_t v = t; v < n; v += nt)
+                if (single[v]) best_cluster(v, offsets, nbors, cluster_of, threshold, cnt,
+                                            &best[v], &top[v]);
+        });
+    for (auto& x : th) x.join();
+    // expandClusters' loop (Decomposition.h:958-1001): singleton clusters in
+    // id order, each one's first vertex in its set's iteration order moves to
+    // its best cluster; stats recomputed at the end.
+    std::map<int64_t, double> scores;
+    const int64_t size = (int64_t)D.size();
+    for (int64_t i = 0; i < size; ++i) {
+        if (D[i].nVertices > 1 || D[i].vertices.empty()) continue;
+        const int64_t v = *D[i].vertices.cbegin();
+        if (best[v] < 0) continue;
+        D[best[v]].vertices.insert(v);
+        D[i].vertices.erase(v);
+        scores[v] = (double)top[v] / (double)(offsets[v + 1] - offsets[v]);
+    }
+    for (Clus& c : D) stats(offsets, nbors, c);
+    if (expanded_of) {
+        for (int64_t c = 0; c < size; ++c)
+            for (int64_t v : D[c].vertices) expanded_of[v] = c;
+    }
+    if (score) {
+        std::fill(score, score + n, -1.0);
+        for (const auto& kv : scores) score[kv.first] = kv.second;
+    }
+    RT_TRY(write_all(name, n, offsets, nbors, D));
+    // printScores (RTRex.cpp:120-135): moved vertices ascending
+    File fsc(std::string(name) + "-scores.txt");
+    if (!fsc.f) return fail(RTREX_EIO, std::string("cannot write ") + name + "-scores.txt");
+    std::fprintf(fsc.f, "Vertex,Score\n");
+    for (const auto& kv : scores) std::fprintf(fsc.f, "%" PRId64 ",%f\n", kv.first, kv.second);
+    return RTREX_OK;
 }
 
 rtrex_status rtrex_write_components(const char* name, int64_t n, const int64_t* offsets,

@@ -8,6 +8,9 @@ The driver (RTRex.cpp:141-206, mode s) writes, in its working directory:
   RTRex-<name>-{decomposition,stats,cut,freq}.txt            printDecomposition
   connected-components-<name>-{decomposition,stats,cut,freq}.txt
   amazon_cleaned_edges.txt                                   survivor dump
+In mode e (expandClusters) at thresholds 0, 2 and 10 it also records the
+RTRex-expanded-g-{decomposition,stats,cut,freq,scores}.txt files as
+"RTRex-expanded-t<threshold>-<suffix>".
 Beside the file hashes it records the canonical partition hash
 (rtrex_cuda_amd.validate.canonical_sets_sha) of both decomposition files, so
 that a partition computed elsewhere (cluster ids and member order aside) is
@@ -37,6 +40,7 @@ from rtrex_cuda_amd import validate as V  # noqa: E402
 RTREX = os.path.join(ROOT, "oracle", "_ref", "RTRex")
 EPS = (0.3, 0.1)
 SUFFIXES = ("decomposition", "stats", "cut", "freq")
+THRESHOLDS = (0, 2, 10)
 
 
 def sha_file(p: str) -> str:
@@ -78,6 +82,20 @@ def main():
                     files[f"{pre.split('-g')[0]}-sets"] = V.canonical_sets_sha(sets)
                 p = os.path.join(d, "amazon_cleaned_edges.txt")
                 files["survivors"] = [sha_file(p), os.path.getsize(p)]
+                # mode e: expandClusters at several thresholds (RTRex.cpp:166-201)
+                for thr in THRESHOLDS:
+                    subprocess.run([RTREX, ef, "g", str(eps), "e", str(thr)], cwd=d, check=True,
+                                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                    for suf in SUFFIXES + ("scores",):
+                        p = os.path.join(d, f"RTRex-expanded-g-{suf}.txt")
+                        ent = [sha_file(p), os.path.getsize(p)]
+                        if suf == "freq":
+                            with open(p, "rb") as f:
+                                lines = f.read().split(b"\n")
+                            keep = [ln for ln in lines if not ln.startswith(b"Size = %d," % n)]
+                            if len(keep) != len(lines):
+                                ent.append(hashlib.sha256(b"\n".join(keep)).hexdigest())
+                        files[f"RTRex-expanded-t{thr}-{suf}"] = ent
             out[f"{name}_eps{eps}"] = files
             print(name, eps, flush=True)
     with open(os.path.join(HERE, "files.json"), "w") as f:

@@ -56,3 +56,22 @@ def test_cli_matches_reference_driver(tag, tmp_path):
     check_files(tmp_path, tag, "RTRex-g", "RTRex", n)
     check_files(tmp_path, tag, "connected-components-g", "connected-components", n)
     assert sha_file(tmp_path / "amazon_cleaned_edges.txt") == FILES[tag]["survivors"][0]
+
+
+@pytest.mark.parametrize("tag", ["clean_email_eps0.3", "clean_astro_eps0.1"])
+def test_cli_mode_e_matches_reference_driver(tag, tmp_path):
+    """The driver's mode e (expandClusters at threshold 2): the RTRex,
+    RTRex-expanded (with scores) and connected-components files."""
+    name = tag.rsplit("_eps", 1)[0]
+    eps = golden_meta()[tag]["eps"]
+    n, u, v = golden_graph(name)
+    ef = tmp_path / "g.txt"
+    O.write_escape(str(ef), n, u, v)
+    print(f"[gpu] cli mode e {tag}", flush=True)
+    r = subprocess.run([CLI, str(ef), "g", str(eps), "e", "2"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    check_files(tmp_path, tag, "RTRex-g", "RTRex", n)
+    check_files(tmp_path, tag, "RTRex-expanded-g", "RTRex-expanded-t2", n,
+                ("decomposition", "stats", "cut", "freq", "scores"))
+    check_files(tmp_path, tag, "connected-components-g", "connected-components", n)

@@ -25,8 +25,8 @@ def sha_file(p) -> str:
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def check_files(tmp_path, tag, prefix, kind, n):
-    for suf in SUFFIXES:
+def check_files(tmp_path, tag, prefix, kind, n, suffixes=SUFFIXES):
+    for suf in suffixes:
         exp = FILES[tag][f"{kind}-{suf}"]
         p = tmp_path / f"{prefix}-{suf}.txt"
         if len(exp) > 2:
@@ -55,6 +55,25 @@ def test_writers_match_reference_driver(tag, tmp_path):
     check_files(tmp_path, tag, "cc-g", "connected-components", n)
     R.write_survivors(str(tmp_path / "surv.txt"), off, nb, d.status_after_clean)
     assert sha_file(tmp_path / "surv.txt") == FILES[tag]["survivors"][0]
+
+
+@pytest.mark.parametrize("thr", [0, 2, 10])
+@pytest.mark.parametrize("tag", sorted(FILES))
+def test_expanded_match_reference_driver(tag, thr, tmp_path):
+    """expandClusters (mode e) on the oracle's clusters: the expanded files
+    and the scores file equal the reference driver's at each threshold."""
+    name = tag.rsplit("_eps", 1)[0]
+    eps = golden_meta()[tag]["eps"]
+    n, u, v = golden_graph(name)
+    off, nb = O.csr_from_undirected(n, u, v)
+    d = O.decompose(off, nb, eps)
+    eo, sc = R.write_expanded(str(tmp_path / "RTRex-expanded-g"), off, nb, d.cluster_map,
+                              d.member_rank, d.n_clusters, thr)
+    check_files(tmp_path, tag, "RTRex-expanded-g", f"RTRex-expanded-t{thr}", n,
+                SUFFIXES + ("scores",))
+    moved = sc >= 0
+    assert np.all(eo[~moved] == d.cluster_map[~moved])
+    assert np.all(eo[moved] != d.cluster_map[moved])
 
 
 def test_writer_rejects_bad_ranks(tmp_path):
