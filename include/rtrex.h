@@ -177,6 +177,13 @@ rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
 rtrex_status rtrex_cluster_stats_noalloc(rtrex_graph g, int64_t* n_vertices,
                                          int64_t* n_edges, int64_t* cut);
 
+/* Connected components of the original graph.  Replaces connectedComp
+ * (Decomposition.h:1023-1068), which RTRex.cpp:203-204 runs after the
+ * extraction: comp[n] (nullable) receives each vertex's component, numbered
+ * as the reference's BFS finds them (by smallest vertex), *n_comp their
+ * number.  A union-find on the device; callable once the graph is built. */
+rtrex_status rtrex_connected_components(rtrex_graph g, int64_t* comp, int64_t* n_comp);
+
 /* Decision audit.  Every threshold or ratio decision whose margin is within
  * the FP64 error bound of the reference's own summation is counted as
  * ambiguous; with zero ambiguous decisions the survivor and cluster sets are

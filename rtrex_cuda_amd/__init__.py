@@ -41,7 +41,7 @@ EXPORTS = [
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
     "rtrex_get_survivors", "rtrex_host_alloc", "rtrex_host_free",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
-    "rtrex_cluster_stats_noalloc",
+    "rtrex_cluster_stats_noalloc", "rtrex_connected_components",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
@@ -144,6 +144,7 @@ def lib() -> C.CDLL:
             "rtrex_decompose": [G, C.c_double, P64, P64],
             "rtrex_cluster_stats": [G, P64, P64, P64],
             "rtrex_cluster_stats_noalloc": [G, P64, P64, P64],
+            "rtrex_connected_components": [G, P64, P64],
             "rtrex_get_audit": [G, C.POINTER(Audit)],
             "rtrex_get_timings": [G, C.POINTER(Timings)],
             "rtrex_batch_decompose": [C.c_int64, P64, P64, PU32, PU32, C.c_double, C.c_int,
@@ -376,6 +377,14 @@ class DeviceGraph:
         cut = np.empty(n_clusters, np.int64)
         _check(lib().rtrex_cluster_stats(self._h, _p(nv, P64), _p(ne, P64), _p(cut, P64)))
         return nv, ne, cut
+
+    def connected_components(self):
+        """connectedComp: (component per vertex, count), components numbered
+        by smallest vertex as the reference's BFS finds them."""
+        comp = np.empty(self.n, np.int64)
+        k = C.c_int64()
+        _check(lib().rtrex_connected_components(self._h, _p(comp, P64), C.byref(k)))
+        return comp, int(k.value)
 
     def populate_stats_noalloc(self, n_clusters: int):
         """populate_stats through the allocation-free entry point (at most

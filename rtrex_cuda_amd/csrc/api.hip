@@ -38,17 +38,14 @@ struct ArenaScope {
         Arena& a = h->arena;
         if (a.top == 0 && a.demand > a.cap) {
             (void)hipStreamSynchronize(h->stream);
-            if (a.base && a.owned) (void)hipFree(a.base);
+            if (a.base && a.owned) dev_free(a.base);
             a.base = nullptr;
             a.cap = 0;
             a.owned = true;
             const uint64_t want = a.demand + a.demand / 4;
-            void* q = nullptr;
-            if (hipMalloc(&q, want) == hipSuccess) {
+            if (void* q = dev_alloc(want, nullptr)) {
                 a.base = static_cast<char*>(q);
                 a.cap = want;
-            } else {
-                (void)hipGetLastError();
             }
         }
         tl_arena = &a;
@@ -58,6 +55,19 @@ struct ArenaScope {
         tl_stream = prev_stream;
     }
 };
+
+// The arena sized up front for the largest phase scratch of a graph this
+// size (from the cache), so that not even the first calls on a new handle
+// allocate scratch per request.
+void presize_arena(rtrex_graph h, int64_t n, int64_t m) {
+    if (n < 0 || m < 0 || h->arena.base) return;
+    const uint64_t want = scratch_bytes((uint64_t)n, (uint64_t)m);
+    if (void* q = dev_alloc(want, nullptr)) {
+        h->arena.base = static_cast<char*>(q);
+        h->arena.cap = want;
+        h->arena.owned = true;
+    }
+}
 }  // namespace
 
 __global__ void k_publish(uint64_t* box, uint32_t seq, const unsigned* src) {
@@ -182,14 +192,20 @@ rtrex_status new_handle(int device, rtrex_graph* out, hipStream_t stream = nullp
     // order (regions of all removed edges), whatever the certification says.
     if (const char* ra = std::getenv("RTREX_REPLAY_ALL")) h->replay_all = std::atoi(ra) != 0;
     hipError_t e = hipSuccess;
+    StreamSet ss;
     if (stream) {
         h->stream = stream;
         h->own_stream = false;
+        for (auto& ev : h->ev)
+            if (e == hipSuccess) e = hipEventCreate(&ev);
+    } else if (stream_take(&ss)) {  // a drained stream of an earlier handle
+        h->stream = ss.s;
+        for (int i = 0; i < NEV; ++i) h->ev[i] = ss.ev[i];
     } else {
         e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        for (auto& ev : h->ev)
+            if (e == hipSuccess) e = hipEventCreate(&ev);
     }
-    for (auto& ev : h->ev)
-        if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e != hipSuccess) {
         rtrex_graph_destroy(h);
         return hip_fail(e, "stream/event creation", __FILE__, __LINE__);
@@ -302,6 +318,8 @@ rtrex_status create_edges_in(const uint32_t* u, const uint32_t* v, int64_t n, in
         ArenaScope scope_(h);
         st = build_from_edges(h, u, v, n, m);
     } else {
+        presize_arena(h, n, m);
+        ArenaScope scope_(h);
         st = build_from_edges(h, u, v, n, m);
     }
     if (st != RTREX_OK) {
@@ -396,7 +414,12 @@ rtrex_status rtrex_graph_create_csr(const int64_t* offsets,
                                     int64_t nnz, int device, rtrex_graph* out) {
     rtrex_graph h = nullptr;
     RT_TRY(new_handle(device, &h));
-    rtrex_status s = build_from_csr(h, offsets, nbors, n, nnz);
+    presize_arena(h, n, nnz / 2);
+    rtrex_status s;
+    {
+        ArenaScope scope_(h);
+        s = build_from_csr(h, offsets, nbors, n, nnz);
+    }
     if (s != RTREX_OK) {
         std::string msg = g_last_error;
         rtrex_graph_destroy(h);
@@ -412,7 +435,12 @@ rtrex_status rtrex_graph_create_edges(const uint32_t* u, const uint32_t* v,
                                       rtrex_graph* out) {
     rtrex_graph h = nullptr;
     RT_TRY(new_handle(device, &h));
-    rtrex_status s = build_from_edges(h, u, v, n, m);
+    presize_arena(h, n, m);
+    rtrex_status s;
+    {
+        ArenaScope scope_(h);
+        s = build_from_edges(h, u, v, n, m);
+    }
     if (s != RTREX_OK) {
         std::string msg = g_last_error;
         rtrex_graph_destroy(h);
@@ -427,13 +455,30 @@ void rtrex_graph_destroy(rtrex_graph g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
-    for (void* p : g->allocs) (void)hipFree(p);
-    if (g->arena.base && g->arena.owned) (void)hipFree(g->arena.base);
-    if (g->cm_pin) (void)hipHostFree(g->cm_pin);
+    // Everything goes back to the process-wide caches (the stream drained).
+    for (void* p : g->allocs) dev_free(p);
+    if (g->arena.base && g->arena.owned) dev_free(g->arena.base);
+    pin_free(g->cm_pin);
+    pin_free(g->up_pin);
     box_close(g);  // after the stream drained above
-    for (auto& ev : g->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
+    bool pooled = false;
+    if (g->stream && g->own_stream) {
+        bool all = true;
+        for (auto& ev : g->ev) all = all && ev;
+        if (all && hipStreamQuery(g->stream) == hipSuccess) {
+            StreamSet ss;
+            ss.s = g->stream;
+            for (int i = 0; i < NEV; ++i) ss.ev[i] = g->ev[i];
+            stream_give(ss);
+            pooled = true;
+        }
+        (void)hipGetLastError();
+    }
+    if (!pooled) {
+        for (auto& ev : g->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
+    }
     delete g;
 }
 
@@ -779,6 +824,12 @@ rtrex_status rtrex_cluster_stats(rtrex_graph g, int64_t* n_vertices,
     RT_TRY(check(g));
     ArenaScope scope_(g);
     return run_cluster_stats(g, n_vertices, n_edges, cut);
+}
+
+rtrex_status rtrex_connected_components(rtrex_graph g, int64_t* comp, int64_t* n_comp) {
+    RT_TRY(check(g));
+    ArenaScope scope_(g);
+    return run_connected_components(g, comp, n_comp);
 }
 
 rtrex_status rtrex_cluster_stats_noalloc(rtrex_graph g, int64_t* n_vertices,

@@ -1386,14 +1386,13 @@ bool try_alloc_terms(rtrex_graph h, uint64_t total) {
         (void)hipGetLastError();
         return false;
     }
+    freeb += dev_cached_bytes();  // the cache gives its blocks back on demand
     if (bytes >= freeb / 5 * 3) return false;
-    void* q = nullptr;
-    if (hipMalloc(&q, bytes) == hipSuccess) {
+    if (void* q = dev_alloc(bytes, nullptr)) {
         h->allocs.push_back(q);
         h->zt = static_cast<Term*>(q);
         return true;
     }
-    (void)hipGetLastError();
     return false;
 }
 

@@ -1032,6 +1032,54 @@ __global__ void __launch_bounds__(T) k_cluster_stats(const GraphDev g,
     }
 }
 
+// ---- connectedComp over the original graph (Decomposition.h:1023-1068)
+__global__ void __launch_bounds__(T) k_cc_init(uint32_t n, uint32_t* parent) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) parent[v] = v;
+}
+
+// Every edge joins its endpoints' trees, the larger root under the smaller,
+// so each component's root is its smallest vertex.
+__global__ void __launch_bounds__(T) k_cc_hook(const GraphDev g, uint32_t* parent) {
+    for (uint32_t e = blockIdx.x * T + threadIdx.x; e < g.m; e += gridDim.x * T) {
+        uint32_t a = uf_find(parent, g.esrc[e]);
+        uint32_t b = uf_find(parent, g.nb[g.eslot[e]]);
+        while (a != b) {
+            if (a > b) {
+                const uint32_t t = a;
+                a = b;
+                b = t;
+            }
+            const uint32_t old = atomicCAS(&parent[b], b, a);
+            if (old == b) break;
+            b = uf_find(parent, old);
+            a = uf_find(parent, a);
+        }
+    }
+}
+
+// parent[] := root (roots only written, as in k_uf_compress); isroot flags.
+__global__ void __launch_bounds__(T) k_cc_roots(uint32_t n, uint32_t* parent,
+                                               uint32_t* __restrict__ isroot) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) {
+        uint32_t r = v, p = parent[v];
+        while (p != r) {
+            r = p;
+            p = parent[r];
+        }
+        parent[v] = r;
+        isroot[v] = r == v ? 1u : 0u;
+    }
+}
+
+// The reference numbers components as its BFS meets them: by smallest
+// vertex, i.e. the rank of the root among the roots.
+__global__ void __launch_bounds__(T) k_cc_label(uint32_t n, const uint32_t* __restrict__ parent,
+                                               const uint32_t* __restrict__ rk,
+                                               int64_t* __restrict__ out) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T)
+        out[v] = rk[parent[v]];
+}
+
 int bits_for(uint64_t n) {
     int b = 1;
     while ((1ull << b) < n) ++b;
@@ -1322,7 +1370,8 @@ rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out) {
         RT_HIP(hipStreamSynchronize(h->stream));
         return RTREX_OK;
     }
-    if (!h->cm_pin) RT_HIP(hipHostMalloc((void**)&h->cm_pin, (uint64_t)n * 4, hipHostMallocDefault));
+    if (!h->cm_pin) h->cm_pin = static_cast<uint32_t*>(pin_alloc((uint64_t)n * 4));
+    if (!h->cm_pin) return fail(RTREX_ENOMEM, "page-locked staging allocation failed");
     RT_HIP(hipMemcpyAsync(h->cm_pin, h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, h->stream));
     RT_HIP(hipStreamSynchronize(h->stream));
     const size_t ns = h->seg_bounds.empty() ? 0 : h->seg_bounds.size() - 1;
@@ -1363,6 +1412,41 @@ rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
         if (ne) ne[i] = (int64_t)(hb[i] / 2);
         if (cut) cut[i] = (int64_t)hc[i];
     }
+    return RTREX_OK;
+}
+
+rtrex_status run_connected_components(rtrex_graph h, int64_t* comp, int64_t* n_comp) {
+    if (h->phase < PH_BUILT) return fail(RTREX_ESTATE, "graph not built");
+    const GraphDev& g = h->g;
+    const uint32_t n = g.n;
+    hipStream_t s = h->stream;
+    if (!n) {
+        if (n_comp) *n_comp = 0;
+        return RTREX_OK;
+    }
+    Scratch sc;
+    uint32_t *parent, *isroot, *rk;
+    int64_t* lab;
+    RT_TRY(sc.get(&parent, n));
+    RT_TRY(sc.get(&isroot, n));
+    RT_TRY(sc.get(&rk, (uint64_t)n + 1));
+    RT_TRY(sc.get(&lab, n));
+    const unsigned bn = blocks_for(n, T, 8192);
+    k_cc_init<<<bn, T, 0, s>>>(n, parent);
+    k_cc_hook<<<blocks_for(g.m, T, 8192), T, 0, s>>>(g, parent);
+    k_cc_roots<<<bn, T, 0, s>>>(n, parent, isroot);
+    RT_HIP(hipGetLastError());
+    RT_TRY(exclusive_scan_u32_32(isroot, rk, n, s));
+    void* dout = nullptr;
+    const bool direct = comp && mapped_host(comp, &dout);
+    k_cc_label<<<bn, T, 0, s>>>(n, parent, rk, direct ? static_cast<int64_t*>(dout) : lab);
+    RT_HIP(hipGetLastError());
+    uint32_t nc = 0;
+    RT_HIP(hipMemcpyAsync(&nc, rk + n, 4, hipMemcpyDeviceToHost, s));
+    if (comp && !direct)
+        RT_HIP(hipMemcpyAsync(comp, lab, (uint64_t)n * 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (n_comp) *n_comp = nc;
     return RTREX_OK;
 }
 

@@ -13,6 +13,10 @@
 // outer loop); edge ids are the ranks of the canonical slots (src < dst).
 #include <algorithm>
 
+#include <cstring>
+#include <thread>
+#include <vector>
+
 #include "dev.hpp"
 #include "internal.hpp"
 
@@ -167,6 +171,27 @@ __global__ void __launch_bounds__(T) k_split_keys(const uint64_t* __restrict__ k
         atomicAdd(&deg[s], 1u);
         if (i > 0 && keys[i - 1] == k) atomicOr(err, ERR_UNSORTED_OR_DUP);
     }
+}
+
+// memcpy split over a few host threads when large (page-locked staging).
+void parallel_copy(void* dst, const void* src, uint64_t bytes) {
+    const uint64_t chunk = 4ull << 20;
+    const unsigned nt = (unsigned)std::min<uint64_t>(
+        std::max(1u, std::min(8u, std::thread::hardware_concurrency())), bytes / chunk + 1);
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t per = (bytes / nt + 63) & ~63ull;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t a = std::min<uint64_t>(bytes, t * per), b = std::min<uint64_t>(bytes, a + per);
+        if (a < b)
+            th.emplace_back([=] {
+                std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+            });
+    }
+    for (auto& x : th) x.join();
 }
 
 int bits_for(uint64_t n) {
@@ -325,8 +350,21 @@ rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
     RT_TRY(sc.get(&tmp, nnz));
     RT_HIP(hipEventRecord(h->ev[0], s));
     if (m) {
-        RT_HIP(hipMemcpyAsync(du, u, m * 4, hipMemcpyHostToDevice, s));
-        RT_HIP(hipMemcpyAsync(dv, v, m * 4, hipMemcpyHostToDevice, s));
+        // Through page-locked staging (from the pin cache, released when the
+        // build has drained): the host copies in parallel, the DMA engine
+        // runs at link speed instead of the runtime's pageable path.
+        const uint64_t bytes = (uint64_t)m * 4;
+        if (!h->up_pin) h->up_pin = pin_alloc(2 * bytes);
+        if (h->up_pin) {
+            uint32_t* su = static_cast<uint32_t*>(h->up_pin);
+            parallel_copy(su, u, bytes);
+            parallel_copy(su + m, v, bytes);
+            RT_HIP(hipMemcpyAsync(du, su, bytes, hipMemcpyHostToDevice, s));
+            RT_HIP(hipMemcpyAsync(dv, su + m, bytes, hipMemcpyHostToDevice, s));
+        } else {
+            RT_HIP(hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, s));
+            RT_HIP(hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, s));
+        }
     }
     const int B = bits_for((uint64_t)n + 1);
     const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
@@ -338,7 +376,15 @@ rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
                                   &h->ctr->error);
     RT_HIP(hipGetLastError());
     RT_TRY(exclusive_scan_u32_32(g.deg, g.off, g.n, s));
-    RT_TRY(finish(h, src));
+    const rtrex_status fs = finish(h, src);  // ends with the stream drained
+    if (fs != RTREX_OK) {
+        (void)hipStreamSynchronize(s);
+        pin_free(h->up_pin);
+        h->up_pin = nullptr;
+        return fs;
+    }
+    pin_free(h->up_pin);
+    h->up_pin = nullptr;
     RT_HIP(hipEventRecord(h->ev[1], s));
     RT_HIP(hipEventSynchronize(h->ev[1]));
     float ms = 0;

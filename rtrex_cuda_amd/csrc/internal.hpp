@@ -157,6 +157,26 @@ struct Arena {
 
 enum Phase { PH_NONE = 0, PH_BUILT = 1, PH_COUNTED = 2, PH_CLEANED = 3, PH_EXTRACTED = 4 };
 
+// ---- process-wide caches (memcache.hip) -------------------------------------
+constexpr int NEV = 9;  // timing events per handle
+struct StreamSet {
+    hipStream_t s = nullptr;
+    hipEvent_t ev[NEV] = {};
+};
+// Device memory of the current device (null and *err on failure); blocks
+// come back to a cache that later requests reuse.
+void* dev_alloc(uint64_t bytes, hipError_t* err);
+void dev_free(void* p);
+void dev_trim();  // every cached block of the current device released
+uint64_t dev_cached_bytes();
+// Page-locked host memory (null on failure), cached the same way.
+void* pin_alloc(uint64_t bytes);
+void pin_free(void* p);
+// A stream of the current device with its events, from the pool (false: none
+// pooled); given back once drained.
+bool stream_take(StreamSet* out);
+void stream_give(const StreamSet& s);
+
 }  // namespace rtrex
 
 // The opaque handle of the C ABI.
@@ -194,6 +214,7 @@ struct rtrex_graph_s {
     // Cluster ids stay in HBM (cid); copy_cluster_map brings them to the host
     // on request through this pinned staging buffer (n entries, lazily).
     uint32_t* cm_pin = nullptr;
+    void* up_pin = nullptr;  // page-locked staging of an edge-list upload (pin cache)
     std::vector<uint32_t> seg_bounds;  // batch: first cluster id of each segment, and the total
     uint32_t* seg_pos_d = nullptr;     // batch: seg_vtx on the device (u32)
     // Batch layout (one graph = one segment): vertex segment boundaries.
@@ -238,7 +259,7 @@ struct rtrex_graph_s {
     // Events: 0-1 build, 2-4 count, 5-6 clean, 7-8 extract.  The count and
     // clean times are read once something waits for the stream anyway
     // (resolve_timings), so those phases end without a host round trip.
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[rtrex::NEV] = {};
     bool tm_count_pending = false, tm_clean_pending = false, tm_extract_pending = false;
     // Host mailbox: pinned words a kernel publishes small results into
     // (value | sequence << 32), polled by the host in place of a copy and a
@@ -323,12 +344,11 @@ rtrex_status dalloc(rtrex_graph h, T** p, uint64_t count) {
         *p = static_cast<T*>(q);
         return RTREX_OK;
     }
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(RTREX_ENOMEM, "hipMalloc of " + std::to_string(bytes) +
+    hipError_t e = hipSuccess;
+    q = dev_alloc(bytes, &e);
+    if (!q)
+        return fail(RTREX_ENOMEM, "device allocation of " + std::to_string(bytes) +
                                       " bytes failed: " + hipGetErrorString(e));
-    }
     h->allocs.push_back(q);
     *p = static_cast<T*>(q);
     return RTREX_OK;
@@ -360,7 +380,7 @@ struct Scratch {
             if (stream) (void)hipStreamSynchronize(stream);
             else (void)hipDeviceSynchronize();
         }
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : ptrs) dev_free(p);
         if (a) {
             a->top = mark;
             a->vtop = vmark;
@@ -380,12 +400,11 @@ struct Scratch {
                 return RTREX_OK;
             }
         }
-        hipError_t e = hipMalloc(&q, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(RTREX_ENOMEM, "scratch hipMalloc of " +
-                                          std::to_string(bytes) + " bytes failed");
-        }
+        hipError_t e = hipSuccess;
+        q = dev_alloc(bytes, &e);
+        if (!q)
+            return fail(RTREX_ENOMEM, "scratch allocation of " + std::to_string(bytes) +
+                                          " bytes failed");
         ptrs.push_back(q);
         *p = static_cast<T*>(q);
         return RTREX_OK;
@@ -423,6 +442,7 @@ rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out);
 bool mapped_host(const void* p, void** dptr);
 rtrex_status run_cluster_stats(rtrex_graph h, int64_t* nv, int64_t* ne,
                                int64_t* cut);
+rtrex_status run_connected_components(rtrex_graph h, int64_t* comp, int64_t* n_comp);
 rtrex_status run_cluster_stats_noalloc(rtrex_graph h, int64_t* nv, int64_t* ne,
                                        int64_t* cut);
 // Per-vertex supports of the listed vertices (all when list is null) on

@@ -300,6 +300,11 @@ rtrex_status rtrex_server_start(const rtrex_server_config* cfg, rtrex_server* ou
     if (s->cfg.batch_max_graphs <= 0) s->cfg.batch_max_graphs = 1024;
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, (size_t)cfg->hbm_budget_bytes);
+    if (e != hipSuccess) {  // blocks cached by earlier handles go back first
+        (void)hipGetLastError();
+        dev_trim();
+        e = hipMalloc(&q, (size_t)cfg->hbm_budget_bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         delete s;

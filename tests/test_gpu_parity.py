@@ -334,3 +334,23 @@ def test_results_into_page_locked_host_memory():
         _, k = g.decompose(0.1, out=cm)
         g.survivor_words(out=words)
     assert k == k0 and np.array_equal(cm, cm0) and np.array_equal(words, w0)
+
+
+@pytest.mark.parametrize("name", ["small_test", "random_graph_rtr", "clean_email", "config1"])
+def test_connected_components_on_device(name):
+    """rtrex_connected_components (connectedComp) equals the oracle's
+    partition and numbering (by smallest vertex); the oracle is pinned to
+    the reference driver's connected-components file."""
+    n, u, v = golden_graph(name)
+    off, nb = O.csr_from_undirected(n, u, v)
+    ref, k0 = O.connected_components(off, nb)
+    with R.DeviceGraph.from_edges(n, u, v) as g:
+        comp, k = g.connected_components()
+        pinned = R.host_empty(n, np.int64)
+        lib = R.lib()
+        import ctypes as C
+        kk = C.c_int64()
+        R._check(lib.rtrex_connected_components(g._h, pinned.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                C.byref(kk)))
+    assert k == k0 and np.array_equal(comp, ref)
+    assert kk.value == k0 and np.array_equal(pinned, ref)
