@@ -58,6 +58,26 @@ rtrex_status rtrex_escape_header(const char* path, int64_t* n, int64_t* m);
 rtrex_status rtrex_escape_read(const char* path, int64_t m, uint32_t* u,
                                uint32_t* v);
 
+/* The same rules parsed on the device, and the reference's SNAP conversion
+ * (RTRex_CUDA/snap_to_rtr.py:47-142): '#'/'%' comment lines and blank lines
+ * skipped, lines without two integer tokens skipped, self-loops dropped, ids
+ * remapped to 0..n-1 by sorted order when they are not already, each
+ * undirected pair kept once, edges sorted. */
+enum { RTREX_FORMAT_ESCAPE = 0, RTREX_FORMAT_SNAP = 1 };
+
+/* Reads an edge-list text file (format RTREX_FORMAT_ESCAPE or _SNAP), parses
+ * it on the device and builds the graph there from the parsed edges, with no
+ * edge list on the host: loadGraph_Escape + makeCSR (GraphIO.cpp:19-88,
+ * Graph.cpp:344-384), or snap_to_rtr.py then both.  Errors as
+ * rtrex_escape_read and rtrex_graph_create_edges. */
+rtrex_status rtrex_graph_load(const char* path, int format, int device, rtrex_graph* out);
+
+/* snap_to_rtr.py's conversion on the device: the SNAP file in_path written
+ * as the Escape file out_path ("n m", then "u v" per edge, u < v, sorted);
+ * *n and *m (nullable) receive its vertex and edge counts. */
+rtrex_status rtrex_snap_convert(const char* in_path, const char* out_path, int device,
+                                int64_t* n, int64_t* m);
+
 /* ------------------------------------------------------------------------ */
 /* Graph lifecycle.                                                          */
 
@@ -140,7 +160,13 @@ rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
 /* Greedy extraction after rtrex_initial_clean.  Replaces disjointExtractCUDA
  * (RTRex_CUDA/main.cu:130-338) / disjointExtract (Decomposition.h:538-940).
  * cluster_map[n] receives the reference's cluster id of every vertex (ids in
- * the order the sequential reference creates them); *n_clusters the count. */
+ * the order the sequential reference creates them); *n_clusters the count.
+ * Every decision is certified against the reference's own arithmetic or
+ * re-run in its order; a ratio-prefix decision within the per-vertex
+ * supports' error bound cannot be re-run (the reference subtracts from a
+ * vertex's support in its global stack order) and is counted in
+ * rtrex_audit.ambiguous_ratio: when that count is 0 the clusters are provably
+ * the reference's. */
 rtrex_status rtrex_extract(rtrex_graph g, double eps, int64_t* cluster_map,
                            int64_t* n_clusters);
 

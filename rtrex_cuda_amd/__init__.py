@@ -41,7 +41,8 @@ EXPORTS = [
     "rtrex_count", "rtrex_get_supports", "rtrex_initial_clean", "rtrex_get_edge_status",
     "rtrex_get_survivors", "rtrex_host_alloc", "rtrex_host_free",
     "rtrex_delete_and_clean", "rtrex_extract", "rtrex_decompose", "rtrex_cluster_stats",
-    "rtrex_cluster_stats_noalloc", "rtrex_connected_components",
+    "rtrex_cluster_stats_noalloc", "rtrex_connected_components", "rtrex_graph_load",
+    "rtrex_snap_convert",
     "rtrex_get_audit", "rtrex_get_timings", "rtrex_batch_create",
     "rtrex_batch_cluster_counts", "rtrex_batch_decompose",
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
@@ -145,6 +146,8 @@ def lib() -> C.CDLL:
             "rtrex_cluster_stats": [G, P64, P64, P64],
             "rtrex_cluster_stats_noalloc": [G, P64, P64, P64],
             "rtrex_connected_components": [G, P64, P64],
+            "rtrex_graph_load": [C.c_char_p, C.c_int, C.c_int, C.POINTER(G)],
+            "rtrex_snap_convert": [C.c_char_p, C.c_char_p, C.c_int, P64, P64],
             "rtrex_get_audit": [G, C.POINTER(Audit)],
             "rtrex_get_timings": [G, C.POINTER(Timings)],
             "rtrex_batch_decompose": [C.c_int64, P64, P64, PU32, PU32, C.c_double, C.c_int,
@@ -255,6 +258,18 @@ class DeviceGraph:
         _check(lib().rtrex_graph_create_edges(_p(u, PU32), _p(v, PU32), n, len(u), device,
                                               C.byref(h)))
         return cls(h, n, 2 * len(u))
+
+    @classmethod
+    def load(cls, path: str, fmt: str = "escape", device: int = 0) -> "DeviceGraph":
+        """An edge-list text file parsed on the device and built there: the
+        reference's Escape loader (fmt "escape") or snap_to_rtr.py's
+        conversion (fmt "snap")."""
+        code = {"escape": 0, "snap": 1}[fmt]
+        h = C.c_void_p()
+        _check(lib().rtrex_graph_load(str(path).encode(), code, device, C.byref(h)))
+        n, nnz = C.c_int64(), C.c_int64()
+        _check(lib().rtrex_graph_info(h, C.byref(n), C.byref(nnz)))
+        return cls(h, int(n.value), int(nnz.value))
 
     def close(self) -> None:  # freeDeviceGraph
         if self._h:
@@ -519,6 +534,16 @@ def write_decomposition(name: str, offsets, nbors, cluster_of, member_rank, n_cl
     mr = np.ascontiguousarray(member_rank, np.int64)
     _check(lib().rtrex_write_decomposition(name.encode(), len(off) - 1, _p(off, P64), _p(nb, P64),
                                            _p(co, P64), _p(mr, P64), int(n_clusters)))
+
+
+def snap_convert(in_path: str, out_path: str, device: int = 0):
+    """snap_to_rtr.py's conversion (RTRex_CUDA/snap_to_rtr.py:47-142) on the
+    device: the SNAP edge list in_path written as the Escape file out_path.
+    Returns (n, m)."""
+    n, m = C.c_int64(), C.c_int64()
+    _check(lib().rtrex_snap_convert(str(in_path).encode(), str(out_path).encode(), device,
+                                    C.byref(n), C.byref(m)))
+    return int(n.value), int(m.value)
 
 
 def write_expanded(name: str, offsets, nbors, cluster_of, member_rank, n_clusters: int,

@@ -24,7 +24,7 @@ BENCH_LIB = os.path.join(PKG, "librtrex_bench.so")
 CLI = os.path.join(PKG, "rtrex")
 
 HIP_SOURCES = ["api.hip", "sort.hip", "graph.hip", "count.hip", "clean.hip", "extract.hip",
-               "serve.hip", "writers.hip", "memcache.hip"]
+               "serve.hip", "writers.hip", "memcache.hip", "ingest.hip"]
 
 # IEEE FP64 throughout (no contraction into FMA, no fast-math): the supports
 # must round exactly like the reference's separate multiply/add/divide.

@@ -451,6 +451,40 @@ rtrex_status rtrex_graph_create_edges(const uint32_t* u, const uint32_t* v,
     return RTREX_OK;
 }
 
+rtrex_status rtrex_graph_load(const char* path, int format, int device, rtrex_graph* out) {
+    rtrex_graph h = nullptr;
+    RT_TRY(new_handle(device, &h));
+    rtrex_status s;
+    {
+        ArenaScope scope_(h);
+        s = load_text(h, path, format);
+    }
+    if (s != RTREX_OK) {
+        std::string msg = g_last_error;
+        rtrex_graph_destroy(h);
+        set_error(msg);
+        return s;
+    }
+    *out = h;
+    return RTREX_OK;
+}
+
+rtrex_status rtrex_snap_convert(const char* in_path, const char* out_path, int device,
+                                int64_t* n, int64_t* m) {
+    if (!in_path || !out_path) return fail(RTREX_EINVAL, "null path");
+    rtrex_graph h = nullptr;  // a handle for its stream and scratch only
+    RT_TRY(new_handle(device, &h));
+    rtrex_status s;
+    {
+        ArenaScope scope_(h);
+        s = convert_text(h, in_path, out_path, n, m);
+    }
+    std::string msg = g_last_error;
+    rtrex_graph_destroy(h);
+    if (s != RTREX_OK) set_error(msg);
+    return s;
+}
+
 void rtrex_graph_destroy(rtrex_graph g) {
     if (!g) return;
     (void)hipSetDevice(g->device);

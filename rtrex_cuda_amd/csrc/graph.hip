@@ -330,6 +330,34 @@ rtrex_status build_from_csr(rtrex_graph h, const int64_t* off,
     return RTREX_OK;
 }
 
+// The CSR of an edge list already on the device (du, dv: m entries each;
+// the build reads them before any of its own scratch is taken).
+rtrex_status build_from_device_edges(rtrex_graph h, const uint32_t* du, const uint32_t* dv,
+                                     int64_t n, int64_t m) {
+    if (n < 0 || m < 0 || n >= (int64_t)UINT32_MAX || 2 * m >= (int64_t)UINT32_MAX)
+        return fail(RTREX_EINVAL, "n and 2m must be in [0, 2^32-1)");
+    const uint64_t nnz = 2 * (uint64_t)m;
+    if (h->g.nb == nullptr) RT_TRY(alloc_state(h, (uint32_t)n, nnz));
+    GraphDev& g = h->g;
+    hipStream_t s = h->stream;
+    Scratch sc;
+    uint32_t* src;
+    uint64_t *keys, *tmp;
+    RT_TRY(sc.get(&src, nnz));
+    RT_TRY(sc.get(&keys, nnz));
+    RT_TRY(sc.get(&tmp, nnz));
+    const int B = bits_for((uint64_t)n + 1);
+    const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
+    k_make_keys<<<bs, T, 0, s>>>(du, dv, m, g.n, B, keys, &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    RT_TRY(radix_sort_u64(keys, tmp, nnz, 2 * B, s));
+    RT_HIP(hipMemsetAsync(g.deg, 0, (uint64_t)g.n * 4, s));
+    k_split_keys<<<bs, T, 0, s>>>(keys, nnz, B, g.nb, src, g.deg, &h->ctr->error);
+    RT_HIP(hipGetLastError());
+    RT_TRY(exclusive_scan_u32_32(g.deg, g.off, g.n, s));
+    return finish(h, src);  // ends with the stream drained
+}
+
 rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
                               const uint32_t* v, int64_t n, int64_t m) {
     if (n < 0 || m < 0 || n >= (int64_t)UINT32_MAX ||
@@ -338,16 +366,11 @@ rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
     if (m && (!u || !v)) return fail(RTREX_EINVAL, "null edge array");
     const uint64_t nnz = 2 * (uint64_t)m;
     RT_TRY(alloc_state(h, (uint32_t)n, nnz));
-    GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
-    uint32_t *du, *dv, *src;
-    uint64_t *keys, *tmp;
+    uint32_t *du, *dv;
     RT_TRY(sc.get(&du, m));
     RT_TRY(sc.get(&dv, m));
-    RT_TRY(sc.get(&src, nnz));
-    RT_TRY(sc.get(&keys, nnz));
-    RT_TRY(sc.get(&tmp, nnz));
     RT_HIP(hipEventRecord(h->ev[0], s));
     if (m) {
         // Through page-locked staging (from the pin cache, released when the
@@ -366,25 +389,11 @@ rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
             RT_HIP(hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, s));
         }
     }
-    const int B = bits_for((uint64_t)n + 1);
-    const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
-    k_make_keys<<<bs, T, 0, s>>>(du, dv, m, g.n, B, keys, &h->ctr->error);
-    RT_HIP(hipGetLastError());
-    RT_TRY(radix_sort_u64(keys, tmp, nnz, 2 * B, s));
-    RT_HIP(hipMemsetAsync(g.deg, 0, (uint64_t)g.n * 4, s));
-    k_split_keys<<<bs, T, 0, s>>>(keys, nnz, B, g.nb, src, g.deg,
-                                  &h->ctr->error);
-    RT_HIP(hipGetLastError());
-    RT_TRY(exclusive_scan_u32_32(g.deg, g.off, g.n, s));
-    const rtrex_status fs = finish(h, src);  // ends with the stream drained
-    if (fs != RTREX_OK) {
-        (void)hipStreamSynchronize(s);
-        pin_free(h->up_pin);
-        h->up_pin = nullptr;
-        return fs;
-    }
+    const rtrex_status fs = build_from_device_edges(h, du, dv, n, m);
+    if (fs != RTREX_OK) (void)hipStreamSynchronize(s);
     pin_free(h->up_pin);
     h->up_pin = nullptr;
+    RT_TRY(fs);
     RT_HIP(hipEventRecord(h->ev[1], s));
     RT_HIP(hipEventSynchronize(h->ev[1]));
     float ms = 0;

@@ -427,6 +427,15 @@ rtrex_status build_from_csr(rtrex_graph h, const int64_t* off,
                             const int64_t* nb, int64_t n, int64_t nnz);
 rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
                               const uint32_t* v, int64_t n, int64_t m);
+// Edge-list text parsed on the device (ingest.hip): load_text builds the
+// graph of an Escape or SNAP file; convert_text writes the Escape file of a
+// SNAP file (snap_to_rtr.py's conversion).
+rtrex_status load_text(rtrex_graph h, const char* path, int format);
+rtrex_status convert_text(rtrex_graph h, const char* in, const char* out, int64_t* n,
+                          int64_t* m);
+// The same from an edge list already in device memory.
+rtrex_status build_from_device_edges(rtrex_graph h, const uint32_t* du, const uint32_t* dv,
+                                     int64_t n, int64_t m);
 rtrex_status run_count(rtrex_graph h);
 // Phase times whose events are still pending (waits for them).
 rtrex_status resolve_timings(rtrex_graph h);
