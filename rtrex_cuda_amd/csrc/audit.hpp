@@ -34,11 +34,15 @@ __device__ __forceinline__ double ulp_of(double x) {
     return ldexp(1.0, ilogb(x) - 52);
 }
 
-// Cheap test valid for any history of at most k subtractions from s0: both
-// this library's value v and the reference's lie within k ulp(s0)/2 of the
-// exact difference, hence within k ulp(s0) of each other.
+// Cheap test valid for any history of at most k subtractions from s0.  The
+// reference's value lies within k ulp(s0)/2 of the exact difference (its
+// subtractions round to nearest; running values never exceed s0).  This
+// library's value v may come from FP64 atomics (global atomic add, built with
+// -munsafe-fp-atomics), whose rounding is not relied upon: each is counted as
+// off by up to a whole ulp(s0), so v lies within k ulp(s0) of the exact
+// difference and the two within 3k/2 ulp(s0) of each other.
 __device__ __forceinline__ double cheap_bound(uint32_t k, double s0) {
-    return (double)k * ulp_of(s0) + 0x1.0p-1060;
+    return 1.5 * (double)k * ulp_of(s0) + 0x1.0p-1060;
 }
 
 __device__ __forceinline__ void two_sum(double a, double b, double& s,
