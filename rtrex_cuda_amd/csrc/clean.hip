@@ -1510,7 +1510,7 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
             A.nF = nF;
             A.nFd = nullptr;
             unsigned* acur = L.akc[0];
-            if (!ak_valid)  // its counts were cleared by the round that used it last
+            if (!ak_valid)  // its counts were cleared by the round before (k_end_round)
                 k_alive_buckets<<<blocks_for((g.m + SEED_K - 1) / SEED_K, T, 4096), T, 0, s>>>(
                     g, L.ak[0], acur);
             if (h->n_alive) {
@@ -1575,11 +1575,15 @@ rtrex_status peel(rtrex_graph h, double eps, uint32_t* F, uint32_t* Fn,
         else k_near_check<<<NEAR_BLOCKS, T, 0, s>>>(A);
         // The frontier dies, the claimed edges are queued.  Both lists are
         // bounded by the edges alive at the start of the round.  The next
-        // round's counters are cleared here, and after a pull round the
-        // counts of the lists just walked (the next round's output lists).
+        // round's counters are cleared here, and so are the counts of
+        // L.ak[0]: after a pull round it is the list just walked (the next
+        // round's output list, after the swap); after a push round it is
+        // stale, and a later pull round rebuilds it from a zero count (the
+        // rebuild appends; on top of the old counts every alive edge would be
+        // listed twice and a claimed edge queued twice).
         k_end_round<<<blocks_for(std::max<uint64_t>(nF, h->n_alive), T, 4096), T, 0, s>>>(
             g, F, nF, Fn, A.nFn, h->ctr, terms, rc + 2 * ((round + 1) & 1),
-            pull ? L.akc[0] : nullptr, h->box_dev, ++h->box_seq);
+            terms ? L.akc[0] : nullptr, h->box_dev, ++h->box_seq);
         if (pull) {  // ak[1] (now ak[0]): the next pull round's work list
             std::swap(L.ak[0], L.ak[1]);
             std::swap(L.akc[0], L.akc[1]);

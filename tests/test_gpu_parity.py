@@ -116,6 +116,39 @@ def test_audit_counts_every_edge(tag):
     assert r["audit"]["edges_removed"] == len(r["nb"]) // 2
 
 
+def _small_config4_indices(limit: int = 200):
+    out = []
+    for i in range(4096):
+        rng = np.random.default_rng(23 * 1_000_003 + i)  # graphs.config4_graph's size draw
+        if int(round(np.exp(rng.uniform(np.log(50), np.log(20_000))))) <= limit:
+            out.append(i)
+    return out
+
+
+def test_small_dense_graphs_each_edge_removed_once():
+    """Every graph of the 4,096-graph batch with at most 200 vertices (954
+    small dense graphs), each alone: the clean mixes pull rounds (frontier
+    larger than the alive set) with push rounds, and a pull round after a push
+    round rebuilds the alive lists.  Every edge is removed exactly once over
+    the decomposition (the audit's count equals m) and the clusters are the
+    oracle's.  Graph 1074 (51 vertices, 409 edges) is one whose clean goes
+    push, pull, push, push, push, push, pull, pull (its frontier sizes
+    against its alive edges, computed on the CPU)."""
+    idx = _small_config4_indices()
+    assert 1074 in idx and len(idx) == 954
+    bad = []
+    for i in idx:
+        n, u, v = G.config4_graph(i)
+        with R.DeviceGraph.from_edges(n, u, v) as g:
+            cm, k = g.decompose(0.3)
+            a = g.audit()
+        off, nb = O.csr_from_undirected(n, u, v)
+        d = O.decompose(off, nb, 0.3, with_clusters=False)
+        if a["edges_removed"] != len(u) or k != d.n_clusters or not np.array_equal(cm, d.cluster_map):
+            bad.append((i, len(u), a["edges_removed"], k, d.n_clusters))
+    assert not bad, bad[:10]
+
+
 def test_some_golden_case_replays():
     # the exact replay (and its region search) is exercised by the golden set
     assert any(result(t)["audit"]["replayed_edges"] > 0 for t in GOLDEN_TAGS)

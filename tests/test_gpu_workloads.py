@@ -116,7 +116,8 @@ def test_config4_batch_equals_alone_and_oracle():
     for i, (n, u, v) in enumerate(graphs):
         with R.DeviceGraph.from_edges(n, u, v) as g:
             cm, k = g.decompose(eps)
-        if k != counts[i] or not np.array_equal(cm, maps[i]):
+            removed = g.audit()["edges_removed"]
+        if k != counts[i] or not np.array_equal(cm, maps[i]) or removed != len(u):
             bad_alone.append(i)
         if i % 512 == 511:
             print(f"[gpu] alone {i + 1}/4096 ({time.time() - t0:.1f} s)", flush=True)
