@@ -1374,15 +1374,26 @@ rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out) {
     if (!h->cm_pin) return fail(RTREX_ENOMEM, "page-locked staging allocation failed");
     RT_HIP(hipMemcpyAsync(h->cm_pin, h->cid, (uint64_t)n * 4, hipMemcpyDeviceToHost, h->stream));
     RT_HIP(hipStreamSynchronize(h->stream));
+    const uint32_t* cm = h->cm_pin;
     const size_t ns = h->seg_bounds.empty() ? 0 : h->seg_bounds.size() - 1;
     if (!ns) {
-        for (uint32_t v = 0; v < n; ++v) out[v] = h->cm_pin[v];
+        host_parallel(n, 1u << 20, [cm, out](uint64_t a, uint64_t b) {
+            for (uint64_t v = a; v < b; ++v) out[v] = cm[v];
+        });
         return RTREX_OK;
     }
-    for (size_t q = 0; q < ns; ++q) {  // per-graph ids
-        const int64_t b0 = h->seg_bounds[q];
-        for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v) out[v] = h->cm_pin[v] - b0;
-    }
+    // Per-graph ids: graph q's vertices [seg_vtx[q], seg_vtx[q+1]) hold ids
+    // from seg_bounds[q] on.
+    const std::vector<int64_t>& sv = h->seg_vtx;
+    const std::vector<uint32_t>& sb = h->seg_bounds;
+    host_parallel(n, 1u << 20, [cm, out, ns, &sv, &sb](uint64_t a, uint64_t b) {
+        size_t q = (size_t)(std::upper_bound(sv.begin(), sv.begin() + ns + 1, (int64_t)a) -
+                            sv.begin()) - 1;
+        for (uint64_t v = a; v < b; ++v) {
+            while ((int64_t)v >= sv[q + 1]) ++q;
+            out[v] = (int64_t)cm[v] - (int64_t)sb[q];
+        }
+    });
     return RTREX_OK;
 }
 

@@ -24,11 +24,32 @@
 #include <cstdint>
 #include <string>
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/rtrex.h"
 
 namespace rtrex {
+
+// f(a, b) over contiguous ranges covering [0, n), on up to 8 host threads
+// (one range when n < 2 * min_range): host loops over per-vertex results.
+template <class F>
+void host_parallel(uint64_t n, uint64_t min_range, F&& f) {
+    const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<uint64_t>(hw, n / std::max<uint64_t>(1, min_range));
+    if (nt <= 1) {
+        f(uint64_t(0), n);
+        return;
+    }
+    const uint64_t per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) {
+        const uint64_t a = std::min<uint64_t>(n, t * per), b = std::min<uint64_t>(n, a + per);
+        if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
+    }
+    f(uint64_t(0), std::min<uint64_t>(n, per));
+    for (auto& x : th) x.join();
+}
 
 // Per-slot status codes.  The reference's characters map as Y=ALIVE,
 // S=QUEUED, N=DEAD; PENDING is an edge claimed for the next peel round.
@@ -169,6 +190,7 @@ void* dev_alloc(uint64_t bytes, hipError_t* err);
 void dev_free(void* p);
 void dev_trim();  // every cached block of the current device released
 uint64_t dev_cached_bytes();
+bool dev_cache_fits(uint64_t bytes);  // a cached block would serve dev_alloc(bytes)
 // Page-locked host memory (null on failure), cached the same way.
 void* pin_alloc(uint64_t bytes);
 void pin_free(void* p);

@@ -138,6 +138,15 @@ void dev_trim() {
     for (void* p : drop) (void)hipFree(p);
 }
 
+bool dev_cache_fits(uint64_t bytes) {
+    const uint64_t want = round_bytes(bytes);
+    const int d = current_device();
+    Caches& c = caches();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.dev_free[d].lower_bound(want);
+    return it != c.dev_free[d].end() && it->first <= want + want / 4;
+}
+
 uint64_t dev_cached_bytes() {
     const int d = current_device();
     Caches& c = caches();
