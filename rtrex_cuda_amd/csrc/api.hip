@@ -191,6 +191,8 @@ rtrex_status new_handle(int device, rtrex_graph* out, hipStream_t stream = nullp
     // Checking mode: every cleaning decision re-run in the reference's own
     // order (regions of all removed edges), whatever the certification says.
     if (const char* ra = std::getenv("RTREX_REPLAY_ALL")) h->replay_all = std::atoi(ra) != 0;
+    if (const char* cw = std::getenv("RTREX_CASCADE_WG_MIN"))
+        h->cascade_min = (uint32_t)std::strtoul(cw, nullptr, 10);
     hipError_t e = hipSuccess;
     StreamSet ss;
     if (stream) {

@@ -51,6 +51,7 @@ constexpr int XT = 64;  // one wave per workgroup
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr uint32_t LDS_CAND = 1024;
 constexpr uint32_t XSTAGE = 2048;  // longest N(hi) the inner loop stages in LDS
+constexpr int CWG = 8;             // waves per handed-over cascade (k_cascade_wg)
 
 using isort::Rec;
 
@@ -289,6 +290,19 @@ struct XArgs {
     uint32_t* hbkt;          // [hoff[c], ...): set buckets
     uint32_t* undec;         // edges no bound decided (for an exact re-run)
     uint32_t undec_cap;
+    // Cascades handed to the workgroup kernel (k_cascade_wg): a cascade whose
+    // stack holds cascade_min entries or more between two pops stops; its
+    // component is listed in dlist (count *ndefer) with the stack length
+    // (dtop) and the next seed (resume), and the component waits.  clist
+    // (count *nclist), when given, is the list of components this launch
+    // resumes instead of every component.  cascade_min 0: no hand-over.
+    uint32_t cascade_min = 0;
+    uint32_t* resume = nullptr;
+    uint32_t* dtop = nullptr;
+    uint32_t* dlist = nullptr;
+    unsigned* ndefer = nullptr;
+    const uint32_t* clist = nullptr;
+    const unsigned* nclist = nullptr;
 };
 
 // Cascade decision for alive edge f whose support just became v (threshold
@@ -368,16 +382,19 @@ __device__ __forceinline__ void set_status_at(const GraphDev& g, uint32_t e, uin
 // no load waits on a branch; the degrees and the records' edges' statuses, supports
 // and decision inputs; the third vertices and mirror slots; the third
 // vertices' supports), and the entry on top of the stack is carried in
-// registers when the pop pushed.
-__device__ void cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
-                              unsigned long long& amb_c, unsigned long long& n_tri,
-                              unsigned long long& n_edges, double* s_ct, uint32_t* s_cn) {
+// registers when the pop pushed.  Returns 0, or the stack length when the
+// cascade is handed to the workgroup kernel (the stack is wl[ebase, +top)
+// between pops).
+__device__ uint32_t cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
+                                  unsigned long long& amb_c, unsigned long long& n_tri,
+                                  unsigned long long& n_edges, double* s_ct, uint32_t* s_cn) {
     const GraphDev& g = A.g;
     const TermBuf& tb = A.tb;
     const int lane = lane_id();
     bool have = false;
     uint64_t next = 0;
     while (top > 0) {
+        if (A.cascade_min && top >= A.cascade_min) return top;
         --top;
         const uint64_t ent = have ? next : A.wl[ebase + top];
         have = false;
@@ -514,6 +531,7 @@ __device__ void cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
         next = last;
         __syncthreads();
     }
+    return 0;
 }
 
 __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
@@ -529,17 +547,18 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     const GraphDev& g = A.g;
     const int lane = lane_id();
     unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
-    const uint32_t ncomp = *A.ncomp;
+    const uint32_t nwork = A.clist ? *A.nclist : *A.ncomp;
 
     while (true) {
-        uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(&A.ctr->work_next, 1u);
-        c = __shfl(c, 0, 64);
-        if (c >= ncomp) break;
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&A.ctr->work_next, 1u);
+        i = __shfl(i, 0, 64);
+        if (i >= nwork) break;
+        const uint32_t c = A.clist ? A.clist[i] : i;
         const uint32_t start = A.cstart[c], len = A.cstart[c + 1] - start;
         const uint64_t ebase = A.dpre[start];
 
-        for (uint32_t r = 0; r < len; ++r) {
+        for (uint32_t r = A.resume ? A.resume[c] : 0; r < len; ++r) {
             const uint32_t u = A.cv[start + r];
             if (A.cseed[u] != NONE) continue;
             const uint32_t du = g.deg[u];
@@ -787,7 +806,16 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
 
             // ... and cascade (deleteAndClean, Decomposition.h:147-283)
             if (A.tb.t) {
-                cascade_terms(A, ebase, top, amb_c, n_tri, n_edges, s_ct, &s_cn);
+                const uint32_t left = cascade_terms(A, ebase, top, amb_c, n_tri, n_edges, s_ct,
+                                                    &s_cn);
+                if (left) {  // the workgroup kernel finishes it; seeds resume at r + 1
+                    if (lane == 0) {
+                        A.resume[c] = r + 1;
+                        A.dtop[c] = left;
+                        A.dlist[atomicAdd(A.ndefer, 1u)] = c;
+                    }
+                    break;
+                }
             } else {
             while (top > 0) {
                 --top;
@@ -970,6 +998,167 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     if (lane == 0) {
         if (amb_c) atomicAdd(&A.ctr->ambiguous_clean, amb_c);
         if (amb_r) atomicAdd(&A.ctr->ambiguous_ratio, amb_r);
+    }
+}
+
+// The cascades k_extract handed over (one workgroup of NW waves per listed
+// component): the same LIFO pops over stored triangle lists, each pop's list
+// split into rounds of NW chunks of 64 records, one chunk per wave.  Within
+// a pop the records are distinct triangles {x, y, w}: their third vertices
+// and other edges are distinct, no edge but the popped one dies before the
+// pop ends, and a certification counts DEAD edges only, so the chunks of a
+// round are independent; their pushes go on the stack in chunk order (an
+// exclusive prefix of the waves' push counts), which is the reference's
+// order.  The popped edge's endpoints lose the sum of the pop's live
+// triangle weights in one subtraction (partial sums per lane, then per wave
+// in wave order) instead of one weight at a time: vertex supports are only
+// compared within the error bound that covers any subtraction order
+// (ambiguous_ratio).  Thread 0 of block 0 also clears the work queue head
+// and next_count, the count the next k_extract launch appends to.
+template <int NW>
+__global__ void __launch_bounds__(NW * WAVE) k_cascade_wg(XArgs A, const uint32_t* __restrict__ list,
+                                                           unsigned* next_count) {
+    __shared__ double s_ct[NW][kMaxPerm];
+    __shared__ uint32_t s_cn[NW];
+    __shared__ uint32_t s_tot[2][NW];
+    __shared__ double s_sum[NW];
+    const GraphDev& g = A.g;
+    const TermBuf& tb = A.tb;
+    const int lane = lane_id();
+    const int wid = (int)threadIdx.x / WAVE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *next_count = 0;
+        A.ctr->work_next = 0;
+    }
+    const uint32_t c = list[blockIdx.x];
+    const uint64_t ebase = A.dpre[A.cstart[c]];
+    uint32_t top = A.dtop[c];
+    unsigned long long amb_c = 0, n_tri = 0, n_edges = 0;
+    int par = 0;
+    while (top > 0) {
+        --top;
+        const uint64_t ent = A.wl[ebase + top];
+        const uint32_t e = (uint32_t)(ent >> 32), pa = (uint32_t)ent;
+        const uint32_t x = g.esrc[e], sle = g.eslot[e], tc = g.tcnt[e];
+        const uint64_t c0 = tb.zoff[e], c1 = c0 + tc;
+        const uint32_t y = g.nb[sle];
+        const uint32_t pb = pa == x ? y : x;
+        const uint32_t dpa = g.deg[pa], dpb = g.deg[pb];
+        const uint32_t dx = pa == x ? dpa : dpb, dy = pa == x ? dpb : dpa;
+        const uint32_t tlo = dx <= dy ? x : y;
+        const uint32_t rlo = dpa < dpb ? pa : pb;
+        const uint32_t rhi = rlo == pa ? pb : pa;
+        const bool swp = rlo != tlo;
+        double hsum = 0.0;  // this lane's live triangle weights, for the endpoints
+        const uint32_t nch = (tc + WAVE - 1) / WAVE;
+        for (uint32_t k = 0; k < nch; k += NW) {  // rounds: the same count in every wave
+            const uint64_t kk = c0 + (uint64_t)(k + wid) * WAVE + lane;
+            const bool in = kk < c1;
+            const Term q = term_or_blank(tb.t, kk, in);
+            const uint32_t fj = q.sa, fh = q.sb;
+            const uint8_t sj = g.est[fj], sh = g.est[fh];
+            const uint32_t aj = g.esrc[fj], slj = g.eslot[fj], slh = g.eslot[fh];
+            const double supj = g.sup[fj], suph = g.sup[fh];
+            const double ewj = g.ew[fj], ewh = g.ew[fh];
+            const bool xj = A.xact && A.xact[fj], xh = A.xact && A.xact[fh];
+            const uint32_t tcj = g.tcnt[fj], tch = g.tcnt[fh];
+            const double s0j = g.sup0[fj], s0h = g.sup0[fh];
+            const uint32_t wj = g.nb[slj];
+            const uint32_t spj = g.part[slj], sph = g.part[slh];
+            const bool hit = in && sj != ST_DEAD && sh != ST_DEAD;
+            const uint32_t w = aj == tlo ? wj : aj;  // third vertex
+            const double t = q.w;
+            n_tri += lane == 0 ? (unsigned long long)__popcll(ballot64(hit)) : 0ull;
+            const uint32_t jl = swp ? fh : fj, jh = swp ? fj : fh;
+            const uint8_t sl = swp ? sh : sj, shh = swp ? sj : sh;
+            const double supl = swp ? suph : supj, suph2 = swp ? supj : suph;
+            const double ewl = swp ? ewh : ewj, ewh2 = swp ? ewj : ewh;
+            const bool xl = swp ? xh : xj, xh2 = swp ? xj : xh;
+            const uint32_t tcl = swp ? tch : tcj, tch2 = swp ? tcj : tch;
+            const double s0l = swp ? s0h : s0j, s0h2 = swp ? s0j : s0h;
+            bool push1 = false, push2 = false, need1 = false, need2 = false;
+            double v1 = 0.0, v2 = 0.0, thr1 = 0.0, thr2 = 0.0;
+            if (hit) {
+                hsum += t;
+                g.pv[w] -= t;
+                if (sl == ST_ALIVE) {
+                    v1 = supl - t;
+                    g.sup[jl] = v1;
+                    g.touched[jl] = 1;
+                    thr1 = A.eps * ewl;
+                    push1 = decide_cheap_v(xl, tcl, s0l, v1, thr1, need1);
+                }
+                if (shh == ST_ALIVE) {
+                    v2 = suph2 - t;
+                    g.sup[jh] = v2;
+                    g.touched[jh] = 1;
+                    thr2 = A.eps * ewh2;
+                    push2 = decide_cheap_v(xh2, tch2, s0h2, v2, thr2, need2);
+                }
+            }
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                uint64_t nb = ballot64(side == 0 ? need1 : need2);
+                while (nb) {  // wave-uniform
+                    const int bl = __ffsll((unsigned long long)nb) - 1;
+                    nb &= nb - 1;
+                    const uint32_t f = __shfl(side == 0 ? jl : jh, bl, 64);
+                    const double th = __shfl(side == 0 ? thr1 : thr2, bl, 64);
+                    const int cc = certify_terms_wave(g, tb, f, th, false, e, s_ct[wid], &s_cn[wid]);
+                    if (lane == bl) {
+                        if (side == 0) push1 = settle(A, f, v1, th, cc, amb_c);
+                        else push2 = settle(A, f, v2, th, cc, amb_c);
+                    }
+                }
+            }
+            const uint32_t np2 = (uint32_t)push1 + (uint32_t)push2;
+            uint32_t pre = np2;  // inclusive prefix of pushes over the wave's lanes
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(pre, d, 64);
+                if (lane >= d) pre += o;
+            }
+            if (lane == 63) s_tot[par][wid] = pre;
+            pre -= np2;
+            __syncthreads();
+            uint32_t base = 0, rtot = 0;
+#pragma unroll
+            for (int w2 = 0; w2 < NW; ++w2) {
+                const uint32_t tw = s_tot[par][w2];
+                base += w2 < wid ? tw : 0u;
+                rtot += tw;
+            }
+            par ^= 1;
+            const uint64_t e1 = ((uint64_t)jl << 32) | rlo, e2 = ((uint64_t)jh << 32) | rhi;
+            if (push1) {
+                A.wl[ebase + top + base + pre] = e1;
+                set_status_at(g, jl, swp ? slh : slj, swp ? sph : spj, ST_QUEUED);
+            }
+            if (push2) {
+                A.wl[ebase + top + base + pre + (uint32_t)push1] = e2;
+                set_status_at(g, jh, swp ? slj : slh, swp ? spj : sph, ST_QUEUED);
+            }
+            top += rtot;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) hsum += __shfl_xor(hsum, d, 64);
+        if (lane == 0) s_sum[wid] = hsum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0.0;
+            for (int w2 = 0; w2 < NW; ++w2) tot += s_sum[w2];
+            g.pv[x] -= tot;
+            g.pv[y] -= tot;
+            set_status_at(g, e, sle, g.part[sle], ST_DEAD);
+            g.sup[e] = 0.0;
+            n_edges++;
+        }
+        __syncthreads();  // the stack, statuses and supports for the next pop
+    }
+    for (int d = 32; d >= 1; d >>= 1) amb_c += __shfl_xor(amb_c, d, 64);
+    if (lane == 0) {
+        if (n_tri) atomicAdd(&A.ctr->tri_removed, n_tri);
+        if (n_edges) atomicAdd(&A.ctr->edges_removed, n_edges);
+        if (amb_c) atomicAdd(&A.ctr->ambiguous_clean, amb_c);
     }
 }
 
@@ -1269,8 +1458,46 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.undec = undec;
         a.undec_cap = g.m;
         const unsigned grid = std::min<uint32_t>(nk / 2, 8192);
+        // Long cascades go to a workgroup each (k_cascade_wg): listed by
+        // k_extract, finished, then their components' seed loops resumed,
+        // until no launch hands one over.  Components have two vertices or
+        // more, so at most nk / 2 are listed per launch.
+        const bool handoff = h->cascade_min && a.tb.t;
+        const uint32_t dcap = nk / 2 + 1;
+        uint32_t *resume = nullptr, *dtop = nullptr, *dlist = nullptr;
+        unsigned* dcount = nullptr;
+        if (handoff) {
+            RT_TRY(sc.get(&resume, nk));
+            RT_TRY(sc.get(&dtop, nk));
+            RT_TRY(sc.get(&dlist, 2ull * dcap));
+            RT_TRY(sc.get(&dcount, 2));
+            RT_HIP(hipMemsetAsync(resume, 0, (uint64_t)nk * 4, s));
+            RT_HIP(hipMemsetAsync(dcount, 0, 2 * sizeof(unsigned), s));
+            a.cascade_min = h->cascade_min;
+            a.resume = resume;
+            a.dtop = dtop;
+            a.dlist = dlist;
+            a.ndefer = dcount;
+        }
         k_extract<<<grid, XT, 0, s>>>(a);
         RT_HIP(hipGetLastError());
+        for (int cur = 0; handoff; cur ^= 1) {
+            k_publish<<<1, WAVE, 0, s>>>(h->box_dev, ++h->box_seq, dcount + cur);
+            RT_HIP(hipGetLastError());
+            uint32_t nd = 0;
+            RT_TRY(box_wait(h, 0, h->box_seq, &nd));
+            if (!nd) break;
+            if (nd > dcap) return fail(RTREX_EDEVICE, "cascade hand-over list overflow");
+            const uint32_t* lst = dlist + (uint64_t)cur * dcap;
+            k_cascade_wg<CWG><<<nd, CWG * WAVE, 0, s>>>(a, lst, dcount + (cur ^ 1));
+            XArgs b = a;
+            b.clist = lst;
+            b.nclist = dcount + cur;
+            b.dlist = dlist + (uint64_t)(cur ^ 1) * dcap;
+            b.ndefer = dcount + (cur ^ 1);
+            k_extract<<<std::min<uint32_t>(nd, grid), XT, 0, s>>>(b);
+            RT_HIP(hipGetLastError());
+        }
         // Undecided cascade decisions (inexact post-clean supports): kept for
         // an exact re-run of their regions (rtrex_extract); their count is
         // read with the phase's last results below.

@@ -228,6 +228,9 @@ struct rtrex_graph_s {
     std::vector<uint32_t> dlog_e, dlog_a, dlog_b, dlog_c;
     uint32_t dlog_calls = 0;
     bool replay_all = false;  // every clean decision re-run in the reference's order (checking)
+    // Extraction: a cascade whose stack reaches this many entries is finished
+    // by a workgroup of several waves instead of one wave (0: never).
+    uint32_t cascade_min = 4096;
     std::vector<uint32_t> force_replay;  // edges whose regions the clean re-runs
     bool force_all = false;
     double force_eps = -1;  // the eps force_replay was found at

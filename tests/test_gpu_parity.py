@@ -149,6 +149,25 @@ def test_small_dense_graphs_each_edge_removed_once():
     assert not bad, bad[:10]
 
 
+@pytest.mark.parametrize("tag", GOLDEN_TAGS)
+def test_extract_with_every_cascade_in_a_workgroup(tag, monkeypatch):
+    # RTREX_CASCADE_WG_MIN=1: every extraction cascade is finished by the
+    # workgroup kernel (k_cascade_wg) and every component's seed loop resumed
+    # after each cluster; results as the reference's.
+    name = tag.rsplit("_eps", 1)[0]
+    eps = golden_meta()[tag]["eps"]
+    n, u, v = golden_graph(name)
+    monkeypatch.setenv("RTREX_CASCADE_WG_MIN", "1")
+    print(f"[gpu] {tag} cascades in workgroups", flush=True)
+    with R.DeviceGraph.from_edges(n, u, v) as g:
+        cm, k = g.decompose(eps)
+        a = g.audit()
+    ref = golden_result(tag)
+    assert a["ambiguous_clean"] == 0 and a["ambiguous_ratio"] == 0
+    assert a["edges_removed"] == len(u)
+    assert np.array_equal(cm, ref["cluster_map"])
+
+
 def test_some_golden_case_replays():
     # the exact replay (and its region search) is exercised by the golden set
     assert any(result(t)["audit"]["replayed_edges"] > 0 for t in GOLDEN_TAGS)

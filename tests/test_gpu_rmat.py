@@ -65,3 +65,26 @@ def test_rmat15_against_oracle(plan, eps):
     assert k == d.n_clusters and np.array_equal(cm, d.cluster_map)
     stats = np.array([[x[0], x[1], x[2]] for x in d.clusters], np.int64)
     assert np.array_equal(np.stack([nv, ne, cut], 1), stats)
+
+
+@pytest.mark.parametrize("cascade_min", [1, 64])
+@pytest.mark.parametrize("eps", [0.05, 0.02])
+def test_rmat15_cascades_in_workgroups(cascade_min, eps, monkeypatch):
+    """The extraction's cascades handed to the workgroup kernel (8 waves per
+    cascade, the pop's triangle list split over the waves) from a stack of
+    cascade_min entries on, through the RTREX_CASCADE_WG_MIN setting read at
+    handle creation: 1 sends every cascade there, 64 hands cascades over in
+    the middle.  Clusters and stats equal the oracle's."""
+    g = graph()
+    d = oracle_decomposition(eps)
+    monkeypatch.setenv("RTREX_CASCADE_WG_MIN", str(cascade_min))
+    print(f"[gpu] rmat15 cascades in workgroups from {cascade_min}, eps={eps}", flush=True)
+    with R.DeviceGraph.from_edges(g["n"], g["u"], g["v"]) as dg:
+        cm, k = dg.decompose(eps)
+        nv, ne, cut = dg.populate_stats(k)
+        a = dg.audit()
+    assert a["ambiguous_clean"] == 0 and a["ambiguous_ratio"] == 0
+    assert a["edges_removed"] == len(g["u"])
+    assert k == d.n_clusters and np.array_equal(cm, d.cluster_map)
+    stats = np.array([[x[0], x[1], x[2]] for x in d.clusters], np.int64)
+    assert np.array_equal(np.stack([nv, ne, cut], 1), stats)
