@@ -4,7 +4,10 @@ reference:
   skewed large  R-MAT scale 22 (64.2 M edges): CSR, triangle total, FP64
                 supports, survivors and clusters against hashes of the
                 reference's own run (tests/golden/config3.json, written by
-                make_golden_c3.py from the reference sources).
+                make_golden_c3.py from the reference sources); the clusters
+                of a fixture with more than 100,000 survivors (one giant
+                alive component, minutes of extraction) with
+                RTREX_SLOW_TESTS=1 only.
   batched       the 4,096 planted-community graphs in one batch: every graph's
                 cluster map equals the graph decomposed alone through the plain
                 path AND the CPU oracle's.
@@ -46,8 +49,9 @@ def _json(name):
         return json.load(f)
 
 
-def _full_hashes(n, u, v, eps, label):
-    """Every hash the reference fixtures hold, from the HIP path."""
+def _full_hashes(n, u, v, eps, label, extract=True):
+    """Every hash the reference fixtures hold, from the HIP path (the
+    extraction's only when extract)."""
     t0 = time.time()
     with R.DeviceGraph.from_edges(n, u, v) as g:
         off, nb = g.csr()
@@ -62,18 +66,27 @@ def _full_hashes(n, u, v, eps, label):
         surv = survivors_of(off, nb, st)
         out["survivors"], out["survivors_sha"] = int(len(surv)), sha(surv)
         del st, surv
-        cm, k = g.disjoint_extract(eps)
-        out["n_clusters"], out["cluster_map_sha"] = k, sha(cm)
-        out["cluster_sets_sha"] = V.canonical_cluster_sha(cm)
+        print(f"[gpu] {label}: cleaned ({time.time() - t0:.1f} s)", flush=True)
+        if extract:
+            cm, k = g.disjoint_extract(eps)
+            out["n_clusters"], out["cluster_map_sha"] = k, sha(cm)
+            out["cluster_sets_sha"] = V.canonical_cluster_sha(cm)
+            print(f"[gpu] {label}: extracted ({time.time() - t0:.1f} s)", flush=True)
         out["audit"] = g.audit()
-        print(f"[gpu] {label}: decomposed ({time.time() - t0:.1f} s)", flush=True)
     return out
 
 
-KEYS = ("csr_sha", "count_total", "count_perEdge_sha", "count_perVertex_sha", "survivors",
-        "survivors_sha", "n_clusters", "cluster_map_sha", "cluster_sets_sha")
+COUNT_CLEAN_KEYS = ("csr_sha", "count_total", "count_perEdge_sha", "count_perVertex_sha",
+                    "survivors", "survivors_sha")
+EXTRACT_KEYS = ("n_clusters", "cluster_map_sha", "cluster_sets_sha")
 
 C3 = _json("config3.json")
+
+# The extraction of a fixture whose clean leaves more survivors than this runs
+# one giant alive component (R-MAT scale 22 at eps 0.02: 2.9 M edges on 5,900
+# vertices, about 150 s on one MI355X) and only with RTREX_SLOW_TESTS=1.
+C3_EXTRACT_SURVIVORS = 100_000
+SLOW = os.environ.get("RTREX_SLOW_TESTS") == "1"
 
 
 @pytest.fixture(scope="module")
@@ -91,11 +104,13 @@ def test_config3_matches_reference(tag, config3_graph):
     ref = C3[tag]
     n, u, v = config3_graph
     assert (n, len(u)) == (ref["n"], ref["m"])
-    got = _full_hashes(n, u, v, ref["eps"], tag)
-    for k in KEYS:
+    extract = SLOW or ref["survivors"] <= C3_EXTRACT_SURVIVORS
+    got = _full_hashes(n, u, v, ref["eps"], tag, extract)
+    for k in COUNT_CLEAN_KEYS + (EXTRACT_KEYS if extract else ()):
         assert got[k] == ref[k], k
     assert got["audit"]["ambiguous_clean"] == 0
-    assert got["audit"]["ambiguous_ratio"] == 0
+    if extract:
+        assert got["audit"]["ambiguous_ratio"] == 0
 
 
 def _oracle_map(g, eps):
