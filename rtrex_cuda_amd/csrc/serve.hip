@@ -16,11 +16,13 @@
 // join the next launch that a worker forms.  A launch that does not fit while
 // nothing else runs is run outside the pool.  A batch is the batched path:
 // per-graph results equal running each graph alone.
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -34,7 +36,7 @@ namespace {
 
 struct Job {
     int64_t id = 0;
-    std::vector<uint32_t> u, v;
+    std::unique_ptr<uint32_t[]> u, v;  // the submitted edges (m each), copied at submission
     int64_t n = 0, m = 0;
     double eps = 0;
     uint64_t cap = 0;  // upper bound of stored triangle-list records
@@ -99,8 +101,9 @@ struct rtrex_server_s {
     std::condition_variable cv;
     std::deque<Job*> queue;
     std::deque<Job*> intake;  // submitted, footprint not yet computed
+    int intake_busy = 0;      // jobs an intake thread is sizing
     std::condition_variable cv_in;
-    std::thread intake_thread;
+    std::vector<std::thread> intake_threads;
     JobTable<Job> jobs;
     int running = 0;
     bool stop = false;
@@ -119,7 +122,7 @@ void fail_job(Job* j, rtrex_status st) {
 
 void run_single(rtrex_server s, Job* j, hipStream_t st, char* slab, uint64_t bytes) {
     rtrex_graph g = nullptr;
-    rtrex_status r = create_edges_in(j->u.data(), j->v.data(), j->n, j->m, s->cfg.device, st,
+    rtrex_status r = create_edges_in(j->u.get(), j->v.get(), j->n, j->m, s->cfg.device, st,
                                      slab, bytes, &g);
     if (r == RTREX_OK) {
         j->cmap.resize((size_t)j->n);
@@ -139,8 +142,8 @@ void run_batch(rtrex_server s, std::vector<Job*>& js, hipStream_t st, char* slab
     }
     std::vector<uint32_t> u(eoff[B]), v(eoff[B]);
     for (int64_t i = 0; i < B; ++i) {
-        std::copy(js[i]->u.begin(), js[i]->u.end(), u.begin() + eoff[i]);
-        std::copy(js[i]->v.begin(), js[i]->v.end(), v.begin() + eoff[i]);
+        std::copy(js[i]->u.get(), js[i]->u.get() + js[i]->m, u.begin() + eoff[i]);
+        std::copy(js[i]->v.get(), js[i]->v.get() + js[i]->m, v.begin() + eoff[i]);
     }
     std::vector<int64_t> vbase;
     std::vector<uint32_t> U, V;
@@ -185,8 +188,10 @@ uint64_t term_cap(const Job* j) {
     return cap;
 }
 
-// Moves submitted jobs to the run queue in submission order once their
-// footprint is known (kept off the submitting thread).
+// Moves submitted jobs to the run queue once their footprint is known, off
+// the submitting thread.  Several intake threads size jobs side by side, so a
+// job enters the queue when it is sized: a large graph (tens of milliseconds
+// of degree counting) does not hold up the small jobs submitted after it.
 void intake_main(rtrex_server s) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
@@ -196,10 +201,12 @@ void intake_main(rtrex_server s) {
             continue;
         }
         Job* j = s->intake.front();
+        s->intake.pop_front();
+        s->intake_busy++;
         lk.unlock();
         j->cap = term_cap(j);
         lk.lock();
-        s->intake.pop_front();
+        s->intake_busy--;
         s->queue.push_back(j);
         s->cv.notify_all();
     }
@@ -209,19 +216,19 @@ void worker_main(rtrex_server s, int w) {
     (void)hipSetDevice(s->cfg.device);
     hipStream_t st = s->streams[w];
     std::unique_lock<std::mutex> lk(s->mu);
-    for (;;) {
-        s->cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
-        if (s->queue.empty()) {
-            if (s->stop && s->intake.empty()) return;
-            if (s->stop) s->cv.wait(lk);  // the intake still has jobs
-            continue;
-        }
-        // The launch: the head job, plus the small jobs right behind it.
-        std::vector<Job*> take{s->queue.front()};
+    // The launch starting at queue position p: that job, plus the small jobs
+    // right behind it.  The stored triangle lists count at most a quarter of
+    // the pool: a job whose upper bound is larger (skewed graphs: the bound can
+    // be several times the exact size) sizes its lists exactly after a
+    // counting pass and takes them from its range when they fit, else from
+    // the device cache -- so it runs beside other jobs instead of waiting for
+    // the whole server to drain.
+    auto form = [&](size_t p, std::vector<Job*>& take) {
+        take.assign(1, s->queue[p]);
         const Job* h0 = take[0];
         uint64_t N = h0->n, M = h0->m, Cp = h0->cap;
         if ((int64_t)h0->m <= s->cfg.small_job_edges) {
-            for (size_t i = 1; i < s->queue.size(); ++i) {
+            for (size_t i = p + 1; i < s->queue.size(); ++i) {
                 const Job* j = s->queue[i];
                 if ((int64_t)j->m > s->cfg.small_job_edges || j->eps != h0->eps) break;
                 if ((int64_t)(M + j->m) > s->cfg.batch_max_edges ||
@@ -233,21 +240,59 @@ void worker_main(rtrex_server s, int w) {
                 Cp += j->cap;
             }
         }
-        // The stored triangle lists count at most half the pool: a job whose
-        // upper bound is larger (skewed graphs: the bound can be several
-        // times the exact size) sizes its lists exactly after a counting pass
-        // and takes them from its range when they fit, else outside the pool
-        // -- so it runs beside other jobs instead of waiting for the whole
-        // server to drain.
-        const uint64_t bytes =
-            launch_bytes(N, M, std::min<uint64_t>(Cp, s->pool.cap / 2 / sizeof(Term)));
-        uint64_t off = 0;
-        const bool pooled = s->pool.take(bytes, &off);
-        if (!pooled && s->running > 0) {
-            s->cv.wait(lk);  // admission: wait for a running launch to free memory
+        return launch_bytes(N, M, std::min<uint64_t>(Cp, s->pool.cap / 4 / sizeof(Term)));
+    };
+    // Reserved workers: with three workers or more, worker 0 takes only
+    // launches headed by a small job, and with four or more, worker 1 only
+    // jobs of at most 32 times that many edges -- each the first such job in
+    // the queue -- so that small and medium jobs always find a worker while
+    // large graphs hold the others for hundreds of milliseconds.
+    const int64_t lim = w == 0 && s->cfg.workers >= 3   ? s->cfg.small_job_edges
+                        : w == 1 && s->cfg.workers >= 4 ? 32 * s->cfg.small_job_edges
+                                                        : INT64_MAX;
+    const bool small_only = lim != INT64_MAX;
+    auto first_small = [&]() -> size_t {
+        const size_t look = std::min<size_t>(s->queue.size(), 256);
+        for (size_t p = 0; p < look; ++p)
+            if ((int64_t)s->queue[p]->m <= lim) return p;
+        return SIZE_MAX;
+    };
+    std::vector<Job*> take;
+    for (;;) {
+        s->cv.wait(lk, [&] {
+            return s->stop || (small_only ? first_small() != SIZE_MAX : !s->queue.empty());
+        });
+        const size_t p0 = small_only ? first_small() : (s->queue.empty() ? SIZE_MAX : 0);
+        if (p0 == SIZE_MAX) {
+            if (s->stop && s->queue.empty() && s->intake.empty() && s->intake_busy == 0) return;
+            if (s->stop) s->cv.wait(lk);  // jobs still in the intake or for other workers
             continue;
         }
-        for (size_t i = 0; i < take.size(); ++i) s->queue.pop_front();
+        uint64_t bytes = form(p0, take);
+        size_t at = p0;
+        uint64_t off = 0;
+        bool pooled = s->pool.take(bytes, &off);
+        if (!pooled && s->running > 0) {
+            // Admission: the head waits for a running launch to free memory.
+            // Meanwhile a small launch further back (at most 1/16 of the pool)
+            // that fits now may go first; the head keeps its place and is
+            // tried first whenever a launch ends.
+            const size_t look = std::min<size_t>(s->queue.size(), 64);
+            for (size_t p = p0 + 1; p < look && !pooled; ++p) {
+                const uint64_t b = form(p, take);
+                if (b > s->pool.cap / 16) continue;
+                if (s->pool.take(b, &off)) {
+                    pooled = true;
+                    bytes = b;
+                    at = p;
+                }
+            }
+            if (!pooled) {
+                s->cv.wait(lk);
+                continue;
+            }
+        }
+        s->queue.erase(s->queue.begin() + at, s->queue.begin() + at + take.size());
         s->running++;
         const double t_start = s->now();
         lk.unlock();
@@ -324,7 +369,7 @@ rtrex_status rtrex_server_start(const rtrex_server_config* cfg, rtrex_server* ou
             return hip_fail(e, "server stream creation", __FILE__, __LINE__);
         }
     }
-    s->intake_thread = std::thread(intake_main, s);
+    for (int t = 0; t < 4; ++t) s->intake_threads.emplace_back(intake_main, s);
     for (int w = 0; w < cfg->workers; ++w) s->workers.emplace_back(worker_main, s, w);
     *out = s;
     return RTREX_OK;
@@ -337,12 +382,25 @@ rtrex_status rtrex_server_submit(rtrex_server s, const uint32_t* u, const uint32
         return fail(RTREX_EINVAL, "n and 2m must be in [0, 2^32-1)");
     if (m && (!u || !v)) return fail(RTREX_EINVAL, "null edge array");
     if (!(eps > 0)) return fail(RTREX_EINVAL, "eps must be > 0");
-    for (int64_t i = 0; i < m; ++i)
-        if ((int64_t)u[i] >= n || (int64_t)v[i] >= n)
-            return fail(RTREX_EINVAL, "edge endpoint >= n");
-    auto* j = new Job();
-    j->u.assign(u, u + m);
-    j->v.assign(v, v + m);
+    // The edges are copied (the caller may reuse its arrays) and checked in
+    // the same pass, on several host threads for a large graph.
+    std::unique_ptr<Job> jp(new Job());
+    jp->u.reset(new uint32_t[std::max<int64_t>(m, 1)]);
+    jp->v.reset(new uint32_t[std::max<int64_t>(m, 1)]);
+    std::atomic<bool> bad{false};
+    uint32_t* du = jp->u.get();
+    uint32_t* dv = jp->v.get();
+    host_parallel((uint64_t)m, 1u << 20, [&](uint64_t a, uint64_t b) {
+        uint32_t mx = 0;
+        for (uint64_t i = a; i < b; ++i) {
+            du[i] = u[i];
+            dv[i] = v[i];
+            mx = std::max(mx, std::max(u[i], v[i]));
+        }
+        if (b > a && (int64_t)mx >= n) bad = true;
+    });
+    if (bad) return fail(RTREX_EINVAL, "edge endpoint >= n");
+    Job* j = jp.release();
     j->n = n;
     j->m = m;
     j->eps = eps;
@@ -396,7 +454,7 @@ rtrex_status rtrex_server_stop(rtrex_server s) {
         s->cv.notify_all();
         s->cv_in.notify_all();
     }
-    s->intake_thread.join();
+    for (auto& t : s->intake_threads) t.join();
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->cv.notify_all();
