@@ -5,10 +5,11 @@
 // (RTRex_CUDA/gpu_graph.cu:46-135), makeCSR + sortById (Graph.cpp:344-384,
 // 201-205) and getPartnerMap (Graph.cpp:320-340).
 //
-// Edge-list build: both directions of every edge become a u64 key
-// (src << B | dst), B = bits needed for n; an LSD radix sort orders them by
-// (src, dst) exactly as makeCSR's std::sort of pairs does, offsets come from a
-// degree histogram + exclusive scan.  Partner slots are found by a binary
+// Edge-list build: degrees by atomics, offsets by an exclusive scan, both
+// directions of every edge scattered into their lists, then each list sorted
+// on its own (by a thread, a wave or a wave in LDS by length; lists over 4,096
+// entries together by the LSD radix sort) -- the order makeCSR's std::sort of
+// (src, dst) pairs gives, without sorting all 2m pairs.  Partner slots are found by a binary
 // search of src in dst's sorted list (one dependent search per slot, coalesced
 // outer loop); edge ids are the ranks of the canonical slots (src < dst).
 #include <algorithm>
@@ -134,43 +135,203 @@ __global__ void __launch_bounds__(T) k_pack_slots(const uint32_t* __restrict__ n
     }
 }
 
-// ---- edge-list build --------------------------------------------------------
-__global__ void __launch_bounds__(T) k_make_keys(const uint32_t* __restrict__ u,
-                                                 const uint32_t* __restrict__ v,
-                                                 uint64_t m, uint32_t n, int B,
-                                                 uint64_t* __restrict__ keys,
-                                                 unsigned* err) {
+// ---- edge-list build: the CSR without a global sort -- degrees, offsets, each
+// edge scattered into both endpoints' lists, then every list sorted on its
+// own.  An out-of-range id is flagged and the edge taken as (0, 0), a
+// self-loop flagged (the build then fails in finish()), so every slot holds
+// ids in range.
+__device__ __forceinline__ void edge_ends(const uint32_t* __restrict__ u,
+                                          const uint32_t* __restrict__ v, uint64_t i,
+                                          uint32_t n, uint32_t& a, uint32_t& b, unsigned& e) {
+    a = u[i];
+    b = v[i];
+    e = 0;
+    if (a >= n || b >= n) {
+        e |= ERR_RANGE;
+        a = b = 0;
+    }
+    if (a == b) e |= ERR_SELF_LOOP;
+}
+
+__global__ void __launch_bounds__(T) k_edge_degrees(const uint32_t* __restrict__ u,
+                                                    const uint32_t* __restrict__ v, uint64_t m,
+                                                    uint32_t n, uint32_t* __restrict__ deg,
+                                                    unsigned* err) {
     for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < m;
          i += (uint64_t)gridDim.x * T) {
-        uint32_t a = u[i], b = v[i];
-        unsigned e = 0;
-        if (a >= n || b >= n) {
-            e |= ERR_RANGE;
-            a = b = 0;
-        }
-        if (a == b) e |= ERR_SELF_LOOP;
+        uint32_t a, b;
+        unsigned e;
+        edge_ends(u, v, i, n, a, b, e);
         if (e) atomicOr(err, e);
-        keys[2 * i] = ((uint64_t)a << B) | b;
-        keys[2 * i + 1] = ((uint64_t)b << B) | a;
+        atomicAdd(&deg[a], 1u);
+        atomicAdd(&deg[b], 1u);
     }
 }
 
-__global__ void __launch_bounds__(T) k_split_keys(const uint64_t* __restrict__ keys,
-                                                  uint64_t nnz, int B,
-                                                  uint32_t* __restrict__ nb,
-                                                  uint32_t* __restrict__ src,
-                                                  uint32_t* __restrict__ deg,
-                                                  unsigned* err) {
-    const uint64_t mask = (1ull << B) - 1;
-    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < nnz;
+// cur: per-vertex fill counts, zero on entry.  Lists come out unordered.
+__global__ void __launch_bounds__(T) k_edge_fill(const uint32_t* __restrict__ u,
+                                                 const uint32_t* __restrict__ v, uint64_t m,
+                                                 uint32_t n, const uint32_t* __restrict__ off,
+                                                 uint32_t* __restrict__ cur,
+                                                 uint32_t* __restrict__ nb,
+                                                 uint32_t* __restrict__ src) {
+    for (uint64_t i = (uint64_t)blockIdx.x * T + threadIdx.x; i < m;
          i += (uint64_t)gridDim.x * T) {
-        uint64_t k = keys[i];
-        uint32_t s = (uint32_t)(k >> B);
-        nb[i] = (uint32_t)(k & mask);
-        src[i] = s;
-        atomicAdd(&deg[s], 1u);
-        if (i > 0 && keys[i - 1] == k) atomicOr(err, ERR_UNSORTED_OR_DUP);
+        uint32_t a, b;
+        unsigned e;
+        edge_ends(u, v, i, n, a, b, e);
+        const uint32_t pa = off[a] + atomicAdd(&cur[a], 1u);
+        const uint32_t pb = off[b] + atomicAdd(&cur[b], 1u);
+        nb[pa] = b;
+        src[pa] = a;
+        nb[pb] = a;
+        src[pb] = b;
     }
+}
+
+// Every list sorted ascending, each list by the cheapest unit for its
+// length: up to 16 entries by one thread (a bitonic network in registers),
+// up to 64 by one wave (a bitonic network across the lanes), up to SL_LDS by
+// one wave in LDS; longer lists are listed in big (count *nbig) for the radix
+// sort.  A repeated neighbour is an error (duplicate edge).
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+constexpr uint32_t SL_REG = 16;
+__global__ void __launch_bounds__(T) k_sort_short(const uint32_t* __restrict__ off, uint32_t n,
+                                                  uint32_t* __restrict__ nb, unsigned* err) {
+    for (uint32_t x = blockIdx.x * T + threadIdx.x; x < n; x += gridDim.x * T) {
+        const uint32_t b0 = off[x], d = off[x + 1] - b0;
+        if (d < 2 || d > SL_REG) continue;
+        uint32_t r[SL_REG];
+#pragma unroll
+        for (uint32_t i = 0; i < SL_REG; ++i) r[i] = i < d ? nb[b0 + i] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t k = 2; k <= SL_REG; k <<= 1) {
+#pragma unroll
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (uint32_t i = 0; i < SL_REG; ++i) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        if ((i & k) == 0) cswap(r[i], r[l]);
+                        else cswap(r[l], r[i]);
+                    }
+                }
+            }
+        }
+        bool dup = false;
+#pragma unroll
+        for (uint32_t i = 0; i < SL_REG; ++i) {
+            if (i < d) nb[b0 + i] = r[i];
+            if (i + 1 < d) dup |= r[i] == r[i + 1];
+        }
+        if (dup) atomicOr(err, ERR_UNSORTED_OR_DUP);
+    }
+}
+
+constexpr int SL_W = 4;
+constexpr uint32_t SL_LDS = 4096;
+__global__ void __launch_bounds__(SL_W * WAVE) k_sort_long(const uint32_t* __restrict__ off,
+                                                           uint32_t n, uint32_t* __restrict__ nb,
+                                                           uint32_t* __restrict__ big,
+                                                           unsigned* nbig, unsigned* err) {
+    __shared__ uint32_t s_a[SL_W][SL_LDS];
+    const int lane = lane_id();
+    const uint32_t wave = (blockIdx.x * SL_W * WAVE + threadIdx.x) / WAVE;
+    const uint32_t nwaves = gridDim.x * SL_W;
+    uint32_t* a = s_a[threadIdx.x / WAVE];
+    // 64 vertices per step; the lists longer than SL_REG among them in turn
+    for (uint32_t c = wave * WAVE; c < n; c += nwaves * WAVE) {
+        const uint32_t xl = c + lane;
+        const uint32_t o0 = xl < n ? off[xl] : 0, o1 = xl < n ? off[xl + 1] : 0;
+        uint64_t todo = ballot64(o1 - o0 > SL_REG);
+        while (todo) {  // wave-uniform
+            const int q = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t x = c + (uint32_t)q;
+            const uint32_t b0 = __shfl(o0, q, WAVE), d = __shfl(o1, q, WAVE) - b0;
+            if (d <= (uint32_t)WAVE) {
+                uint32_t y = (uint32_t)lane < d ? nb[b0 + lane] : 0xffffffffu;
+#pragma unroll
+                for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+                    for (int j = k >> 1; j > 0; j >>= 1) {
+                        const uint32_t o = __shfl_xor(y, j, WAVE);
+                        const bool asc = (lane & k) == 0, low = (lane & j) == 0;
+                        y = low == asc ? min(y, o) : max(y, o);
+                    }
+                }
+                const uint32_t nx = __shfl_down(y, 1, WAVE);
+                if ((uint32_t)lane + 1 < d && y == nx) atomicOr(err, ERR_UNSORTED_OR_DUP);
+                if ((uint32_t)lane < d) nb[b0 + lane] = y;
+            } else if (d <= SL_LDS) {
+                uint32_t np = 2 * WAVE;
+                while (np < d) np <<= 1;
+                for (uint32_t i = lane; i < np; i += WAVE) a[i] = i < d ? nb[b0 + i] : 0xffffffffu;
+                wave_sync();
+                for (uint32_t k = 2; k <= np; k <<= 1) {
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                        for (uint32_t i = lane; i < np; i += WAVE) {
+                            const uint32_t l = i ^ j;
+                            if (l > i) {
+                                const uint32_t ai = a[i], al = a[l];
+                                if ((ai > al) == ((i & k) == 0)) {
+                                    a[i] = al;
+                                    a[l] = ai;
+                                }
+                            }
+                        }
+                        wave_sync();
+                    }
+                }
+                for (uint32_t i = lane; i < d; i += WAVE) {
+                    const uint32_t y = a[i];
+                    if (i + 1 < d && a[i + 1] == y) atomicOr(err, ERR_UNSORTED_OR_DUP);
+                    nb[b0 + i] = y;
+                }
+                wave_sync();  // a[] is reused by the wave's next list
+            } else if (lane == 0) {
+                big[atomicAdd(nbig, 1u)] = x;
+            }
+        }
+    }
+}
+
+// Lists longer than SL_LDS: keys (list index << B | neighbour) gathered at
+// pos[i] for big list i, radix-sorted, written back; one workgroup per list.
+__global__ void __launch_bounds__(T) k_big_keys(const uint32_t* __restrict__ off,
+                                                const uint32_t* __restrict__ big,
+                                                const uint64_t* __restrict__ pos, int B,
+                                                const uint32_t* __restrict__ nb,
+                                                uint64_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x, x = big[i];
+    const uint32_t b0 = off[x], d = off[x + 1] - b0;
+    for (uint32_t j = threadIdx.x; j < d; j += T) keys[pos[i] + j] = ((uint64_t)i << B) | nb[b0 + j];
+}
+
+__global__ void __launch_bounds__(T) k_big_back(const uint32_t* __restrict__ off,
+                                                const uint32_t* __restrict__ big,
+                                                const uint64_t* __restrict__ pos, int B,
+                                                const uint64_t* __restrict__ keys,
+                                                uint32_t* __restrict__ nb, unsigned* err) {
+    const uint32_t i = blockIdx.x, x = big[i];
+    const uint32_t b0 = off[x], d = off[x + 1] - b0;
+    const uint64_t mask = (1ull << B) - 1;
+    for (uint32_t j = threadIdx.x; j < d; j += T) {
+        const uint64_t k = keys[pos[i] + j];
+        if (j + 1 < d && keys[pos[i] + j + 1] == k) atomicOr(err, ERR_UNSORTED_OR_DUP);
+        nb[b0 + j] = (uint32_t)(k & mask);
+    }
+}
+
+__global__ void k_big_bounds(const uint32_t* __restrict__ off, const uint32_t* __restrict__ big,
+                             uint32_t nbig, uint32_t* __restrict__ len) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nbig; i += gridDim.x * T)
+        len[i] = off[big[i] + 1] - off[big[i]];
 }
 
 // memcpy split over a few host threads when large (page-locked staging).
@@ -341,20 +502,49 @@ rtrex_status build_from_device_edges(rtrex_graph h, const uint32_t* du, const ui
     GraphDev& g = h->g;
     hipStream_t s = h->stream;
     Scratch sc;
-    uint32_t* src;
-    uint64_t *keys, *tmp;
+    uint32_t *src, *cur, *big;
     RT_TRY(sc.get(&src, nnz));
-    RT_TRY(sc.get(&keys, nnz));
-    RT_TRY(sc.get(&tmp, nnz));
-    const int B = bits_for((uint64_t)n + 1);
-    const unsigned bs = blocks_for(std::max<uint64_t>(nnz, n), T, 65536);
-    k_make_keys<<<bs, T, 0, s>>>(du, dv, m, g.n, B, keys, &h->ctr->error);
-    RT_HIP(hipGetLastError());
-    RT_TRY(radix_sort_u64(keys, tmp, nnz, 2 * B, s));
+    RT_TRY(sc.get(&cur, (uint64_t)g.n + 2));
+    RT_TRY(sc.get(&big, (uint64_t)g.n + 1));
+    unsigned* nbig = cur + g.n;  // one counter after the fill counts
+    const unsigned bm = blocks_for(std::max<uint64_t>(m, 1), T, 65536);
+    unsigned* err = &h->ctr->error;
     RT_HIP(hipMemsetAsync(g.deg, 0, (uint64_t)g.n * 4, s));
-    k_split_keys<<<bs, T, 0, s>>>(keys, nnz, B, g.nb, src, g.deg, &h->ctr->error);
+    RT_HIP(hipMemsetAsync(cur, 0, ((uint64_t)g.n + 1) * 4, s));
+    if (m) k_edge_degrees<<<bm, T, 0, s>>>(du, dv, m, g.n, g.deg, err);
     RT_HIP(hipGetLastError());
     RT_TRY(exclusive_scan_u32_32(g.deg, g.off, g.n, s));
+    if (m) {
+        k_edge_fill<<<bm, T, 0, s>>>(du, dv, m, g.n, g.off, cur, g.nb, src);
+        k_sort_short<<<blocks_for(g.n, T, 8192), T, 0, s>>>(g.off, g.n, g.nb, err);
+        k_sort_long<<<blocks_for(g.n, SL_W * WAVE, 4096), SL_W * WAVE, 0, s>>>(g.off, g.n, g.nb,
+                                                                             big, nbig, err);
+        RT_HIP(hipGetLastError());
+        uint32_t nb_host = 0;
+        RT_HIP(hipMemcpyAsync(&nb_host, nbig, 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        if (nb_host) {  // lists longer than SL_LDS: one radix sort over all of them
+            uint32_t* len;
+            uint64_t* pos;
+            RT_TRY(sc.get(&len, nb_host));
+            RT_TRY(sc.get(&pos, (uint64_t)nb_host + 1));
+            k_big_bounds<<<blocks_for(nb_host, T, 1024), T, 0, s>>>(g.off, big, nb_host, len);
+            RT_HIP(hipGetLastError());
+            RT_TRY(exclusive_scan_u32(len, pos, nb_host, s));
+            uint64_t total = 0;
+            RT_HIP(hipMemcpyAsync(&total, pos + nb_host, 8, hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            uint64_t *keys, *tmp;
+            RT_TRY(sc.get(&keys, total));
+            RT_TRY(sc.get(&tmp, total));
+            const int B = bits_for((uint64_t)n + 1);
+            k_big_keys<<<nb_host, T, 0, s>>>(g.off, big, pos, B, g.nb, keys);
+            RT_HIP(hipGetLastError());
+            RT_TRY(radix_sort_u64(keys, tmp, total, bits_for(nb_host) + B, s));
+            k_big_back<<<nb_host, T, 0, s>>>(g.off, big, pos, B, keys, g.nb, err);
+            RT_HIP(hipGetLastError());
+        }
+    }
     return finish(h, src);  // ends with the stream drained
 }
 
