@@ -334,27 +334,6 @@ __global__ void k_big_bounds(const uint32_t* __restrict__ off, const uint32_t* _
         len[i] = off[big[i] + 1] - off[big[i]];
 }
 
-// memcpy split over a few host threads when large (page-locked staging).
-void parallel_copy(void* dst, const void* src, uint64_t bytes) {
-    const uint64_t chunk = 4ull << 20;
-    const unsigned nt = (unsigned)std::min<uint64_t>(
-        std::max(1u, std::min(8u, std::thread::hardware_concurrency())), bytes / chunk + 1);
-    if (nt <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    std::vector<std::thread> th;
-    const uint64_t per = (bytes / nt + 63) & ~63ull;
-    for (unsigned t = 0; t < nt; ++t) {
-        const uint64_t a = std::min<uint64_t>(bytes, t * per), b = std::min<uint64_t>(bytes, a + per);
-        if (a < b)
-            th.emplace_back([=] {
-                std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
-            });
-    }
-    for (auto& x : th) x.join();
-}
-
 int bits_for(uint64_t n) {
     int b = 1;
     while ((1ull << b) < n) ++b;
@@ -569,11 +548,39 @@ rtrex_status build_from_edges(rtrex_graph h, const uint32_t* u,
         const uint64_t bytes = (uint64_t)m * 4;
         if (!h->up_pin) h->up_pin = pin_alloc(2 * bytes);
         if (h->up_pin) {
+            // Slices of both arrays, one per host thread; the caller copies
+            // slice 0 and then, in slice order, waits for each thread and
+            // queues its slice's two transfers, so the DMA of the first
+            // slices overlaps the copies of the later ones.
             uint32_t* su = static_cast<uint32_t*>(h->up_pin);
-            parallel_copy(su, u, bytes);
-            parallel_copy(su + m, v, bytes);
-            RT_HIP(hipMemcpyAsync(du, su, bytes, hipMemcpyHostToDevice, s));
-            RT_HIP(hipMemcpyAsync(dv, su + m, bytes, hipMemcpyHostToDevice, s));
+            uint32_t* sv = su + m;
+            const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            const uint64_t ns = std::min<uint64_t>(hw, (uint64_t)m / (256u << 10) + 1);
+            const uint64_t per = ((uint64_t)m + ns - 1) / ns;
+            auto slice = [&](uint64_t t) {
+                const uint64_t a = std::min<uint64_t>(m, t * per), b = std::min<uint64_t>(m, a + per);
+                if (a < b) {
+                    std::memcpy(su + a, u + a, (b - a) * 4);
+                    std::memcpy(sv + a, v + a, (b - a) * 4);
+                }
+            };
+            auto queue = [&](uint64_t t) -> hipError_t {
+                const uint64_t a = std::min<uint64_t>(m, t * per), b = std::min<uint64_t>(m, a + per);
+                if (a >= b) return hipSuccess;
+                hipError_t e = hipMemcpyAsync(du + a, su + a, (b - a) * 4, hipMemcpyHostToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(dv + a, sv + a, (b - a) * 4, hipMemcpyHostToDevice, s);
+                return e;
+            };
+            std::vector<std::thread> th;
+            for (uint64_t t = 1; t < ns; ++t) th.emplace_back(slice, t);
+            slice(0);
+            hipError_t e = queue(0);
+            for (uint64_t t = 1; t < ns; ++t) {
+                th[t - 1].join();
+                if (e == hipSuccess) e = queue(t);
+            }
+            RT_HIP(e);
         } else {
             RT_HIP(hipMemcpyAsync(du, u, bytes, hipMemcpyHostToDevice, s));
             RT_HIP(hipMemcpyAsync(dv, v, bytes, hipMemcpyHostToDevice, s));
