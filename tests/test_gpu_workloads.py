@@ -179,7 +179,7 @@ def test_config5_trace_served_equals_alone_and_reference():
     for tag, ref in sorted(SERVE.items()):
         n, u, v = pool[ref["kind"]][ref["index"]]
         got = _full_hashes(n, u, v, ref["eps"], tag)
-        for k in KEYS:
+        for k in COUNT_CLEAN_KEYS + EXTRACT_KEYS:
             assert got[k] == ref[k], (tag, k)
         cm0, _ = alone.get((ref["kind"], ref["index"]), (None, None))
         if cm0 is not None:
