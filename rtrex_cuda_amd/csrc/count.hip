@@ -602,205 +602,304 @@ __global__ void __launch_bounds__(HW) k_count_hub_w(GraphDev g, const uint4* __r
     }
 }
 
-// Small hubs (d_hub <= 64) with a term buffer: one wave per unit, the
-// unit's owned edges' shorter lists streamed as ONE sequence, 128 elements per
-// step (two per lane, two loads in flight), whatever edge each belongs to
-// (edge k covers elements [L_k, L_k + d_k) of the sequence, L the exclusive
-// prefix sum of the list lengths d_k).  A lane finds its edge from the edge
-// starts that fall in the step: each start sets a per-step tag at its
-// position in LDS, one ballot per 64-element part gathers them, and the lane's
-// edge is the previous step's last edge plus the starts at or below it.
+// Small hubs (d_hub <= 64) with a term buffer, batched: a wave takes a chunk
+// of consecutive units (one hub each) and splits it into batches of whole
+// units with at most 64 owned edges and 256 hub-list slots (each hub's list
+// padded to a multiple of 8).  A batch stages all its hub lists in LDS at
+// once and streams ALL its owned edges' shorter lists as one sequence, so the
+// per-unit work of one hub per wave pass (staging, prefix scans, a partial
+// batch of matches, a step half empty at the end of the hub) is paid once per
+// batch, about six hubs.  Edge k of the batch covers elements [L_k, L_k + d_k)
+// of the sequence; a lane finds its edge from the edge starts in the step
+// (per-step tags in LDS, one ballot per 64-element part).
 //
-// The scan is split from the work a match needs.  Streaming reads only the
-// 4-byte ids of the lists; a match (the id found in the hub's list) is
-// appended to the wave's ring of pending matches in LDS -- its slot in the
-// streamed list, its position in the hub's list and its edge -- in stream
-// order, so every edge's matches are in ascending z and contiguous.  Each time
-// 64 matches are pending they are handled at once, one per lane: the edge id
-// of {lo, z} (gathered by slot), the FP64 weight, the 16-byte term at the
-// edge's term offset plus the edge's earlier matches, and the support fold:
-// the weights go to LDS in ring order and the last lane of every edge's run
-// adds the run's weights, one after another, to the edge's running sum
-// (the reference's ascending-z order).  So the divisions, the term stores and
-// the folds run with every lane busy instead of once per probed element
-// step, and no edge id is read for the ids that do not match.  The next
-// unit's record and hub staging loads are issued before the current unit's
-// steps (unit records two units ahead), so a unit's setup does not wait on
-// memory.
-constexpr int FW = 256;
-constexpr int FH = 2;        // 64-element parts per step
-constexpr uint32_t FQ = 128;  // pending-match ring per wave: < 64 left + 64 appended
-struct FlatStage {
-    uint32_t z, e, d, o, zl, zh, oi;  // hub slot lane: id, edge id, degree, offset, term offset; own_idx
-};
-__device__ __forceinline__ FlatStage flat_stage(const GraphDev& g, const TermBuf& tb,
-                                                const uint32_t* __restrict__ own_idx,
-                                                const uint4& U, bool valid, int lane) {
-    FlatStage r{PAD_ID, 0u, 0u, 0u, 0u, 0u, 0u};
-    if (!valid) return r;
-    const uint32_t oh = U.y, k0 = U.z, dh = U.w & 0xffffu, cnt = U.w >> 16;
-    if ((uint32_t)lane < dh) {
-        const uint2 a = g.nbe[oh + lane];
-        const uint2 b = g.ndo[oh + lane];
-        const uint64_t zo = tb.zsl[oh + lane];
-        r.z = a.x;
-        r.e = a.y;
-        r.d = b.x;
-        r.o = b.y;
-        r.zl = (uint32_t)zo;
-        r.zh = (uint32_t)(zo >> 32);
+// Every lane probes its own edge's hub: the hub's list is an 8-ary tree whose
+// eight top pivots are kept per hub in LDS, indexed by the hub's first block
+// (the last entries of the hub's full blocks; UINT32_MAX from the hub's last
+// block on, so a probe never leaves the hub's blocks), and the leaf is the
+// hub's list itself.  The step
+// after the current one has its edge lookup done and its (id, edge id) pairs
+// loaded before the current step is probed, so the loads of one step are in
+// flight while the previous one is worked on.
+//
+// Matches go to the wave's ring in LDS in stream order -- the edge id of
+// {lo, z} from the streamed pair, the hub slot of z, the owned edge -- and are
+// handled 64 at a time, one per lane: the weight 1/(d_lo d_hub d_z) (the
+// product d_lo d_hub < 2^12 times d_z is exact in FP64, so the division sees
+// the same value as the reference's integer product), the 16-byte term at the
+// edge's term offset plus its earlier matches, and the support fold: the
+// weights go to LDS in ring order and the last lane of every edge's run adds
+// the run, one weight after another, to the edge's running sum (ascending z,
+// the reference's order).  A batch's last matches are handled when it ends.
+constexpr int SW = 64;
+constexpr uint32_t S_HS = 256;   // hub-list slots per batch
+constexpr uint32_t S_OE = 64;    // owned edges per batch
+constexpr uint32_t S_Q = 128;    // pending-match ring per wave: < 64 left + 64 appended
+constexpr uint32_t S_WORK = 1024; // streamed elements per chunk (wave task), about
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(x, (unsigned)d, 64);
+        if (lane >= d) x += y;
     }
-    if ((uint32_t)lane < cnt) r.oi = own_idx[k0 + lane];
-    return r;
+    return x;
 }
 
-__global__ void __launch_bounds__(FW) k_count_hub_flat(GraphDev g, const uint4* __restrict__ units,
-                                                       uint32_t nunits,
-                                                       const uint32_t* __restrict__ own_idx,
-                                                       TermBuf tb) {
-    constexpr int NW = FW / WAVE;
-    __shared__ __attribute__((aligned(16))) uint32_t s_nb[NW][WAVE];
-    __shared__ uint32_t s_ed[NW][WAVE], s_dg[NW][WAVE];
-    __shared__ uint4 s_ek[NW][WAVE];     // per owned edge k: (L_k, list offset - L_k, d_k, 0)
-    __shared__ uint64_t s_zb[NW][WAVE];  // term offset of edge k
-    __shared__ uint32_t s_cnt[NW][WAVE];  // matches so far
-    __shared__ double s_S[NW][WAVE];      // support so far
+// The leaf block of z in a hub list of at most 64 entries, given its top
+// pivots pa = p0..p3, pb = p4..p7 (p7 and every pivot from the last block on
+// are UINT32_MAX): the block is the count of p0..p6 below z, by three
+// selects.  True and pos = index of z when z is in the list.
+__device__ __forceinline__ bool probe64(const uint32_t* leaf, const uint4& pa, const uint4& pb,
+                                        uint32_t z, uint32_t& pos) {
+    const bool c2 = pa.w < z;
+    const bool c1 = (c2 ? pb.y : pa.y) < z;
+    const uint32_t q1 = c2 ? pb.z : pa.z;
+    const uint32_t q0 = c2 ? pb.x : pa.x;
+    const bool c0 = (c1 ? q1 : q0) < z;
+    const uint32_t b = (c2 ? 4u : 0u) | (c1 ? 2u : 0u) | (c0 ? 1u : 0u);
+    const uint4* q = reinterpret_cast<const uint4*>(leaf) + 2 * b;
+    const uint4 a = q[0], c = q[1];
+    const uint32_t mk = (uint32_t)(a.x == z) | (uint32_t)(a.y == z) << 1 |
+                        (uint32_t)(a.z == z) << 2 | (uint32_t)(a.w == z) << 3 |
+                        (uint32_t)(c.x == z) << 4 | (uint32_t)(c.y == z) << 5 |
+                        (uint32_t)(c.z == z) << 6 | (uint32_t)(c.w == z) << 7;
+    pos = 8 * b + (mk ? (uint32_t)__ffs(mk) - 1u : 0u);
+    return mk != 0;
+}
+
+__global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* __restrict__ units,
+                                                    const uint32_t* __restrict__ chunk,
+                                                    uint32_t nchunks,
+                                                    const uint32_t* __restrict__ own_idx,
+                                                    TermBuf tb) {
+    constexpr int NW = SW / WAVE;
+    __shared__ __attribute__((aligned(16))) uint32_t s_nb[NW][S_HS];  // hub lists, padded
+    __shared__ uint2 s_hd[NW][S_HS];      // hub slot: (edge id of {hub, z}, d_z)
+    __shared__ uint2 s_ek[NW][S_OE];      // owned edge k: (list offset - L_k, hub base slot)
+    __shared__ __attribute__((aligned(16))) uint4 s_pv[NW][S_HS / 8][2];  // top pivots per hub
+    __shared__ uint32_t s_D[NW][S_OE];    // d_lo * d_hub
+    __shared__ uint64_t s_zb[NW][S_OE];   // term offset of edge k
+    __shared__ uint32_t s_cnt[NW][S_OE];  // matches so far
+    __shared__ double s_S[NW][S_OE];      // support so far
     __shared__ double s_t[NW][WAVE];      // a batch's match weights, in ring order
-    __shared__ uint32_t s_flag[NW][FH * WAVE];
-    __shared__ uint2 s_q[NW][FQ];         // pending matches: (slot, hub position | edge << 8)
+    // Tagged marks: a batch's starts (owned edges at [0, 64), hub blocks at
+    // [64, 96)), then each step's edge starts (positions in the step); one
+    // tag per use, so a mark counts only in the use that wrote it.
+    __shared__ uint32_t s_mark[NW][2 * WAVE];
+    __shared__ uint2 s_q[NW][S_Q];        // pending matches: (edge id of {lo, z}, slot | k << 8)
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     const uint64_t upto = below | (1ull << lane);
-    uint32_t tag = 0;
-#pragma unroll
-    for (int q = 0; q < FH; ++q) s_flag[wid][q * WAVE + lane] = 0;  // tags start at 1
-    const uint32_t stride = gridDim.x * NW;
-    uint32_t un = blockIdx.x * NW + wid;
-    uint4 Ucur = un < nunits ? units[un] : make_uint4(0u, 0u, 0u, 0u);
-    uint4 Unext = un + stride < nunits ? units[un + stride] : make_uint4(0u, 0u, 0u, 0u);
-    FlatStage cur = flat_stage(g, tb, own_idx, Ucur, un < nunits, lane);
-    for (; un < nunits; un += stride) {
-        const uint32_t dh = Ucur.w & 0xffffu, cnt = Ucur.w >> 16;
-        if ((uint32_t)lane < dh) {
-            s_ed[wid][lane] = cur.e;
-            s_dg[wid][lane] = cur.d;
-        }
-        s_nb[wid][lane] = cur.z;
-        Piv8 top;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) top.p[i] = __builtin_amdgcn_readlane(cur.z, 8 * i + 7);
-        // Owned edge k = lane: the data of its hub slot li, held by lane li.
-        const bool own = (uint32_t)lane < cnt;
-        const int li = own ? (int)cur.oi : 0;
-        const uint32_t vd = __shfl(cur.d, li, 64), vo = __shfl(cur.o, li, 64);
-        const uint32_t ve = __shfl(cur.e, li, 64);
-        const uint32_t zl = __shfl(cur.zl, li, 64), zh = __shfl(cur.zh, li, 64);
-        const uint32_t dk = own ? vd : 0u;
-        uint32_t incl = dk;
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const uint32_t y = __shfl_up(incl, (unsigned)d, 64);
-            if (lane >= d) incl += y;
-        }
-        const uint32_t Lk = incl - dk;
-        const uint32_t W = __builtin_amdgcn_readlane(incl, WAVE - 1);
-        if (own) {
-            s_ek[wid][lane] = make_uint4(Lk, vo - Lk, dk, 0u);
-            s_zb[wid][lane] = ((uint64_t)zh << 32) | zl;
-            s_cnt[wid][lane] = 0;
-            s_S[wid][lane] = 0.0;
-        }
-        // Next unit's staging loads now; the record after it as well.
-        const uint32_t un1 = un + stride, un2 = un + 2 * stride;
-        cur = flat_stage(g, tb, own_idx, Unext, un1 < nunits, lane);
-        Ucur = Unext;
-        Unext = un2 < nunits ? units[un2] : make_uint4(0u, 0u, 0u, 0u);
-        wave_sync();
-        // The pending matches [qh, qt) of the ring (nh <= 64 of them from qh
-        // on) handled one per lane, the support folded in ring order.
-        auto handle = [&](uint32_t qh, uint32_t nh) {
-            const bool act = (uint32_t)lane < nh;
-            const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (FQ - 1)];
-            const uint32_t pos = q.y & 0xffu, k = act ? q.y >> 8 : 0u;
-            const uint32_t elo = act ? g.s2e[q.x] : 0u;  // edge {lo, z}
-            const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
-            const bool start = act && (lane == 0 || kup != k);
-            const uint64_t sm = ballot64(start) | 1ull;
-            const int r0 = 63 - __clzll((long long)(sm & upto));  // first lane of the run
-            const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
-            uint32_t c0 = 0;
-            if (act) {
-                c0 = s_cnt[wid][k];
-                const double t = tri_w(dh, s_ek[wid][k].z, s_dg[wid][pos]);
-                tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, elo, s_ed[wid][pos]};
-                s_t[wid][lane] = t;
-            }
-            wave_sync();
-            if (last) {
-                double S = s_S[wid][k];
-                int i = r0;
-                for (; i + 3 <= lane; i += 4) {
-                    const double a0 = s_t[wid][i], a1 = s_t[wid][i + 1];
-                    const double a2 = s_t[wid][i + 2], a3 = s_t[wid][i + 3];
-                    S += a0;
-                    S += a1;
-                    S += a2;
-                    S += a3;
-                }
-                for (; i <= lane; ++i) S += s_t[wid][i];
-                s_S[wid][k] = S;
-                s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
-            }
-            wave_sync();
-        };
-        uint32_t qh = 0, qt = 0;
-        int kprev = -1;
-        for (uint32_t base = 0; base < W; base += FH * WAVE) {
+    uint32_t tag = 0;  // the marks start at 0
+    s_mark[wid][lane] = 0;
+    s_mark[wid][WAVE + lane] = 0;
+    // Chunks of about equal work (the plan's chunk offsets), round robin.
+    for (uint32_t ch = blockIdx.x * NW + wid; ch < nchunks; ch += gridDim.x * NW) {
+        const uint32_t u0 = chunk[ch], nu = chunk[ch + 1] - u0;
+        for (uint32_t s = 0; s < nu;) {
+            // Batch: units [s, s + nb), the longest run from s within both
+            // caps; lane j holds unit s + j.  (Wave-uniform values are read
+            // with readlane so that they live in scalar registers and the
+            // loops below stay uniform.)
+            const uint4 U = (uint32_t)lane < nu - s ? units[u0 + s + lane]
+                                                    : make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t udh = U.w & 0xffffu, ucnt = U.w >> 16, upad = (udh + 7u) & ~7u;
+            const uint32_t pc = wave_incl_scan(ucnt, lane), ph = wave_incl_scan(upad, lane);
+            const bool fit = (uint32_t)lane < nu - s && pc <= S_OE && ph <= S_HS;
+            const uint32_t nb = (uint32_t)__popcll(ballot64(fit));
+            const uint32_t noe = __builtin_amdgcn_readlane(pc, (int)nb - 1);
+            const uint32_t nhs = __builtin_amdgcn_readlane(ph, (int)nb - 1);
+            const bool inb = (uint32_t)lane < nb;
+            const uint32_t ucb = pc - ucnt, uhb = ph - upad;  // unit's first edge, slot
             ++tag;
-            if (own && Lk >= base && Lk - base < (uint32_t)(FH * WAVE))
-                s_flag[wid][Lk - base] = tag;
-            wave_sync();
-            int kq[FH];
-            bool aq[FH];
-            uint32_t sq[FH], zq[FH];
-            int kb = kprev;  // edge before this part
-#pragma unroll
-            for (int q = 0; q < FH; ++q) {
-                const bool st = s_flag[wid][q * WAVE + lane] == tag;
-                const uint64_t sm = ballot64(st);
-                kq[q] = kb + (int)__popcll(sm & below) + (st ? 1 : 0);
-                kb += (int)__popcll(sm);
-                aq[q] = base + q * WAVE + lane < W;  // then 0 <= k < cnt
-                sq[q] = base + q * WAVE + lane + s_ek[wid][aq[q] ? kq[q] : 0].y;
+            if (inb) {
+                s_mark[wid][ucb] = tag;
+                s_mark[wid][WAVE + (uhb >> 3)] = tag;
             }
-            // all of the step's list loads in flight at once
+            wave_sync();
+            const uint64_t kst = ballot64(s_mark[wid][lane] == tag);
+            const uint64_t hst = ballot64(lane < 32 && s_mark[wid][WAVE + lane] == tag);
+            // Hub slots, four rounds of 64: list entry, its edge id and degree.
+            uint2 hA[4];
+            uint32_t hD[4];
 #pragma unroll
-            for (int q = 0; q < FH; ++q) zq[q] = aq[q] ? g.nb[sq[q]] : PAD_ID;
-#pragma unroll
-            for (int hf = 0; hf < FH; ++hf) {
-                uint32_t pos = 0;
-                const bool hit =
-                    aq[hf] && tree_probe<1>(s_nb[wid], nullptr, nullptr, top, zq[hf], pos);
-                const uint64_t hm = ballot64(hit);
-                if (hit)
-                    s_q[wid][(qt + (uint32_t)__popcll(hm & below)) & (FQ - 1)] =
-                        make_uint2(sq[hf], pos | ((uint32_t)kq[hf] << 8));
-                qt += (uint32_t)__popcll(hm);
-                if (qt - qh >= (uint32_t)WAVE) {
-                    wave_sync();
-                    handle(qh, WAVE);
-                    qh += WAVE;
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = (uint32_t)lane + 64u * r;
+                hA[r] = make_uint2(PAD_ID, 0u);
+                hD[r] = 0u;
+                if (64u * r < nhs) {
+                    const uint32_t blk = min(i >> 3, 31u);
+                    const int j = __popcll(hst & ((2ull << blk) - 1ull)) - 1;
+                    const uint32_t oh = __shfl(U.y, j, 64), dh = __shfl(udh, j, 64);
+                    const uint32_t t = i - __shfl(uhb, j, 64);
+                    if (i < nhs && t < dh) {
+                        hA[r] = g.nbe[oh + t];
+                        hD[r] = g.ndo[oh + t].x;
+                    }
                 }
             }
-            kprev = __builtin_amdgcn_readlane(kq[FH - 1], WAVE - 1);
-        }
-        if (qt != qh) {
+            // Owned edge k = lane: its unit, hub-side slot and list data.
+            const bool ko = (uint32_t)lane < noe;
+            const int jk = __popcll(kst & upto) - 1;
+            const uint32_t k0 = __shfl(U.z, jk, 64), cb = __shfl(ucb, jk, 64);
+            const uint32_t koh = __shfl(U.y, jk, 64), kdh = __shfl(udh, jk, 64);
+            const uint32_t khb = __shfl(uhb, jk, 64), kpad = __shfl(upad, jk, 64);
+            const uint32_t li = ko ? own_idx[k0 + (uint32_t)lane - cb] : 0u;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = (uint32_t)lane + 64u * r;
+                if (64u * r < nhs) {
+                    s_nb[wid][i] = hA[r].x;
+                    s_hd[wid][i] = make_uint2(hA[r].y, hD[r]);
+                }
+            }
+            // (loaded by every lane, slot 0 for lanes past the batch's edges,
+            // so no load is left in flight on a path the compiler cannot
+            // see through)
+            const uint32_t olo = g.ndo[ko ? koh + li : 0u].y;
+            const uint64_t zo = tb.zsl[ko ? koh + li : 0u];
             wave_sync();
-            handle(qh, qt - qh);
+            uint32_t dlo = 0, ve = 0;
+            if (ko) {
+                const uint2 hd = s_hd[wid][khb + li];  // slot li of the hub: lo
+                ve = hd.x;
+                dlo = hd.y;
+            }
+            if (inb) {  // the unit's hub pivots
+                const uint32_t nblk = upad >> 3;
+                uint32_t p[8];
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    p[b] = (uint32_t)b + 1u < nblk ? s_nb[wid][uhb + 8u * b + 7u] : PAD_ID;
+                s_pv[wid][uhb >> 3][0] = make_uint4(p[0], p[1], p[2], p[3]);
+                s_pv[wid][uhb >> 3][1] = make_uint4(p[4], p[5], p[6], p[7]);
+            }
+            const uint32_t incl = wave_incl_scan(dlo, lane);
+            const uint32_t Lk = incl - dlo;
+            const uint32_t W = __builtin_amdgcn_readlane(incl, WAVE - 1);
+            if (ko) {
+                s_ek[wid][lane] = make_uint2(olo - Lk, khb);
+                s_D[wid][lane] = dlo * kdh;
+                s_zb[wid][lane] = zo;
+                s_cnt[wid][lane] = 0;
+                s_S[wid][lane] = 0.0;
+            }
+            wave_sync();
+
+            // The pending matches [qh, qh + nh) of the ring handled one per
+            // lane, the support folded in ring order.
+            auto handle = [&](uint32_t qh, uint32_t nh) {
+                const bool act = (uint32_t)lane < nh;
+                const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (S_Q - 1)];
+                const uint32_t hs = q.y & 0xffu, k = act ? q.y >> 8 : 0u;
+                const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
+                const bool start = act && (lane == 0 || kup != k);
+                const uint64_t sm = ballot64(start) | 1ull;
+                const int r0 = 63 - __clzll((long long)(sm & upto));  // first lane of the run
+                const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
+                uint32_t c0 = 0;
+                if (act) {
+                    c0 = s_cnt[wid][k];
+                    const uint2 hd = s_hd[wid][hs];
+                    const double t = 1.0 / ((double)s_D[wid][k] * (double)hd.y);
+                    tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, hd.x};
+                    s_t[wid][lane] = t;
+                }
+                wave_sync();
+                if (last) {
+                    double S = s_S[wid][k];
+                    int i = r0;
+                    for (; i + 3 <= lane; i += 4) {
+                        double a[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) a[j] = s_t[wid][i + j];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) S += a[j];
+                    }
+                    for (; i <= lane; ++i) S += s_t[wid][i];
+                    s_S[wid][k] = S;
+                    s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
+                }
+                wave_sync();
+            };
+            // Edge lookup and loads of the step at base: the step's edge
+            // starts tagged in LDS, then per part the lane's edge (kb: the
+            // edge of the element before the step; -1 past the end) and its
+            // list's (id, edge id) pair.  The loads are issued by every lane
+            // (slot 0 past the end; that pair is never used), so the wait
+            // before a step's probe leaves the later steps' loads in flight.
+            int kb = -1;
+            struct Step {
+                int k[2];
+                uint2 z[2];
+            };
+            auto prep = [&](uint32_t base, Step& S) {
+                ++tag;
+                if (ko && Lk >= base && Lk - base < 2u * WAVE) s_mark[wid][Lk - base] = tag;
+                wave_sync();
+                uint32_t sq[2];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const bool st = s_mark[wid][q * WAVE + lane] == tag;
+                    const uint64_t m = ballot64(st);
+                    const int k = kb + (int)__popcll(m & below) + (st ? 1 : 0);
+                    kb += (int)__popcll(m);
+                    const bool aq = base + q * WAVE + (uint32_t)lane < W;  // then 0 <= k < noe
+                    sq[q] = aq ? base + q * WAVE + (uint32_t)lane + s_ek[wid][k].x : 0u;
+                    S.k[q] = aq ? k : -1;
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) S.z[q] = g.nbe[sq[q]];
+            };
+            uint32_t qh = 0, qt = 0;
+            auto probe = [&](const Step& S) {
+#pragma unroll
+                for (int hf = 0; hf < 2; ++hf) {
+                    // Branch-free: every lane probes (a lane past the end
+                    // probes edge 0's hub and is masked), so the step's
+                    // loads are always waited for here.
+                    const bool act = S.k[hf] >= 0;
+                    const uint32_t hb = s_ek[wid][act ? S.k[hf] : 0].y;
+                    const uint4 pa = s_pv[wid][hb >> 3][0], pb = s_pv[wid][hb >> 3][1];
+                    uint32_t pos;
+                    const bool hit = probe64(s_nb[wid] + hb, pa, pb, S.z[hf].x, pos) & act;
+                    const uint64_t hm = ballot64(hit);
+                    if (hit)
+                        s_q[wid][(qt + (uint32_t)__popcll(hm & below)) & (S_Q - 1)] =
+                            make_uint2(S.z[hf].y, (hb + pos) | ((uint32_t)S.k[hf] << 8));
+                    qt += (uint32_t)__popcll(hm);
+                    if (qt - qh >= (uint32_t)WAVE) {
+                        wave_sync();
+                        handle(qh, WAVE);
+                        qh += WAVE;
+                    }
+                }
+            };
+            // Two step buffers used in turn (one step's loads in flight
+            // while the other is probed; no register copies of loads in
+            // flight).
+            Step A, B;
+            prep(0, A);
+            for (uint32_t base = 0;;) {
+                prep(base + 2 * WAVE, B);
+                probe(A);
+                base += 2 * WAVE;
+                if (base >= W) break;
+                prep(base + 2 * WAVE, A);
+                probe(B);
+                base += 2 * WAVE;
+                if (base >= W) break;
+            }
+            if (qt != qh) {
+                wave_sync();
+                handle(qh, qt - qh);
+            }
+            if (ko) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
+            wave_sync();  // the next batch restages the slices
+            s += nb;
         }
-        if (own) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
-        wave_sync();  // the next unit restages the slices
     }
 }
 
@@ -1278,10 +1377,10 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         uint64_t nu = 0;
         for (int c = 0; c < HUB_NG; ++c) nu += h->hub_ucnt[c];
         if (nu) {
-            const uint32_t blocks =
-                (uint32_t)std::min<uint64_t>((nu + FW / WAVE - 1) / (FW / WAVE), 16384);
-            k_count_hub_flat<<<blocks, FW, 0, h->stream>>>(h->g, h->hub_units, (uint32_t)nu,
-                                                           h->own_idx, tb);
+            const uint32_t tasks = h->small_nchunks;
+            const uint32_t blocks = std::min<uint32_t>((tasks + SW / WAVE - 1) / (SW / WAVE), 65536);
+            k_count_small<<<blocks, SW, 0, h->stream>>>(h->g, h->hub_units, h->small_chunk, tasks,
+                                                        h->own_idx, tb);
         }
         at = nu;
         c0 = HUB_NG;
@@ -1312,6 +1411,34 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
             }
         }
         at += nu;
+    }
+}
+
+// Streamed work of every small-hub unit: the sum of its owned edges' shorter
+// lists (the degrees of the hub's neighbours at the owned slots).
+__global__ void __launch_bounds__(T) k_small_work(GraphDev g, const uint4* __restrict__ units,
+                                                  uint32_t nu, const uint32_t* __restrict__ own_idx,
+                                                  uint32_t* __restrict__ w) {
+    for (uint32_t u = blockIdx.x * T + threadIdx.x; u < nu; u += gridDim.x * T) {
+        const uint4 U = units[u];
+        const uint32_t cnt = U.w >> 16;
+        uint32_t sum = 0;
+        for (uint32_t i = 0; i < cnt; ++i) sum += g.ndo[U.y + own_idx[U.z + i]].x;
+        w[u] = sum;
+    }
+}
+
+// Chunk c takes the units that start in the work window [c wt, (c + 1) wt)
+// (P: exclusive prefix sums of the unit work): chunk[c] = the first unit with
+// P >= c wt; chunk[nchunks] = nu.  A window no unit starts in is empty.
+__global__ void __launch_bounds__(T) k_small_chunks(const uint64_t* __restrict__ P, uint32_t nu,
+                                                    uint32_t wt, uint32_t nchunks,
+                                                    uint32_t* __restrict__ chunk) {
+    for (uint32_t u = blockIdx.x * T + threadIdx.x; u < nu; u += gridDim.x * T) {
+        const uint64_t c0 = u ? P[u - 1] / wt + 1 : 0, c1 = P[u] / wt;
+        for (uint64_t c = c0; c <= c1; ++c) chunk[c] = u;
+        if (u == nu - 1)
+            for (uint64_t c = c1 + 1; c <= nchunks; ++c) chunk[c] = nu;
     }
 }
 
@@ -1366,6 +1493,31 @@ rtrex_status prepare_hubs(rtrex_graph h) {
         k_hub_units<false><<<bv, T, 0, s>>>(g, h->own_off, hwork, ccnt, h->hub_units, cbase, P,
                                              nullptr);
         RT_HIP(hipGetLastError());
+    }
+    // Small-hub chunks (the classes below l0_max come first in hub_units).
+    h->small_nchunks = 0;
+    uint32_t nsmall = 0;
+    for (int c = 0; c < HUB_NG; ++c) nsmall += h->hub_ucnt[c];
+    if (nsmall) {
+        uint32_t* w;
+        uint64_t* pre;
+        RT_TRY(sc.get(&w, nsmall));
+        RT_TRY(sc.get(&pre, nsmall + 1));
+        k_small_work<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(g, h->hub_units, nsmall,
+                                                                h->own_idx, w);
+        RT_TRY(exclusive_scan_u32(w, pre, nsmall, s));
+        uint64_t last = 0;
+        RT_HIP(hipMemcpyAsync(&last, pre + nsmall - 1, 8, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        const uint64_t nch = last / S_WORK + 1;  // windows up to the last unit's start
+        if (nch + 1 > h->small_chunk_cap) {
+            h->small_chunk_cap = 2 * (nch + 1);
+            RT_TRY(dalloc(h, &h->small_chunk, h->small_chunk_cap));
+        }
+        k_small_chunks<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(pre, nsmall, S_WORK,
+                                                                  (uint32_t)nch, h->small_chunk);
+        RT_HIP(hipGetLastError());
+        h->small_nchunks = (uint32_t)nch;
     }
     RT_HIP(hipStreamSynchronize(s));
     return RTREX_OK;

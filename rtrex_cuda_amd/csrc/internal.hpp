@@ -259,6 +259,11 @@ struct rtrex_graph_s {
     uint32_t* own_off = nullptr;
     uint32_t* own_idx = nullptr;
     uint4* hub_units = nullptr;
+    // Small-hub units split into chunks of about equal streamed work: chunk c
+    // is units [small_chunk[c], small_chunk[c + 1]).
+    uint32_t* small_chunk = nullptr;
+    uint64_t small_chunk_cap = 0;
+    uint32_t small_nchunks = 0;
     uint32_t hub_ucnt[20] = {};
     uint32_t hub_umax[20] = {};  // largest hub degree per class
     unsigned long long* red = nullptr;  // per-block partial sums (count phase)
