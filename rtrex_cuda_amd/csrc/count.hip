@@ -306,14 +306,17 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         if (oc) {
             const uint32_t dh = g.deg[h];
             const uint64_t work = hwork[h];
-            // Units of at least max(unit_min, d_h) elements; the large class
-            // restages its tree per unit: unit_hash_mult * d_h, but at most
-            // LARGE_UNIT owned edges apart so that hubs still spread out.
+            // Units of at least max(unit_min, d_h) elements (small hubs); the
+            // medium and large classes restage their tree per unit:
+            // max(2 d_h, 4 unit_min) (a unit feeds four waves); the huge class
+            // unit_hash_mult * d_h, but at most LARGE_UNIT owned edges apart
+            // so that hubs still spread out.
             const int lc_ = hub_lclass(dh, Pm);
-            const uint64_t uw = lc_ >= 2 ? (uint64_t)Pm.unit_hash_mult * dh
-                                         : max<uint64_t>(Pm.unit_min, dh);
+            const uint64_t uw = lc_ == 0 ? max<uint64_t>(Pm.unit_min, dh)
+                                : lc_ < 3 ? max<uint64_t>(2ull * dh, 4ull * Pm.unit_min)
+                                          : (uint64_t)Pm.unit_hash_mult * dh;
             P = (work + uw - 1) / uw;
-            if (lc_ >= 2) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
+            if (lc_ == 3) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = lc_ * HUB_NG + hub_gclass(work, oc, Pm);
             at = atomicAdd(&lc[c], (uint32_t)P);
@@ -1012,6 +1015,195 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
     }
 }
 
+// Medium and large hubs with a term buffer (degree in (l0_max, l2_max], tree
+// depth LV = 2 up to 512 entries, 3 up to 4096): one workgroup of four waves
+// per unit (a hub and part of its owned edges).  The hub's list is staged in
+// LDS once per unit as an 8-ary tree (leaf, every 8th and every 64th entry;
+// the top pivots in registers) with the degree of every slot beside it.  The
+// unit's owned edges are taken 32 at a time by the waves (an LDS counter);
+// a wave streams its edges' shorter lists as one sequence, 128 elements per
+// step with the next step's loads in flight, probes every id in the tree and
+// handles the matches 64 at a time exactly like the small-hub kernel (the
+// edge id of {hub, z} gathered from the hub's slots, L2-resident meanwhile).
+constexpr int HS_T = 256;
+constexpr uint32_t HS_E = 16;  // owned edges per wave batch
+
+template <int LV>
+__global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
+                                                           const uint4* __restrict__ units,
+                                                           uint32_t nunits,
+                                                           const uint32_t* __restrict__ own_idx,
+                                                           TermBuf tb, uint32_t lcap) {
+    constexpr uint32_t L = LV == 2 ? 512 : 4096;
+    constexpr int NW = HS_T / WAVE;
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    uint32_t* s_leaf = s_dyn;       // lcap entries: the list, padded
+    uint32_t* s_dg = s_dyn + lcap;  // lcap entries: degree of each slot's vertex
+    __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
+    __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
+    __shared__ uint32_t s_next;  // the unit's next batch
+    __shared__ uint32_t s_ek[NW][HS_E];   // owned edge k: list offset - L_k
+    __shared__ uint32_t s_D[NW][HS_E];    // d_lo * d_hub
+    __shared__ uint64_t s_zb[NW][HS_E];   // term offset
+    __shared__ uint32_t s_cnt[NW][HS_E];  // matches so far
+    __shared__ double s_S[NW][HS_E];      // support so far
+    __shared__ double s_t[NW][WAVE];
+    __shared__ uint32_t s_mark[NW][2 * WAVE];
+    __shared__ uint2 s_q[NW][S_Q];  // pending matches: (edge id of {lo, z}, hub slot | k << 16)
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t upto = below | (1ull << lane);
+    uint32_t tag = 0;
+    s_mark[wid][lane] = 0;
+    s_mark[wid][WAVE + lane] = 0;
+    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+        uint32_t h, oh, k0, dh, cnt;
+        unit_fields(units[un], h, oh, k0, dh, cnt);
+        const uint32_t nleaf = min(L, ((dh + 7) & ~7u) + 8);  // <= lcap
+        for (uint32_t i = threadIdx.x; i < nleaf; i += HS_T) {
+            uint32_t z = PAD_ID, d = 0;
+            if (i < dh) {
+                z = g.nb[oh + i];
+                d = g.ndo[oh + i].x;
+            }
+            s_leaf[i] = z;
+            s_dg[i] = d;
+        }
+        for (uint32_t i = threadIdx.x; i < L / 8; i += HS_T) s_l1[i] = hub_id(g, oh, dh, 8 * i + 7);
+        if (LV == 3)
+            for (uint32_t i = threadIdx.x; i < L / 64; i += HS_T)
+                s_l2[i] = hub_id(g, oh, dh, 64 * i + 63);
+        Piv8 top;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
+        if (threadIdx.x == 0) s_next = 0;
+        __syncthreads();
+        const TreeProbe<LV> probe_tree{s_leaf, s_l1, s_l2, top};
+        for (;;) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(&s_next, 1u);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if (b * HS_E >= cnt) break;
+            const uint32_t a0 = b * HS_E, noe = min(HS_E, cnt - a0);
+            // Owned edge k = lane of the batch: its hub slot li, the shorter
+            // list's offset and length, the term offset.
+            const bool ko = (uint32_t)lane < noe;
+            const uint32_t li = own_idx[k0 + a0 + (ko ? (uint32_t)lane : 0u)];
+            const uint32_t olo = g.ndo[oh + li].y;
+            const uint64_t zo = tb.zsl[oh + li];
+            const uint32_t ve = g.nbe[oh + li].y;
+            const uint32_t dlo = ko ? s_dg[li] : 0u;
+            const uint32_t incl = wave_incl_scan(dlo, lane);
+            const uint32_t Lk = incl - dlo;
+            const uint32_t W = __builtin_amdgcn_readlane(incl, WAVE - 1);
+            if (ko) {
+                s_ek[wid][lane] = olo - Lk;
+                s_D[wid][lane] = dlo * dh;
+                s_zb[wid][lane] = zo;
+                s_cnt[wid][lane] = 0;
+                s_S[wid][lane] = 0.0;
+            }
+            wave_sync();
+            auto handle = [&](uint32_t qh, uint32_t nh) {
+                const bool act = (uint32_t)lane < nh;
+                const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (S_Q - 1)];
+                const uint32_t hs = act ? q.y & 0xffffu : 0u, k = act ? q.y >> 16 : 0u;
+                const uint32_t ez = g.nbe[oh + hs].y;  // edge {hub, z}
+                const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
+                const bool start = act && (lane == 0 || kup != k);
+                const uint64_t sm = ballot64(start) | 1ull;
+                const int r0 = 63 - __clzll((long long)(sm & upto));
+                const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
+                uint32_t c0 = 0;
+                if (act) {
+                    c0 = s_cnt[wid][k];
+                    const double t = 1.0 / ((double)s_D[wid][k] * (double)s_dg[hs]);
+                    tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, ez};
+                    s_t[wid][lane] = t;
+                }
+                wave_sync();
+                if (last) {
+                    double S = s_S[wid][k];
+                    int i = r0;
+                    for (; i + 3 <= lane; i += 4) {
+                        double a[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) a[j] = s_t[wid][i + j];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) S += a[j];
+                    }
+                    for (; i <= lane; ++i) S += s_t[wid][i];
+                    s_S[wid][k] = S;
+                    s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
+                }
+                wave_sync();
+            };
+            int kb = -1;
+            struct Step {
+                int k[2];
+                uint2 z[2];
+            };
+            auto prep = [&](uint32_t base, Step& S) {
+                ++tag;
+                if (ko && Lk >= base && Lk - base < 2u * WAVE) s_mark[wid][Lk - base] = tag;
+                wave_sync();
+                uint32_t sq[2];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const bool st = s_mark[wid][q * WAVE + lane] == tag;
+                    const uint64_t m = ballot64(st);
+                    const int k = kb + (int)__popcll(m & below) + (st ? 1 : 0);
+                    kb += (int)__popcll(m);
+                    const bool aq = base + q * WAVE + (uint32_t)lane < W;
+                    sq[q] = aq ? base + q * WAVE + (uint32_t)lane + s_ek[wid][k] : 0u;
+                    S.k[q] = aq ? k : -1;
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) S.z[q] = g.nbe[sq[q]];
+            };
+            uint32_t qh = 0, qt = 0;
+            auto probe = [&](const Step& S) {
+#pragma unroll
+                for (int hf = 0; hf < 2; ++hf) {
+                    const bool act = S.k[hf] >= 0;
+                    uint32_t pos;
+                    const bool hit = probe_tree(S.z[hf].x, pos) & act;
+                    const uint64_t hm = ballot64(hit);
+                    if (hit)
+                        s_q[wid][(qt + (uint32_t)__popcll(hm & below)) & (S_Q - 1)] =
+                            make_uint2(S.z[hf].y, pos | ((uint32_t)S.k[hf] << 16));
+                    qt += (uint32_t)__popcll(hm);
+                    if (qt - qh >= (uint32_t)WAVE) {
+                        wave_sync();
+                        handle(qh, WAVE);
+                        qh += WAVE;
+                    }
+                }
+            };
+            Step A, B;
+            prep(0, A);
+            for (uint32_t base = 0;;) {
+                prep(base + 2 * WAVE, B);
+                probe(A);
+                base += 2 * WAVE;
+                if (base >= W) break;
+                prep(base + 2 * WAVE, A);
+                probe(B);
+                base += 2 * WAVE;
+                if (base >= W) break;
+            }
+            if (qt != qh) {
+                wave_sync();
+                handle(qh, qt - qh);
+            }
+            if (ko) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
+            wave_sync();
+        }
+        __syncthreads();  // the next unit restages the hub
+    }
+}
+
 // Huge hubs (degree in (l2_max, HUB_MAX_LIMIT]): a four-level 8-ary tree whose
 // upper levels (every 8th, 64th and 512th entry; 4096 + 512 + 64 ids, 18 KB)
 // are staged in LDS and whose leaf is the hub's list itself in HBM: a probe
@@ -1369,7 +1561,8 @@ void launch_hub_l(int gc, const GraphDev& g, const uint4* units, uint32_t nu,
 }
 
 // Every hub-path unit class, one launch each; with a term buffer, the small
-// hubs' classes (stored first) in one flattened launch.
+// hubs' classes (stored first) in one batched launch and the medium and large
+// classes in one streaming launch each.
 void hub_pass(rtrex_graph h, const TermBuf& tb) {
     uint64_t at = 0;
     int c0 = 0;
@@ -1383,7 +1576,27 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
                                                         h->own_idx, tb);
         }
         at = nu;
-        c0 = HUB_NG;
+        for (int lc = 1; lc <= 2; ++lc) {
+            uint32_t n = 0, mx = 0;
+            for (int c = lc * HUB_NG; c < (lc + 1) * HUB_NG; ++c) {
+                n += h->hub_ucnt[c];
+                mx = std::max(mx, h->hub_umax[c]);
+            }
+            if (n) {
+                const uint4* u = h->hub_units + at;
+                const uint32_t blocks = std::min<uint32_t>(n, 16384);
+                if (lc == 1)
+                    k_count_hub_stream<2><<<blocks, HS_T, 2 * 512 * 4, h->stream>>>(
+                        h->g, u, n, h->own_idx, tb, 512);
+                else {
+                    const uint32_t lcap = (std::min<uint32_t>(mx, 4096) + 8 + 15) & ~15u;
+                    k_count_hub_stream<3><<<blocks, HS_T, 2ull * lcap * 4, h->stream>>>(
+                        h->g, u, n, h->own_idx, tb, lcap);
+                }
+            }
+            at += n;
+        }
+        c0 = 3 * HUB_NG;
     }
     for (int c = c0; c < HUB_NL * HUB_NG; ++c) {
         const uint32_t nu = h->hub_ucnt[c];
