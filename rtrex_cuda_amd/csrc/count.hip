@@ -672,6 +672,7 @@ __device__ __forceinline__ bool probe64(const uint32_t* leaf, const uint4& pa, c
 }
 
 __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* __restrict__ units,
+                                                    uint32_t nunits,
                                                     const uint32_t* __restrict__ chunk,
                                                     uint32_t nchunks,
                                                     const uint32_t* __restrict__ own_idx,
@@ -698,19 +699,71 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
     uint32_t tag = 0;  // the marks start at 0
     s_mark[wid][lane] = 0;
     s_mark[wid][WAVE + lane] = 0;
-    // Chunks of about equal work (the plan's chunk offsets), round robin.
-    for (uint32_t ch = blockIdx.x * NW + wid; ch < nchunks; ch += gridDim.x * NW) {
-        const uint32_t u0 = chunk[ch], nu = chunk[ch + 1] - u0;
-        for (uint32_t s = 0; s < nu;) {
-            // Batch: units [s, s + nb), the longest run from s within both
-            // caps; lane j holds unit s + j.  (Wave-uniform values are read
-            // with readlane so that they live in scalar registers and the
-            // loops below stay uniform.)
-            const uint4 U = (uint32_t)lane < nu - s ? units[u0 + s + lane]
-                                                    : make_uint4(0u, 0u, 0u, 0u);
+    // The pending matches [qh, qh + nh) of the ring handled one per lane, the
+    // support folded in ring order.
+    auto handle = [&](uint32_t qh, uint32_t nh) {
+        const bool act = (uint32_t)lane < nh;
+        const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (S_Q - 1)];
+        const uint32_t hs = q.y & 0xffu, k = act ? q.y >> 8 : 0u;
+        const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
+        const bool start = act && (lane == 0 || kup != k);
+        const uint64_t sm = ballot64(start) | 1ull;
+        const int r0 = 63 - __clzll((long long)(sm & upto));  // first lane of the run
+        const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
+        uint32_t c0 = 0;
+        if (act) {
+            c0 = s_cnt[wid][k];
+            const uint2 hd = s_hd[wid][hs];
+            const double t = 1.0 / ((double)s_D[wid][k] * (double)hd.y);
+            tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, hd.x};
+            s_t[wid][lane] = t;
+        }
+        wave_sync();
+        if (last) {
+            double S = s_S[wid][k];
+            int i = r0;
+            for (; i + 3 <= lane; i += 4) {
+                double a[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a[j] = s_t[wid][i + j];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) S += a[j];
+            }
+            for (; i <= lane; ++i) S += s_t[wid][i];
+            s_S[wid][k] = S;
+            s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
+        }
+        wave_sync();
+    };
+    // A batch's end: its last pending matches, then every owned edge's
+    // (support, triangles) record.  Runs at the next batch's setup.
+    bool pend = false, pko = false;
+    uint32_t pqh = 0, pqt = 0, pve = 0;
+    auto flush = [&]() {
+        if (pqt != pqh) {
+            wave_sync();
+            handle(pqh, pqt - pqh);
+        }
+        if (pko) tb.rec[pve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
+        wave_sync();
+        pend = false;
+    };
+    // Chunks of about equal work (the plan's chunk offsets), round robin; the
+    // units of the next batch are loaded while the current one streams.
+    uint32_t ch = blockIdx.x * NW + wid;
+    if (ch >= nchunks) return;
+    uint32_t u1 = chunk[ch + 1], s = chunk[ch];
+    uint4 Un = (uint32_t)lane < u1 - s ? units[s + lane] : make_uint4(0u, 0u, 0u, 0u);
+    for (;;) {
+        {
+            // Batch: units [s, s + nb) of the chunk [.., u1), the longest run
+            // from s within both caps; lane j holds unit s + j.  (Wave-uniform
+            // values are read with readlane so that they live in scalar
+            // registers and the loops below stay uniform.)
+            const uint4 U = (uint32_t)lane < u1 - s ? Un : make_uint4(0u, 0u, 0u, 0u);
             const uint32_t udh = U.w & 0xffffu, ucnt = U.w >> 16, upad = (udh + 7u) & ~7u;
             const uint32_t pc = wave_incl_scan(ucnt, lane), ph = wave_incl_scan(upad, lane);
-            const bool fit = (uint32_t)lane < nu - s && pc <= S_OE && ph <= S_HS;
+            const bool fit = (uint32_t)lane < u1 - s && pc <= S_OE && ph <= S_HS;
             const uint32_t nb = (uint32_t)__popcll(ballot64(fit));
             const uint32_t noe = __builtin_amdgcn_readlane(pc, (int)nb - 1);
             const uint32_t nhs = __builtin_amdgcn_readlane(ph, (int)nb - 1);
@@ -750,6 +803,10 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
             const uint32_t koh = __shfl(U.y, jk, 64), kdh = __shfl(udh, jk, 64);
             const uint32_t khb = __shfl(uhb, jk, 64), kpad = __shfl(upad, jk, 64);
             const uint32_t li = ko ? own_idx[k0 + (uint32_t)lane - cb] : 0u;
+            // The previous batch's last matches and results, while this
+            // batch's staging loads are in flight (its LDS is overwritten
+            // only below).
+            if (pend) flush();
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t i = (uint32_t)lane + 64u * r;
@@ -791,42 +848,20 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
             }
             wave_sync();
 
-            // The pending matches [qh, qh + nh) of the ring handled one per
-            // lane, the support folded in ring order.
-            auto handle = [&](uint32_t qh, uint32_t nh) {
-                const bool act = (uint32_t)lane < nh;
-                const uint2 q = s_q[wid][(qh + (uint32_t)lane) & (S_Q - 1)];
-                const uint32_t hs = q.y & 0xffu, k = act ? q.y >> 8 : 0u;
-                const uint32_t kup = __shfl_up(k, 1u, 64), kdn = __shfl_down(k, 1u, 64);
-                const bool start = act && (lane == 0 || kup != k);
-                const uint64_t sm = ballot64(start) | 1ull;
-                const int r0 = 63 - __clzll((long long)(sm & upto));  // first lane of the run
-                const bool last = act && ((uint32_t)lane + 1 == nh || kdn != k);
-                uint32_t c0 = 0;
-                if (act) {
-                    c0 = s_cnt[wid][k];
-                    const uint2 hd = s_hd[wid][hs];
-                    const double t = 1.0 / ((double)s_D[wid][k] * (double)hd.y);
-                    tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, hd.x};
-                    s_t[wid][lane] = t;
+            // The next batch's units (loaded by every lane: the last unit
+            // past the end, never used).
+            uint32_t sn = s + nb;
+            bool last_batch = false;
+            if (sn >= u1) {
+                ch += gridDim.x * NW;
+                if (ch < nchunks) {
+                    sn = chunk[ch];
+                    u1 = chunk[ch + 1];
+                } else {
+                    last_batch = true;
                 }
-                wave_sync();
-                if (last) {
-                    double S = s_S[wid][k];
-                    int i = r0;
-                    for (; i + 3 <= lane; i += 4) {
-                        double a[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) a[j] = s_t[wid][i + j];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) S += a[j];
-                    }
-                    for (; i <= lane; ++i) S += s_t[wid][i];
-                    s_S[wid][k] = S;
-                    s_cnt[wid][k] = c0 + (uint32_t)(lane - r0 + 1);
-                }
-                wave_sync();
-            };
+            }
+            Un = units[min(sn + (uint32_t)lane, nunits - 1)];
             // Edge lookup and loads of the step at base: the step's edge
             // starts tagged in LDS, then per part the lane's edge (kb: the
             // edge of the element before the step; -1 past the end) and its
@@ -895,15 +930,16 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
                 base += 2 * WAVE;
                 if (base >= W) break;
             }
-            if (qt != qh) {
-                wave_sync();
-                handle(qh, qt - qh);
-            }
-            if (ko) tb.rec[ve] = SupRec{s_S[wid][lane], s_cnt[wid][lane], 0u};
-            wave_sync();  // the next batch restages the slices
-            s += nb;
+            pend = true;
+            pko = ko;
+            pve = ve;
+            pqh = qh;
+            pqt = qt;
+            if (last_batch) break;
+            s = sn;
         }
     }
+    flush();
 }
 
 // Medium and large hubs (d_hub <= 512: LV = 2; <= 4096: LV = 3): one
@@ -1572,8 +1608,8 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         if (nu) {
             const uint32_t tasks = h->small_nchunks;
             const uint32_t blocks = std::min<uint32_t>((tasks + SW / WAVE - 1) / (SW / WAVE), 65536);
-            k_count_small<<<blocks, SW, 0, h->stream>>>(h->g, h->hub_units, h->small_chunk, tasks,
-                                                        h->own_idx, tb);
+            k_count_small<<<blocks, SW, 0, h->stream>>>(h->g, h->hub_units, (uint32_t)nu,
+                                                        h->small_chunk, tasks, h->own_idx, tb);
         }
         at = nu;
         for (int lc = 1; lc <= 2; ++lc) {
