@@ -748,9 +748,11 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
         wave_sync();
         pend = false;
     };
-    // Chunks of about equal work (the plan's chunk offsets), round robin; the
-    // units of the next batch are loaded while the current one streams.
-    uint32_t ch = blockIdx.x * NW + wid;
+    // Chunks of about equal work (the plan's chunk offsets), round robin over
+    // the launch, neighbouring chunks on one XCD (hubs of one community share
+    // lists); the units of the next batch are loaded while the current one
+    // streams.
+    uint32_t ch = xcd_swizzle(blockIdx.x, gridDim.x) * NW + wid;
     if (ch >= nchunks) return;
     uint32_t u1 = chunk[ch + 1], s = chunk[ch];
     uint4 Un = (uint32_t)lane < u1 - s ? units[s + lane] : make_uint4(0u, 0u, 0u, 0u);
@@ -1093,7 +1095,8 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
     uint32_t tag = 0;
     s_mark[wid][lane] = 0;
     s_mark[wid][WAVE + lane] = 0;
-    for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
+    // (units of one hub are consecutive: one XCD stages the hub from its L2)
+    for (uint32_t un = xcd_swizzle(blockIdx.x, gridDim.x); un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);
         const uint32_t nleaf = min(L, ((dh + 7) & ~7u) + 8);  // <= lcap

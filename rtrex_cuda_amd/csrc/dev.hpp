@@ -48,6 +48,18 @@ __device__ __forceinline__ double edge_thr(double eps, uint32_t a, uint32_t b) {
     return eps * ew;
 }
 
+// The hardware deals a launch's workgroups to the chip's 8 XCDs (each with
+// its own L2) round robin by workgroup id.  This maps workgroup b of an
+// n-workgroup launch to a work index such that the workgroups of one XCD take
+// one contiguous range of the work, in order: neighbouring work items (which
+// share data) then run on the same XCD and share its L2.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t n) {
+    const uint32_t x = b % N_XCD, i = b / N_XCD;
+    const uint32_t per = n / N_XCD, rem = n % N_XCD;
+    return x * per + min(x, rem) + i;
+}
+
 __device__ __forceinline__ uint64_t ballot64(bool p) {
     return __ballot(p);
 }
