@@ -54,6 +54,13 @@ __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountPara
 __host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub, const CountParams& P) {
     return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : dhub <= P.l2_max ? 2 : 3;
 }
+// Medium and large hubs whose edges' shorter lists are short on average are
+// counted by the streaming kernel (with a term buffer); long ones (over
+// g_thr[3] elements on average, R-MAT's high-degree pairs) keep 64 lanes per
+// edge.
+__host__ __device__ __forceinline__ bool hub_stream_class(int lc, int gc) {
+    return (lc == 1 || lc == 2) && gc < HUB_NG - 1;
+}
 // Lanes per edge from the mean shorter-list length of the hub's edges.
 __host__ __device__ __forceinline__ int hub_gclass(uint64_t work, uint32_t edges,
                                                    const CountParams& P) {
@@ -311,14 +318,19 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
             // max(2 d_h, 4 unit_min) (a unit feeds four waves); the huge class
             // unit_hash_mult * d_h, but at most LARGE_UNIT owned edges apart
             // so that hubs still spread out.
-            const int lc_ = hub_lclass(dh, Pm);
+            // Classes whose shorter lists average over g_thr[3] elements keep
+            // the lanes-per-edge kernels and their units (below); the others
+            // of the medium and large classes stream (hub_stream_class).
+            const int lc_ = hub_lclass(dh, Pm), gc_ = hub_gclass(work, oc, Pm);
+            const bool stream = hub_stream_class(lc_, gc_);
             const uint64_t uw = lc_ == 0 ? max<uint64_t>(Pm.unit_min, dh)
-                                : lc_ < 3 ? max<uint64_t>(2ull * dh, 4ull * Pm.unit_min)
-                                          : (uint64_t)Pm.unit_hash_mult * dh;
+                                : stream ? max<uint64_t>(2ull * dh, 4ull * Pm.unit_min)
+                                : lc_ == 1 ? max<uint64_t>(Pm.unit_min, dh)
+                                           : (uint64_t)Pm.unit_hash_mult * dh;
             P = (work + uw - 1) / uw;
-            if (lc_ == 3) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
+            if (lc_ >= 2 && !stream) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
-            c = lc_ * HUB_NG + hub_gclass(work, oc, Pm);
+            c = lc_ * HUB_NG + gc_;
             at = atomicAdd(&lc[c], (uint32_t)P);
             if (COUNT) atomicMax(&lm[c], dh);
         }
@@ -1615,11 +1627,17 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
                                                         h->small_chunk, tasks, h->own_idx, tb);
         }
         at = nu;
-        for (int lc = 1; lc <= 2; ++lc) {
+        c0 = HUB_NG;
+    }
+    for (int c = c0; c < HUB_NL * HUB_NG; ++c) {
+        const int lc = c / HUB_NG, gc = c % HUB_NG;
+        if (tb.t && hub_stream_class(lc, gc)) {
+            // the class's streaming classes (consecutive) in one launch
+            if (gc) continue;
             uint32_t n = 0, mx = 0;
-            for (int c = lc * HUB_NG; c < (lc + 1) * HUB_NG; ++c) {
-                n += h->hub_ucnt[c];
-                mx = std::max(mx, h->hub_umax[c]);
+            for (int cc = c; cc < c + HUB_NG && hub_stream_class(lc, cc % HUB_NG); ++cc) {
+                n += h->hub_ucnt[cc];
+                mx = std::max(mx, h->hub_umax[cc]);
             }
             if (n) {
                 const uint4* u = h->hub_units + at;
@@ -1634,14 +1652,11 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
                 }
             }
             at += n;
+            continue;
         }
-        c0 = 3 * HUB_NG;
-    }
-    for (int c = c0; c < HUB_NL * HUB_NG; ++c) {
         const uint32_t nu = h->hub_ucnt[c];
         if (!nu) continue;
         const uint4* u = h->hub_units + at;
-        const int lc = c / HUB_NG, gc = c % HUB_NG;
         if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 3) {
