@@ -1121,14 +1121,17 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
             s_leaf[i] = z;
             s_dg[i] = d;
         }
-        for (uint32_t i = threadIdx.x; i < L / 8; i += HS_T) s_l1[i] = hub_id(g, oh, dh, 8 * i + 7);
+        if (threadIdx.x == 0) s_next = 0;
+        __syncthreads();
+        // Pivot levels from the staged leaf (entries from nleaf on are
+        // padding).
+        auto leaf_at = [&](uint32_t i) { return i < nleaf ? s_leaf[i] : PAD_ID; };
+        for (uint32_t i = threadIdx.x; i < L / 8; i += HS_T) s_l1[i] = leaf_at(8 * i + 7);
         if (LV == 3)
-            for (uint32_t i = threadIdx.x; i < L / 64; i += HS_T)
-                s_l2[i] = hub_id(g, oh, dh, 64 * i + 63);
+            for (uint32_t i = threadIdx.x; i < L / 64; i += HS_T) s_l2[i] = leaf_at(64 * i + 63);
         Piv8 top;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
-        if (threadIdx.x == 0) s_next = 0;
+        for (int i = 0; i < 8; ++i) top.p[i] = leaf_at((L / 8) * i + L / 8 - 1);
         __syncthreads();
         const TreeProbe<LV> probe_tree{s_leaf, s_l1, s_l2, top};
         for (;;) {
