@@ -1134,12 +1134,16 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
         for (int i = 0; i < 8; ++i) top.p[i] = leaf_at((L / 8) * i + L / 8 - 1);
         __syncthreads();
         const TreeProbe<LV> probe_tree{s_leaf, s_l1, s_l2, top};
+        // The unit's owned edges split into nbat batches of at most HS_E
+        // edges, at least one per wave (long lists: few edges per unit).
+        const uint32_t nbat = min(cnt, max((uint32_t)NW, (cnt + HS_E - 1) / HS_E));
         for (;;) {
             uint32_t b = 0;
             if (lane == 0) b = atomicAdd(&s_next, 1u);
             b = __builtin_amdgcn_readfirstlane(b);
-            if (b * HS_E >= cnt) break;
-            const uint32_t a0 = b * HS_E, noe = min(HS_E, cnt - a0);
+            if (b >= nbat) break;
+            const uint32_t a0 = (uint32_t)((uint64_t)cnt * b / nbat);
+            const uint32_t noe = (uint32_t)((uint64_t)cnt * (b + 1) / nbat) - a0;
             // Owned edge k = lane of the batch: its hub slot li, the shorter
             // list's offset and length, the term offset.
             const bool ko = (uint32_t)lane < noe;
