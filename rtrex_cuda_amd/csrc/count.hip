@@ -1688,6 +1688,37 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
     }
 }
 
+// Processing order of the small-hub units (results do not depend on it): the
+// units sorted by a MinHash of their hub's closed neighbourhood, min over
+// z in N(h) + {h} of a mixed 32-bit hash of z.  Two hubs share that value with
+// probability the Jaccard similarity of their neighbourhoods, so hubs of one
+// dense community tend to land in one chunk and stream the same lists while
+// they are in that XCD's caches.  key = hash << 32 | unit index.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__global__ void __launch_bounds__(T) k_small_keys(GraphDev g, const uint4* __restrict__ units,
+                                                  uint32_t nu, uint64_t* __restrict__ keys) {
+    for (uint32_t u = blockIdx.x * T + threadIdx.x; u < nu; u += gridDim.x * T) {
+        const uint4 U = units[u];
+        const uint32_t dh = U.w & 0xffffu;
+        uint32_t m = mix32(U.x);
+        for (uint32_t i = 0; i < dh; ++i) m = min(m, mix32(g.nb[U.y + i]));
+        keys[u] = ((uint64_t)m << 32) | u;
+    }
+}
+__global__ void __launch_bounds__(T) k_small_permute(const uint64_t* __restrict__ keys,
+                                                     const uint4* __restrict__ in, uint32_t nu,
+                                                     uint4* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nu; i += gridDim.x * T)
+        out[i] = in[(uint32_t)keys[i]];
+}
+
 // Streamed work of every small-hub unit: the sum of its owned edges' shorter
 // lists (the degrees of the hub's neighbours at the owned slots).
 __global__ void __launch_bounds__(T) k_small_work(GraphDev g, const uint4* __restrict__ units,
@@ -1773,6 +1804,22 @@ rtrex_status prepare_hubs(rtrex_graph h) {
     uint32_t nsmall = 0;
     for (int c = 0; c < HUB_NG; ++c) nsmall += h->hub_ucnt[c];
     if (nsmall) {
+        {
+            uint64_t *keys, *tmp;
+            uint4* cpy;
+            RT_TRY(sc.get(&keys, nsmall));
+            RT_TRY(sc.get(&tmp, nsmall));
+            RT_TRY(sc.get(&cpy, nsmall));
+            k_small_keys<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(g, h->hub_units, nsmall, keys);
+            int bits = 32;
+            while ((1ull << (bits - 32)) < nsmall) ++bits;
+            RT_TRY(radix_sort_u64(keys, tmp, nsmall, bits, s));
+            RT_HIP(hipMemcpyAsync(cpy, h->hub_units, (uint64_t)nsmall * sizeof(uint4),
+                                  hipMemcpyDeviceToDevice, s));
+            k_small_permute<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(keys, cpy, nsmall,
+                                                                      h->hub_units);
+            RT_HIP(hipGetLastError());
+        }
         uint32_t* w;
         uint64_t* pre;
         RT_TRY(sc.get(&w, nsmall));
