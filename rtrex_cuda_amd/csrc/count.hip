@@ -41,6 +41,16 @@ __device__ __forceinline__ int bucket_of(uint32_t w) {
     return w <= 4u ? 0 : w <= 16u ? 1 : w <= 96u ? 2 : 3;
 }
 
+// A term written once here and read only by later phases: a streaming
+// (nontemporal) store, so the term buffer does not push the lists the count
+// kernels re-read out of the L2.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_term(Term* p, double w, uint32_t a, uint32_t b) {
+    const uint64_t wb = (uint64_t)__double_as_longlong(w);
+    const u32x4 v = {(uint32_t)wb, (uint32_t)(wb >> 32), a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(p));
+}
+
 // Hub classes (CountParams).  Edge {x, y} streams the list of
 // lo = (d_x <= d_y ? x : y) (x < y) and probes the list of the other
 // endpoint, its hub.  Edges whose hub degree is in (hub_min, hub_max] are
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(T) k_count_edges(GraphDev g,
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
                 if (hit)  // edges {lo, z} and {hi, z}
-                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, g.s2e[p + gl], g.s2e[hp]};
+                    store_term(tb.t + wbase + c + __popcll(hm & below), t, g.s2e[p + gl], g.s2e[hp]);
             }
             group_fold<G>(hit, t, S, c);
             if (go) p += G;
@@ -535,7 +545,7 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
                 const uint64_t hm = G == 64 ? ballot64(hit)
                                             : (ballot64(hit) >> gbase) & ((1ull << G) - 1);
                 if (tb.t && hit)  // edges {lo, z} and {hub, z}
-                    tb.t[wbase + c + __popcll(hm & below)] = Term{t, er[r], hm_.ed(g, pos)};
+                    store_term(tb.t + wbase + c + __popcll(hm & below), t, er[r], hm_.ed(g, pos));
                 c += (uint32_t)__popcll(hm);
                 // Ordered fold: the group's match weights in lane order
                 // (ascending z).
@@ -727,7 +737,7 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
             c0 = s_cnt[wid][k];
             const uint2 hd = s_hd[wid][hs];
             const double t = 1.0 / ((double)s_D[wid][k] * (double)hd.y);
-            tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, hd.x};
+            store_term(tb.t + s_zb[wid][k] + c0 + (uint32_t)(lane - r0), t, q.x, hd.x);
             s_t[wid][lane] = t;
         }
         wave_sync();
@@ -1177,7 +1187,7 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
                 if (act) {
                     c0 = s_cnt[wid][k];
                     const double t = 1.0 / ((double)s_D[wid][k] * (double)s_dg[hs]);
-                    tb.t[s_zb[wid][k] + c0 + (uint32_t)(lane - r0)] = Term{t, q.x, ez};
+                    store_term(tb.t + s_zb[wid][k] + c0 + (uint32_t)(lane - r0), t, q.x, ez);
                     s_t[wid][lane] = t;
                 }
                 wave_sync();
@@ -1693,7 +1703,11 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
 // z in N(h) + {h} of a mixed 32-bit hash of z.  Two hubs share that value with
 // probability the Jaccard similarity of their neighbourhoods, so hubs of one
 // dense community tend to land in one chunk and stream the same lists while
-// they are in that XCD's caches.  key = hash << 32 | unit index.
+// they are in that XCD's caches.  key = unit index << 32 | the hash's low 20
+// bits, sorted on those bits (stable: ties keep their order).  The groups
+// come in pseudo-random order; ordered by the whole hash (whose small values
+// come from large neighbourhoods) or grouped by its top bits (minima are
+// small, so the top bits merge many groups) measured 7-8 % slower.
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7feb352du;
@@ -1709,14 +1723,14 @@ __global__ void __launch_bounds__(T) k_small_keys(GraphDev g, const uint4* __res
         const uint32_t dh = U.w & 0xffffu;
         uint32_t m = mix32(U.x);
         for (uint32_t i = 0; i < dh; ++i) m = min(m, mix32(g.nb[U.y + i]));
-        keys[u] = ((uint64_t)m << 32) | u;
+        keys[u] = ((uint64_t)u << 32) | (m & 0xfffffu);
     }
 }
 __global__ void __launch_bounds__(T) k_small_permute(const uint64_t* __restrict__ keys,
                                                      const uint4* __restrict__ in, uint32_t nu,
                                                      uint4* __restrict__ out) {
     for (uint32_t i = blockIdx.x * T + threadIdx.x; i < nu; i += gridDim.x * T)
-        out[i] = in[(uint32_t)keys[i]];
+        out[i] = in[(uint32_t)(keys[i] >> 32)];
 }
 
 // Streamed work of every small-hub unit: the sum of its owned edges' shorter
@@ -1811,9 +1825,7 @@ rtrex_status prepare_hubs(rtrex_graph h) {
             RT_TRY(sc.get(&tmp, nsmall));
             RT_TRY(sc.get(&cpy, nsmall));
             k_small_keys<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(g, h->hub_units, nsmall, keys);
-            int bits = 32;
-            while ((1ull << (bits - 32)) < nsmall) ++bits;
-            RT_TRY(radix_sort_u64(keys, tmp, nsmall, bits, s));
+            RT_TRY(radix_sort_u64(keys, tmp, nsmall, 20, s));
             RT_HIP(hipMemcpyAsync(cpy, h->hub_units, (uint64_t)nsmall * sizeof(uint4),
                                   hipMemcpyDeviceToDevice, s));
             k_small_permute<<<blocks_for(nsmall, T, 4096), T, 0, s>>>(keys, cpy, nsmall,
