@@ -452,8 +452,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "count phase (k_count_small, "
-                     "k_count_hub, k_count_hub_large, k_count_hub_huge, k_count_edges, "
-                     "k_sum_tcnt)",
+                     "k_count_hub_stream, k_count_hub, k_count_hub_large, k_count_hub_huge, "
+                     "k_count_edges, k_sum_tcnt)",
                      "algorithmic_bytes": B_count, "s_min": s_min},
         "audit": plain["audit"],
         "autotune": plain["autotune"],
