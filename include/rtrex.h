@@ -248,10 +248,14 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 /* 4096 (<= l2_max) with an 8-ary pivot tree; above l2_max (hub_max <= 32760)  */
 /* the tree's upper levels are staged and its leaf is the list in device      */
 /* memory; other edges are counted per edge.                                   */
-/* A hub's edges get 4, 8, 16, 32 or 64 lanes each by their mean shorter-list */
-/* length against g_thr[0..3].  Units of work hold at least unit_min streamed  */
-/* elements (large class: unit_hash_mult * hub degree).  Results never depend   */
-/* on the parameters, only the time does.                                      */
+/* With stored triangle lists, hubs up to l0_max are counted in batches of    */
+/* whole hubs (units of at least unit_min streamed elements, grouped into     */
+/* chunks of about 1,024), and hubs up to l2_max whose edges' shorter lists   */
+/* average at most g_thr[3] elements by a streaming kernel (units of          */
+/* max(2 * hub degree, 4 * unit_min) elements); other hubs' edges get 4, 8,   */
+/* 16, 32 or 64 lanes each by their mean shorter-list length against          */
+/* g_thr[0..3] (units: unit_min, or unit_hash_mult * hub degree above l1_max).*/
+/* Results never depend on the parameters, only the time does.                */
 typedef struct {
     int32_t hub_min, hub_max, l0_max, l1_max;
     int32_t g_thr[4];
