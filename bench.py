@@ -10,7 +10,7 @@ medium planted-community workload (C2).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--eps 0.3]
                     [--workload c2|c1|c3|rmat20] [--no-cpu] [--c3-steps S]
-                    [--batch-graphs B] [--serve-seconds T]
+                    [--batch-graphs B] [--serve-seconds T] [--serve-capacity R1,R2]
 
 Beside the headline: the same steps with the results left in HBM, with the
 upload and device CSR build, the skewed large R-MAT scale-22 graph (C3), the
@@ -161,20 +161,13 @@ def copy_calibration(nbytes: int, iters: int) -> int:
     return 0
 
 
-def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb: float) -> dict:
-    """C5: Poisson request stream through the server (continuous batching,
-    HBM-footprint admission) vs the same requests one at a time through the
-    plain path.  Latency = result completion - scheduled arrival."""
+def run_trace(R, pool, trace, eps: float, workers: int, budget_gb: float, note: bool = True):
+    """Submits a (time, kind, index) trace to a fresh server at its scheduled
+    times; returns per request (clusters, job info, map hash) and the clock
+    origin of the schedule."""
     import queue
     import threading
 
-    import rtrex_cuda_amd as R
-    from rtrex_cuda_amd import graphs as G
-    sizes = (64, 32, 3)
-    t0 = time.perf_counter()
-    pool = G.serve_pool(*sizes)
-    gen_s = time.perf_counter() - t0
-    trace = G.serve_trace(seconds, rate, pool_sizes=sizes)
     graph = lambda kind, i: pool[kind][i]  # noqa: E731
     done = [None] * len(trace)
     with R.Server(workers=workers, hbm_budget_bytes=int(budget_gb * (1 << 30))) as srv:
@@ -196,7 +189,7 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
         start = srv.clock() + 0.05
         next_note = 10.0
         for i, (t, kind, idx) in enumerate(trace):
-            if t >= next_note:
+            if note and t >= next_note:
                 progress(f"c5: {t:.0f} s of the trace submitted")
                 next_note += 10.0
             while srv.clock() < start + t:
@@ -204,6 +197,25 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
             q.put((i, srv.submit(*graph(kind, idx), eps)))
         q.put(None)
         th.join()
+    return done, start
+
+
+def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb: float,
+                capacity_rates=(), capacity_seconds: float = 10.0) -> dict:
+    """C5: Poisson request stream through the server (continuous batching,
+    HBM-footprint admission) vs the same requests one at a time through the
+    plain path.  Latency = result completion - scheduled arrival.  Capacity:
+    shorter traces of the same mix at higher rates; the highest rate whose p99
+    stays under a second with every request served at the offered rate."""
+    import rtrex_cuda_amd as R
+    from rtrex_cuda_amd import graphs as G
+    sizes = (64, 32, 3)
+    t0 = time.perf_counter()
+    pool = G.serve_pool(*sizes)
+    gen_s = time.perf_counter() - t0
+    trace = G.serve_trace(seconds, rate, pool_sizes=sizes)
+    graph = lambda kind, i: pool[kind][i]  # noqa: E731
+    done, start = run_trace(R, pool, trace, eps, workers, budget_gb)
     lat = np.array([done[i][1]["end_s"] - (start + t) for i, (t, _, _) in enumerate(trace)])
     kinds = np.array([k for _, k, _ in trace])
     by_kind = {}
@@ -216,6 +228,25 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
     edges = np.array([len(graph(k, i)[1]) for _, k, i in trace], np.float64)
     makespan = max(d[1]["end_s"] for d in done) - start
     batched = np.array([d[1]["batch_graphs"] for d in done])
+    # Capacity sweep: the same mix at higher arrival rates.
+    sweep = []
+    for r in capacity_rates:
+        tr = G.serve_trace(capacity_seconds, r, pool_sizes=sizes)
+        dn, st0 = run_trace(R, pool, tr, eps, workers, budget_gb, note=False)
+        lt = np.array([dn[i][1]["end_s"] - (st0 + t) for i, (t, _, _) in enumerate(tr)])
+        ed = float(sum(len(graph(k, i)[1]) for _, k, i in tr))
+        span = max(d[1]["end_s"] for d in dn) - st0
+        row = {"rate": r, "requests": len(tr), "p50_ms": round(float(np.percentile(lt, 50)) * 1e3, 2),
+               "p99_ms": round(float(np.percentile(lt, 99)) * 1e3, 2),
+               "offered_Medges_per_s": round(ed / capacity_seconds / 1e6, 2),
+               "served_Medges_per_s": round(ed / span / 1e6, 2)}
+        sweep.append(row)
+        progress(f"c5: {r:g} req/s p99 {row['p99_ms']:.0f} ms")
+    main_ok = (float(np.percentile(lat, 99)) < 1.0 and
+               edges.sum() / makespan >= 0.97 * edges.sum() / seconds)
+    ok = ([rate] if main_ok else []) + [
+        row["rate"] for row in sweep
+        if row["p99_ms"] < 1000.0 and row["served_Medges_per_s"] >= 0.97 * row["offered_Medges_per_s"]]
     # Capacity: the first requests of the same trace submitted at once (no
     # arrival process); throughput = their edges / (last completion - first
     # submission).
@@ -257,6 +288,10 @@ def serving_leg(eps: float, seconds: float, rate: float, workers: int, budget_gb
             "serial_plain_Medges_per_s": round(sm / st / 1e6, 2),
             "serial_plain_sample": f"{len(sample)} requests (every 10th), {st:.2f} s",
             "served_vs_serial": round((edges.sum() / makespan) / (sm / st), 2),
+            "capacity": {"seconds_per_rate": capacity_seconds, "rates": sweep,
+                         "highest_rate_p99_under_1s": max(ok) if ok else None,
+                         "note": "traces of the same mix at higher Poisson rates; a rate "
+                                 "counts when p99 < 1 s and served >= 97 % of offered"},
             "saturated": {"requests": cap_n, "Medges_per_s": round(cap_rate, 2),
                           "vs_serial": round(cap_rate / (sm / st / 1e6), 2),
                           "note": "the trace's first requests submitted at once"},
@@ -397,6 +432,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="C5 serving leg: length of the request trace (0 skips it)")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-capacity", default="260,320",
+                    help="C5 capacity sweep: comma-separated Poisson rates (req/s) run for 10 s "
+                         "each after the main trace ('' skips it)")
     ap.add_argument("--serve-budget-gb", type=float, default=128.0,
                     help="serving pool size (HBM footprint admission), GiB of the 288 GB")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -467,8 +505,9 @@ def main() -> int:
         out["batched"] = batched_vs_plain(args.eps, args.batch_graphs, max(1, args.steps // 3))
         progress(f"c4: {out['batched']['batched_ms']:.1f} ms/step batched")
     if args.serve_seconds > 0:
+        rates = tuple(float(x) for x in args.serve_capacity.split(",") if x.strip())
         out["serving"] = serving_leg(args.eps, args.serve_seconds, 200.0, args.serve_workers,
-                                     args.serve_budget_gb)
+                                     args.serve_budget_gb, rates)
         progress("c5: done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, u, v, args.eps)
