@@ -1701,9 +1701,10 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
 // Processing order of the small-hub units (results do not depend on it): the
 // units sorted by a MinHash of their hub's closed neighbourhood, min over
 // z in N(h) + {h} of a mixed 32-bit hash of z.  Two hubs share that value with
-// probability the Jaccard similarity of their neighbourhoods, so hubs of one
-// dense community tend to land in one chunk and stream the same lists while
-// they are in that XCD's caches.  key = unit index << 32 | the hash's low 20
+// probability the Jaccard similarity of their neighbourhoods, so hubs with
+// common neighbours tend to run near each other (one XCD, close in time) and
+// re-read the same lists from its L2.  Measured on C2 against the units'
+// build order: 1.6 % faster; against the hub id's hash alone: 0-4 %.  key = unit index << 32 | the hash's low 20
 // bits, sorted on those bits (stable: ties keep their order).  The groups
 // come in pseudo-random order; ordered by the whole hash (whose small values
 // come from large neighbourhoods) or grouped by its top bits (minima are
