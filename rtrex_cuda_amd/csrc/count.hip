@@ -57,7 +57,11 @@ __device__ __forceinline__ void store_term(Term* p, double w, uint32_t a, uint32
 // counted per hub (k_count_hub*), the hub's list staged once in LDS; the rest
 // per edge (k_count_edges).
 constexpr int HUB_NL = 4, HUB_NG = 5;  // staging classes 64/512/4096/32768; G = 4..64
-constexpr uint32_t LARGE_UNIT = 32;   // owned edges per unit, at most, in the large class
+// Owned edges per unit, at most, in the large and huge classes.  A huge
+// unit restages the tree and the filter of a hub of up to 32760 entries, so
+// its units are larger (R-MAT scale 21 count: 123 -> 110 ms with 128 instead
+// of 32); 32 keeps the few large hubs of planted graphs spread over the CUs.
+constexpr uint32_t LARGE_UNIT = 32, HUGE_UNIT = 128;
 __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
     return dhub > P.hub_min && dhub <= P.hub_max;
 }
@@ -326,8 +330,8 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
             // Units of at least max(unit_min, d_h) elements (small hubs); the
             // medium and large classes restage their tree per unit:
             // max(2 d_h, 4 unit_min) (a unit feeds four waves); the huge class
-            // unit_hash_mult * d_h, but at most LARGE_UNIT owned edges apart
-            // so that hubs still spread out.
+            // unit_hash_mult * d_h, but at most LARGE_UNIT (huge: HUGE_UNIT)
+            // owned edges apart so that hubs still spread out.
             // Classes whose shorter lists average over g_thr[3] elements keep
             // the lanes-per-edge kernels and their units (below); the others
             // of the medium and large classes stream (hub_stream_class).
@@ -338,7 +342,8 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
                                 : lc_ == 1 ? max<uint64_t>(Pm.unit_min, dh)
                                            : (uint64_t)Pm.unit_hash_mult * dh;
             P = (work + uw - 1) / uw;
-            if (lc_ >= 2 && !stream) P = max<uint64_t>(P, (oc + LARGE_UNIT - 1) / LARGE_UNIT);
+            const uint32_t ucap = lc_ == 3 ? HUGE_UNIT : LARGE_UNIT;
+            if (lc_ >= 2 && !stream) P = max<uint64_t>(P, (oc + ucap - 1) / ucap);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = lc_ * HUB_NG + gc_;
             at = atomicAdd(&lc[c], (uint32_t)P);
@@ -580,6 +585,129 @@ __device__ __forceinline__ void hub_edges(const GraphDev& g, const TermBuf& tb, 
             }
         }
         if (act && gl == 0) tb.rec[e] = SupRec{S, c, 0u};
+    }
+}
+
+// Membership filter of a staged hub list: one bit per hash value of
+// 2^FB, set for every id of the list (a multiplicative hash, so ids that
+// share low bits, as R-MAT's do, still spread).  A clear bit proves z is not
+// in the list; a set bit is confirmed by the tree probe.
+constexpr uint32_t FILT_MUL = 0x9E3779B1u;
+template <int FB>
+__device__ __forceinline__ uint32_t filt_hash(uint32_t z) {
+    return (z * FILT_MUL) >> (32 - FB);
+}
+template <int FB>
+__device__ __forceinline__ void filt_set(uint32_t* filt, uint32_t z) {
+    const uint32_t h = filt_hash<FB>(z);
+    atomicOr(&filt[h >> 5], 1u << (h & 31u));
+}
+template <int FB>
+__device__ __forceinline__ bool filt_test(const uint32_t* filt, uint32_t z) {
+    const uint32_t h = filt_hash<FB>(z);
+    return (filt[h >> 5] >> (h & 31u)) & 1u;
+}
+constexpr uint32_t RING = 128;  // filter-passing ids queued per wave
+// Filter sizes of the 64-lane large- and huge-hub kernels (2^FB bits): at
+// most 1/16 and 1/4 of the bits are set by the class's largest hub.  The huge
+// kernel's 16 KB filter leaves room for three workgroups per CU; a 32 KB one
+// (two) measured 13 % slower on R-MAT scale 20 and 21, an 8 KB one (more
+// false positives) 3-5 % slower.
+constexpr int LARGE_FB = 16, HUGE_FB = 17;
+// Whether the large-hub kernel stages its slots' edge ids and degrees beside
+// the tree: without a filter, where every id is probed.  With one, a match
+// reads them from the hub's slots in HBM (L2-resident) and the smaller LDS
+// footprint fits four workgroups per CU instead of two (R-MAT scale 20 count
+// 54.5 -> 45.9 ms).
+__host__ __device__ constexpr bool large_slot_meta(int fb) {
+    return fb == 0;
+}
+
+// hub_edges for G = 64 (one owned edge per wave at a time; R-MAT's long
+// shorter lists against hubs of thousands of entries, where most ids miss):
+// each streamed id is first tested against the hub's filter, and only the
+// ids that pass are queued, in stream order, in the wave's LDS ring (id and
+// edge id {lo, z}); whenever 64 are queued they are probed in the tree
+// together, so the probe, the match weight and the term store run on full
+// waves of candidates instead of on every streamed id.  The queue keeps
+// ascending z, so terms and the ordered fold are those of hub_edges.
+template <int FB, class PR, class HM, bool PRE = false>
+__device__ __forceinline__ void hub_edges_filt(const GraphDev& g, const TermBuf& tb, uint32_t dh,
+                                               const PR& probe, const HM& hm_,
+                                               const uint32_t* own, const uint32_t* filt,
+                                               uint2* ring, uint32_t first, uint32_t cnt,
+                                               uint32_t stride, const uint4* pre = nullptr,
+                                               const uint64_t* prez = nullptr) {
+    constexpr int R = 4;  // chunks of 64 ids per load round
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t idx = first; idx < cnt; idx += stride) {  // wave-uniform
+        uint32_t e;  // edge {hub, lo}
+        uint2 yo;
+        uint64_t wb0;
+        if (PRE) {
+            const uint4 q = pre[idx];
+            e = q.x;
+            yo = make_uint2(q.y, q.z);
+            wb0 = prez[idx];
+        } else {
+            const uint32_t li = own[idx];
+            e = hm_.ed(g, li);
+            yo = hm_.dego(g, li);
+            wb0 = tb.t ? hm_.zo(tb, li) : 0;
+        }
+        const uint32_t dy = yo.x;
+        uint32_t p = yo.y;
+        const uint32_t le = p + dy;
+        const uint64_t wbase = tb.t ? wb0 : 0;
+        double S = 0.0;
+        uint32_t c = 0, head = 0, tail = 0;  // ring positions (mod RING)
+        // Probes the n <= 64 queued ids from head, one per lane.
+        auto drain = [&](uint32_t n) {
+            wave_sync();  // the ring entries written by other lanes
+            const uint2 a = ring[(head + (uint32_t)lane) & (RING - 1)];
+            const bool in = (uint32_t)lane < n;
+            const uint32_t z = in ? a.x : 0u;
+            uint32_t pos = 0;
+            const bool hit = in && probe(z, pos);
+            const double t = hit ? tri_w(dh, dy, hm_.deg(g, pos)) : 0.0;
+            const uint64_t hm = ballot64(hit);
+            if (tb.t && hit)  // edges {lo, z} and {hub, z}
+                store_term(tb.t + wbase + c + __popcll(hm & below), t, a.y, hm_.ed(g, pos));
+            c += (uint32_t)__popcll(hm);
+            uint64_t mask = hm;  // ordered fold: ascending z
+            while (mask) {
+                const int b = __ffsll((unsigned long long)mask) - 1;
+                S += readlane_f64(t, b);
+                mask &= mask - 1;
+            }
+            head += n;
+            wave_sync();  // these slots are written again only after the reads
+        };
+        while (p < le) {
+            uint32_t zr[R], er[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t q = p + r * WAVE + lane;
+                const uint2 a = q < le ? g.nbe[q] : make_uint2(0u, 0u);
+                zr[r] = a.x;
+                er[r] = a.y;  // edge {lo, z}
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool in = p + r * WAVE + lane < le;
+                const bool pass = in && filt_test<FB>(filt, zr[r]);
+                const uint64_t pm = ballot64(pass);
+                if (pass)
+                    ring[(tail + (uint32_t)__popcll(pm & below)) & (RING - 1)] =
+                        make_uint2(zr[r], er[r]);
+                tail += (uint32_t)__popcll(pm);
+                if (tail - head >= WAVE) drain(WAVE);
+            }
+            p += R * WAVE;
+        }
+        if (tail != head) drain(tail - head);
+        if (lane == 0) tb.rec[e] = SupRec{S, c, 0u};
     }
 }
 
@@ -1024,41 +1152,58 @@ __global__ void __launch_bounds__(NT) k_count_hub(GraphDev g, const uint4* __res
 }
 
 // Large hubs (degree in (l1_max, l2_max], l2_max <= 4096): one workgroup of HTL lanes per unit,
-// the hub's 8-ary tree staged in LDS once and shared by its waves, with the
-// edge id and degree of every hub slot beside it so that a match needs no
-// memory access; the owned edges' own data is read from HBM (the hub's slots,
-// L2-resident; fetching it during the staging measured slower, it delays the
-// staging barrier).  The leaf and slot arrays live in dynamic LDS of lcap
-// entries each, lcap >= the launch's largest hub degree + 8: a probe ends in
-// the leaf block holding the hub's first padding entry at the latest.
+// the hub's 8-ary tree staged in LDS once and shared by its waves; the owned
+// edges' own data is read from HBM (the hub's slots, L2-resident; fetching it
+// during the staging measured slower, it delays the staging barrier).  The
+// leaf lives in dynamic LDS of lcap entries, lcap >= the launch's largest hub
+// degree + 8: a probe ends in the leaf block holding the hub's first padding
+// entry at the latest.  FB = 0 (fewer than 64 lanes per edge): the edge id and
+// degree of every hub slot are staged beside the leaf (two more arrays of
+// lcap), so that a match needs no memory access.  FB > 0 (64 lanes): the ids
+// are filtered first (hub_edges_filt) against a filter of 2^FB bits built
+// while the hub is staged, and a match reads its slot's data from HBM.
 constexpr int HTL = 512;
-template <int G>
+template <int G, int FB = 0>
 __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
                                                          const uint4* __restrict__ units,
                                                          uint32_t nunits,
                                                          const uint32_t* __restrict__ own_idx,
                                                          TermBuf tb, uint32_t lcap) {
+    static_assert(FB == 0 || G == WAVE, "the filtered path runs one edge per wave");
     constexpr uint32_t L = 4096;
+    constexpr uint32_t NF = FB ? (1u << FB) / 32 : 1;
+    constexpr bool SM = large_slot_meta(FB);  // slots' edge ids and degrees in LDS
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     uint32_t* s_leaf = s_dyn;
     uint32_t* s_ed = s_dyn + lcap;
     uint32_t* s_dg = s_dyn + 2 * lcap;
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
-    __shared__ double sfold[HTL];
+    __shared__ double sfold[FB ? 1 : HTL];
+    __shared__ __attribute__((aligned(16))) uint32_t s_filt[NF];
+    __shared__ uint2 s_ring[FB ? (HTL / WAVE) * RING : 1];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
         unit_fields(units[un], h, oh, k0, dh, cnt);
+        if constexpr (FB > 0) {  // the previous unit's probes ended at the barrier below
+            for (uint32_t i = threadIdx.x; i < NF; i += HTL) s_filt[i] = 0u;
+            __syncthreads();
+        }
         const uint32_t nleaf = ((dh + 7) & ~7u) + 8;  // <= lcap
         for (uint32_t i = threadIdx.x; i < nleaf; i += HTL) {
             uint32_t z = PAD_ID;
             if (i < dh) {
-                const uint2 a = g.nbe[oh + i];
-                z = a.x;
-                s_ed[i] = a.y;
-                s_dg[i] = g.ndo[oh + i].x;
+                if constexpr (SM) {
+                    const uint2 a = g.nbe[oh + i];
+                    z = a.x;
+                    s_ed[i] = a.y;
+                    s_dg[i] = g.ndo[oh + i].x;
+                } else {
+                    z = g.nb[oh + i];
+                }
+                if constexpr (FB > 0) filt_set<FB>(s_filt, z);
             }
             s_leaf[i] = z;
         }
@@ -1068,9 +1213,14 @@ __global__ void __launch_bounds__(HTL) k_count_hub_large(GraphDev g,
 #pragma unroll
         for (int i = 0; i < 8; ++i) top.p[i] = hub_id(g, oh, dh, (L / 8) * i + L / 8 - 1);
         __syncthreads();
-        const HubMeta<false, true> meta{s_ed, s_dg, nullptr, nullptr, oh};
-        hub_edges<G>(g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta, own_idx + k0,
-                     sfold + wid * WAVE, wid * per_wave, cnt, per_wave * (HTL / WAVE));
+        const HubMeta<false, SM> meta{s_ed, s_dg, nullptr, nullptr, oh};
+        if constexpr (FB > 0)
+            hub_edges_filt<FB>(g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta,
+                               own_idx + k0, s_filt, s_ring + wid * RING, wid, cnt,
+                               HTL / WAVE);
+        else
+            hub_edges<G>(g, tb, dh, TreeProbe<3>{s_leaf, s_l1, s_l2, top}, meta, own_idx + k0,
+                         sfold + wid * WAVE, wid * per_wave, cnt, per_wave * (HTL / WAVE));
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -1282,6 +1432,7 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
 // (sh = oh mod 8 entries earlier): those sh entries read as 0, so the sequence
 // stays non-decreasing, and are never a match.
 static_assert(HUB_MAX_LIMIT + 7 < 32768, "a huge hub's list and its shift fit the tree");
+static_assert(HUGE_UNIT <= HTL, "a unit's owned edges are prefetched one per thread");
 __device__ __forceinline__ uint32_t huge_id(const GraphDev& g, uint32_t ob, uint32_t sh,
                                             uint32_t dh, uint32_t s) {
     return s < sh ? 0u : s < sh + dh ? g.nb[ob + s] : PAD_ID;
@@ -1321,24 +1472,35 @@ struct HugeProbe {
     }
 };
 
-template <int G>
+// FB > 0 (G = 64 only): the ids are filtered first (hub_edges_filt) against a
+// filter of 2^FB bits, built from one pass over the hub's list that also
+// fills the tree's lowest pivot level.
+template <int G, int FB = 0>
 __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
                                                         const uint4* __restrict__ units,
                                                         uint32_t nunits,
                                                         const uint32_t* __restrict__ own_idx,
                                                         TermBuf tb) {
+    static_assert(FB == 0 || G == WAVE, "the filtered path runs one edge per wave");
     constexpr uint32_t L = 32768;
+    constexpr uint32_t NF = FB ? (1u << FB) / 32 : 1;
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
     __shared__ __attribute__((aligned(16))) uint32_t s_l3[L / 512];
-    __shared__ double sfold[HTL];
-    __shared__ uint4 s_pre[LARGE_UNIT];
-    __shared__ uint64_t s_prez[LARGE_UNIT];
+    __shared__ double sfold[FB ? 1 : HTL];
+    __shared__ uint4 s_pre[HUGE_UNIT];
+    __shared__ uint64_t s_prez[HUGE_UNIT];
+    __shared__ __attribute__((aligned(16))) uint32_t s_filt[NF];
+    __shared__ uint2 s_ring[FB ? (HTL / WAVE) * RING : 1];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
-        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= LARGE_UNIT in this class
+        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= HUGE_UNIT in this class
+        if constexpr (FB > 0) {  // the previous unit's probes ended at the barrier below
+            for (uint32_t i = threadIdx.x; i < NF; i += HTL) s_filt[i] = 0u;
+            __syncthreads();
+        }
         if (threadIdx.x < cnt) {
             const uint32_t li = own_idx[k0 + threadIdx.x];
             const uint2 yo = g.ndo[oh + li];
@@ -1346,7 +1508,20 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
             s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
         }
         const uint32_t sh = oh & 7u, ob = oh - sh;  // dh + sh <= L
-        for (uint32_t i = threadIdx.x; i < L / 8; i += HTL) s_l1[i] = huge_id(g, ob, sh, dh, 8 * i + 7);
+        if constexpr (FB > 0) {
+            // every entry into the filter; position s = sh + j of the shifted
+            // sequence is pivot s / 8 of the lowest level when s % 8 == 7
+            for (uint32_t j = threadIdx.x; j < dh; j += HTL) {
+                const uint32_t z = g.nb[oh + j];
+                filt_set<FB>(s_filt, z);
+                if (((sh + j) & 7u) == 7u) s_l1[(sh + j) >> 3] = z;
+            }
+            for (uint32_t i = ((sh + dh) >> 3) + threadIdx.x; i < L / 8; i += HTL)
+                s_l1[i] = PAD_ID;  // 8 i + 7 >= sh + dh: past the list
+        } else {
+            for (uint32_t i = threadIdx.x; i < L / 8; i += HTL)
+                s_l1[i] = huge_id(g, ob, sh, dh, 8 * i + 7);
+        }
         for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
             s_l2[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
         for (uint32_t i = threadIdx.x; i < L / 512; i += HTL)
@@ -1357,9 +1532,14 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
             pr.top.p[i] = huge_id(g, ob, sh, dh, (L / 8) * i + L / 8 - 1);
         __syncthreads();
         const HubMeta<false, false> meta{nullptr, nullptr, nullptr, nullptr, oh};
-        hub_edges<G, HugeProbe, HubMeta<false, false>, true>(
-            g, tb, dh, pr, meta, nullptr, sfold + wid * WAVE, wid * per_wave, cnt,
-            per_wave * (HTL / WAVE), s_pre, s_prez);
+        if constexpr (FB > 0)
+            hub_edges_filt<FB, HugeProbe, HubMeta<false, false>, true>(
+                g, tb, dh, pr, meta, nullptr, s_filt, s_ring + wid * RING, wid, cnt,
+                HTL / WAVE, s_pre, s_prez);
+        else
+            hub_edges<G, HugeProbe, HubMeta<false, false>, true>(
+                g, tb, dh, pr, meta, nullptr, sfold + wid * WAVE, wid * per_wave, cnt,
+                per_wave * (HTL / WAVE), s_pre, s_prez);
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -1606,14 +1786,18 @@ template <int G>
 void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
                      const TermBuf& tb, uint32_t maxdh, hipStream_t s) {
     const uint32_t lcap = (std::min<uint32_t>(maxdh, 4096) + 8 + 15) & ~15u;
-    k_count_hub_large<G><<<std::min<uint32_t>(nu, 4096), HTL, 3ull * lcap * 4, s>>>(
+    constexpr int FB = G == WAVE ? LARGE_FB : 0;
+    const uint64_t arrays = large_slot_meta(FB) ? 3 : 1;
+    k_count_hub_large<G, FB><<<std::min<uint32_t>(nu, 4096), HTL, arrays * lcap * 4, s>>>(
         g, units, nu, own_idx, tb, lcap);
 }
 
 template <int G>
 void launch_hub_huge(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
                      const TermBuf& tb, hipStream_t s) {
-    k_count_hub_huge<G><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu, own_idx, tb);
+    constexpr int FB = G == WAVE ? HUGE_FB : 0;
+    k_count_hub_huge<G, FB><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu, own_idx,
+                                                                        tb);
 }
 
 template <int L>

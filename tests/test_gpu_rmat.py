@@ -3,7 +3,8 @@
 ~5.9k) against the CPU oracle.  Its hubs span every count class: lists staged
 whole in LDS (degree <= 4096) and the huge-hub kernel above (upper tree levels
 in LDS, leaf blocks read from the hub's list), with the default plan and with
-a plan that sends every hub above degree 64 to the huge-hub kernel.  Counts
+a plan that sends every hub above degree 64 to the huge-hub kernel, and with
+both at 64 lanes per edge for every hub (their filtered paths).  Counts
 and FP64 supports bit for bit, survivors, clusters and stats exactly.  At eps
 0.3 the clean removes every edge (R-MAT has little clustering); eps 0.05 and
 0.02 leave 132k and 347k survivors and non-trivial clusters."""
@@ -21,6 +22,9 @@ pytestmark = pytest.mark.gpu
 PLANS = {
     "default": {},
     "huge_above_64": {"l1_max": 64, "l2_max": 64},
+    # 64 lanes per edge for every hub: the filtered large- and huge-hub paths
+    "large_64_lanes": {"l1_max": 64, "g_thr": [0, 0, 0, 0]},
+    "huge_64_lanes": {"l1_max": 64, "l2_max": 64, "g_thr": [0, 0, 0, 0]},
 }
 
 _graph = {}

@@ -30,6 +30,10 @@ SETTINGS = [
     {"hub_min": 0, "hub_max": 32760, "l0_max": 0, "l1_max": 0, "l2_max": 0},
     {"hub_min": 2, "hub_max": 32760, "l0_max": 8, "l1_max": 32, "l2_max": 64,
      "g_thr": [2, 4, 8, 16]},
+    # 64 lanes per edge: the filtered paths of the 4096-entry and huge-hub kernels
+    {"hub_min": 1, "hub_max": 4096, "l0_max": 0, "l1_max": 0, "g_thr": [0, 0, 0, 0]},
+    {"hub_min": 0, "hub_max": 32760, "l0_max": 0, "l1_max": 0, "l2_max": 0,
+     "g_thr": [0, 0, 0, 0]},
 ]
 
 
