@@ -245,9 +245,9 @@ rtrex_status rtrex_get_timings(rtrex_graph g, rtrex_timings* out);
 /* An edge whose longer-list endpoint (its hub) has degree in                 */
 /* (hub_min, hub_max] is counted per hub with the hub's list staged in LDS:   */
 /* a sorted list of up to 64 entries (degree <= l0_max), 512 (<= l1_max) or   */
-/* 4096 (<= l2_max) with an 8-ary pivot tree; above l2_max (hub_max <= 32760)  */
-/* the tree's upper levels are staged and its leaf is the list in device      */
-/* memory; other edges are counted per edge.                                   */
+/* 4096 (<= l2_max) with an 8-ary pivot tree; above l2_max (hub_max <=       */
+/* 262136) the tree's upper levels are staged and its leaf is the list in     */
+/* device memory; other edges are counted per edge.                            */
 /* With stored triangle lists, hubs up to l0_max are counted in batches of    */
 /* whole hubs (units of at least unit_min streamed elements, grouped into     */
 /* chunks of about 1,024), and hubs up to l2_max whose edges' shorter lists   */
@@ -261,6 +261,8 @@ typedef struct {
     int32_t g_thr[4];
     int32_t unit_min, unit_hash_mult;
     int32_t l2_max; /* l1_max <= l2_max <= 4096 */
+    int32_t l3_max; /* l2_max <= l3_max <= 32760: hubs above take a tree of */
+                    /* five levels instead of four (up to 262136 entries)   */
 } rtrex_count_params;
 
 rtrex_status rtrex_get_count_params(rtrex_graph g, rtrex_count_params* out);

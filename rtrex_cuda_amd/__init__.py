@@ -79,7 +79,7 @@ class Timings(C.Structure):
 class CountParams(C.Structure):
     _fields_ = [("hub_min", C.c_int32), ("hub_max", C.c_int32), ("l0_max", C.c_int32),
                 ("l1_max", C.c_int32), ("g_thr", C.c_int32 * 4), ("unit_min", C.c_int32),
-                ("unit_hash_mult", C.c_int32), ("l2_max", C.c_int32)]
+                ("unit_hash_mult", C.c_int32), ("l2_max", C.c_int32), ("l3_max", C.c_int32)]
 
     def as_dict(self) -> dict:
         d = {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "g_thr"}

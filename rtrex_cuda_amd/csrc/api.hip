@@ -710,16 +710,20 @@ rtrex_count_params to_abi(const CountParams& c) {
     p.unit_min = (int32_t)c.unit_min;
     p.unit_hash_mult = (int32_t)c.unit_hash_mult;
     p.l2_max = (int32_t)c.l2_max;
+    p.l3_max = (int32_t)c.l3_max;
     return p;
 }
 
 rtrex_status from_abi(const rtrex_count_params& p, CountParams* c) {
     if (p.hub_min < 0 || p.hub_min >= p.hub_max || p.hub_max > (int32_t)HUB_MAX_LIMIT)
-        return fail(RTREX_EINVAL, "need 0 <= hub_min < hub_max <= 32760");
+        return fail(RTREX_EINVAL,
+                    "need 0 <= hub_min < hub_max <= " + std::to_string(HUB_MAX_LIMIT));
     if (p.l0_max < 0 || p.l0_max > 64 || p.l1_max < p.l0_max || p.l1_max > 512)
         return fail(RTREX_EINVAL, "need 0 <= l0_max <= 64 and l0_max <= l1_max <= 512");
     if (p.l2_max < p.l1_max || p.l2_max > 4096)
         return fail(RTREX_EINVAL, "need l1_max <= l2_max <= 4096");
+    if (p.l3_max < p.l2_max || p.l3_max > 32760)
+        return fail(RTREX_EINVAL, "need l2_max <= l3_max <= 32760");
     for (int i = 0; i < 4; ++i)
         if (p.g_thr[i] < 0 || (i && p.g_thr[i] < p.g_thr[i - 1]))
             return fail(RTREX_EINVAL, "g_thr must be non-negative and non-decreasing");
@@ -733,6 +737,7 @@ rtrex_status from_abi(const rtrex_count_params& p, CountParams* c) {
     c->unit_min = (uint32_t)p.unit_min;
     c->unit_hash_mult = (uint32_t)p.unit_hash_mult;
     c->l2_max = (uint32_t)p.l2_max;
+    c->l3_max = (uint32_t)p.l3_max;
     return RTREX_OK;
 }
 
@@ -813,7 +818,7 @@ rtrex_status rtrex_autotune_count(rtrex_graph g, int repeats, rtrex_count_params
                 for (uint32_t v : {4u, 8u, 16u}) { c.unit_hash_mult = v; cand.push_back(c); }
                 break;
             default:
-                for (uint32_t v : {1024u, 4096u, HUB_MAX_LIMIT}) {
+                for (uint32_t v : {1024u, 4096u, 32760u, HUB_MAX_LIMIT}) {
                     c.hub_max = v;
                     if (c.hub_min < v) cand.push_back(c);
                 }

@@ -62,6 +62,13 @@ constexpr int HUB_NL = 4, HUB_NG = 5;  // staging classes 64/512/4096/32768; G =
 // its units are larger (R-MAT scale 21 count: 123 -> 110 ms with 128 instead
 // of 32); 32 keeps the few large hubs of planted graphs spread over the CUs.
 constexpr uint32_t LARGE_UNIT = 32, HUGE_UNIT = 128;
+// A unit's w field: d_hub in the low UNIT_DH_BITS bits (up to HUB_MAX_LIMIT),
+// its owned-edge count above (at most d_hub, and at most 4096 below the huge
+// class, whose units hold at most HUGE_UNIT).
+constexpr uint32_t UNIT_DH_BITS = 18, UNIT_DH_MASK = (1u << UNIT_DH_BITS) - 1u;
+static_assert(HUB_MAX_LIMIT <= UNIT_DH_MASK && HUGE_UNIT < (1u << (32 - UNIT_DH_BITS)) &&
+                  4096u < (1u << (32 - UNIT_DH_BITS)),
+              "a unit's degree and edge count fit its w field");
 __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
     return dhub > P.hub_min && dhub <= P.hub_max;
 }
@@ -362,7 +369,8 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
         for (uint64_t p = 0; p < P; ++p) {
             const uint32_t k0 = o0 + (uint32_t)((uint64_t)oc * p / P);
             const uint32_t k1 = o0 + (uint32_t)((uint64_t)oc * (p + 1) / P);
-            units[cls_base[c] + lb[c] + at + p] = make_uint4(h, oh, k0, dh | ((k1 - k0) << 16));
+            units[cls_base[c] + lb[c] + at + p] =
+                make_uint4(h, oh, k0, dh | ((k1 - k0) << UNIT_DH_BITS));
         }
     }
 }
@@ -376,15 +384,14 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
 constexpr int HT = 256;
 
 // Hub-path work unit: part of one hub's owned edges.  x = hub, y = offset of
-// its list, z = first own_idx entry, w = d_hub | (edge count << 16) (d_hub <=
-// 32768, hub_max's limit; at most 512 edges per unit).
+// its list, z = first own_idx entry, w = d_hub | (edge count << UNIT_DH_BITS).
 __device__ __forceinline__ void unit_fields(const uint4& U, uint32_t& h, uint32_t& oh,
                                             uint32_t& k0, uint32_t& dh, uint32_t& cnt) {
     h = U.x;
     oh = U.y;
     k0 = U.z;
-    dh = U.w & 0xffffu;
-    cnt = U.w >> 16;
+    dh = U.w & UNIT_DH_MASK;
+    cnt = U.w >> UNIT_DH_BITS;
 }
 
 // Probe structure of a hub list staged in LDS: an 8-ary search tree.  The
@@ -913,7 +920,8 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
             // values are read with readlane so that they live in scalar
             // registers and the loops below stay uniform.)
             const uint4 U = (uint32_t)lane < u1 - s ? Un : make_uint4(0u, 0u, 0u, 0u);
-            const uint32_t udh = U.w & 0xffffu, ucnt = U.w >> 16, upad = (udh + 7u) & ~7u;
+            const uint32_t udh = U.w & UNIT_DH_MASK, ucnt = U.w >> UNIT_DH_BITS;
+            const uint32_t upad = (udh + 7u) & ~7u;
             const uint32_t pc = wave_incl_scan(ucnt, lane), ph = wave_incl_scan(upad, lane);
             const bool fit = (uint32_t)lane < u1 - s && pc <= S_OE && ph <= S_HS;
             const uint32_t nb = (uint32_t)__popcll(ballot64(fit));
@@ -953,7 +961,7 @@ __global__ void __launch_bounds__(SW, 5) k_count_small(GraphDev g, const uint4* 
             const int jk = __popcll(kst & upto) - 1;
             const uint32_t k0 = __shfl(U.z, jk, 64), cb = __shfl(ucb, jk, 64);
             const uint32_t koh = __shfl(U.y, jk, 64), kdh = __shfl(udh, jk, 64);
-            const uint32_t khb = __shfl(uhb, jk, 64), kpad = __shfl(upad, jk, 64);
+            const uint32_t khb = __shfl(uhb, jk, 64);
             const uint32_t li = ko ? own_idx[k0 + (uint32_t)lane - cb] : 0u;
             // The previous batch's last matches and results, while this
             // batch's staging loads are in flight (its LDS is overwritten
@@ -1431,11 +1439,37 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
 // tree indexes the list from the 32-byte boundary at or before its start
 // (sh = oh mod 8 entries earlier): those sh entries read as 0, so the sequence
 // stays non-decreasing, and are never a match.
-static_assert(HUB_MAX_LIMIT + 7 < 32768, "a huge hub's list and its shift fit the tree");
+//
+// Hubs above HUGE_TREE_MAX entries (R-MAT's largest) take a five-level tree
+// over 262144 positions (MegaProbe): levels of every 64th, 512th and 4096th
+// entry in the same LDS arrays, the top 8 in registers, then the 8 pivots of
+// every 8th entry inside the chosen 64-entry stretch are read from the list
+// itself (eight 4-byte loads, two lines), then the leaf block.
+constexpr uint32_t HUGE_TREE_MAX = 32760;
+static_assert(HUGE_TREE_MAX + 7 < 32768 && HUB_MAX_LIMIT + 7 < 262144,
+              "a huge hub's list and its shift fit its tree");
 static_assert(HUGE_UNIT <= HTL, "a unit's owned edges are prefetched one per thread");
 __device__ __forceinline__ uint32_t huge_id(const GraphDev& g, uint32_t ob, uint32_t sh,
                                             uint32_t dh, uint32_t s) {
     return s < sh ? 0u : s < sh + dh ? g.nb[ob + s] : PAD_ID;
+}
+// Leaf block b of a huge hub's shifted list: positions 8b .. 8b + 7, read
+// from memory as they are (the entries before sh belong to the previous list,
+// those from sh + dh on to the next one); a match counts only inside the list,
+// k in [kmin, kend).  True and pos = z's index in the list when found.
+__device__ __forceinline__ bool huge_leaf(const uint4* blk, uint32_t sh, uint32_t dh, uint32_t b,
+                                          uint32_t z, uint32_t& pos) {
+    const uint4 a = blk[2 * b], c = blk[2 * b + 1];
+    const uint32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+    const uint32_t base = 8 * b;
+    const uint32_t kmin = base >= sh ? 0u : sh - base;
+    const uint32_t kend = sh + dh > base ? min(8u, sh + dh - base) : 0u;
+    int j = -1;
+#pragma unroll
+    for (int k = 7; k >= 0; --k)
+        if (v[k] == z && (uint32_t)k >= kmin && (uint32_t)k < kend) j = k;
+    pos = base + (uint32_t)max(j, 0) - sh;
+    return j >= 0;
 }
 struct HugeProbe {
     const uint4* blk;  // the list's 32-byte blocks, from its aligned base
@@ -1454,21 +1488,40 @@ struct HugeProbe {
         b = 8 * b + min(below8(q[0], q[1], z), 7u);
         q = reinterpret_cast<const uint4*>(l1) + 2 * b;
         b = 8 * b + min(below8(q[0], q[1], z), 7u);
-        // leaf block b: positions 8b .. 8b + 7 of the shifted sequence, read
-        // from memory as they are (the entries before sh belong to the
-        // previous list, those from sh + dh on to the next one); a match
-        // counts only inside the list, k in [kmin, kend)
-        const uint4 a = blk[2 * b], c = blk[2 * b + 1];
-        const uint32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
-        const uint32_t base = 8 * b;
-        const uint32_t kmin = base >= sh ? 0u : sh - base;
-        const uint32_t kend = sh + dh > base ? min(8u, sh + dh - base) : 0u;
-        int j = -1;
+        return ok && huge_leaf(blk, sh, dh, b, z, pos);
+    }
+};
+
+struct MegaProbe {
+    const uint4* blk;    // the list's 32-byte blocks, from its aligned base
+    const uint32_t* nb;  // the list's entries, from the same base
+    uint32_t sh, dh;
+    const uint32_t *l2, *l3, *l4;  // every 64th, 512th and 4096th position
+    Piv8 top;                      // every 32768th
+    __device__ __forceinline__ bool operator()(uint32_t z, uint32_t& pos) const {
+        uint32_t b = 0;
 #pragma unroll
-        for (int k = 7; k >= 0; --k)
-            if (v[k] == z && (uint32_t)k >= kmin && (uint32_t)k < kend) j = k;
-        pos = base + (uint32_t)max(j, 0) - sh;
-        return ok && j >= 0;
+        for (int i = 0; i < 8; ++i) b += top.p[i] < z ? 1u : 0u;
+        const bool ok = b < 8;
+        b = min(b, 7u);
+        const uint4* q = reinterpret_cast<const uint4*>(l4) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        q = reinterpret_cast<const uint4*>(l3) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        q = reinterpret_cast<const uint4*>(l2) + 2 * b;
+        b = 8 * b + min(below8(q[0], q[1], z), 7u);
+        // positions 64 b + 8 k + 7 (never below sh <= 7): the list's entry,
+        // PAD_ID past its end (the load clamped into the list)
+        const uint32_t end = sh + dh;
+        uint32_t nbelow = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t s = 64 * b + 8 * k + 7;
+            const uint32_t x = nb[min(s, end - 1u)];
+            nbelow += (s < end && x < z) ? 1u : 0u;
+        }
+        b = 8 * b + min(nbelow, 7u);
+        return ok && huge_leaf(blk, sh, dh, b, z, pos);
     }
 };
 
@@ -1480,7 +1533,7 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
                                                         const uint4* __restrict__ units,
                                                         uint32_t nunits,
                                                         const uint32_t* __restrict__ own_idx,
-                                                        TermBuf tb) {
+                                                        TermBuf tb, uint32_t tree_max) {
     static_assert(FB == 0 || G == WAVE, "the filtered path runs one edge per wave");
     constexpr uint32_t L = 32768;
     constexpr uint32_t NF = FB ? (1u << FB) / 32 : 1;
@@ -1507,39 +1560,62 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
             s_pre[threadIdx.x] = make_uint4(g.nbe[oh + li].y, yo.x, yo.y, 0u);
             s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
         }
-        const uint32_t sh = oh & 7u, ob = oh - sh;  // dh + sh <= L
-        if constexpr (FB > 0) {
-            // every entry into the filter; position s = sh + j of the shifted
-            // sequence is pivot s / 8 of the lowest level when s % 8 == 7
-            for (uint32_t j = threadIdx.x; j < dh; j += HTL) {
-                const uint32_t z = g.nb[oh + j];
-                filt_set<FB>(s_filt, z);
-                if (((sh + j) & 7u) == 7u) s_l1[(sh + j) >> 3] = z;
+        const uint32_t sh = oh & 7u, ob = oh - sh;
+        // the five-level tree (no filter) above tree_max <= HUGE_TREE_MAX
+        const bool mega = dh > tree_max;
+        Piv8 top;
+        if (!mega) {  // dh + sh <= L
+            if constexpr (FB > 0) {
+                // every entry into the filter; position s = sh + j of the
+                // shifted sequence is pivot s / 8 of the lowest level when
+                // s % 8 == 7
+                for (uint32_t j = threadIdx.x; j < dh; j += HTL) {
+                    const uint32_t z = g.nb[oh + j];
+                    filt_set<FB>(s_filt, z);
+                    if (((sh + j) & 7u) == 7u) s_l1[(sh + j) >> 3] = z;
+                }
+                for (uint32_t i = ((sh + dh) >> 3) + threadIdx.x; i < L / 8; i += HTL)
+                    s_l1[i] = PAD_ID;  // 8 i + 7 >= sh + dh: past the list
+            } else {
+                for (uint32_t i = threadIdx.x; i < L / 8; i += HTL)
+                    s_l1[i] = huge_id(g, ob, sh, dh, 8 * i + 7);
             }
-            for (uint32_t i = ((sh + dh) >> 3) + threadIdx.x; i < L / 8; i += HTL)
-                s_l1[i] = PAD_ID;  // 8 i + 7 >= sh + dh: past the list
-        } else {
-            for (uint32_t i = threadIdx.x; i < L / 8; i += HTL)
-                s_l1[i] = huge_id(g, ob, sh, dh, 8 * i + 7);
-        }
-        for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
-            s_l2[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
-        for (uint32_t i = threadIdx.x; i < L / 512; i += HTL)
-            s_l3[i] = huge_id(g, ob, sh, dh, 512 * i + 511);
-        HugeProbe pr{reinterpret_cast<const uint4*>(g.nb + ob), sh, dh, s_l1, s_l2, s_l3, {}};
+            for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
+                s_l2[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
+            for (uint32_t i = threadIdx.x; i < L / 512; i += HTL)
+                s_l3[i] = huge_id(g, ob, sh, dh, 512 * i + 511);
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            pr.top.p[i] = huge_id(g, ob, sh, dh, (L / 8) * i + L / 8 - 1);
+            for (int i = 0; i < 8; ++i) top.p[i] = huge_id(g, ob, sh, dh, (L / 8) * i + L / 8 - 1);
+        } else {  // dh + sh <= 8 L: every 64th, 512th, 4096th, 32768th position
+            for (uint32_t i = threadIdx.x; i < L / 8; i += HTL)
+                s_l1[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
+            for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
+                s_l2[i] = huge_id(g, ob, sh, dh, 512 * i + 511);
+            for (uint32_t i = threadIdx.x; i < L / 512; i += HTL)
+                s_l3[i] = huge_id(g, ob, sh, dh, 4096 * i + 4095);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) top.p[i] = huge_id(g, ob, sh, dh, L * i + L - 1);
+        }
         __syncthreads();
         const HubMeta<false, false> meta{nullptr, nullptr, nullptr, nullptr, oh};
-        if constexpr (FB > 0)
-            hub_edges_filt<FB, HugeProbe, HubMeta<false, false>, true>(
-                g, tb, dh, pr, meta, nullptr, s_filt, s_ring + wid * RING, wid, cnt,
-                HTL / WAVE, s_pre, s_prez);
-        else
-            hub_edges<G, HugeProbe, HubMeta<false, false>, true>(
-                g, tb, dh, pr, meta, nullptr, sfold + wid * WAVE, wid * per_wave, cnt,
-                per_wave * (HTL / WAVE), s_pre, s_prez);
+        const uint4* blk = reinterpret_cast<const uint4*>(g.nb + ob);
+        double* fold = FB ? nullptr : sfold + wid * WAVE;  // G < 64 only
+        if (mega) {
+            const MegaProbe pr{blk, g.nb + ob, sh, dh, s_l1, s_l2, s_l3, top};
+            hub_edges<G, MegaProbe, HubMeta<false, false>, true>(
+                g, tb, dh, pr, meta, nullptr, fold, wid * per_wave, cnt, per_wave * (HTL / WAVE),
+                s_pre, s_prez);
+        } else {
+            const HugeProbe pr{blk, sh, dh, s_l1, s_l2, s_l3, top};
+            if constexpr (FB > 0)
+                hub_edges_filt<FB, HugeProbe, HubMeta<false, false>, true>(
+                    g, tb, dh, pr, meta, nullptr, s_filt, s_ring + wid * RING, wid, cnt,
+                    HTL / WAVE, s_pre, s_prez);
+            else
+                hub_edges<G, HugeProbe, HubMeta<false, false>, true>(
+                    g, tb, dh, pr, meta, nullptr, fold, wid * per_wave, cnt,
+                    per_wave * (HTL / WAVE), s_pre, s_prez);
+        }
         __syncthreads();  // the next unit restages the hub
     }
 }
@@ -1794,10 +1870,10 @@ void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const 
 
 template <int G>
 void launch_hub_huge(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
-                     const TermBuf& tb, hipStream_t s) {
+                     const TermBuf& tb, uint32_t tree_max, hipStream_t s) {
     constexpr int FB = G == WAVE ? HUGE_FB : 0;
     k_count_hub_huge<G, FB><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu, own_idx,
-                                                                        tb);
+                                                                        tb, tree_max);
 }
 
 template <int L>
@@ -1861,12 +1937,13 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 3) {
+            const uint32_t tl3 = std::min(h->cparams.l3_max, HUGE_TREE_MAX);
             switch (gc) {
-                case 0: launch_hub_huge<4>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_huge<8>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_huge<16>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_huge<32>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                default: launch_hub_huge<64>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 0: launch_hub_huge<4>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 1: launch_hub_huge<8>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 2: launch_hub_huge<16>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 3: launch_hub_huge<32>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                default: launch_hub_huge<64>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
             }
         } else {
             const uint32_t mx = h->hub_umax[c];
@@ -1905,7 +1982,7 @@ __global__ void __launch_bounds__(T) k_small_keys(GraphDev g, const uint4* __res
                                                   uint32_t nu, uint64_t* __restrict__ keys) {
     for (uint32_t u = blockIdx.x * T + threadIdx.x; u < nu; u += gridDim.x * T) {
         const uint4 U = units[u];
-        const uint32_t dh = U.w & 0xffffu;
+        const uint32_t dh = U.w & UNIT_DH_MASK;
         uint32_t m = mix32(U.x);
         for (uint32_t i = 0; i < dh; ++i) m = min(m, mix32(g.nb[U.y + i]));
         keys[u] = ((uint64_t)u << 32) | (m & 0xfffffu);
@@ -1925,7 +2002,7 @@ __global__ void __launch_bounds__(T) k_small_work(GraphDev g, const uint4* __res
                                                   uint32_t* __restrict__ w) {
     for (uint32_t u = blockIdx.x * T + threadIdx.x; u < nu; u += gridDim.x * T) {
         const uint4 U = units[u];
-        const uint32_t cnt = U.w >> 16;
+        const uint32_t cnt = U.w >> UNIT_DH_BITS;
         uint32_t sum = 0;
         for (uint32_t i = 0; i < cnt; ++i) sum += g.ndo[U.y + own_idx[U.z + i]].x;
         w[u] = sum;

@@ -150,9 +150,9 @@ struct AuditHost {
     int64_t components = 0, largest_component = 0;
 };
 
-// Largest hub degree the count kernels accept (the huge-hub tree spans
-// 32768 entries from the 8-entry boundary at or before the list's start).
-constexpr uint32_t HUB_MAX_LIMIT = 32760;
+// Largest hub degree the count kernels accept (the largest hubs' tree spans
+// 262144 entries from the 8-entry boundary at or before the list's start).
+constexpr uint32_t HUB_MAX_LIMIT = 262136;
 
 // Count-plan parameters (the autotuner's search space).  An edge whose
 // longer-list endpoint (its hub) has degree in (hub_min, hub_max] is counted
@@ -165,6 +165,7 @@ constexpr uint32_t HUB_MAX_LIMIT = 32760;
 struct CountParams {
     uint32_t hub_min = 0, hub_max = HUB_MAX_LIMIT;
     uint32_t l0_max = 64, l1_max = 512, l2_max = 4096;
+    uint32_t l3_max = 32760;  // above: the huge-hub tree of five levels
     uint32_t g_thr[4] = {3, 6, 24, 80};
     uint32_t unit_min = 1024, unit_hash_mult = 8;
 };

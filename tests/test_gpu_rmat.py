@@ -4,7 +4,8 @@
 whole in LDS (degree <= 4096) and the huge-hub kernel above (upper tree levels
 in LDS, leaf blocks read from the hub's list), with the default plan and with
 a plan that sends every hub above degree 64 to the huge-hub kernel, and with
-both at 64 lanes per edge for every hub (their filtered paths).  Counts
+both at 64 lanes per edge for every hub (their filtered paths), and through
+the five-level tree the hubs above degree 32760 take.  Counts
 and FP64 supports bit for bit, survivors, clusters and stats exactly.  At eps
 0.3 the clean removes every edge (R-MAT has little clustering); eps 0.05 and
 0.02 leave 132k and 347k survivors and non-trivial clusters."""
@@ -25,6 +26,9 @@ PLANS = {
     # 64 lanes per edge for every hub: the filtered large- and huge-hub paths
     "large_64_lanes": {"l1_max": 64, "g_thr": [0, 0, 0, 0]},
     "huge_64_lanes": {"l1_max": 64, "l2_max": 64, "g_thr": [0, 0, 0, 0]},
+    # every hub above degree 64 through the five-level tree of the largest hubs
+    "five_levels_above_64": {"l1_max": 64, "l2_max": 64, "l3_max": 64},
+    "five_levels_64_lanes": {"l1_max": 64, "l2_max": 64, "l3_max": 64, "g_thr": [0, 0, 0, 0]},
 }
 
 _graph = {}

@@ -34,6 +34,11 @@ SETTINGS = [
     {"hub_min": 1, "hub_max": 4096, "l0_max": 0, "l1_max": 0, "g_thr": [0, 0, 0, 0]},
     {"hub_min": 0, "hub_max": 32760, "l0_max": 0, "l1_max": 0, "l2_max": 0,
      "g_thr": [0, 0, 0, 0]},
+    # all hubs through the five-level tree (levels from every 64th entry up in
+    # LDS, the 8th-entry pivots and the leaf read from the list)
+    {"hub_min": 0, "hub_max": 262136, "l0_max": 0, "l1_max": 0, "l2_max": 0, "l3_max": 0},
+    {"hub_min": 0, "hub_max": 262136, "l0_max": 0, "l1_max": 0, "l2_max": 0, "l3_max": 0,
+     "g_thr": [0, 0, 0, 0]},
 ]
 
 
@@ -56,7 +61,8 @@ def test_supports_do_not_depend_on_parameters(i):
 def test_invalid_parameters_rejected():
     n, u, v = golden_graph("small_test")
     with R.DeviceGraph.from_edges(n, u, v) as g:
-        for bad in ({"hub_max": 40000}, {"l0_max": 65}, {"l1_max": 600},
+        for bad in ({"hub_max": 262137}, {"l3_max": 32761}, {"l2_max": 512, "l3_max": 256},
+                    {"l0_max": 65}, {"l1_max": 600},
                     {"l2_max": 5000}, {"l1_max": 256, "l2_max": 128},
                     {"g_thr": [5, 4, 3, 2]}, {"hub_min": 100, "hub_max": 50}):
             with pytest.raises(R.RtrexError) as e:
