@@ -432,7 +432,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="C5 serving leg: length of the request trace (0 skips it)")
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--serve-capacity", default="260,320",
+    ap.add_argument("--serve-capacity", default="300,360",
                     help="C5 capacity sweep: comma-separated Poisson rates (req/s) run for 10 s "
                          "each after the main trace ('' skips it)")
     ap.add_argument("--serve-budget-gb", type=float, default=128.0,
