@@ -157,6 +157,30 @@ rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords);
 rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
                                     int64_t k, double eps);
 
+/* One seed's inner loop on the current state (after rtrex_initial_clean and
+ * any rtrex_delete_and_clean calls, before rtrex_extract): the two-hop
+ * candidates of seed u in ascending id with their triangle weights (the
+ * first max_cand written; *n_cand = how many there are), the internal and
+ * the potential weight.  Replaces gpuExtractInner (gpu_extract.cuh:28-37),
+ * with the sequential reference's loop (Decomposition.h:728-827): pairs of
+ * alive neighbours v, w with slot(w) >= slot(v), both of degree at most
+ * d_u / eps; weights added in that loop's order.  A seed without alive
+ * neighbours returns no candidates and zero weights.  The statuses are left
+ * unchanged. */
+rtrex_status rtrex_extract_inner(rtrex_graph g, int64_t u, double eps, int64_t* candidates,
+                                 double* cand_wgt, int64_t max_cand, int64_t* n_cand,
+                                 double* internal, double* potential);
+
+/* The (member, neighbour) pairs that delete a cluster's alive edges, for
+ * rtrex_delete_and_clean: members in the order given (the reference iterates
+ * its unordered_set), each member's alive slots in order, an edge listed from
+ * its first member only (Decomposition.h:897-914).  pairs[2 * max_pairs]
+ * receives the first max_pairs; *n_pairs = how many there are.  Replaces
+ * gpuMarkClusterEdges (gpu_extract.cuh:53-59) without marking: the statuses
+ * are left unchanged, rtrex_delete_and_clean marks and cascades. */
+rtrex_status rtrex_cluster_edges(rtrex_graph g, const int64_t* verts, int64_t n_verts,
+                                 int64_t* pairs, int64_t max_pairs, int64_t* n_pairs);
+
 /* Greedy extraction after rtrex_initial_clean.  Replaces disjointExtractCUDA
  * (RTRex_CUDA/main.cu:130-338) / disjointExtract (Decomposition.h:538-940).
  * cluster_map[n] receives the reference's cluster id of every vertex (ids in

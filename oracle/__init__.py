@@ -49,9 +49,14 @@ def lib() -> C.CDLL:
         L.oracle_connected_components.argtypes = [C.c_int64, p64, p64, p64, p64]
         L.oracle_extract_state.argtypes = [C.c_int64, p64, p64, C.c_double, pd, pd, p64, pu8,
                                            p64, p64]
+        L.oracle_extract_inner.argtypes = [C.c_int64, p64, p64, pu8, C.c_int64, C.c_double, p64,
+                                           pd, C.c_int64, p64, pd, pd]
+        L.oracle_cluster_edges.argtypes = [C.c_int64, p64, p64, pu8, p64, C.c_int64, p64,
+                                           C.c_int64, p64]
         for f in (L.oracle_make_csr, L.oracle_count, L.oracle_initial_clean,
                   L.oracle_delete_and_clean, L.oracle_decompose,
-                  L.oracle_connected_components, L.oracle_extract_state):
+                  L.oracle_connected_components, L.oracle_extract_state,
+                  L.oracle_extract_inner, L.oracle_cluster_edges):
             f.restype = C.c_int
         _lib = L
     return _lib
@@ -203,6 +208,47 @@ def extract_from_state(off, nb, eps: float, state: Clean):
     if rc:
         raise ValueError(f"oracle_extract_state failed rc={rc}")
     return cm, int(nc[0])
+
+
+def extract_inner(off, nb, status, u: int, eps: float):
+    """The inner loop of seed u on the given slot statuses: (candidates in
+    ascending id, their weights, internal, potential)."""
+    n = len(off) - 1
+    off = np.ascontiguousarray(off, np.int64)
+    nb = np.ascontiguousarray(nb, np.int64)
+    st = np.ascontiguousarray(status, np.uint8)
+    cap = max(n, 1)
+    cand = np.zeros(cap, np.int64)
+    wgt = np.zeros(cap, np.float64)
+    nc = np.zeros(1, np.int64)
+    iw = np.zeros(2, np.float64)
+    rc = lib().oracle_extract_inner(n, _p(off, C.c_int64), _p(nb, C.c_int64), _p(st, C.c_uint8),
+                                    u, eps, _p(cand, C.c_int64), _p(wgt, C.c_double), cap,
+                                    _p(nc, C.c_int64), _p(iw[:1], C.c_double),
+                                    _p(iw[1:], C.c_double))
+    if rc:
+        raise ValueError(f"oracle_extract_inner failed rc={rc}")
+    k = int(nc[0])
+    return cand[:k].copy(), wgt[:k].copy(), float(iw[0]), float(iw[1])
+
+
+def cluster_edges(off, nb, status, verts):
+    """The deletion pairs of a cluster with members in the given order, on
+    the given slot statuses: shape (k, 2)."""
+    n = len(off) - 1
+    off = np.ascontiguousarray(off, np.int64)
+    nb = np.ascontiguousarray(nb, np.int64)
+    st = np.ascontiguousarray(status, np.uint8)
+    vs = np.ascontiguousarray(verts, np.int64)
+    cap = max(len(nb) // 2, 1)
+    out = np.zeros(2 * cap, np.int64)
+    k = np.zeros(1, np.int64)
+    rc = lib().oracle_cluster_edges(n, _p(off, C.c_int64), _p(nb, C.c_int64), _p(st, C.c_uint8),
+                                    _p(vs, C.c_int64), len(vs), _p(out, C.c_int64), cap,
+                                    _p(k, C.c_int64))
+    if rc:
+        raise ValueError(f"oracle_cluster_edges failed rc={rc}")
+    return out[:2 * int(k[0])].reshape(-1, 2).copy()
 
 
 def connected_components(off, nb):

@@ -220,6 +220,76 @@ struct Cand {
     double wgt;
 };
 
+// The deletion pairs of a cluster (Decomposition.h:897-914): members in the
+// given order (the reference iterates its unordered_set), each one's 'Y'
+// slots in order, both slots of a listed edge marked 'S'.
+void cluster_pairs(State& s, const std::vector<i64>& members, Work& todo) {
+    const Graph& g = s.g;
+    for (i64 z : members)
+        for (i64 j = g.off[z]; j < g.off[z + 1]; ++j) {
+            if (s.st[j] != 'Y') continue;
+            todo.emplace_back(z, g.nb[j]);
+            mark(s, j, 'S');
+        }
+}
+
+// The inner loop of one seed u (Decomposition.h:728-827, sequential branch):
+// every pair of alive neighbours v, w (slot k >= j, both of degree at most
+// d_u / eps) adds the triangle (u, v, w) to internal and enumerates the
+// triangles (v, w, x) over the lower-degree side's list; x a neighbour of u
+// (nbd) adds to internal when it is the largest of v, w, x, any other x is a
+// two-hop candidate collecting the weight in triwgt.  cand receives the
+// candidates in ascending id (std::set iteration) with their weights, and
+// potential their sum in that order; triwgt is left zero.
+void inner_loop(const State& s, i64 u, const std::vector<char>& nbd, std::vector<double>& triwgt,
+                double& internal, std::vector<Cand>& cand, double& potential) {
+    const Graph& g = s.g;
+    const i64 du = g.deg(u);
+    std::set<i64> two_hop;
+    const double lim = (double)du / s.eps;
+    for (i64 j = g.off[u]; j < g.off[u + 1]; ++j) {  // :728-805
+        if (s.st[j] == 'N') continue;
+        const i64 v = g.nb[j];
+        const i64 dv = g.deg(v);
+        if ((double)dv > lim) continue;
+        for (i64 k = j; k < g.off[u + 1]; ++k) {
+            if (s.st[k] == 'N') continue;
+            const i64 w = g.nb[k];
+            const i64 dw = g.deg(w);
+            if ((double)dw > lim) continue;
+            // The reference re-finds w in u's list (:754-756); with the
+            // sorted simple CSR that is slot k itself, already alive.
+            const i64 loc = g.slot(u, w);
+            if (loc < 0 || s.st[loc] == 'N') continue;
+            internal += tri_weight(du, dv, dw);
+            const i64 lo = dv < dw ? v : w;
+            const i64 hi = dv < dw ? w : v;
+            for (i64 l = g.off[lo]; l < g.off[lo + 1]; ++l) {
+                if (s.st[l] == 'N') continue;
+                const i64 x = g.nb[l];
+                if (x == lo || x == hi || x == u) continue;
+                const i64 lhx = g.slot(hi, x);
+                if (lhx < 0 || s.st[lhx] == 'N') continue;
+                const double t = tri_weight(g.deg(x), dv, dw);
+                if (nbd[x]) {
+                    if (x > v && x > w) internal += t;
+                    continue;
+                }
+                two_hop.insert(x);
+                triwgt[x] += t;
+            }
+        }
+    }
+    cand.clear();  // :811-827
+    cand.reserve(two_hop.size());
+    potential = 0;
+    for (i64 x : two_hop) {
+        cand.push_back({x, triwgt[x]});
+        potential += triwgt[x];
+        triwgt[x] = 0;
+    }
+}
+
 // disjointExtract, sequential inner-loop branch (Decomposition.h:538-940),
 // entered after count and initial clean.
 std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
@@ -240,7 +310,7 @@ std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
     i64 nclus = 0;
 
     for (i64 r = 0; r < n; ++r) {
-        const i64 u = order[r].first, du = order[r].second;
+        const i64 u = order[r].first;
         if (clustered[u]) continue;
         Cluster c;
         c.vertices.insert(u);
@@ -248,7 +318,6 @@ std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
         clustered[u] = 1;
         i64 joined = 0;  // order in which vertices join the cluster
         if (member_rank) member_rank[u] = joined;
-        std::set<i64> two_hop;
         double internal = 0;
 
         for (i64 j = g.off[u]; j < g.off[u + 1]; ++j) {  // :620-631
@@ -260,49 +329,9 @@ std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
             cluster_map[v] = nclus;
             clustered[v] = 1;
         }
-        const double lim = (double)du / s.eps;
-        for (i64 j = g.off[u]; j < g.off[u + 1]; ++j) {  // :728-805
-            if (s.st[j] == 'N') continue;
-            const i64 v = g.nb[j];
-            const i64 dv = g.deg(v);
-            if ((double)dv > lim) continue;
-            for (i64 k = j; k < g.off[u + 1]; ++k) {
-                if (s.st[k] == 'N') continue;
-                const i64 w = g.nb[k];
-                const i64 dw = g.deg(w);
-                if ((double)dw > lim) continue;
-                // The reference re-finds w in u's list (:754-756); with the
-                // sorted simple CSR that is slot k itself, already alive.
-                const i64 loc = g.slot(u, w);
-                if (loc < 0 || s.st[loc] == 'N') continue;
-                internal += tri_weight(du, dv, dw);
-                const i64 lo = dv < dw ? v : w;
-                const i64 hi = dv < dw ? w : v;
-                for (i64 l = g.off[lo]; l < g.off[lo + 1]; ++l) {
-                    if (s.st[l] == 'N') continue;
-                    const i64 x = g.nb[l];
-                    if (x == lo || x == hi || x == u) continue;
-                    const i64 lhx = g.slot(hi, x);
-                    if (lhx < 0 || s.st[lhx] == 'N') continue;
-                    const double t = tri_weight(g.deg(x), dv, dw);
-                    if (nbd[x]) {
-                        if (x > v && x > w) internal += t;
-                        continue;
-                    }
-                    two_hop.insert(x);
-                    triwgt[x] += t;
-                }
-            }
-        }
-
-        std::vector<Cand> cand;  // :811-827
-        cand.reserve(two_hop.size());
+        std::vector<Cand> cand;
         double potential = 0;
-        for (i64 x : two_hop) {
-            cand.push_back({x, triwgt[x]});
-            potential += triwgt[x];
-            triwgt[x] = 0;
-        }
+        inner_loop(s, u, nbd, triwgt, internal, cand, potential);
         std::sort(cand.begin(), cand.end(),
                   [](Cand a, Cand b) { return a.wgt > b.wgt; });
 
@@ -333,12 +362,7 @@ std::vector<Cluster> extract(State& s, i64* cluster_map, i64* member_rank) {
         nclus++;
 
         Work todo;  // :897-918
-        for (i64 z : c.vertices)
-            for (i64 j = g.off[z]; j < g.off[z + 1]; ++j) {
-                if (s.st[j] != 'Y') continue;
-                todo.emplace_back(z, g.nb[j]);
-                mark(s, j, 'S');
-            }
+        cluster_pairs(s, std::vector<i64>(c.vertices.begin(), c.vertices.end()), todo);
         delete_and_clean(s, todo);
     }
     return out;
@@ -462,6 +486,53 @@ int oracle_extract_state(int64_t n, const int64_t* offsets, const int64_t* nbors
     std::vector<Cluster> dec = extract(s, cluster_map, nullptr);
     *n_clusters = (i64)dec.size();
     *total = s.total;
+    return 0;
+}
+
+int oracle_extract_inner(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         const uint8_t* status, int64_t u, double eps, int64_t* cand,
+                         double* wgt, int64_t cap, int64_t* ncand, double* internal,
+                         double* potential) {
+    if (!valid(n, offsets, nbors) || u < 0 || u >= n || cap < 0) return 1;
+    Graph g{n, offsets, nbors};
+    State s(g);
+    s.eps = eps;
+    s.st.assign(status, status + offsets[n]);
+    std::vector<char> nbd(n, 0);
+    for (i64 j = g.off[u]; j < g.off[u + 1]; ++j)  // :620-631
+        if (s.st[j] != 'N') nbd[g.nb[j]] = 1;
+    std::vector<double> triwgt(n, 0.0);
+    std::vector<Cand> c;
+    double in = 0, pot = 0;
+    inner_loop(s, u, nbd, triwgt, in, c, pot);
+    for (i64 i = 0; i < (i64)c.size() && i < cap; ++i) {
+        cand[i] = c[i].vertex;
+        wgt[i] = c[i].wgt;
+    }
+    *ncand = (i64)c.size();
+    *internal = in;
+    *potential = pot;
+    return 0;
+}
+
+int oracle_cluster_edges(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         const uint8_t* status, const int64_t* verts, int64_t nverts,
+                         int64_t* pairs, int64_t cap, int64_t* npairs) {
+    if (!valid(n, offsets, nbors) || nverts < 0 || cap < 0) return 1;
+    Graph g{n, offsets, nbors};
+    State s(g);
+    s.part = partner_slots(g);
+    s.st.assign(status, status + offsets[n]);
+    std::vector<i64> members(verts, verts + nverts);
+    for (i64 z : members)
+        if (z < 0 || z >= n) return 1;
+    Work todo;
+    cluster_pairs(s, members, todo);
+    for (i64 i = 0; i < (i64)todo.size() && i < cap; ++i) {
+        pairs[2 * i] = todo[i].first;
+        pairs[2 * i + 1] = todo[i].second;
+    }
+    *npairs = (i64)todo.size();
     return 0;
 }
 

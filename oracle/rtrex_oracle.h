@@ -55,6 +55,24 @@ int oracle_extract_state(int64_t n, const int64_t* offsets, const int64_t* nbors
                          int64_t* total, const uint8_t* status, int64_t* cluster_map,
                          int64_t* n_clusters);
 
+/* The inner loop of one seed u (Decomposition.h:728-827) on the given slot
+ * statuses ('N' deleted): the two-hop candidates in ascending id with their
+ * triangle weights (the first cap of them written; *ncand = how many),
+ * internal and potential weight.  The same function runs inside
+ * oracle_extract_state and oracle_decompose. */
+int oracle_extract_inner(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         const uint8_t* status, int64_t u, double eps, int64_t* cand,
+                         double* wgt, int64_t cap, int64_t* ncand, double* internal,
+                         double* potential);
+
+/* The deletion pairs of a cluster (Decomposition.h:897-914) on the given slot
+ * statuses: members in the given order, each one's 'Y' slots in order, an
+ * edge listed once (its slots marked 'S' on a copy of the statuses).  The
+ * same function builds the seed loop's worklists. */
+int oracle_cluster_edges(int64_t n, const int64_t* offsets, const int64_t* nbors,
+                         const uint8_t* status, const int64_t* verts, int64_t nverts,
+                         int64_t* pairs, int64_t cap, int64_t* npairs);
+
 /* Full disjointExtract (Decomposition.h:538-940): count, clean, then greedy
  * extraction.  Outputs: cluster_map[n] (cluster id per vertex, ids in
  * extraction order), *n_clusters, survivors status[nnz] after the initial

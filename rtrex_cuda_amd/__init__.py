@@ -48,7 +48,8 @@ EXPORTS = [
     "rtrex_server_start", "rtrex_server_submit", "rtrex_server_wait", "rtrex_server_clock",
     "rtrex_server_stop", "rtrex_get_member_order", "rtrex_get_cluster_map", "rtrex_write_decomposition",
     "rtrex_write_components", "rtrex_write_survivors", "rtrex_write_expanded", "rtrex_get_count_params",
-    "rtrex_set_count_params", "rtrex_autotune_count",
+    "rtrex_set_count_params", "rtrex_autotune_count", "rtrex_extract_inner",
+    "rtrex_cluster_edges",
 ]
 
 
@@ -141,6 +142,8 @@ def lib() -> C.CDLL:
             "rtrex_get_edge_status": [G, PU8],
             "rtrex_get_survivors": [G, PU32, C.c_int64],
             "rtrex_delete_and_clean": [G, P64, C.c_int64, C.c_double],
+            "rtrex_extract_inner": [G, C.c_int64, C.c_double, P64, PD, C.c_int64, P64, PD, PD],
+            "rtrex_cluster_edges": [G, P64, C.c_int64, P64, C.c_int64, P64],
             "rtrex_extract": [G, C.c_double, P64, P64],
             "rtrex_decompose": [G, C.c_double, P64, P64],
             "rtrex_cluster_stats": [G, P64, P64, P64],
@@ -334,6 +337,32 @@ class DeviceGraph:
     def gpu_delete_and_clean(self, pairs, eps: float) -> None:
         pr = np.ascontiguousarray(pairs, np.int64).reshape(-1)
         _check(lib().rtrex_delete_and_clean(self._h, _p(pr, P64), len(pr) // 2, float(eps)))
+
+    def extract_inner(self, u: int, eps: float):
+        """gpuExtractInner: seed u's two-hop candidates (ascending id), their
+        weights, internal and potential weight on the current state."""
+        cap = self.n
+        cand = np.empty(max(cap, 1), np.int64)
+        wgt = np.empty(max(cap, 1), np.float64)
+        nc = C.c_int64()
+        iw = C.c_double()
+        pw = C.c_double()
+        _check(lib().rtrex_extract_inner(self._h, int(u), float(eps), _p(cand, P64), _p(wgt, PD),
+                                         cap, C.byref(nc), C.byref(iw), C.byref(pw)))
+        k = int(nc.value)
+        return cand[:k].copy(), wgt[:k].copy(), float(iw.value), float(pw.value)
+
+    def cluster_edges(self, verts):
+        """gpuMarkClusterEdges without marking: the (member, neighbour) pairs
+        deleting the cluster's alive edges, shape (k, 2), for
+        gpu_delete_and_clean."""
+        vs = np.ascontiguousarray(verts, np.int64).reshape(-1)
+        cap = self.nnz // 2
+        out = np.empty(2 * max(cap, 1), np.int64)
+        k = C.c_int64()
+        _check(lib().rtrex_cluster_edges(self._h, _p(vs, P64), len(vs), _p(out, P64), cap,
+                                         C.byref(k)))
+        return out[:2 * int(k.value)].reshape(-1, 2).copy()
 
     def disjoint_extract(self, eps: float):
         cm = np.empty(self.n, np.int64)

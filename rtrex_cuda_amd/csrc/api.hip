@@ -628,6 +628,22 @@ rtrex_status rtrex_delete_and_clean(rtrex_graph g, const int64_t* pairs,
     return run_delete_and_clean(g, pairs, k, eps);
 }
 
+rtrex_status rtrex_extract_inner(rtrex_graph g, int64_t u, double eps, int64_t* candidates,
+                                 double* cand_wgt, int64_t max_cand, int64_t* n_cand,
+                                 double* internal, double* potential) {
+    RT_TRY(check(g));
+    ArenaScope scope_(g);
+    return run_extract_inner(g, u, eps, candidates, cand_wgt, max_cand, n_cand, internal,
+                             potential);
+}
+
+rtrex_status rtrex_cluster_edges(rtrex_graph g, const int64_t* verts, int64_t n_verts,
+                                 int64_t* pairs, int64_t max_pairs, int64_t* n_pairs) {
+    RT_TRY(check(g));
+    ArenaScope scope_(g);
+    return run_cluster_edges(g, verts, n_verts, pairs, max_pairs, n_pairs);
+}
+
 namespace {
 // Extraction whose every decision is the reference's.  A cascade decision the
 // certified post-clean supports cannot settle is recorded; the clean then

@@ -534,6 +534,142 @@ __device__ uint32_t cascade_terms(const XArgs& A, uint64_t ebase, uint32_t top,
     return 0;
 }
 
+// The inner loop of seed u (Decomposition.h:728-805): every pair of alive
+// neighbours nbr[a], nbr[b] (b >= a, both of degree at most lim = d_u / eps)
+// adds the triangle (u, v, w) to internal and enumerates the triangles
+// (v, w, x) over the lower-degree side's list, searched in the other side's;
+// x a neighbour of u (mk[x] == stamp) adds to internal when it is the largest
+// of v, w, x, any other x is a two-hop candidate: appended to cand (in
+// discovery order) the first time (cm[x] = stamp, tw[x] = its weight), its
+// weight added to tw[x] after that.  Called by the whole wave; returns the
+// number of candidates and adds to internal in the reference's order.
+__device__ uint32_t seed_pairs(const GraphDev& g, uint32_t u, uint32_t du, double lim,
+                               uint32_t stamp, const uint32_t* nbr, uint32_t nA,
+                               const uint32_t* mk, uint32_t* cm, double* tw, uint32_t* cand,
+                               uint32_t* s_hn, uint8_t* s_ha, double& internal) {
+    const int lane = lane_id();
+    uint32_t nc = 0;
+    // Up to 64 alive neighbours: ids, degrees and list bounds one per
+    // lane, broadcast per pair (else read per pair).
+    const bool regs = nA <= (uint32_t)XT;
+    uint32_t r_v = 0, r_d = 0, r_o0 = 0, r_o1 = 0;
+    if (regs && (uint32_t)lane < nA) {
+        r_v = nbr[lane];
+        r_d = g.deg[r_v];
+        r_o0 = g.off[r_v];
+        r_o1 = g.off[r_v + 1];
+    }
+    for (uint32_t a = 0; a < nA; ++a) {
+        const uint32_t v = regs ? __shfl(r_v, (int)a, 64) : nbr[a];
+        const uint32_t dv = regs ? __shfl(r_d, (int)a, 64) : g.deg[v];
+        if ((double)dv > lim) continue;
+        const uint32_t v0 = regs ? __shfl(r_o0, (int)a, 64) : g.off[v];
+        const uint32_t v1 = regs ? __shfl(r_o1, (int)a, 64) : g.off[v + 1];
+        for (uint32_t b = a; b < nA; ++b) {
+            const uint32_t w = regs ? __shfl(r_v, (int)b, 64) : nbr[b];
+            const uint32_t dw = regs ? __shfl(r_d, (int)b, 64) : g.deg[w];
+            if ((double)dw > lim) continue;
+            const uint32_t w0 = regs ? __shfl(r_o0, (int)b, 64) : g.off[w];
+            const uint32_t w1 = regs ? __shfl(r_o1, (int)b, 64) : g.off[w + 1];
+            internal += tri_w(du, dv, dw);
+            const uint32_t lo = dv < dw ? v : w, hi = dv < dw ? w : v;
+            uint32_t p = dv < dw ? v0 : w0;
+            const uint32_t le = dv < dw ? v1 : w1;
+            uint32_t hlo = dv < dw ? w0 : v0;
+            const uint32_t he = dv < dw ? w1 : v1;
+            // N(hi) and its alive flags staged in LDS when short
+            // enough (the first chunk of N(lo) loaded alongside), so
+            // that each lane's search reads LDS; else searched in HBM.
+            const uint32_t dh = he - hlo;
+            const bool staged = dh <= XSTAGE;
+            uint32_t x0 = 0;
+            uint8_t st0 = ST_DEAD;
+            if (staged) {
+                if (p + lane < le) {
+                    x0 = g.nb[p + lane];
+                    st0 = g.st[p + lane];
+                }
+                for (uint32_t i = lane; i < dh; i += XT) {
+                    s_hn[i] = g.nb[hlo + i];
+                    s_ha[i] = g.st[hlo + i] != ST_DEAD;
+                }
+                wave_sync();
+            }
+            for (bool first = true; p < le; p += XT, first = false) {
+                const uint32_t l = p + lane;
+                const bool in = l < le;
+                const uint32_t x = !in ? 0 : staged && first ? x0 : g.nb[l];
+                const uint8_t sx = !in ? (uint8_t)ST_DEAD : staged && first ? st0 : g.st[l];
+                const bool ok = in && sx != ST_DEAD && x != lo && x != hi && x != u;
+                bool hit;
+                if (staged) {
+                    uint32_t a0 = 0, a1 = dh;  // lower bound of x in the staged list
+                    while (a0 < a1) {
+                        const uint32_t mid = (a0 + a1) >> 1;
+                        if (s_hn[mid] < x) a0 = mid + 1;
+                        else a1 = mid;
+                    }
+                    hit = ok && a0 < dh && s_hn[a0] == x && s_ha[a0];
+                } else {
+                    const uint32_t pos = in ? lower_bound(g.nb, hlo, he, x) : hlo;
+                    const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
+                    hit = ok && pos < he && g.nb[pos] == x && g.st[pos] != ST_DEAD;
+                    hlo = nh;
+                }
+                const double t = hit ? tri_w(g.deg[x], dv, dw) : 0.0;
+                const bool isn = hit && mk[x] == stamp;
+                uint64_t ib = ballot64(isn && x > v && x > w);
+                while (ib) {  // wave-uniform
+                    const int bl = __ffsll((unsigned long long)ib) - 1;
+                    internal += __shfl(t, bl, 64);
+                    ib &= ib - 1;
+                }
+                const bool ct = hit && !isn;
+                const bool fresh = ct && cm[x] != stamp;
+                const uint64_t fb = ballot64(fresh);
+                if (fresh) {
+                    cm[x] = stamp;
+                    tw[x] = t;
+                    cand[nc + __popcll(fb & lanes_below())] = x;
+                } else if (ct) {
+                    tw[x] += t;
+                }
+                nc += (uint32_t)__popcll(fb);
+            }
+            __syncthreads();  // candidate weights of this pair are final
+        }
+    }
+    return nc;
+}
+
+// The nc candidates of a seed (cand, discovery order) in ascending id, the
+// std::set order of Decomposition.h:811-827: rec[i] = (x, tw[x]); returns
+// their weights' sum in that order (potentialWgt).  keys: room for nc
+// rounded up to a power of two (LDS, or cand itself).  Called by the wave.
+__device__ double seed_candidates(const uint32_t* cand, uint32_t nc, uint32_t* keys,
+                                  const double* tw, Rec* rec) {
+    const int lane = lane_id();
+    uint32_t np = 1;
+    while (np < nc) np <<= 1;
+    for (uint32_t i = lane; i < np; i += XT) keys[i] = i < nc ? cand[i] : NONE;
+    __syncthreads();
+    if (nc > 1) wave_bitonic(keys, np);
+    double potential = 0.0;
+    for (uint32_t p = 0; p < nc; p += XT) {  // ascending x (Decomposition.h:815-827)
+        const uint32_t i = p + lane;
+        const bool in = i < nc;
+        const uint32_t x = in ? keys[i] : 0;
+        const double wx = in ? tw[x] : 0.0;
+        if (in) {
+            rec[i].vertex = x;
+            rec[i].wgt = wx;
+        }
+        const uint32_t cnt = min(64u, nc - p);
+        for (uint32_t b2 = 0; b2 < cnt; ++b2) potential += __shfl(wx, (int)b2, 64);
+    }
+    return potential;
+}
+
 __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ Rec srec[LDS_CAND];
     __shared__ double spv[LDS_CAND];
@@ -593,124 +729,19 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
 
             // 2. inner loop (Decomposition.h:728-805)
             double internal = 0.0;
-            uint32_t nc = 0;
             const uint64_t cb = 2ull * start;
-            // Up to 64 alive neighbours: ids, degrees and list bounds one per
-            // lane, broadcast per pair (else read per pair).
-            const bool regs = nA <= (uint32_t)XT;
-            uint32_t r_v = 0, r_d = 0, r_o0 = 0, r_o1 = 0;
-            if (regs && (uint32_t)lane < nA) {
-                r_v = A.mem[start + 1 + lane];
-                r_d = g.deg[r_v];
-                r_o0 = g.off[r_v];
-                r_o1 = g.off[r_v + 1];
-            }
-            for (uint32_t a = 0; a < nA; ++a) {
-                const uint32_t v = regs ? __shfl(r_v, (int)a, 64) : A.mem[start + 1 + a];
-                const uint32_t dv = regs ? __shfl(r_d, (int)a, 64) : g.deg[v];
-                if ((double)dv > lim) continue;
-                const uint32_t v0 = regs ? __shfl(r_o0, (int)a, 64) : g.off[v];
-                const uint32_t v1 = regs ? __shfl(r_o1, (int)a, 64) : g.off[v + 1];
-                for (uint32_t b = a; b < nA; ++b) {
-                    const uint32_t w = regs ? __shfl(r_v, (int)b, 64) : A.mem[start + 1 + b];
-                    const uint32_t dw = regs ? __shfl(r_d, (int)b, 64) : g.deg[w];
-                    if ((double)dw > lim) continue;
-                    const uint32_t w0 = regs ? __shfl(r_o0, (int)b, 64) : g.off[w];
-                    const uint32_t w1 = regs ? __shfl(r_o1, (int)b, 64) : g.off[w + 1];
-                    internal += tri_w(du, dv, dw);
-                    const uint32_t lo = dv < dw ? v : w, hi = dv < dw ? w : v;
-                    uint32_t p = dv < dw ? v0 : w0;
-                    const uint32_t le = dv < dw ? v1 : w1;
-                    uint32_t hlo = dv < dw ? w0 : v0;
-                    const uint32_t he = dv < dw ? w1 : v1;
-                    // N(hi) and its alive flags staged in LDS when short
-                    // enough (the first chunk of N(lo) loaded alongside), so
-                    // that each lane's search reads LDS; else searched in HBM.
-                    const uint32_t dh = he - hlo;
-                    const bool staged = dh <= XSTAGE;
-                    uint32_t x0 = 0;
-                    uint8_t st0 = ST_DEAD;
-                    if (staged) {
-                        if (p + lane < le) {
-                            x0 = g.nb[p + lane];
-                            st0 = g.st[p + lane];
-                        }
-                        for (uint32_t i = lane; i < dh; i += XT) {
-                            s_hn[i] = g.nb[hlo + i];
-                            s_ha[i] = g.st[hlo + i] != ST_DEAD;
-                        }
-                        wave_sync();
-                    }
-                    for (bool first = true; p < le; p += XT, first = false) {
-                        const uint32_t l = p + lane;
-                        const bool in = l < le;
-                        const uint32_t x = !in ? 0 : staged && first ? x0 : g.nb[l];
-                        const uint8_t sx = !in ? (uint8_t)ST_DEAD : staged && first ? st0 : g.st[l];
-                        const bool ok = in && sx != ST_DEAD && x != lo && x != hi && x != u;
-                        bool hit;
-                        if (staged) {
-                            uint32_t a0 = 0, a1 = dh;  // lower bound of x in the staged list
-                            while (a0 < a1) {
-                                const uint32_t mid = (a0 + a1) >> 1;
-                                if (s_hn[mid] < x) a0 = mid + 1;
-                                else a1 = mid;
-                            }
-                            hit = ok && a0 < dh && s_hn[a0] == x && s_ha[a0];
-                        } else {
-                            const uint32_t pos = in ? lower_bound(g.nb, hlo, he, x) : hlo;
-                            const uint32_t nh = __shfl(pos, (int)min(63u, le - p - 1), 64);
-                            hit = ok && pos < he && g.nb[pos] == x && g.st[pos] != ST_DEAD;
-                            hlo = nh;
-                        }
-                        const double t = hit ? tri_w(g.deg[x], dv, dw) : 0.0;
-                        const bool isn = hit && A.mk[x] == stamp;
-                        uint64_t ib = ballot64(isn && x > v && x > w);
-                        while (ib) {  // wave-uniform
-                            const int bl = __ffsll((unsigned long long)ib) - 1;
-                            internal += __shfl(t, bl, 64);
-                            ib &= ib - 1;
-                        }
-                        const bool ct = hit && !isn;
-                        const bool fresh = ct && A.cm[x] != stamp;
-                        const uint64_t fb = ballot64(fresh);
-                        if (fresh) {
-                            A.cm[x] = stamp;
-                            A.tw[x] = t;
-                            A.cand[cb + nc + __popcll(fb & lanes_below())] = x;
-                        } else if (ct) {
-                            A.tw[x] += t;
-                        }
-                        nc += (uint32_t)__popcll(fb);
-                    }
-                    __syncthreads();  // candidate weights of this pair are final
-                }
-            }
+            const uint32_t nc = seed_pairs(g, u, du, lim, stamp, A.mem + start + 1, nA, A.mk, A.cm,
+                                           A.tw, A.cand + cb, s_hn, s_ha, internal);
 
             // 3. candidates ascending (std::set), potential, sort, prefix
             uint32_t np = 1;
             while (np < nc) np <<= 1;
             const bool small = np <= LDS_CAND;
-            uint32_t* keys = small ? skey : A.cand + cb;
-            for (uint32_t i = lane; i < np; i += XT)
-                keys[i] = i < nc ? A.cand[cb + i] : NONE;
-            __syncthreads();
-            if (nc > 1) wave_bitonic(keys, np);
             Rec* rec = small ? srec : A.grec + start;
             double* pvv = small ? spv : A.gpv + start;
             double* bdd = sbd;  // per-candidate support bounds (LDS case only)
-            double potential = 0.0;
-            for (uint32_t p = 0; p < nc; p += XT) {  // ascending x (Decomposition.h:815-827)
-                const uint32_t i = p + lane;
-                const bool in = i < nc;
-                const uint32_t x = in ? keys[i] : 0;
-                const double wx = in ? A.tw[x] : 0.0;
-                if (in) {
-                    rec[i].vertex = x;
-                    rec[i].wgt = wx;
-                }
-                const uint32_t cnt = min(64u, nc - p);
-                for (uint32_t b2 = 0; b2 < cnt; ++b2) potential += __shfl(wx, (int)b2, 64);
-            }
+            const double potential =
+                seed_candidates(A.cand + cb, nc, small ? skey : A.cand + cb, A.tw, rec);
             __syncthreads();
             if (lane == 0) isort::sort(rec, rec + nc);
             __syncthreads();
@@ -1160,6 +1191,101 @@ __global__ void __launch_bounds__(NW * WAVE) k_cascade_wg(XArgs A, const uint32_
         if (n_edges) atomicAdd(&A.ctr->edges_removed, n_edges);
         if (amb_c) atomicAdd(&A.ctr->ambiguous_clean, amb_c);
     }
+}
+
+// ---------------------------------------------------------------- per seed
+// One seed's inner loop on the current state (rtrex_extract_inner): the alive
+// neighbours of u marked with stamp in mk (Decomposition.h:620-631), the
+// pairs (seed_pairs), the candidates in ascending id with their weights
+// (seed_candidates) into rec, and out = {internal, potential}, nout = the
+// candidate count.  One wave.
+struct InnerArgs {
+    GraphDev g;
+    uint32_t u, stamp;
+    double eps;
+    uint32_t *mk, *cm;  // n each, stamps (never reset: stamps grow per call)
+    double* tw;         // n
+    uint32_t* nbr;      // deg(u): the alive neighbours
+    uint32_t* cand;     // 2 n: candidates, then sort keys when LDS is short
+    Rec* rec;           // n: the result
+    double* out;        // internal, potential
+    uint32_t* nout;
+};
+__global__ void __launch_bounds__(XT) k_extract_inner(InnerArgs A) {
+    __shared__ Rec srec[LDS_CAND];
+    __shared__ uint32_t skey[LDS_CAND];
+    __shared__ uint32_t s_hn[XSTAGE];
+    __shared__ uint8_t s_ha[XSTAGE];
+    const GraphDev& g = A.g;
+    const int lane = lane_id();
+    const uint32_t u = A.u, du = g.deg[u];
+    uint32_t nA = 0;
+    for (uint32_t p = g.off[u]; p < g.off[u + 1]; p += XT) {
+        const uint32_t j = p + lane;
+        const bool alive = j < g.off[u + 1] && g.st[j] != ST_DEAD;
+        const uint32_t v = alive ? g.nb[j] : 0u;
+        const uint64_t bal = ballot64(alive);
+        if (alive) {
+            A.nbr[nA + (uint32_t)__popcll(bal & lanes_below())] = v;
+            A.mk[v] = A.stamp;
+        }
+        nA += (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    double internal = 0.0;
+    const uint32_t nc = seed_pairs(g, u, du, (double)du / A.eps, A.stamp, A.nbr, nA, A.mk, A.cm,
+                                   A.tw, A.cand, s_hn, s_ha, internal);
+    uint32_t np = 1;
+    while (np < nc) np <<= 1;
+    const bool small = np <= LDS_CAND;
+    Rec* rec = small ? srec : A.rec;
+    const double potential = seed_candidates(A.cand, nc, small ? skey : A.cand, A.tw, rec);
+    __syncthreads();
+    if (small)
+        for (uint32_t i = lane; i < nc; i += XT) A.rec[i] = srec[i];
+    if (lane == 0) {
+        A.out[0] = internal;
+        A.out[1] = potential;
+        A.nout[0] = nc;
+    }
+}
+
+// The pairs deleting a cluster's edges (rtrex_cluster_edges): members in the
+// given order, each one's alive slots in order, an edge listed from its first
+// member only (pos: member index per vertex, NONE elsewhere) -- the push order
+// of Decomposition.h:897-914, where the first listing marks both slots 'S'.
+// One wave; out[2i], out[2i + 1] = (member, neighbour) for i < cap; *nout =
+// the number of pairs.
+__global__ void __launch_bounds__(XT) k_cluster_edges(GraphDev g, const uint32_t* __restrict__ mem,
+                                                      uint32_t nm, const uint32_t* __restrict__ pos,
+                                                      uint32_t* __restrict__ out, uint64_t cap,
+                                                      unsigned long long* nout) {
+    const int lane = lane_id();
+    uint64_t c = 0;
+    for (uint32_t q = 0; q < nm; ++q) {
+        const uint32_t z = mem[q];
+        const uint32_t zb = g.off[z], ze = g.off[z + 1];
+        for (uint32_t p = zb; p < ze; p += XT) {
+            const uint32_t j = p + lane;
+            const bool in = j < ze;
+            const uint32_t y = in ? g.nb[j] : 0u;
+            const bool take = in && g.st[j] == ST_ALIVE && !(pos[y] < q);
+            const uint64_t bal = ballot64(take);
+            const uint64_t i = c + (uint64_t)__popcll(bal & lanes_below());
+            if (take && i < cap) {
+                out[2 * i] = z;
+                out[2 * i + 1] = y;
+            }
+            c += (uint64_t)__popcll(bal);
+        }
+    }
+    if (lane == 0) *nout = c;
+}
+
+__global__ void __launch_bounds__(T) k_set_pos(const uint32_t* __restrict__ mem, uint32_t nm,
+                                               uint32_t* __restrict__ pos, bool clear) {
+    for (uint32_t q = blockIdx.x * T + threadIdx.x; q < nm; q += gridDim.x * T)
+        pos[mem[q]] = clear ? NONE : q;
 }
 
 }  // namespace
@@ -1766,6 +1892,126 @@ rtrex_status run_cluster_stats_noalloc(rtrex_graph h, int64_t* nv, int64_t* ne, 
     if (ne) RT_HIP(hipMemcpyAsync(ne, b, k * 8, hipMemcpyDeviceToHost, s));
     if (cut) RT_HIP(hipMemcpyAsync(cut, c, k * 8, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
+    return RTREX_OK;
+}
+
+// One seed's inner loop (gpuExtractInner, gpu_extract.cuh:28-37; the
+// sequential Decomposition.h:728-827) on the handle's current state.
+rtrex_status run_extract_inner(rtrex_graph h, int64_t u, double eps, int64_t* cand,
+                               double* wgt, int64_t cap, int64_t* ncand, double* internal,
+                               double* potential) {
+    if (h->phase < PH_CLEANED || h->phase >= PH_EXTRACTED)
+        return fail(RTREX_ESTATE, "extract_inner needs a cleaned, not yet extracted graph");
+    if (!(eps > 0)) return fail(RTREX_EINVAL, "eps must be > 0");
+    GraphDev& g = h->g;
+    if (u < 0 || u >= (int64_t)g.n) return fail(RTREX_EINVAL, "seed out of range");
+    if (cap < 0 || (cap && (!cand || !wgt)) || !ncand || !internal || !potential)
+        return fail(RTREX_EINVAL, "bad output arguments");
+    hipStream_t s = h->stream;
+    const uint32_t n = g.n;
+    if (!h->xi_mk) {  // stamps start at 1: zeroed arrays hold none
+        RT_TRY(dalloc(h, &h->xi_mk, n));
+        RT_TRY(dalloc(h, &h->xi_cm, n));
+        RT_TRY(dalloc(h, &h->xi_tw, n));
+        h->xi_stamp = 0xffffffffu;
+    }
+    if (h->xi_stamp == 0xffffffffu) {
+        RT_HIP(hipMemsetAsync(h->xi_mk, 0, (uint64_t)n * 4, s));
+        RT_HIP(hipMemsetAsync(h->xi_cm, 0, (uint64_t)n * 4, s));
+        h->xi_stamp = 0;
+    }
+    Scratch sc;
+    InnerArgs A;
+    A.g = g;
+    A.u = (uint32_t)u;
+    A.stamp = ++h->xi_stamp;
+    A.eps = eps;
+    A.mk = h->xi_mk;
+    A.cm = h->xi_cm;
+    A.tw = h->xi_tw;
+    RT_TRY(sc.get(&A.nbr, (uint64_t)n));
+    RT_TRY(sc.get(&A.cand, 2ull * n));
+    RT_TRY(sc.get(&A.rec, (uint64_t)n));
+    RT_TRY(sc.get(&A.out, 2));
+    RT_TRY(sc.get(&A.nout, 1));
+    k_extract_inner<<<1, XT, 0, s>>>(A);
+    RT_HIP(hipGetLastError());
+    double o[2];
+    uint32_t nc = 0;
+    RT_HIP(hipMemcpyAsync(o, A.out, sizeof o, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(&nc, A.nout, 4, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    const uint64_t k = std::min<uint64_t>(nc, (uint64_t)cap);
+    if (k) {
+        std::vector<Rec> r(k);
+        RT_HIP(hipMemcpyAsync(r.data(), A.rec, k * sizeof(Rec), hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < k; ++i) {
+            cand[i] = r[i].vertex;
+            wgt[i] = r[i].wgt;
+        }
+    }
+    *ncand = nc;
+    *internal = o[0];
+    *potential = o[1];
+    return RTREX_OK;
+}
+
+// The pairs that delete a cluster's alive edges (gpuMarkClusterEdges,
+// gpu_extract.cuh:53-59; the sequential Decomposition.h:897-914), statuses
+// untouched: the caller hands them to rtrex_delete_and_clean.
+rtrex_status run_cluster_edges(rtrex_graph h, const int64_t* verts, int64_t nverts,
+                               int64_t* pairs, int64_t cap, int64_t* npairs) {
+    if (h->phase < PH_CLEANED || h->phase >= PH_EXTRACTED)
+        return fail(RTREX_ESTATE, "cluster_edges needs a cleaned, not yet extracted graph");
+    GraphDev& g = h->g;
+    if (nverts < 0 || (nverts && !verts) || cap < 0 || (cap && !pairs) || !npairs)
+        return fail(RTREX_EINVAL, "bad arguments");
+    if ((uint64_t)nverts > g.n) return fail(RTREX_EINVAL, "more vertices than the graph has");
+    std::vector<uint32_t> mv(nverts);
+    for (int64_t i = 0; i < nverts; ++i) {
+        if (verts[i] < 0 || verts[i] >= (int64_t)g.n)
+            return fail(RTREX_EINVAL, "vertex out of range");
+        mv[i] = (uint32_t)verts[i];
+    }
+    {
+        std::vector<uint32_t> sorted(mv);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(RTREX_EINVAL, "a vertex is listed twice");
+    }
+    *npairs = 0;
+    if (!nverts) return RTREX_OK;
+    hipStream_t s = h->stream;
+    const uint32_t n = g.n;
+    if (!h->xi_pos) {
+        RT_TRY(dalloc(h, &h->xi_pos, n));
+        RT_HIP(hipMemsetAsync(h->xi_pos, 0xff, (uint64_t)n * 4, s));  // NONE
+    }
+    Scratch sc;
+    uint32_t *dm, *dout;
+    unsigned long long* dn;
+    const uint64_t k = (uint64_t)cap;
+    RT_TRY(sc.get(&dm, (uint64_t)nverts));
+    RT_TRY(sc.get(&dout, 2 * std::max<uint64_t>(k, 1)));
+    RT_TRY(sc.get(&dn, 1));
+    RT_HIP(hipMemcpyAsync(dm, mv.data(), mv.size() * 4, hipMemcpyHostToDevice, s));
+    const unsigned bm = blocks_for((uint64_t)nverts, T, 1024);
+    k_set_pos<<<bm, T, 0, s>>>(dm, (uint32_t)nverts, h->xi_pos, false);
+    k_cluster_edges<<<1, XT, 0, s>>>(g, dm, (uint32_t)nverts, h->xi_pos, dout, k, dn);
+    k_set_pos<<<bm, T, 0, s>>>(dm, (uint32_t)nverts, h->xi_pos, true);
+    RT_HIP(hipGetLastError());
+    unsigned long long total = 0;
+    RT_HIP(hipMemcpyAsync(&total, dn, 8, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    const uint64_t w = std::min<uint64_t>(total, k);
+    if (w) {
+        std::vector<uint32_t> hp(2 * w);
+        RT_HIP(hipMemcpyAsync(hp.data(), dout, 2 * w * 4, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        for (uint64_t i = 0; i < 2 * w; ++i) pairs[i] = hp[i];
+    }
+    *npairs = (int64_t)total;
     return RTREX_OK;
 }
 

@@ -216,6 +216,11 @@ struct rtrex_graph_s {
     uint32_t* cseed = nullptr;
     uint32_t* cid = nullptr;  // cluster id per vertex (global numbering)
     uint32_t* mrank = nullptr;  // position at which each vertex joined its cluster
+    // Per-seed calls (rtrex_extract_inner, rtrex_cluster_edges): neighbour and
+    // candidate stamps (xi_stamp per call), candidate weights, member index.
+    uint32_t *xi_mk = nullptr, *xi_cm = nullptr, *xi_pos = nullptr;
+    double* xi_tw = nullptr;
+    uint32_t xi_stamp = 0;
     uint8_t* xact = nullptr;    // per edge: post-clean support is the reference's bits
     std::vector<uint32_t> bcnt;     // unordered_set bucket count after k insertions
     uint32_t* bcnt_dev = nullptr;   // device copy of bcnt[0, bcnt_dev_len), capacity n + 1
@@ -474,6 +479,11 @@ rtrex_status run_initial_clean(rtrex_graph h, double eps);
 rtrex_status run_delete_and_clean(rtrex_graph h, const int64_t* pairs,
                                   int64_t k, double eps);
 rtrex_status run_extract(rtrex_graph h, double eps);
+rtrex_status run_extract_inner(rtrex_graph h, int64_t u, double eps, int64_t* cand,
+                               double* wgt, int64_t cap, int64_t* ncand, double* internal,
+                               double* potential);
+rtrex_status run_cluster_edges(rtrex_graph h, const int64_t* verts, int64_t nverts,
+                               int64_t* pairs, int64_t cap, int64_t* npairs);
 // The cluster map of the last extraction, as int64 per vertex (per-graph ids
 // on a batch handle).
 rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out);

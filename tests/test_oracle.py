@@ -13,7 +13,12 @@ here, on the CPU, by:
   * connected_components: the partition hash of the connected-components file
     the reference's own driver wrote, scipy's connected_components, and a
     closed form (cliques, a path, isolated vertices, whose ids would be -1
-    if a start vertex were left unlabelled).
+    if a start vertex were left unlabelled);
+  * extract_inner (one seed's inner loop, the function the seed loop runs,
+    so the golden clusters pin it too): values worked out by hand from
+    Decomposition.h:728-827 on a four-vertex graph, with and without a
+    deleted edge; cluster_edges (the seed loop's worklist builder) likewise
+    from Decomposition.h:897-914.
 """
 from __future__ import annotations
 
@@ -269,3 +274,53 @@ def test_extract_from_state_equals_decompose(tag):
     cm, k = O.extract_from_state(off, nb, meta["eps"], cl)
     assert k == meta["n_clusters"]
     assert np.array_equal(cm, golden_result(tag)["cluster_map"])
+
+
+def _fold(xs):
+    acc = 0.0
+    for x in xs:
+        acc += x
+    return acc
+
+
+def test_extract_inner_worked_example():
+    """Seed 0 of the graph 0-1, 0-2, 1-2, 1-3, 2-3 (degrees 2, 3, 3, 2) at eps
+    0.3, worked out from the reference's loop (Decomposition.h:728-805): the
+    pairs of alive neighbours are (1,1), (1,2), (2,2) (k >= j, so a neighbour
+    pairs with itself); each adds 1/(2*3*3) to internal; the pair (1,1)
+    enumerates N(1) and meets 2, a neighbour of 0 larger than both, adding
+    1/(3*3*3); vertex 3 is a candidate reached from all three pairs, 1/18 each.
+    A deleted edge {1,3} leaves only the pair (2,2) reaching it."""
+    off, nb = O.csr_from_undirected(4, np.array([0, 0, 1, 1, 2]), np.array([1, 2, 2, 3, 3]))
+    st = np.full(len(nb), ord("Y"), np.uint8)
+    cand, wgt, internal, potential = O.extract_inner(off, nb, st, 0, 0.3)
+    t = 1.0 / 18.0
+    assert list(cand) == [3]
+    assert wgt[0] == _fold([t, t, t]) and potential == wgt[0]
+    assert internal == _fold([t, 1.0 / 27.0, t, t])
+    src = np.repeat(np.arange(4), np.diff(off))
+    dead = ((src == 1) & (nb == 3)) | ((src == 3) & (nb == 1))
+    st[dead] = ord("N")
+    cand, wgt, internal, potential = O.extract_inner(off, nb, st, 0, 0.3)
+    assert list(cand) == [3] and wgt[0] == t and potential == t
+    assert internal == _fold([t, 1.0 / 27.0, t, t])
+    # seed 3 once {1,3} is deleted: its one alive neighbour 2 pairs with
+    # itself; N(2) = {0, 1, 3} gives candidates 0 (1/(2*3*3)) and 1
+    # (1/(3*3*3)), 1 no longer being a neighbour of 3
+    cand, wgt, internal, potential = O.extract_inner(off, nb, st, 3, 0.3)
+    assert list(cand) == [0, 1] and list(wgt) == [t, 1.0 / 27.0]
+    assert internal == t and potential == t + 1.0 / 27.0
+
+
+def test_cluster_edges_worked_example():
+    """Members [1, 0] of the same graph (Decomposition.h:897-914): member 1
+    lists its three 'Y' slots (1,0), (1,2), (1,3) and marks them and their
+    partners 'S'; member 0 then skips (0,1) and lists (0,2).  A deleted slot
+    pair is never listed; a member order [0, 1] lists {0,1} from 0."""
+    off, nb = O.csr_from_undirected(4, np.array([0, 0, 1, 1, 2]), np.array([1, 2, 2, 3, 3]))
+    st = np.full(len(nb), ord("Y"), np.uint8)
+    assert O.cluster_edges(off, nb, st, [1, 0]).tolist() == [[1, 0], [1, 2], [1, 3], [0, 2]]
+    assert O.cluster_edges(off, nb, st, [0, 1]).tolist() == [[0, 1], [0, 2], [1, 2], [1, 3]]
+    src = np.repeat(np.arange(4), np.diff(off))
+    st[((src == 1) & (nb == 2)) | ((src == 2) & (nb == 1))] = ord("N")
+    assert O.cluster_edges(off, nb, st, [1, 0]).tolist() == [[1, 0], [1, 3], [0, 2]]
