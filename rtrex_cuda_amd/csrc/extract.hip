@@ -51,6 +51,9 @@ constexpr int XT = 64;  // one wave per workgroup
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr uint32_t LDS_CAND = 1024;
 constexpr uint32_t XSTAGE = 2048;  // longest N(hi) the inner loop stages in LDS
+// Clusters of up to US_LDS members emulate the set's order in LDS, when the
+// set has at most US_LDS_BKT buckets at that size.
+constexpr uint32_t US_LDS = 128, US_LDS_BKT = 512;
 constexpr int CWG = 8;             // waves per handed-over cascade (k_cascade_wg)
 
 using isort::Rec;
@@ -234,15 +237,63 @@ __global__ void __launch_bounds__(T) k_comp_starts(const uint32_t* __restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) cstart[fpos[nk]] = nk;
 }
 
-// Ascending sort of up to SB_MAX u64 keys by one workgroup (bitonic in LDS);
-// small component layouts skip the multi-pass radix sort.
+// Layouts of up to SB_MAX component keys run in one workgroup of SB_T
+// threads (k_comp_small), bitonic sort in LDS included; larger ones take the
+// multi-pass radix sort and the per-element kernels above.
 constexpr int SB_T = 1024;
 constexpr uint32_t SB_MAX = 8192;
-__global__ void __launch_bounds__(SB_T) k_sort_small(uint64_t* keys, uint32_t n) {
+
+// Exclusive scan over the workgroup of one value per thread (SB_T threads):
+// returns this thread's prefix, *total the sum.  part: SB_T / WAVE entries.
+template <class V>
+__device__ V block_excl_scan(V x, V* part, V* total) {
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    V incl = x;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const V o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == WAVE - 1) part[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        V acc = 0;
+        for (int i = 0; i < SB_T / WAVE; ++i) {
+            const V t = part[i];
+            part[i] = acc;
+            acc += t;
+        }
+        *total = acc;
+    }
+    __syncthreads();
+    return part[w] + incl - x;
+}
+
+// The alive components' layout for up to SB_MAX keys (root << 32 | seed
+// rank) in one workgroup, in place of the sort, layout, start, bucket-size
+// kernels and their three scans: cv (vertices grouped by component, seed
+// order inside), cstart (ncomp + 1 boundaries), *ncomp, dpre (nk + 1 prefix
+// of degrees), hoff (nk + 1 prefix of the components' set bucket counts
+// bcnt[size], 0 past ncomp), *big (largest component).  Thread t takes keys
+// [8t, 8t + 8).
+constexpr uint32_t CS_E = SB_MAX / SB_T;
+__global__ void __launch_bounds__(SB_T) k_comp_small(const uint64_t* __restrict__ keys,
+                                                     uint32_t nk,
+                                                     const uint32_t* __restrict__ order,
+                                                     const GraphDev g,
+                                                     const uint32_t* __restrict__ bcnt,
+                                                     uint32_t* __restrict__ cv,
+                                                     uint32_t* __restrict__ cstart,
+                                                     uint32_t* __restrict__ ncomp,
+                                                     uint64_t* __restrict__ dpre,
+                                                     uint32_t* __restrict__ hoff,
+                                                     uint32_t* __restrict__ big) {
     __shared__ uint64_t a[SB_MAX];
+    __shared__ uint32_t s_start[SB_MAX / 2 + 1];
+    __shared__ uint32_t p32[SB_T / WAVE], t32;
+    __shared__ uint64_t p64[SB_T / WAVE], t64;
     uint32_t np = 1;
-    while (np < n) np <<= 1;
-    for (uint32_t i = threadIdx.x; i < np; i += SB_T) a[i] = i < n ? keys[i] : ~0ull;
+    while (np < nk) np <<= 1;
+    for (uint32_t i = threadIdx.x; i < np; i += SB_T) a[i] = i < nk ? keys[i] : ~0ull;
     __syncthreads();
     for (uint32_t k = 2; k <= np; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -259,7 +310,70 @@ __global__ void __launch_bounds__(SB_T) k_sort_small(uint64_t* keys, uint32_t n)
             __syncthreads();
         }
     }
-    for (uint32_t i = threadIdx.x; i < n; i += SB_T) keys[i] = a[i];
+    const uint32_t i0 = threadIdx.x * CS_E;
+    uint32_t fl = 0, fmask = 0;
+    uint64_t dg = 0;
+    uint32_t d[CS_E];
+#pragma unroll
+    for (uint32_t k = 0; k < CS_E; ++k) {
+        const uint32_t i = i0 + k;
+        d[k] = 0;
+        if (i < nk) {
+            const uint32_t v = order[(uint32_t)(a[i] & 0xffffffffu)];
+            cv[i] = v;
+            d[k] = g.deg[v];
+            if (i == 0 || (a[i] >> 32) != (a[i - 1] >> 32)) {
+                fmask |= 1u << k;
+                ++fl;
+            }
+        }
+        dg += d[k];
+    }
+    uint32_t f = block_excl_scan<uint32_t>(fl, p32, &t32);
+    uint64_t dp = block_excl_scan<uint64_t>(dg, p64, &t64);
+    const uint32_t nc = t32;
+#pragma unroll
+    for (uint32_t k = 0; k < CS_E; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < nk) {
+            dpre[i] = dp;
+            if (fmask >> k & 1u) {
+                s_start[f] = i;
+                cstart[f] = i;
+                ++f;
+            }
+        }
+        dp += d[k];
+    }
+    if (threadIdx.x == 0) {
+        s_start[nc] = nk;
+        cstart[nc] = nk;
+        *ncomp = nc;
+        dpre[nk] = t64;
+    }
+    __syncthreads();
+    uint32_t hs = 0, mx = 0, hv[CS_E];
+#pragma unroll
+    for (uint32_t k = 0; k < CS_E; ++k) {
+        const uint32_t c = i0 + k;
+        hv[k] = 0;
+        if (c < nc) {
+            const uint32_t len = s_start[c + 1] - s_start[c];
+            hv[k] = bcnt[len];
+            mx = max(mx, len);
+        }
+        hs += hv[k];
+    }
+    uint32_t h = block_excl_scan<uint32_t>(hs, p32, &t32);
+#pragma unroll
+    for (uint32_t k = 0; k < CS_E; ++k) {
+        const uint32_t c = i0 + k;
+        if (c < nk) hoff[c] = h;
+        h += hv[k];
+    }
+    if (threadIdx.x == 0) hoff[nk] = t32;
+    for (int dd = 32; dd >= 1; dd >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, dd, 64));
+    if (lane_id() == 0 && mx) atomicMax(big, mx);
 }
 
 // ---------------------------------------------------------------- extraction
@@ -285,6 +399,7 @@ struct XArgs {
     Counters* ctr;
     const uint8_t* xact;     // per edge: post-clean support is the reference's bits
     const uint32_t* bcnt;    // unordered_set bucket count after k insertions
+    bool uset_lds;           // bcnt[min(US_LDS, largest cluster)] <= US_LDS_BKT
     const uint32_t* hoff;    // per component: start of its bucket scratch
     uint32_t* hnext;         // [start, ...): set node links
     uint32_t* hbkt;          // [hoff[c], ...): set buckets
@@ -582,12 +697,13 @@ __device__ uint32_t seed_pairs(const GraphDev& g, uint32_t u, uint32_t du, doubl
             // that each lane's search reads LDS; else searched in HBM.
             const uint32_t dh = he - hlo;
             const bool staged = dh <= XSTAGE;
-            uint32_t x0 = 0;
+            uint32_t x0 = 0, d0 = 0;
             uint8_t st0 = ST_DEAD;
             if (staged) {
                 if (p + lane < le) {
                     x0 = g.nb[p + lane];
                     st0 = g.st[p + lane];
+                    d0 = g.ndo[p + lane].x;
                 }
                 for (uint32_t i = lane; i < dh; i += XT) {
                     s_hn[i] = g.nb[hlo + i];
@@ -600,6 +716,8 @@ __device__ uint32_t seed_pairs(const GraphDev& g, uint32_t u, uint32_t du, doubl
                 const bool in = l < le;
                 const uint32_t x = !in ? 0 : staged && first ? x0 : g.nb[l];
                 const uint8_t sx = !in ? (uint8_t)ST_DEAD : staged && first ? st0 : g.st[l];
+                // d_x from the slot's copy (loaded with x, not after it)
+                const uint32_t dx = !in ? 0u : staged && first ? d0 : g.ndo[l].x;
                 const bool ok = in && sx != ST_DEAD && x != lo && x != hi && x != u;
                 bool hit;
                 if (staged) {
@@ -616,8 +734,16 @@ __device__ uint32_t seed_pairs(const GraphDev& g, uint32_t u, uint32_t du, doubl
                     hit = ok && pos < he && g.nb[pos] == x && g.st[pos] != ST_DEAD;
                     hlo = nh;
                 }
-                const double t = hit ? tri_w(g.deg[x], dv, dw) : 0.0;
-                const bool isn = hit && mk[x] == stamp;
+                // x's marks and weight in one round of loads
+                uint32_t mkx = 0, cmx = 0;
+                double twx = 0.0;
+                if (hit) {
+                    mkx = mk[x];
+                    cmx = cm[x];
+                    twx = tw[x];
+                }
+                const double t = hit ? tri_w(dx, dv, dw) : 0.0;
+                const bool isn = hit && mkx == stamp;
                 uint64_t ib = ballot64(isn && x > v && x > w);
                 while (ib) {  // wave-uniform
                     const int bl = __ffsll((unsigned long long)ib) - 1;
@@ -625,14 +751,14 @@ __device__ uint32_t seed_pairs(const GraphDev& g, uint32_t u, uint32_t du, doubl
                     ib &= ib - 1;
                 }
                 const bool ct = hit && !isn;
-                const bool fresh = ct && cm[x] != stamp;
+                const bool fresh = ct && cmx != stamp;
                 const uint64_t fb = ballot64(fresh);
                 if (fresh) {
                     cm[x] = stamp;
                     tw[x] = t;
                     cand[nc + __popcll(fb & lanes_below())] = x;
                 } else if (ct) {
-                    tw[x] += t;
+                    tw[x] = twx + t;  // x is one lane's alone within a pair
                 }
                 nc += (uint32_t)__popcll(fb);
             }
@@ -670,6 +796,7 @@ __device__ double seed_candidates(const uint32_t* cand, uint32_t nc, uint32_t* k
     return potential;
 }
 
+template <bool TERMS>
 __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ Rec srec[LDS_CAND];
     __shared__ double spv[LDS_CAND];
@@ -680,6 +807,10 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
     __shared__ uint32_t s_cn;
     __shared__ uint32_t s_hn[XSTAGE];  // inner loop: the pair's N(hi) ...
     __shared__ uint8_t s_ha[XSTAGE];   // ... and its alive flags
+    // the cluster's set order: members, iteration order, links, buckets,
+    // bucket counts after k insertions
+    __shared__ uint32_t s_uin[US_LDS], s_uord[US_LDS], s_unxt[US_LDS], s_ubkt[US_LDS_BKT];
+    __shared__ uint32_t s_ubc[US_LDS + 1];
     const GraphDev& g = A.g;
     const int lane = lane_id();
     unsigned long long n_seeds = 0, n_tri = 0, n_edges = 0, amb_c = 0, amb_r = 0;
@@ -793,14 +924,22 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             // iteration order of the reference's unordered_set (filled as the
             // cluster grew: seed, alive neighbours, chosen candidates), their
             // slots in order (Decomposition.h:897-914)
+            // Up to US_LDS members: the set is emulated in LDS (its bucket
+            // counts staged first), so lane 0's dependent steps read LDS.
             uint32_t top = 0;
             const uint32_t nmem = 1 + nA + (uint32_t)(maxind + 1);
-            uint32_t* ins = A.cand + cb;   // members in insertion order
-            uint32_t* ord = ins + len;     // ... in iteration order
+            const bool ulds = A.uset_lds && nmem <= US_LDS;
+            uint32_t* ins = ulds ? s_uin : A.cand + cb;  // members in insertion order
+            uint32_t* ord = ulds ? s_uord : A.cand + cb + len;  // ... in iteration order
+            if (ulds)
+                for (uint32_t q = lane; q <= nmem; q += XT) s_ubc[q] = A.bcnt[q];
             for (uint32_t q = lane; q < nmem; q += XT)
                 ins[q] = q < 1 + nA ? A.mem[start + q] : rec[q - 1 - nA].vertex;
             __syncthreads();
-            if (lane == 0) uset_order(ins, nmem, A.bcnt, A.hnext + start, A.hbkt + A.hoff[c], ord);
+            if (lane == 0) {
+                if (ulds) uset_order(ins, nmem, s_ubc, s_unxt, s_ubkt, ord);
+                else uset_order(ins, nmem, A.bcnt, A.hnext + start, A.hbkt + A.hoff[c], ord);
+            }
             __syncthreads();
             // Up to 64 members: ids and list bounds one per lane, read once.
             // A member's statuses are read after the earlier members' writes
@@ -836,7 +975,7 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
             }
 
             // ... and cascade (deleteAndClean, Decomposition.h:147-283)
-            if (A.tb.t) {
+            if constexpr (TERMS) {
                 const uint32_t left = cascade_terms(A, ebase, top, amb_c, n_tri, n_edges, s_ct,
                                                     &s_cn);
                 if (left) {  // the workgroup kernel finishes it; seeds resume at r + 1
@@ -866,7 +1005,7 @@ __global__ void __launch_bounds__(XT) k_extract(XArgs A) {
                 // vertex, the order the reference visits them), or by
                 // intersecting N(lo) with N(hi).  Either way, per chunk of 64:
                 // slot ja of (lo, w) and jb of (hi, w), both not deleted.
-                const bool terms = A.tb.t != nullptr;
+                constexpr bool terms = false;
                 const uint32_t tlo = terms ? (dx <= dy ? x : y) : lo;
                 const uint64_t c0 = terms ? A.tb.zoff[e] : g.off[lo];
                 const uint64_t c1 = terms ? c0 + g.tcnt[e] : g.off[lo + 1];
@@ -1513,39 +1652,44 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     uint32_t* x_undec_list = nullptr;  // device list of undecided cascade decisions
     const uint32_t* ncomp_d = nullptr;  // component count (device)
     if (nk) {
-        if (nk <= SB_MAX) k_sort_small<<<1, SB_T, 0, s>>>(keys, nk);
-        else RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
-        uint32_t *cv, *vdeg, *flag, *fpos, *cstart;
+        uint32_t *cv, *cstart, *fpos, *hoff, *hnext, *hbkt;
         uint64_t* dpre;
         RT_TRY(sc.get(&cv, nk));
-        RT_TRY(sc.get(&vdeg, nk));
-        RT_TRY(sc.get(&flag, nk));
         RT_TRY(sc.get(&fpos, (uint64_t)nk + 1));
-        RT_TRY(sc.get(&dpre, (uint64_t)nk + 1));
-        const unsigned bk = blocks_for(nk, T, 8192);
-        k_comp_layout<<<bk, T, 0, s>>>(keys, nk, order, g, cv, vdeg, flag);
-        RT_HIP(hipGetLastError());
-        RT_TRY(exclusive_scan_u32_32(flag, fpos, nk, s));
-        // The component count stays on the device (fpos[nk]); everything
-        // below is sized from nk (components have two vertices or more).
-        ncomp_d = fpos + nk;
         RT_TRY(sc.get(&cstart, (uint64_t)nk + 1));
-        k_comp_starts<<<bk, T, 0, s>>>(flag, fpos, nk, cstart);
-        RT_HIP(hipGetLastError());
-        RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
-        RT_TRY(ensure_vertex_supports(h, cv, nk));  // candidates' supports
+        RT_TRY(sc.get(&dpre, (uint64_t)nk + 1));
         // Bucket counts of the reference's unordered_set (measured from the
         // library) for up to nk elements, and each component's share of the
         // set scratch: hoff = prefix of bcnt[size] (at most ratio * nk).
         RT_TRY(ensure_bucket_table(h, nk));
-        uint32_t *hsz, *hoff, *hnext, *hbkt;
-        RT_TRY(sc.get(&hsz, nk));
         RT_TRY(sc.get(&hoff, (uint64_t)nk + 1));
         RT_TRY(sc.get(&hnext, nk));
         RT_TRY(sc.get(&hbkt, (uint64_t)h->bcnt_ratio * nk + 1));
-        k_comp_hsize<<<bk, T, 0, s>>>(cstart, ncomp_d, nk, h->bcnt_dev, hsz, big_d);
-        RT_HIP(hipGetLastError());
-        RT_TRY(exclusive_scan_u32_32(hsz, hoff, nk, s));
+        // The component count stays on the device (fpos[nk]); everything
+        // below is sized from nk (components have two vertices or more).
+        ncomp_d = fpos + nk;
+        if (nk <= SB_MAX) {  // one workgroup: sort, layout and the three scans
+            k_comp_small<<<1, SB_T, 0, s>>>(keys, nk, order, g, h->bcnt_dev, cv, cstart,
+                                            fpos + nk, dpre, hoff, big_d);
+            RT_HIP(hipGetLastError());
+        } else {
+            RT_TRY(radix_sort_u64(keys, tmp, nk, 32 + Bi, s));
+            uint32_t *vdeg, *flag, *hsz;
+            RT_TRY(sc.get(&vdeg, nk));
+            RT_TRY(sc.get(&flag, nk));
+            RT_TRY(sc.get(&hsz, nk));
+            const unsigned bk = blocks_for(nk, T, 8192);
+            k_comp_layout<<<bk, T, 0, s>>>(keys, nk, order, g, cv, vdeg, flag);
+            RT_HIP(hipGetLastError());
+            RT_TRY(exclusive_scan_u32_32(flag, fpos, nk, s));
+            k_comp_starts<<<bk, T, 0, s>>>(flag, fpos, nk, cstart);
+            RT_HIP(hipGetLastError());
+            RT_TRY(exclusive_scan_u32(vdeg, dpre, nk, s));
+            k_comp_hsize<<<bk, T, 0, s>>>(cstart, ncomp_d, nk, h->bcnt_dev, hsz, big_d);
+            RT_HIP(hipGetLastError());
+            RT_TRY(exclusive_scan_u32_32(hsz, hoff, nk, s));
+        }
+        RT_TRY(ensure_vertex_supports(h, cv, nk));  // candidates' supports
         uint32_t *mem, *cand, *undec;
         uint64_t* wl;
         double *tw, *gpv;
@@ -1578,6 +1722,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
         a.ctr = h->ctr;
         a.xact = h->xact;
         a.bcnt = h->bcnt_dev;
+        a.uset_lds = h->bcnt[std::min<uint64_t>(US_LDS, nk)] <= US_LDS_BKT;
         a.hoff = hoff;
         a.hnext = hnext;
         a.hbkt = hbkt;
@@ -1605,7 +1750,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
             a.dlist = dlist;
             a.ndefer = dcount;
         }
-        k_extract<<<grid, XT, 0, s>>>(a);
+        if (a.tb.t) k_extract<true><<<grid, XT, 0, s>>>(a);
+        else k_extract<false><<<grid, XT, 0, s>>>(a);
         RT_HIP(hipGetLastError());
         for (int cur = 0; handoff; cur ^= 1) {
             k_publish<<<1, WAVE, 0, s>>>(h->box_dev, ++h->box_seq, dcount + cur);
@@ -1621,7 +1767,7 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
             b.nclist = dcount + cur;
             b.dlist = dlist + (uint64_t)(cur ^ 1) * dcap;
             b.ndefer = dcount + (cur ^ 1);
-            k_extract<<<std::min<uint32_t>(nd, grid), XT, 0, s>>>(b);
+            k_extract<true><<<std::min<uint32_t>(nd, grid), XT, 0, s>>>(b);
             RT_HIP(hipGetLastError());
         }
         // Undecided cascade decisions (inexact post-clean supports): kept for

@@ -13,7 +13,8 @@ reference:
                 path AND the CPU oracle's.
   serving       a Poisson trace over the serving pool (Chung-Lu, planted
                 communities of 50 k - 2 M edges, R-MAT scale 18) through the
-                server: every request's map equals its graph alone; one graph
+                server, after a burst of small requests at once (batched
+                together): every request's map equals its graph alone; one graph
                 of each kind against hashes of the reference's run
                 (tests/golden/serve_mix.json, make_golden_serve.py).
 """
@@ -152,8 +153,11 @@ def test_config5_trace_served_equals_alone_and_reference():
     t0 = time.time()
     pool = G.serve_pool(n_rmat=1)  # R-MAT scale 18
     trace = G.serve_trace(1.5, pool_sizes=(64, 32, 1))
-    # every kind present at least once, the R-MAT graph among the first requests
-    trace = [(0.0, "rmat", 0), (0.0, "planted", 3)] + trace
+    # every kind present at least once, the R-MAT graph among the first
+    # requests, and a burst of small requests arriving together (they queue
+    # whatever the server's speed, so some are batched)
+    trace = ([(0.0, "rmat", 0), (0.0, "planted", 3)] + [(0.0, "cl", i) for i in range(24)] +
+             trace)
     print(f"[gpu] serving {len(trace)} requests ({time.time() - t0:.1f} s)", flush=True)
     with R.Server(workers=4, hbm_budget_bytes=48 << 30) as s:
         start = time.perf_counter()
