@@ -56,31 +56,24 @@ __device__ __forceinline__ void store_term(Term* p, double w, uint32_t a, uint32
 // endpoint, its hub.  Edges whose hub degree is in (hub_min, hub_max] are
 // counted per hub (k_count_hub*), the hub's list staged once in LDS; the rest
 // per edge (k_count_edges).
-// staging classes 64 / 512 / 4096 / 32760 / 262136 entries; G = 4..64
-constexpr int HUB_NL = 5, HUB_NG = 5;
-static_assert(HUB_NL * HUB_NG == HUB_CLASSES, "the handle's per-class arrays");
+constexpr int HUB_NL = 4, HUB_NG = 5;  // staging classes 64/512/4096/32768; G = 4..64
 // Owned edges per unit, at most, in the large and huge classes.  A huge
 // unit restages the tree and the filter of a hub of up to 32760 entries, so
 // its units are larger (R-MAT scale 21 count: 123 -> 110 ms with 128 instead
 // of 32); 32 keeps the few large hubs of planted graphs spread over the CUs.
 constexpr uint32_t LARGE_UNIT = 32, HUGE_UNIT = 128;
-// Hubs above l3_max restage a filter of 2^MEGA_FB bits and a tree of up to
-// 262136 entries per unit: up to 1024 owned edges per unit, so that staging
-// stays a small share of the unit.
-constexpr uint32_t MEGA_UNIT = 1024;
 // A unit's w field: d_hub in the low UNIT_DH_BITS bits (up to HUB_MAX_LIMIT),
 // its owned-edge count above (at most d_hub, and at most 4096 below the huge
 // class, whose units hold at most HUGE_UNIT).
 constexpr uint32_t UNIT_DH_BITS = 18, UNIT_DH_MASK = (1u << UNIT_DH_BITS) - 1u;
-static_assert(HUB_MAX_LIMIT <= UNIT_DH_MASK && MEGA_UNIT < (1u << (32 - UNIT_DH_BITS)) &&
+static_assert(HUB_MAX_LIMIT <= UNIT_DH_MASK && HUGE_UNIT < (1u << (32 - UNIT_DH_BITS)) &&
                   4096u < (1u << (32 - UNIT_DH_BITS)),
               "a unit's degree and edge count fit its w field");
 __host__ __device__ __forceinline__ bool hub_edge(uint32_t dhub, const CountParams& P) {
     return dhub > P.hub_min && dhub <= P.hub_max;
 }
 __host__ __device__ __forceinline__ int hub_lclass(uint32_t dhub, const CountParams& P) {
-    return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : dhub <= P.l2_max ? 2
-           : dhub <= P.l3_max ? 3 : 4;
+    return dhub <= P.l0_max ? 0 : dhub <= P.l1_max ? 1 : dhub <= P.l2_max ? 2 : 3;
 }
 // Medium and large hubs whose edges' shorter lists are short on average are
 // counted by the streaming kernel (with a term buffer); long ones (over
@@ -356,7 +349,7 @@ __global__ void __launch_bounds__(T) k_hub_units(GraphDev g, const uint32_t* __r
                                 : lc_ == 1 ? max<uint64_t>(Pm.unit_min, dh)
                                            : (uint64_t)Pm.unit_hash_mult * dh;
             P = (work + uw - 1) / uw;
-            const uint32_t ucap = lc_ == 4 ? MEGA_UNIT : lc_ == 3 ? HUGE_UNIT : LARGE_UNIT;
+            const uint32_t ucap = lc_ == 3 ? HUGE_UNIT : LARGE_UNIT;
             if (lc_ >= 2 && !stream) P = max<uint64_t>(P, (oc + ucap - 1) / ucap);
             P = max<uint64_t>(1, min<uint64_t>(P, min<uint64_t>(oc, 65535)));
             c = lc_ * HUB_NG + gc_;
@@ -628,11 +621,6 @@ constexpr uint32_t RING = 128;  // filter-passing ids queued per wave
 // (two) measured 13 % slower on R-MAT scale 20 and 21, an 8 KB one (more
 // false positives) 3-5 % slower.
 constexpr int LARGE_FB = 16, HUGE_FB = 17;
-// The hubs above l3_max (up to 262136 entries) take 2^18 bits (32 KB, two
-// workgroups per CU; C3's largest hub, 162,783 entries, sets about half).
-// Measured on C3: 2^20 bits (one workgroup per CU) 6 % slower, no filter
-// 1.5 % slower.
-constexpr int MEGA_FB = 18;
 // Whether the large-hub kernel stages its slots' edge ids and degrees beside
 // the tree: without a filter, where every id is probed.  With one, a match
 // reads them from the hub's slots in HBM (L2-resident) and the smaller LDS
@@ -1452,7 +1440,7 @@ __global__ void __launch_bounds__(HS_T) k_count_hub_stream(GraphDev g,
 // (sh = oh mod 8 entries earlier): those sh entries read as 0, so the sequence
 // stays non-decreasing, and are never a match.
 //
-// Hubs above l3_max (at most HUGE_TREE_MAX; R-MAT's largest) take a five-level tree
+// Hubs above HUGE_TREE_MAX entries (R-MAT's largest) take a five-level tree
 // over 262144 positions (MegaProbe): levels of every 64th, 512th and 4096th
 // entry in the same LDS arrays, the top 8 in registers, then the 8 pivots of
 // every 8th entry inside the chosen 64-entry stretch are read from the list
@@ -1538,46 +1526,45 @@ struct MegaProbe {
 };
 
 // FB > 0 (G = 64 only): the ids are filtered first (hub_edges_filt) against a
-// filter of 2^FB bits, built from one pass over the hub's list (that pass
-// also fills the four-level tree's lowest pivot level).  MEGA: hubs above
-// l3_max, on the five-level tree (MegaProbe), their owned edges' data read
-// per edge rather than prefetched (their units hold up to MEGA_UNIT edges).
-template <int G, int FB = 0, bool MEGA = false>
+// filter of 2^FB bits, built from one pass over the hub's list that also
+// fills the tree's lowest pivot level.
+template <int G, int FB = 0>
 __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
                                                         const uint4* __restrict__ units,
                                                         uint32_t nunits,
                                                         const uint32_t* __restrict__ own_idx,
-                                                        TermBuf tb) {
+                                                        TermBuf tb, uint32_t tree_max) {
     static_assert(FB == 0 || G == WAVE, "the filtered path runs one edge per wave");
     constexpr uint32_t L = 32768;
     constexpr uint32_t NF = FB ? (1u << FB) / 32 : 1;
-    constexpr uint32_t NPRE = MEGA ? 1 : HUGE_UNIT;
     __shared__ __attribute__((aligned(16))) uint32_t s_l1[L / 8];
     __shared__ __attribute__((aligned(16))) uint32_t s_l2[L / 64];
     __shared__ __attribute__((aligned(16))) uint32_t s_l3[L / 512];
     __shared__ double sfold[FB ? 1 : HTL];
-    __shared__ uint4 s_pre[NPRE];
-    __shared__ uint64_t s_prez[NPRE];
+    __shared__ uint4 s_pre[HUGE_UNIT];
+    __shared__ uint64_t s_prez[HUGE_UNIT];
     __shared__ __attribute__((aligned(16))) uint32_t s_filt[NF];
     __shared__ uint2 s_ring[FB ? (HTL / WAVE) * RING : 1];
     const int wid = threadIdx.x / WAVE;
     constexpr uint32_t per_wave = WAVE / G;
     for (uint32_t un = blockIdx.x; un < nunits; un += gridDim.x) {
         uint32_t h, oh, k0, dh, cnt;
-        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= HUGE_UNIT / MEGA_UNIT
+        unit_fields(units[un], h, oh, k0, dh, cnt);  // cnt <= HUGE_UNIT in this class
         if constexpr (FB > 0) {  // the previous unit's probes ended at the barrier below
             for (uint32_t i = threadIdx.x; i < NF; i += HTL) s_filt[i] = 0u;
             __syncthreads();
         }
-        if (!MEGA && threadIdx.x < cnt) {
+        if (threadIdx.x < cnt) {
             const uint32_t li = own_idx[k0 + threadIdx.x];
             const uint2 yo = g.ndo[oh + li];
             s_pre[threadIdx.x] = make_uint4(g.nbe[oh + li].y, yo.x, yo.y, 0u);
             s_prez[threadIdx.x] = tb.t ? tb.zsl[oh + li] : 0;
         }
         const uint32_t sh = oh & 7u, ob = oh - sh;
+        // the five-level tree (no filter) above tree_max <= HUGE_TREE_MAX
+        const bool mega = dh > tree_max;
         Piv8 top;
-        if constexpr (!MEGA) {  // dh + sh <= L
+        if (!mega) {  // dh + sh <= L
             if constexpr (FB > 0) {
                 // every entry into the filter; position s = sh + j of the
                 // shifted sequence is pivot s / 8 of the lowest level when
@@ -1600,8 +1587,6 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
 #pragma unroll
             for (int i = 0; i < 8; ++i) top.p[i] = huge_id(g, ob, sh, dh, (L / 8) * i + L / 8 - 1);
         } else {  // dh + sh <= 8 L: every 64th, 512th, 4096th, 32768th position
-            if constexpr (FB > 0)
-                for (uint32_t j = threadIdx.x; j < dh; j += HTL) filt_set<FB>(s_filt, g.nb[oh + j]);
             for (uint32_t i = threadIdx.x; i < L / 8; i += HTL)
                 s_l1[i] = huge_id(g, ob, sh, dh, 64 * i + 63);
             for (uint32_t i = threadIdx.x; i < L / 64; i += HTL)
@@ -1615,16 +1600,11 @@ __global__ void __launch_bounds__(HTL) k_count_hub_huge(GraphDev g,
         const HubMeta<false, false> meta{nullptr, nullptr, nullptr, nullptr, oh};
         const uint4* blk = reinterpret_cast<const uint4*>(g.nb + ob);
         double* fold = FB ? nullptr : sfold + wid * WAVE;  // G < 64 only
-        if constexpr (MEGA) {
+        if (mega) {
             const MegaProbe pr{blk, g.nb + ob, sh, dh, s_l1, s_l2, s_l3, top};
-            if constexpr (FB > 0)
-                hub_edges_filt<FB, MegaProbe, HubMeta<false, false>>(
-                    g, tb, dh, pr, meta, own_idx + k0, s_filt, s_ring + wid * RING, wid, cnt,
-                    HTL / WAVE);
-            else
-                hub_edges<G, MegaProbe, HubMeta<false, false>>(g, tb, dh, pr, meta, own_idx + k0,
-                                                              fold, wid * per_wave, cnt,
-                                                              per_wave * (HTL / WAVE));
+            hub_edges<G, MegaProbe, HubMeta<false, false>, true>(
+                g, tb, dh, pr, meta, nullptr, fold, wid * per_wave, cnt, per_wave * (HTL / WAVE),
+                s_pre, s_prez);
         } else {
             const HugeProbe pr{blk, sh, dh, s_l1, s_l2, s_l3, top};
             if constexpr (FB > 0)
@@ -1888,12 +1868,12 @@ void launch_hub_large(const GraphDev& g, const uint4* units, uint32_t nu, const 
         g, units, nu, own_idx, tb, lcap);
 }
 
-template <int G, bool MEGA>
+template <int G>
 void launch_hub_huge(const GraphDev& g, const uint4* units, uint32_t nu, const uint32_t* own_idx,
-                     const TermBuf& tb, hipStream_t s) {
-    constexpr int FB = G == WAVE ? (MEGA ? MEGA_FB : HUGE_FB) : 0;
-    k_count_hub_huge<G, FB, MEGA><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu,
-                                                                              own_idx, tb);
+                     const TermBuf& tb, uint32_t tree_max, hipStream_t s) {
+    constexpr int FB = G == WAVE ? HUGE_FB : 0;
+    k_count_hub_huge<G, FB><<<std::min<uint32_t>(nu, 8192), HTL, 0, s>>>(g, units, nu, own_idx,
+                                                                        tb, tree_max);
 }
 
 template <int L>
@@ -1957,20 +1937,13 @@ void hub_pass(rtrex_graph h, const TermBuf& tb) {
         if (lc == 0) launch_hub_l<WAVE>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 1) launch_hub_l<512>(gc, h->g, u, nu, h->own_idx, tb, h->stream);
         else if (lc == 3) {
+            const uint32_t tl3 = std::min(h->cparams.l3_max, HUGE_TREE_MAX);
             switch (gc) {
-                case 0: launch_hub_huge<4, false>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_huge<8, false>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_huge<16, false>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_huge<32, false>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                default: launch_hub_huge<64, false>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-            }
-        } else if (lc == 4) {  // above l3_max: the five-level tree
-            switch (gc) {
-                case 0: launch_hub_huge<4, true>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 1: launch_hub_huge<8, true>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 2: launch_hub_huge<16, true>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                case 3: launch_hub_huge<32, true>(h->g, u, nu, h->own_idx, tb, h->stream); break;
-                default: launch_hub_huge<64, true>(h->g, u, nu, h->own_idx, tb, h->stream); break;
+                case 0: launch_hub_huge<4>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 1: launch_hub_huge<8>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 2: launch_hub_huge<16>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                case 3: launch_hub_huge<32>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
+                default: launch_hub_huge<64>(h->g, u, nu, h->own_idx, tb, tl3, h->stream); break;
             }
         } else {
             const uint32_t mx = h->hub_umax[c];

@@ -153,8 +153,6 @@ struct AuditHost {
 // Largest hub degree the count kernels accept (the largest hubs' tree spans
 // 262144 entries from the 8-entry boundary at or before the list's start).
 constexpr uint32_t HUB_MAX_LIMIT = 262136;
-// Hub-path unit classes: five staging sizes times five lane counts.
-constexpr int HUB_CLASSES = 25;
 
 // Count-plan parameters (the autotuner's search space).  An edge whose
 // longer-list endpoint (its hub) has degree in (hub_min, hub_max] is counted
@@ -272,8 +270,8 @@ struct rtrex_graph_s {
     uint32_t* small_chunk = nullptr;
     uint64_t small_chunk_cap = 0;
     uint32_t small_nchunks = 0;
-    uint32_t hub_ucnt[rtrex::HUB_CLASSES] = {};
-    uint32_t hub_umax[rtrex::HUB_CLASSES] = {};  // largest hub degree per class
+    uint32_t hub_ucnt[20] = {};
+    uint32_t hub_umax[20] = {};  // largest hub degree per class
     unsigned long long* red = nullptr;  // per-block partial sums (count phase)
     rtrex::CountParams cparams;
     bool plan_valid = false;  // buckets and hub units built for cparams
