@@ -2137,7 +2137,7 @@ rtrex_status run_cluster_edges(rtrex_graph h, const int64_t* verts, int64_t nver
     Scratch sc;
     uint32_t *dm, *dout;
     unsigned long long* dn;
-    const uint64_t k = (uint64_t)cap;
+    const uint64_t k = std::min<uint64_t>((uint64_t)cap, g.m);  // at most m pairs exist
     RT_TRY(sc.get(&dm, (uint64_t)nverts));
     RT_TRY(sc.get(&dout, 2 * std::max<uint64_t>(k, 1)));
     RT_TRY(sc.get(&dn, 1));
