@@ -610,13 +610,30 @@ void rtrex_host_free(void* p) {
     if (p) (void)hipHostFree(p);
 }
 
+namespace {
+// The survivor bitmap written by the device straight into page-locked host
+// memory (rtrex_host_alloc), as the cluster ids are: no copy-engine transfer.
+constexpr int WT = 256;
+__global__ void __launch_bounds__(WT) k_words_out(const uint32_t* __restrict__ src, uint64_t n,
+                                                  uint32_t* __restrict__ dst) {
+    const uint64_t stride = (uint64_t)gridDim.x * WT;
+    for (uint64_t i = (uint64_t)blockIdx.x * WT + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
 rtrex_status rtrex_get_survivors(rtrex_graph g, uint32_t* bits, int64_t nwords) {
     RT_TRY(check(g));
     if (!g->surv) return fail(RTREX_ESTATE, "initial clean has not run");
     const int64_t need = ((int64_t)g->g.m + 31) / 32;
     if (!bits || nwords < need) return fail(RTREX_EINVAL, "survivor buffer too small");
-    if (need)
+    void* dbits = nullptr;
+    if (need && mapped_host(bits, &dbits)) {
+        k_words_out<<<blocks_for((uint64_t)need, WT, 1024), WT, 0, g->stream>>>(
+            g->surv, (uint64_t)need, static_cast<uint32_t*>(dbits));
+        RT_HIP(hipGetLastError());
+    } else if (need) {
         RT_HIP(hipMemcpyAsync(bits, g->surv, need * 4, hipMemcpyDeviceToHost, g->stream));
+    }
     RT_HIP(hipStreamSynchronize(g->stream));
     return RTREX_OK;
 }

@@ -376,7 +376,7 @@ def test_fresh_handles_small_graphs_after_a_large_one():
 
 def test_results_into_page_locked_host_memory():
     """Cluster map and survivor words written into rtrex_host_alloc memory
-    (the device writes the map directly) equal the ordinary copies."""
+    (the device writes both directly) equal the ordinary copies."""
     n, u, v = golden_graph("clean_astro")
     with R.DeviceGraph.from_edges(n, u, v) as g:
         cm0, k0 = g.decompose(0.1)
@@ -384,6 +384,7 @@ def test_results_into_page_locked_host_memory():
         cm = R.host_empty(n, np.int64)
         cm[:] = -7
         words = R.host_empty(len(w0), np.uint32)
+        words[:] = 0xDEADBEEF
         _, k = g.decompose(0.1, out=cm)
         g.survivor_words(out=words)
     assert k == k0 and np.array_equal(cm, cm0) and np.array_equal(words, w0)
