@@ -98,11 +98,12 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
     b.decompose(eps)
     cold_ms = (time.perf_counter() - t0) * 1e3
     params, tbest, tstart = b.autotune(repeats=2)
-    b.decompose(eps)
+    cmb = R.host_empty(n, np.int64)  # page-locked: the device writes the ids
+    b.decompose(eps, out=cmb)
     tms = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        maps, nc = b.decompose(eps)
+        maps, nc = b.decompose(eps, out=cmb)
         tms.append(b.timings())
     bt = (time.perf_counter() - t0) / steps
     audit = b.audit()
@@ -116,9 +117,10 @@ def batched_vs_plain(eps: float, count: int, steps: int) -> dict:
     sample = list(range(0, count, stride))
     for i in sample:
         with R.DeviceGraph.from_edges(*graphs[i]) as p:
-            p.decompose(eps)
+            po = R.host_empty(graphs[i][0], np.int64)
+            p.decompose(eps, out=po)
             t0 = time.perf_counter()
-            c2, _ = p.decompose(eps)
+            c2, _ = p.decompose(eps, out=po)
             pt += time.perf_counter() - t0
         pm += len(graphs[i][1])
         same += int(np.array_equal(c2, maps[i]))

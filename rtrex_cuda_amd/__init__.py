@@ -475,8 +475,11 @@ class Batch(DeviceGraph):
         self.vbase = np.zeros(len(graphs) + 1, np.int64)
         np.cumsum(vtx, out=self.vbase[1:])
 
-    def decompose(self, eps: float):
-        cm, _ = super().decompose(eps)
+    def decompose(self, eps: float, out: np.ndarray | None = None):
+        """out: an int64 array of all graphs' vertices to write the maps into
+        (host_empty memory is written by the device directly); the returned
+        maps are views of it."""
+        cm, _ = super().decompose(eps, out=out)
         nc = np.empty(len(self.vtx), np.int64)
         _check(lib().rtrex_batch_cluster_counts(self._h, _p(nc, P64), len(nc)))
         maps = [cm[self.vbase[i]:self.vbase[i + 1]] for i in range(len(self.vtx))]

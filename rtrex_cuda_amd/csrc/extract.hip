@@ -1610,12 +1610,16 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
     if (!h->seg_vtx.empty() && !h->seed_order) {
         const size_t ns = h->seg_vtx.size() - 1;
         Bs = bits_for(ns + 1);
-        std::vector<uint32_t> seg(n);
-        for (size_t q = 0; q < ns; ++q)
-            for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v) seg[v] = (uint32_t)q;
-        RT_TRY(sc.get(&vseg, n));
-        RT_HIP(hipMemcpyAsync(vseg, seg.data(), (uint64_t)n * 4, hipMemcpyHostToDevice, s));
-        RT_HIP(hipStreamSynchronize(s));
+        if (!h->vseg_d) {  // each vertex's segment, kept for the per-graph ids
+            std::vector<uint32_t> seg(n);
+            for (size_t q = 0; q < ns; ++q)
+                for (int64_t v = h->seg_vtx[q]; v < h->seg_vtx[q + 1]; ++v) seg[v] = (uint32_t)q;
+            RT_TRY(dalloc(h, &h->vseg_d, n));
+            RT_HIP(hipMemcpyAsync(h->vseg_d, seg.data(), (uint64_t)n * 4, hipMemcpyHostToDevice,
+                                  s));
+            RT_HIP(hipStreamSynchronize(s));
+        }
+        vseg = h->vseg_d;
     }
     if (Bs + 2 * Bi > 64) return fail(RTREX_EINVAL, "too many vertices/segments");
     const unsigned bn = blocks_for(n, T, 8192);
@@ -1795,7 +1799,8 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
             RT_HIP(hipMemcpyAsync(h->seg_pos_d, sp.data(), (ns + 1) * 4, hipMemcpyHostToDevice, s));
             RT_HIP(hipStreamSynchronize(s));
         }
-        RT_TRY(sc.get(&bounds_d, ns + 1));
+        if (!h->seg_bounds_d) RT_TRY(dalloc(h, &h->seg_bounds_d, ns + 1));
+        bounds_d = h->seg_bounds_d;
         k_gather_u32<<<blocks_for(ns + 1, T, 1024), T, 0, s>>>(cidr, h->seg_pos_d,
                                                              (uint32_t)(ns + 1), bounds_d);
         RT_HIP(hipGetLastError());
@@ -1844,6 +1849,15 @@ __global__ void __launch_bounds__(T) k_cid_out(const uint32_t* __restrict__ cid,
                                               int64_t* __restrict__ out) {
     for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T) out[v] = cid[v];
 }
+// The same for a batch: per-graph ids, each vertex's id less its graph's
+// first cluster id (vseg: the vertex's graph, bounds: first ids).
+__global__ void __launch_bounds__(T) k_cid_out_seg(const uint32_t* __restrict__ cid, uint32_t n,
+                                                  const uint32_t* __restrict__ vseg,
+                                                  const uint32_t* __restrict__ bounds,
+                                                  int64_t* __restrict__ out) {
+    for (uint32_t v = blockIdx.x * T + threadIdx.x; v < n; v += gridDim.x * T)
+        out[v] = (int64_t)cid[v] - (int64_t)bounds[vseg[v]];
+}
 }  // namespace
 
 bool mapped_host(const void* p, void** dptr) {
@@ -1862,9 +1876,13 @@ rtrex_status copy_cluster_map(rtrex_graph h, int64_t* out) {
     const uint32_t n = h->g.n;
     if (!n) return RTREX_OK;
     void* dout = nullptr;
-    if (h->seg_bounds.empty() && mapped_host(out, &dout)) {
-        k_cid_out<<<blocks_for(n, T, 4096), T, 0, h->stream>>>(h->cid, n,
-                                                               static_cast<int64_t*>(dout));
+    if ((h->seg_bounds.empty() || h->vseg_d) && mapped_host(out, &dout)) {
+        if (h->seg_bounds.empty())
+            k_cid_out<<<blocks_for(n, T, 4096), T, 0, h->stream>>>(h->cid, n,
+                                                                   static_cast<int64_t*>(dout));
+        else
+            k_cid_out_seg<<<blocks_for(n, T, 4096), T, 0, h->stream>>>(
+                h->cid, n, h->vseg_d, h->seg_bounds_d, static_cast<int64_t*>(dout));
         RT_HIP(hipGetLastError());
         RT_HIP(hipStreamSynchronize(h->stream));
         return RTREX_OK;

@@ -248,6 +248,8 @@ struct rtrex_graph_s {
     void* up_pin = nullptr;  // page-locked staging of an edge-list upload (pin cache)
     std::vector<uint32_t> seg_bounds;  // batch: first cluster id of each segment, and the total
     uint32_t* seg_pos_d = nullptr;     // batch: seg_vtx on the device (u32)
+    uint32_t* vseg_d = nullptr;        // batch: each vertex's segment (graph)
+    uint32_t* seg_bounds_d = nullptr;  // batch: seg_bounds on the device
     // Batch layout (one graph = one segment): vertex segment boundaries.
     std::vector<int64_t> seg_vtx;  // empty for a single graph
     uint32_t* nearst = nullptr;  // per edge: last peel round it was near-threshold

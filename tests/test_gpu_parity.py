@@ -345,6 +345,13 @@ def test_cluster_map_kept_in_hbm():
         maps, counts = b.decompose(0.3)
         flat = b.cluster_map()
         assert np.array_equal(flat, np.concatenate(maps))
+        # page-locked output: the device writes the per-graph ids itself
+        pin = R.host_empty(b.n, np.int64)
+        for _ in range(2):
+            pin.fill(-7)
+            maps2, counts2 = b.decompose(0.3, out=pin)
+            assert np.array_equal(counts2, counts)
+            assert all(np.array_equal(x, y) for x, y in zip(maps, maps2))
     finally:
         b.close()
 
