@@ -495,6 +495,7 @@ void rtrex_graph_destroy(rtrex_graph g) {
     for (void* p : g->allocs) dev_free(p);
     if (g->arena.base && g->arena.owned) dev_free(g->arena.base);
     pin_free(g->cm_pin);
+    pin_free(g->sb_pin);
     pin_free(g->up_pin);
     box_close(g);  // after the stream drained above
     bool pooled = false;

@@ -1812,14 +1812,17 @@ rtrex_status run_extract(rtrex_graph h, double eps) {
                                        x_undec_list ? &h->ctr->n_undecided : nullptr,
                                        &h->ctr->error, cidr + n, h->box_dev, fseq);
     RT_HIP(hipGetLastError());
-    h->seg_bounds.assign(ns ? ns + 1 : 0, 0);
+    if (ns && !h->sb_pin) {  // page-locked: a pageable landing takes the slow staged path
+        h->sb_pin = static_cast<uint32_t*>(pin_alloc((ns + 1) * 4));
+        if (!h->sb_pin) return fail(RTREX_ENOMEM, "page-locked staging allocation failed");
+    }
     if (ns)
-        RT_HIP(hipMemcpyAsync(h->seg_bounds.data(), bounds_d, (ns + 1) * 4,
-                              hipMemcpyDeviceToHost, s));
+        RT_HIP(hipMemcpyAsync(h->sb_pin, bounds_d, (ns + 1) * 4, hipMemcpyDeviceToHost, s));
     RT_HIP(hipEventRecord(h->ev[8], s));
     uint32_t hf[5] = {0, 0, 0, 0, 0};
     for (int i = 0; i < 5; ++i) RT_TRY(box_wait(h, 1 + i, fseq, &hf[i]));
     if (ns) RT_HIP(hipEventSynchronize(h->ev[8]));  // the segment bounds
+    h->seg_bounds.assign(h->sb_pin, h->sb_pin + (ns ? ns + 1 : 0));
     const uint32_t ncomp = hf[0], big = hf[1], nu = hf[2], cerr = hf[3], total = hf[4];
     if (cerr) return fail(RTREX_EDEVICE, "extraction left a vertex without a cluster");
     if (ncomp_d) {

@@ -245,6 +245,7 @@ struct rtrex_graph_s {
     // Cluster ids stay in HBM (cid); copy_cluster_map brings them to the host
     // on request through this pinned staging buffer (n entries, lazily).
     uint32_t* cm_pin = nullptr;
+    uint32_t* sb_pin = nullptr;  // batch: page-locked landing of seg_bounds
     void* up_pin = nullptr;  // page-locked staging of an edge-list upload (pin cache)
     std::vector<uint32_t> seg_bounds;  // batch: first cluster id of each segment, and the total
     uint32_t* seg_pos_d = nullptr;     // batch: seg_vtx on the device (u32)
